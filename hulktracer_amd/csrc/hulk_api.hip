@@ -1,0 +1,862 @@
+// hulk_api.hip - C ABI implementation: contexts, uploads, frame pipelines.
+//
+// Replaces GLVisitorGPUTrace's OpenCL host code (HulkEngine/src/
+// GLVisitorGPUTrace.cpp): initializeVisitor (:94-161) -> hulk_create,
+// buildTree / buildPerLeafData (:163-294) -> hulk_upload_scene +
+// hulk_upload_kdtree, traceScene / launchTraceIterations (:296-489) ->
+// hulk_trace_camera + hulk_shade / hulk_render_frame.  Every leaf is
+// resident in HBM, so the residency relaunch loop has no counterpart.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "hulk_abi.h"
+#include "hulk_kernels.hpp"
+
+using namespace hulk;
+
+namespace {
+
+constexpr uint32_t kTopLevelsMax = 12;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct hulk_ctx {
+    hulk_opts opts{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    int num_cus = 256;
+    size_t max_lds = 65536;
+
+    // scene
+    DevBuf verts, tris;
+    uint32_t n_verts = 0, n_tris = 0;
+    SceneBox box{};
+    bool have_scene = false;
+
+    // tree
+    DevBuf nodes, lp, recs;
+    uint32_t n_nodes = 0, n_lp = 0, max_jdepth = 0;
+    uint32_t level_prefix[kTopLevelsMax + 1] = {0};
+    bool have_tree = false;
+    std::vector<hulk_kd_node> host_nodes;  // original order, for ropes
+    bool ropes_ready = false;
+    DevBuf rope_nodes, rope_slots;
+    uint32_t n_slots = 0;
+
+    // lights
+    std::vector<hulk_light> lights;
+    DevBuf d_lights, d_point;
+    uint32_t n_point = 0;
+
+    // work buffers
+    DevBuf rays, hits, hits2, tbound, shadow_rays, shadow_slot, shadow_t, qrays[2], qpix[2], qw[2], acc, rgba,
+        mirror, ctr, stats;
+    unsigned long long* host_stats = nullptr;  // pinned
+    uint32_t* host_ctr = nullptr;              // pinned
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    hulk_counters cnt{};
+};
+
+namespace {
+
+int fail(hulk_ctx* c, int code, const char* msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(hulk_ctx* c, hipError_t e, const char* where) {
+    if (c) {
+        c->err = std::string(where) + ": " + hipGetErrorString(e);
+    }
+    return HULK_E_HIP;
+}
+
+#define HIP_TRY(ctx, expr)                                    \
+    do {                                                      \
+        hipError_t _e = (expr);                               \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
+    } while (0)
+
+int ensure(hulk_ctx* c, DevBuf& b, size_t bytes) {
+    if (bytes <= b.cap && b.p) return HULK_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        if (e != hipSuccess) return hip_fail(c, e, "hipFree");
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return hip_fail(c, e, "hipMalloc");
+    }
+    b.cap = want;
+    return HULK_OK;
+}
+
+void release(DevBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+int check_opts(hulk_ctx* c, const hulk_opts& o) {
+    if (o.variant < 0 || o.variant >= HULK_VARIANT_COUNT) return fail(c, HULK_E_ILLEGAL_INPUT, "unknown variant");
+    if (o.block_size < 64 || o.block_size > 1024 || (o.block_size % 64) != 0)
+        return fail(c, HULK_E_ILLEGAL_INPUT, "block_size must be a multiple of 64 in [64, 1024]");
+    if (o.short_stack_depth < 1 || o.short_stack_depth > kMaxShortStack)
+        return fail(c, HULK_E_ILLEGAL_INPUT, "short_stack_depth must be in [1, 16]");
+    if (o.lds_top_levels > kTopLevelsMax) return fail(c, HULK_E_ILLEGAL_INPUT, "lds_top_levels must be <= 12");
+    return HULK_OK;
+}
+
+Camera to_cam(const hulk_camera& c) {
+    Camera k;
+    k.from[0] = c.from.x; k.from[1] = c.from.y; k.from[2] = c.from.z;
+    k.vf[0] = c.vertical_factor.x; k.vf[1] = c.vertical_factor.y; k.vf[2] = c.vertical_factor.z;
+    k.hf[0] = c.horizontal_factor.x; k.hf[1] = c.horizontal_factor.y; k.hf[2] = c.horizontal_factor.z;
+    k.ll[0] = c.lower_left.x; k.ll[1] = c.lower_left.y; k.ll[2] = c.lower_left.z;
+    return k;
+}
+
+// Rope slots for the rope variant: slot s < n_nodes = tree node s, empty
+// children become virtual empty leaves numbered from n_nodes.  Junction
+// children are rewritten to slots; ropes: the above child's min face on the
+// split axis points at the below child and vice versa, everything else is
+// inherited from the parent; the root's ropes are kRopeNone.
+void build_ropes(const std::vector<hulk_kd_node>& nodes, const SceneBox& box, std::vector<uint4>& rn,
+                 std::vector<RopeSlot>& rs) {
+    const uint32_t n = static_cast<uint32_t>(nodes.size());
+    rn.assign(n, make_uint4(0, 0, 0, 0));
+    rs.assign(n, RopeSlot{});
+    struct Item {
+        uint32_t slot;
+        float lo[3], hi[3];
+        uint32_t r[6];
+    };
+    std::vector<Item> st;
+    Item root;
+    root.slot = 0;
+    for (int a = 0; a < 3; a++) {
+        root.lo[a] = box.mn[a];
+        root.hi[a] = box.mx[a];
+    }
+    for (int k = 0; k < 6; k++) root.r[k] = kRopeNone;
+    st.push_back(root);
+    while (!st.empty()) {
+        Item it = st.back();
+        st.pop_back();
+        RopeSlot s{};
+        s.lo = make_float4(it.lo[0], it.lo[1], it.lo[2], 0.0f);
+        s.hi = make_float4(it.hi[0], it.hi[1], it.hi[2], 0.0f);
+        for (int k = 0; k < 6; k++) s.rope[k] = it.r[k];
+        if (it.slot >= rs.size()) {
+            rs.resize(it.slot + 1);
+            rn.resize(it.slot + 1, make_uint4(0, 0, 0, 0));
+        }
+        rs[it.slot] = s;
+        if (it.slot >= n) {
+            rn[it.slot] = make_uint4(kNodeLeaf, 0, 0, 0);
+            continue;
+        }
+        const hulk_kd_node& nd = nodes[it.slot];
+        if (!(nd.flags & kNodeJunction)) {
+            rn[it.slot] = make_uint4(nd.flags, nd.u.leaf.start, nd.u.leaf.count, nd.u.leaf.leaf_id);
+            continue;
+        }
+        const uint32_t a = nd.flags & 3u;
+        uint32_t above = nd.u.junction.above, below = nd.u.junction.below;
+        uint32_t next = static_cast<uint32_t>(std::max<size_t>(rs.size(), n));
+        if (above == 0) above = next++;
+        if (below == 0) below = next++;
+        if (next > rs.size()) {
+            rs.resize(next);
+            rn.resize(next, make_uint4(0, 0, 0, 0));
+        }
+        uint32_t plane_bits;
+        std::memcpy(&plane_bits, &nd.u.junction.plane, 4);
+        rn[it.slot] = make_uint4(nd.flags, above, below, plane_bits);
+        Item ia = it, ib = it;
+        ia.slot = above;
+        ib.slot = below;
+        ia.lo[a] = nd.u.junction.plane;
+        ib.hi[a] = nd.u.junction.plane;
+        ia.r[2 * a] = below;
+        ib.r[2 * a + 1] = above;
+        st.push_back(ib);
+        st.push_back(ia);
+    }
+}
+
+int ensure_ropes(hulk_ctx* c) {
+    if (c->ropes_ready) return HULK_OK;
+    std::vector<uint4> rn;
+    std::vector<RopeSlot> rs;
+    build_ropes(c->host_nodes, c->box, rn, rs);
+    c->n_slots = static_cast<uint32_t>(rn.size());
+    int r;
+    if ((r = ensure(c, c->rope_nodes, rn.size() * sizeof(uint4)))) return r;
+    if ((r = ensure(c, c->rope_slots, rs.size() * sizeof(RopeSlot)))) return r;
+    HIP_TRY(c, hipMemcpyAsync(c->rope_nodes.p, rn.data(), rn.size() * sizeof(uint4), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->rope_slots.p, rs.data(), rs.size() * sizeof(RopeSlot), hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->ropes_ready = true;
+    return HULK_OK;
+}
+
+// Fill the tree / geometry part of a TraceParams and the LDS budget for the
+// context's options.
+int trace_setup(hulk_ctx* c, TraceParams& P, uint32_t& block, size_t& lds) {
+    if (!c->have_scene) return fail(c, HULK_E_NO_SCENE, "no scene uploaded");
+    const int v = c->opts.variant;
+    if (v != HULK_VARIANT_BRUTEFORCE && !c->have_tree) return fail(c, HULK_E_NO_TREE, "no KD tree uploaded");
+    if (v == HULK_VARIANT_ROPES) {
+        int r = ensure_ropes(c);
+        if (r) return r;
+    }
+    std::memset(&P, 0, sizeof(P));
+    P.nodes = static_cast<const uint4*>(c->nodes.p);
+    P.recs = static_cast<const TriRec*>(c->recs.p);
+    P.verts = static_cast<const float4*>(c->verts.p);
+    P.tris = static_cast<const uint4*>(c->tris.p);
+    P.n_tris = c->n_tris;
+    P.rope_nodes = static_cast<const uint4*>(c->rope_nodes.p);
+    P.rope_slots = static_cast<const RopeSlot*>(c->rope_slots.p);
+    P.box = c->box;
+    P.guard = 4u * (c->n_nodes + c->n_slots) + 4096u;
+    block = c->opts.block_size;
+    uint32_t lds_nodes = 0;
+    if (v != HULK_VARIANT_BRUTEFORCE && v != HULK_VARIANT_ROPES)
+        lds_nodes = c->level_prefix[std::min(c->opts.lds_top_levels, kTopLevelsMax)];
+    P.lds_node_count = lds_nodes;
+    size_t stack_words = 0;
+    if (v == HULK_VARIANT_FULLSTACK) {
+        P.stack_depth = std::max<uint32_t>(c->max_jdepth, 1u);
+        stack_words = 2u * P.stack_depth;
+    } else if (v == HULK_VARIANT_SHORTSTACK || v == HULK_VARIANT_LDSSTACK) {
+        P.stack_depth = c->opts.short_stack_depth;
+        if (v == HULK_VARIANT_LDSSTACK) stack_words = 3u * P.stack_depth;
+    }
+    lds = (size_t)lds_nodes * 16u + stack_words * 4u * block;
+    if (lds > c->max_lds) {
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "LDS need %zu B exceeds the per-block limit %zu B (lower block_size or lds_top_levels)",
+                      lds, c->max_lds);
+        return fail(c, HULK_E_ILLEGAL_INPUT, buf);
+    }
+    P.counters = static_cast<unsigned long long*>(c->stats.p);
+    P.work_counter = static_cast<uint32_t*>(c->ctr.p);
+    return HULK_OK;
+}
+
+int run_trace(hulk_ctx* c, TraceParams& P, uint32_t n_max, uint32_t block, size_t lds, bool count, bool persistent,
+              bool timed) {
+    P.persistent = persistent ? 1u : 0u;
+    uint32_t grid;
+    if (persistent) {
+        HIP_TRY(c, launch_fill_u32(P.work_counter, 1, 0u, c->stream));
+        const uint32_t per_cu = std::max<uint32_t>(1u, 2048u / block);
+        grid = std::max<uint32_t>(1u, std::min<uint32_t>((n_max + block - 1) / block, c->num_cus * per_cu));
+    } else {
+        grid = std::max<uint32_t>(1u, (n_max + block - 1) / block);
+    }
+    if (n_max == 0) return HULK_OK;
+    if (timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, launch_trace(c->opts.variant, P, count, grid, block, lds, c->stream));
+    if (timed) HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return HULK_OK;
+}
+
+int zero_stats(hulk_ctx* c) {
+    HIP_TRY(c, launch_fill_u32(static_cast<uint32_t*>(c->stats.p), 16, 0u, c->stream));
+    return HULK_OK;
+}
+
+int read_stats(hulk_ctx* c) {
+    HIP_TRY(c, hipMemcpyAsync(c->host_stats, c->stats.p, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->cnt.rays += c->host_stats[0];
+    c->cnt.node_visits += c->host_stats[1];
+    c->cnt.leaf_visits += c->host_stats[2];
+    c->cnt.tri_tests += c->host_stats[3];
+    c->cnt.hits += c->host_stats[4];
+    c->cnt.fallbacks += c->host_stats[5];
+    return HULK_OK;
+}
+
+int finish(hulk_ctx* c, uint32_t flags, bool timed) {
+    if (flags & HULK_FLAG_NO_SYNC) return HULK_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (timed) {
+        float ms = 0.0f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->cnt.last_kernel_ms = ms;
+    }
+    return HULK_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ ABI --
+
+extern "C" int hulk_create(hulk_ctx** out, const hulk_opts* opts) {
+    if (!out) return HULK_E_ILLEGAL_INPUT;
+    *out = nullptr;
+    hulk_ctx* c = new (std::nothrow) hulk_ctx;
+    if (!c) return HULK_E_ALLOC;
+    hulk_opts o{};
+    o.device = 0;
+    o.variant = HULK_VARIANT_FULLSTACK;
+    o.block_size = 256;
+    o.short_stack_depth = 6;  // c_shortStackDepth, SharedStructs.hpp:19
+    o.lds_top_levels = 0;
+    o.persistent = 0;
+    o.stream = nullptr;
+    if (opts) o = *opts;
+    int r = check_opts(c, o);
+    if (r) {
+        delete c;
+        return r;
+    }
+    c->opts = o;
+    hipError_t e = hipSetDevice(o.device);
+    if (e != hipSuccess) {
+        delete c;
+        return HULK_E_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) {
+        c->num_cus = prop.multiProcessorCount;
+        c->max_lds = std::min<size_t>(prop.sharedMemPerBlock, 65536);
+    }
+    if (o.stream) {
+        c->stream = static_cast<hipStream_t>(o.stream);
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return HULK_E_HIP;
+        }
+        c->own_stream = true;
+    }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess || ensure(c, c->ctr, 64) || ensure(c, c->stats, 128) ||
+        hipHostMalloc(reinterpret_cast<void**>(&c->host_stats), 128, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&c->host_ctr), 64, hipHostMallocDefault) != hipSuccess) {
+        hulk_destroy(c);
+        return HULK_E_HIP;
+    }
+    *out = c;
+    return HULK_OK;
+}
+
+extern "C" void hulk_destroy(hulk_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DevBuf* bufs[] = {&c->verts, &c->tris, &c->nodes, &c->lp, &c->recs, &c->rope_nodes, &c->rope_slots,
+                      &c->d_lights, &c->d_point, &c->rays, &c->hits, &c->hits2, &c->tbound, &c->shadow_rays,
+                      &c->shadow_slot, &c->shadow_t, &c->qrays[0], &c->qrays[1], &c->qpix[0], &c->qpix[1],
+                      &c->qw[0], &c->qw[1], &c->acc, &c->rgba, &c->mirror, &c->ctr, &c->stats};
+    for (DevBuf* b : bufs) release(*b);
+    if (c->host_stats) (void)hipHostFree(c->host_stats);
+    if (c->host_ctr) (void)hipHostFree(c->host_ctr);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char* hulk_last_error(const hulk_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int hulk_set_options(hulk_ctx* c, const hulk_opts* o) {
+    if (!c || !o) return HULK_E_ILLEGAL_INPUT;
+    if (o->device != c->opts.device || o->stream != c->opts.stream)
+        return fail(c, HULK_E_ILLEGAL_INPUT, "device and stream are fixed at hulk_create");
+    int r = check_opts(c, *o);
+    if (r) return r;
+    c->opts = *o;
+    return HULK_OK;
+}
+
+extern "C" int hulk_get_options(const hulk_ctx* c, hulk_opts* o) {
+    if (!c || !o) return HULK_E_ILLEGAL_INPUT;
+    *o = c->opts;
+    return HULK_OK;
+}
+
+extern "C" void* hulk_stream(hulk_ctx* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
+
+extern "C" int hulk_sync(hulk_ctx* c) {
+    if (!c) return HULK_E_ILLEGAL_INPUT;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HULK_OK;
+}
+
+extern "C" int hulk_upload_scene(hulk_ctx* c, const hulk_float4* verts, uint32_t n_verts, const hulk_tri* tris,
+                                 uint32_t n_tris, const hulk_box* scene_box) {
+    if (!c || !scene_box || (n_verts && !verts) || (n_tris && !tris))
+        return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    for (uint32_t i = 0; i < n_tris; i++)
+        for (int k = 0; k < 3; k++)
+            if (tris[i].idx[k] >= n_verts) return fail(c, HULK_E_ILLEGAL_INPUT, "triangle vertex index out of range");
+    int r;
+    if ((r = ensure(c, c->verts, (size_t)n_verts * 16))) return r;
+    if ((r = ensure(c, c->tris, (size_t)n_tris * 16))) return r;
+    if (n_verts)
+        HIP_TRY(c, hipMemcpyAsync(c->verts.p, verts, (size_t)n_verts * 16, hipMemcpyHostToDevice, c->stream));
+    if (n_tris) HIP_TRY(c, hipMemcpyAsync(c->tris.p, tris, (size_t)n_tris * 16, hipMemcpyHostToDevice, c->stream));
+    c->n_verts = n_verts;
+    c->n_tris = n_tris;
+    c->box.mn[0] = scene_box->min.x; c->box.mn[1] = scene_box->min.y; c->box.mn[2] = scene_box->min.z;
+    c->box.mx[0] = scene_box->max.x; c->box.mx[1] = scene_box->max.y; c->box.mx[2] = scene_box->max.z;
+    c->have_scene = true;
+    c->have_tree = false;  // records depend on the vertices
+    c->ropes_ready = false;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HULK_OK;
+}
+
+extern "C" int hulk_upload_kdtree(hulk_ctx* c, const hulk_kd_node* nodes, uint32_t n_nodes,
+                                  const uint32_t* leaf_polys, uint32_t n_lp) {
+    if (!c) return HULK_E_ILLEGAL_INPUT;
+    if (!c->have_scene) return fail(c, HULK_E_NO_SCENE, "upload the scene before the tree");
+    if (!nodes || n_nodes == 0 || (n_lp && !leaf_polys)) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    // Validate: children in range, every node reached once from the root
+    // (no cycles / sharing), leaf ranges inside leaf_polys, references < n_tris.
+    std::vector<uint8_t> seen(n_nodes, 0);
+    std::vector<uint32_t> depth(n_nodes, 0);
+    std::vector<uint32_t> order;
+    order.reserve(n_nodes);
+    std::deque<uint32_t> q;
+    q.push_back(0);
+    seen[0] = 1;
+    uint32_t max_jdepth = 0;
+    std::vector<uint32_t> perm(n_nodes, 0);
+    // breadth-first for depth < kTopLevelsMax, depth-first below
+    std::vector<uint32_t> dfs;
+    uint32_t prefix[kTopLevelsMax + 1] = {0};
+    auto visit_children = [&](uint32_t i, auto&& push) -> int {
+        const hulk_kd_node& n = nodes[i];
+        if (n.flags & HULK_NODE_LEAF) {
+            if ((uint64_t)n.u.leaf.start + n.u.leaf.count > n_lp)
+                return fail(c, HULK_E_ILLEGAL_INPUT, "leaf range outside leaf_polys");
+            return HULK_OK;
+        }
+        if (!(n.flags & HULK_NODE_JUNCTION)) return fail(c, HULK_E_ILLEGAL_INPUT, "node is neither leaf nor junction");
+        if ((n.flags & 3u) == 3u) return fail(c, HULK_E_ILLEGAL_INPUT, "junction axis out of range");
+        max_jdepth = std::max(max_jdepth, depth[i] + 1);
+        for (uint32_t ch : {n.u.junction.above, n.u.junction.below}) {
+            if (ch == 0) continue;
+            if (ch >= n_nodes) return fail(c, HULK_E_ILLEGAL_INPUT, "child index out of range");
+            if (seen[ch]) return fail(c, HULK_E_ILLEGAL_INPUT, "tree has a cycle or a shared child");
+            seen[ch] = 1;
+            depth[ch] = depth[i] + 1;
+            push(ch);
+        }
+        return HULK_OK;
+    };
+    while (!q.empty()) {
+        const uint32_t i = q.front();
+        q.pop_front();
+        if (depth[i] < kTopLevelsMax) {
+            perm[i] = static_cast<uint32_t>(order.size());
+            order.push_back(i);
+            int r = visit_children(i, [&](uint32_t ch) { q.push_back(ch); });
+            if (r) return r;
+        } else {
+            // depth-first pre-order, above child first
+            dfs.clear();
+            dfs.push_back(i);
+            while (!dfs.empty()) {
+                const uint32_t k = dfs.back();
+                dfs.pop_back();
+                perm[k] = static_cast<uint32_t>(order.size());
+                order.push_back(k);
+                std::vector<uint32_t> kids;
+                int r = visit_children(k, [&](uint32_t ch) { kids.push_back(ch); });
+                if (r) return r;
+                for (auto it = kids.rbegin(); it != kids.rend(); ++it) dfs.push_back(*it);
+            }
+        }
+    }
+    if (order.size() != n_nodes) return fail(c, HULK_E_ILLEGAL_INPUT, "tree has nodes unreachable from the root");
+    for (uint32_t i = 0; i < n_lp; i++)
+        if (leaf_polys[i] >= c->n_tris) return fail(c, HULK_E_ILLEGAL_INPUT, "leaf polygon index out of range");
+    for (uint32_t d = 0; d <= kTopLevelsMax; d++) {
+        uint32_t k = 0;
+        while (k < order.size() && depth[order[k]] < d) k++;
+        prefix[d] = k;
+    }
+    if (max_jdepth > kMaxFullStack) return fail(c, HULK_E_ILLEGAL_INPUT, "tree deeper than 64 junctions");
+    std::vector<uint4> dev(n_nodes);
+    for (uint32_t k = 0; k < n_nodes; k++) {
+        const hulk_kd_node& n = nodes[order[k]];
+        if (n.flags & HULK_NODE_LEAF) {
+            dev[k] = make_uint4(n.flags, n.u.leaf.start, n.u.leaf.count, n.u.leaf.leaf_id);
+        } else {
+            uint32_t plane;
+            std::memcpy(&plane, &n.u.junction.plane, 4);
+            const uint32_t a = n.u.junction.above ? perm[n.u.junction.above] : 0;
+            const uint32_t b = n.u.junction.below ? perm[n.u.junction.below] : 0;
+            dev[k] = make_uint4(n.flags, a, b, plane);
+        }
+    }
+    int r;
+    if ((r = ensure(c, c->nodes, (size_t)n_nodes * 16))) return r;
+    if ((r = ensure(c, c->lp, (size_t)std::max<uint32_t>(n_lp, 1) * 4))) return r;
+    if ((r = ensure(c, c->recs, (size_t)std::max<uint32_t>(n_lp, 1) * sizeof(TriRec)))) return r;
+    HIP_TRY(c, hipMemcpyAsync(c->nodes.p, dev.data(), (size_t)n_nodes * 16, hipMemcpyHostToDevice, c->stream));
+    if (n_lp) HIP_TRY(c, hipMemcpyAsync(c->lp.p, leaf_polys, (size_t)n_lp * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_build_records(static_cast<const float4*>(c->verts.p), static_cast<const uint4*>(c->tris.p),
+                                    static_cast<const uint32_t*>(c->lp.p), n_lp, static_cast<TriRec*>(c->recs.p),
+                                    c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->n_nodes = n_nodes;
+    c->n_lp = n_lp;
+    c->max_jdepth = max_jdepth;
+    std::memcpy(c->level_prefix, prefix, sizeof(prefix));
+    c->host_nodes.assign(nodes, nodes + n_nodes);
+    c->ropes_ready = false;
+    c->have_tree = true;
+    return HULK_OK;
+}
+
+extern "C" int hulk_set_lights(hulk_ctx* c, const hulk_light* lights, uint32_t n) {
+    if (!c || (n && !lights)) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    c->lights.assign(lights, lights + n);
+    std::vector<float4> pts;
+    for (uint32_t k = 0; k < n; k++)
+        if (lights[k].color.w == HULK_LIGHT_POINT)
+            pts.push_back(make_float4(lights[k].data.x, lights[k].data.y, lights[k].data.z, 0.0f));
+    c->n_point = static_cast<uint32_t>(pts.size());
+    int r;
+    if ((r = ensure(c, c->d_lights, std::max<size_t>(n, 1) * sizeof(hulk_light)))) return r;
+    if ((r = ensure(c, c->d_point, std::max<size_t>(pts.size(), 1) * sizeof(float4)))) return r;
+    if (n) HIP_TRY(c, hipMemcpyAsync(c->d_lights.p, lights, n * sizeof(hulk_light), hipMemcpyHostToDevice, c->stream));
+    if (!pts.empty())
+        HIP_TRY(c, hipMemcpyAsync(c->d_point.p, pts.data(), pts.size() * sizeof(float4), hipMemcpyHostToDevice,
+                                  c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HULK_OK;
+}
+
+namespace {
+
+// Resolve an in/out buffer: device pointers are used as is; host pointers
+// are staged through a context buffer.
+template <typename T>
+int stage_in(hulk_ctx* c, const T* src, size_t count, bool device_ptrs, DevBuf& buf, const T** dst) {
+    if (!src) {
+        *dst = nullptr;
+        return HULK_OK;
+    }
+    if (device_ptrs) {
+        *dst = src;
+        return HULK_OK;
+    }
+    int r = ensure(c, buf, count * sizeof(T));
+    if (r) return r;
+    HIP_TRY(c, hipMemcpyAsync(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    *dst = static_cast<const T*>(buf.p);
+    return HULK_OK;
+}
+
+int trace_impl(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H, const hulk_ray* rays, uint32_t n_rays,
+               const float* tbound, hulk_hit* hits, uint32_t flags) {
+    TraceParams P;
+    uint32_t block;
+    size_t lds;
+    int r = trace_setup(c, P, block, lds);
+    if (r) return r;
+    const bool dev = (flags & HULK_FLAG_DEVICE_PTRS) != 0;
+    const bool count = (flags & HULK_FLAG_COUNT) != 0;
+    const uint32_t n = cam ? W * H : n_rays;
+    if (cam) {
+        P.source = kSourceCamera;
+        P.cam = to_cam(*cam);
+        P.W = W;
+        P.H = H;
+    } else {
+        P.source = kSourceRays;
+        if ((r = stage_in(c, rays, n, dev, c->rays, &P.rays))) return r;
+    }
+    P.n = n;
+    if ((r = stage_in(c, tbound, n, dev, c->tbound, &P.tbound))) return r;
+    hulk_hit* out = hits;
+    if (!dev) {
+        if ((r = ensure(c, c->hits, (size_t)n * sizeof(hulk_hit)))) return r;
+        out = static_cast<hulk_hit*>(c->hits.p);
+    }
+    P.hits = out;
+    if (count && (r = zero_stats(c))) return r;
+    if ((r = run_trace(c, P, n, block, lds, count, c->opts.persistent != 0, true))) return r;
+    if (!dev && n) HIP_TRY(c, hipMemcpyAsync(hits, out, (size_t)n * sizeof(hulk_hit), hipMemcpyDeviceToHost, c->stream));
+    if (count && (r = read_stats(c))) return r;
+    return finish(c, flags, true);
+}
+
+}  // namespace
+
+extern "C" int hulk_trace_camera(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H, const float* tbound,
+                                 hulk_hit* hits, uint32_t flags) {
+    if (!c || !cam || !hits) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    if ((uint64_t)W * H > 0xFFFFFFC0ull) return fail(c, HULK_E_ILLEGAL_INPUT, "image too large");
+    return trace_impl(c, cam, W, H, nullptr, 0, tbound, hits, flags);
+}
+
+extern "C" int hulk_trace_rays(hulk_ctx* c, const hulk_ray* rays, uint32_t n, const float* tbound, hulk_hit* hits,
+                               uint32_t flags) {
+    if (!c || (n && (!rays || !hits))) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    if (n > 0xFFFFFFC0u) return fail(c, HULK_E_ILLEGAL_INPUT, "too many rays");
+    return trace_impl(c, nullptr, 0, 0, rays, n, tbound, hits, flags);
+}
+
+namespace {
+
+// Frame pipeline on device buffers: primary hits in `hits0` (already
+// traced), then per generation: shadow rays -> trace -> shade; mirror rays
+// -> trace next generation.  Accumulates into acc and packs into rgba.
+int shade_pipeline(hulk_ctx* c, const hulk_camera& cam, uint32_t W, uint32_t H, const hulk_hit* hits0,
+                   uint32_t max_bounce, const uint8_t* mirror_dev, uint32_t* rgba_dev, bool count) {
+    const uint32_t n = W * H;
+    int r;
+    const uint32_t np = c->n_point;
+    if ((r = ensure(c, c->acc, (size_t)n * 12))) return r;
+    if ((r = ensure(c, c->shadow_rays, (size_t)n * std::max<uint32_t>(np, 1) * sizeof(hulk_ray)))) return r;
+    if ((r = ensure(c, c->shadow_slot, (size_t)n * std::max<uint32_t>(np, 1) * 4))) return r;
+    if ((r = ensure(c, c->shadow_t, (size_t)n * std::max<uint32_t>(np, 1) * 4))) return r;
+    if (max_bounce) {
+        for (int k = 0; k < 2; k++) {
+            if ((r = ensure(c, c->qrays[k], (size_t)n * sizeof(hulk_ray)))) return r;
+            if ((r = ensure(c, c->qpix[k], (size_t)n * 4))) return r;
+            if ((r = ensure(c, c->qw[k], (size_t)n * 4))) return r;
+        }
+        if ((r = ensure(c, c->hits2, (size_t)n * sizeof(hulk_hit)))) return r;
+    }
+    uint32_t* ctr = static_cast<uint32_t*>(c->ctr.p);  // [0] work, [1] shadow count, [2..3] queue counts
+    HIP_TRY(c, launch_fill_f32(static_cast<float*>(c->acc.p), (size_t)n * 3, 0.0f, c->stream));
+    Camera kc = to_cam(cam);
+    TraceParams P;
+    uint32_t block;
+    size_t lds;
+    if ((r = trace_setup(c, P, block, lds))) return r;
+
+    const hulk_hit* cur_hits = hits0;
+    const hulk_ray* cur_rays = nullptr;
+    const uint32_t* cur_pix = nullptr;
+    const float* cur_w = nullptr;
+    const uint32_t* cur_n = nullptr;
+    for (uint32_t g = 0; g <= max_bounce; g++) {
+        const bool spawn = g < max_bounce && mirror_dev;
+        HIP_TRY(c, launch_fill_u32(ctr + 1, 1, 0u, c->stream));
+        const int qo = g & 1;  // output queue of this generation
+        if (spawn) HIP_TRY(c, launch_fill_u32(ctr + 2 + qo, 1, 0u, c->stream));
+        GenParams G;
+        std::memset(&G, 0, sizeof(G));
+        G.hits = cur_hits;
+        G.camera_rays = g == 0;
+        G.cam = kc;
+        G.W = W;
+        G.H = H;
+        G.rays = cur_rays;
+        G.pixel_in = cur_pix;
+        G.weight_in = cur_w;
+        G.n_dev = g == 0 ? nullptr : cur_n;
+        G.n = n;
+        G.point_pos = static_cast<const float4*>(c->d_point.p);
+        G.n_point = np;
+        G.shadow_rays = static_cast<hulk_ray*>(c->shadow_rays.p);
+        G.shadow_slot = static_cast<uint32_t*>(c->shadow_slot.p);
+        G.shadow_count = ctr + 1;
+        G.mirror = mirror_dev;
+        if (spawn) {
+            G.next_rays = static_cast<hulk_ray*>(c->qrays[qo].p);
+            G.pixel_out = static_cast<uint32_t*>(c->qpix[qo].p);
+            G.weight_out = static_cast<float*>(c->qw[qo].p);
+            G.next_count = ctr + 2 + qo;
+        }
+        HIP_TRY(c, launch_gen_secondary(G, n, c->stream));
+        // shadow rays: nearest hit, t scattered to the (ray, light) slot
+        if (np) {
+            TraceParams S = P;
+            S.source = kSourceRays;
+            S.rays = static_cast<const hulk_ray*>(c->shadow_rays.p);
+            S.n_dev = ctr + 1;
+            S.n = n * np;
+            S.t_out = static_cast<float*>(c->shadow_t.p);
+            S.t_slot = static_cast<const uint32_t*>(c->shadow_slot.p);
+            // the work counter of a persistent launch is ctr[0]
+            if ((r = run_trace(c, S, n * np, block, lds, count, true, false))) return r;
+        }
+        ShadeParams SP;
+        std::memset(&SP, 0, sizeof(SP));
+        SP.hits = cur_hits;
+        SP.camera_rays = g == 0;
+        SP.cam = kc;
+        SP.W = W;
+        SP.H = H;
+        SP.rays = cur_rays;
+        SP.pixel_in = cur_pix;
+        SP.weight_in = cur_w;
+        SP.n_dev = g == 0 ? nullptr : cur_n;
+        SP.n = n;
+        SP.lights = static_cast<const float4*>(c->d_lights.p);
+        SP.n_lights = static_cast<uint32_t>(c->lights.size());
+        SP.n_point = np;
+        SP.shadow_t = static_cast<const float*>(c->shadow_t.p);
+        SP.mirror = mirror_dev;
+        SP.spawned = spawn ? 1u : 0u;
+        SP.acc = static_cast<float*>(c->acc.p);
+        HIP_TRY(c, launch_shade_accumulate(SP, n, c->stream));
+        if (!spawn) break;
+        // trace the next generation
+        TraceParams Q = P;
+        Q.source = kSourceRays;
+        Q.rays = static_cast<const hulk_ray*>(c->qrays[qo].p);
+        Q.n_dev = ctr + 2 + qo;
+        Q.n = n;
+        // generation buffers alternate hits2, hits, hits2, ...; the primary
+        // hits (possibly c->hits) are dead once generation 0 is shaded
+        if (g > 0 && (r = ensure(c, c->hits, (size_t)n * sizeof(hulk_hit)))) return r;
+        hulk_hit* nh = static_cast<hulk_hit*>((g & 1) ? c->hits.p : c->hits2.p);
+        Q.hits = nh;
+        if ((r = run_trace(c, Q, n, block, lds, count, true, false))) return r;
+        cur_hits = nh;
+        cur_rays = static_cast<const hulk_ray*>(c->qrays[qo].p);
+        cur_pix = static_cast<const uint32_t*>(c->qpix[qo].p);
+        cur_w = static_cast<const float*>(c->qw[qo].p);
+        cur_n = ctr + 2 + qo;
+    }
+    HIP_TRY(c, launch_pack(static_cast<const float*>(c->acc.p), n, rgba_dev, c->stream));
+    return HULK_OK;
+}
+
+}  // namespace
+
+extern "C" int hulk_shade(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H, const hulk_hit* hits,
+                          uint32_t* rgba8, uint32_t mode, uint32_t flags) {
+    if (!c || !cam || !hits || !rgba8) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    if (mode != HULK_SHADE_FUSED && mode != HULK_SHADE_PICTURE) return fail(c, HULK_E_ILLEGAL_INPUT, "bad shade mode");
+    const bool dev = (flags & HULK_FLAG_DEVICE_PTRS) != 0;
+    const bool count = (flags & HULK_FLAG_COUNT) != 0;
+    const uint32_t n = W * H;
+    int r;
+    const hulk_hit* dh;
+    if ((r = stage_in(c, hits, n, dev, c->hits2, &dh))) return r;
+    uint32_t* out = rgba8;
+    if (!dev) {
+        if ((r = ensure(c, c->rgba, (size_t)n * 4))) return r;
+        out = static_cast<uint32_t*>(c->rgba.p);
+    }
+    if (count && (r = zero_stats(c))) return r;
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    if (mode == HULK_SHADE_PICTURE) {
+        ShadeParams SP;
+        std::memset(&SP, 0, sizeof(SP));
+        SP.hits = dh;
+        SP.camera_rays = 1;
+        SP.cam = to_cam(*cam);
+        SP.W = W;
+        SP.H = H;
+        SP.n = n;
+        SP.lights = static_cast<const float4*>(c->d_lights.p);
+        SP.n_lights = static_cast<uint32_t>(c->lights.size());
+        HIP_TRY(c, launch_shade_picture(SP, out, c->stream));
+    } else {
+        // the primary hits must not alias the generation buffers
+        if (!dev) {
+            if ((r = ensure(c, c->hits, (size_t)n * sizeof(hulk_hit)))) return r;
+            HIP_TRY(c, hipMemcpyAsync(c->hits.p, dh, (size_t)n * sizeof(hulk_hit), hipMemcpyDeviceToDevice, c->stream));
+            dh = static_cast<const hulk_hit*>(c->hits.p);
+        }
+        if ((r = shade_pipeline(c, *cam, W, H, dh, 0, nullptr, out, count))) return r;
+    }
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    if (!dev) HIP_TRY(c, hipMemcpyAsync(rgba8, out, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (count && (r = read_stats(c))) return r;
+    if (flags & HULK_FLAG_NO_SYNC) return HULK_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->cnt.last_shade_ms = ms;
+    return HULK_OK;
+}
+
+extern "C" int hulk_render_frame(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H, uint32_t max_bounce,
+                                 const uint8_t* mirror_mask, const float* tbound, uint32_t* rgba8,
+                                 hulk_hit* primary_hits, uint32_t flags) {
+    if (!c || !cam || !rgba8) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    if (max_bounce > 8) return fail(c, HULK_E_ILLEGAL_INPUT, "max_bounce must be <= 8");
+    const bool dev = (flags & HULK_FLAG_DEVICE_PTRS) != 0;
+    const bool count = (flags & HULK_FLAG_COUNT) != 0;
+    const uint32_t n = W * H;
+    int r;
+    TraceParams P;
+    uint32_t block;
+    size_t lds;
+    if ((r = trace_setup(c, P, block, lds))) return r;
+    const uint8_t* dm = nullptr;
+    if (mirror_mask && max_bounce) {
+        if ((r = stage_in(c, mirror_mask, c->n_tris, dev, c->mirror, &dm))) return r;
+    }
+    const float* dt = nullptr;
+    if ((r = stage_in(c, tbound, n, dev, c->tbound, &dt))) return r;
+    hulk_hit* h0 = nullptr;
+    if (dev && primary_hits) {
+        h0 = primary_hits;
+    } else {
+        if ((r = ensure(c, c->hits, (size_t)n * sizeof(hulk_hit)))) return r;
+        h0 = static_cast<hulk_hit*>(c->hits.p);
+    }
+    uint32_t* out = rgba8;
+    if (!dev) {
+        if ((r = ensure(c, c->rgba, (size_t)n * 4))) return r;
+        out = static_cast<uint32_t*>(c->rgba.p);
+    }
+    if (count && (r = zero_stats(c))) return r;
+    HIP_TRY(c, hipEventRecord(c->ev2, c->stream));
+    P.source = kSourceCamera;
+    P.cam = to_cam(*cam);
+    P.W = W;
+    P.H = H;
+    P.n = n;
+    P.tbound = dt;
+    P.hits = h0;
+    if ((r = run_trace(c, P, n, block, lds, count, c->opts.persistent != 0, true))) return r;
+    if ((r = shade_pipeline(c, *cam, W, H, h0, max_bounce, dm, out, count))) return r;
+    if (!dev) {
+        HIP_TRY(c, hipMemcpyAsync(rgba8, out, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+        if (primary_hits)
+            HIP_TRY(c, hipMemcpyAsync(primary_hits, h0, (size_t)n * sizeof(hulk_hit), hipMemcpyDeviceToHost, c->stream));
+    }
+    if (count && (r = read_stats(c))) return r;
+    return finish(c, flags, true);
+}
+
+extern "C" int hulk_get_counters(const hulk_ctx* c, hulk_counters* out) {
+    if (!c || !out) return HULK_E_ILLEGAL_INPUT;
+    *out = c->cnt;
+    return HULK_OK;
+}
+
+extern "C" int hulk_reset_counters(hulk_ctx* c) {
+    if (!c) return HULK_E_ILLEGAL_INPUT;
+    c->cnt = hulk_counters{};
+    return HULK_OK;
+}

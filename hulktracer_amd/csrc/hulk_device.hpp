@@ -1,0 +1,192 @@
+// hulk_device.hpp - device-side arithmetic and HBM layouts for gfx950.
+//
+// Every function here restates one piece of the reference GPU arithmetic
+// with a fixed operation order, compiled with -ffp-contract=off so that no
+// multiply-add is fused unless written as __builtin_fmaf (the reference
+// writes fma() in exactly three places, SharedFunctions.hpp:86-89).
+// Conventions:
+//   dot(a, b)    = (a.x*b.x + a.y*b.y) + a.z*b.z
+//   cross(a, b)  = (a.y*b.z - a.z*b.y, a.z*b.x - a.x*b.z, a.x*b.y - a.y*b.x)
+//   normalize(v) = v * (1 / sqrt(dot(v, v)))       (IEEE sqrt and divide)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hulk {
+
+constexpr uint32_t kNodeLeaf = 4u;
+constexpr uint32_t kNodeJunction = 8u;
+constexpr uint32_t kNoHit = 0xFFFFFFFFu;
+constexpr uint32_t kRopeNone = 0xFFFFFFFFu;
+constexpr float kFltMax = 3.402823466e+38f;
+constexpr float kFltEpsilon = 1.192092896e-07f;
+
+struct V3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 scale(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ V3 normalize(V3 v) { return scale(v, 1.0f / __builtin_sqrtf(dot(v, v))); }
+__device__ __forceinline__ float lane(V3 v, uint32_t axis) { return axis == 0 ? v.x : (axis == 1 ? v.y : v.z); }
+
+// Per-leaf-reference triangle record, 64 B, 16-B aligned, one record per
+// entry of m_leafPolygons so a leaf's records are contiguous:
+//   a = (n.x, n.y, n.z, triangle id bits)
+//   b = (p1.x, p1.y, p1.z, e00)
+//   c = (e0.x, e0.y, e0.z, e01)
+//   d = (e1.x, e1.y, e1.z, e11)
+// n, e0, e1 and the Gram terms are the values checkRayToPoly computes per
+// test (SharedFunctions.hpp:53-85), computed once by build_records with the
+// same operations, so the per-test arithmetic that depends on the ray is
+// unchanged and bit-identical.
+struct TriRec {
+    float4 a, b, c, d;
+};
+
+// Ray state shared by all traversal variants.
+struct Ray {
+    V3 o, d, inv;
+};
+
+struct Best {
+    float t, s1, s2, det;
+    uint32_t tri;
+    float nx, ny, nz;
+};
+
+__device__ __forceinline__ void best_init(Best& b) {
+    b.t = kFltMax;
+    b.s1 = b.s2 = 0.0f;
+    b.det = 1.0f;
+    b.tri = kNoHit;
+    b.nx = b.ny = b.nz = 0.0f;
+}
+
+// checkRayToPoly (SharedFunctions.hpp:47-102) on a precomputed record, then
+// the strict '<' update of testNodePolygons (GPU_common.cl:62-68).  A test
+// whose t is not below best.t cannot be kept, so it stops there.
+__device__ __forceinline__ bool test_record(const Ray& r, const TriRec& rec, Best& best) {
+    const V3 n = v3(rec.a.x, rec.a.y, rec.a.z);
+    float den = dot(n, r.d);
+    if (__builtin_fabsf(den) < kFltEpsilon) return false;
+    den = 1.0f / den;
+    const V3 p1 = v3(rec.b.x, rec.b.y, rec.b.z);
+    const float t = dot(n, sub(p1, r.o)) * den;
+    if (t < 0.0f) return false;
+    if (!(t < best.t)) return false;
+    const V3 P = add(r.o, scale(r.d, t));
+    const V3 h = sub(P, p1);
+    const V3 e0 = v3(rec.c.x, rec.c.y, rec.c.z);
+    const V3 e1 = v3(rec.d.x, rec.d.y, rec.d.z);
+    const float q0 = dot(e0, h);
+    const float q1 = dot(e1, h);
+    const float e00 = rec.b.w, e01 = rec.c.w, e11 = rec.d.w;
+    const float det = __builtin_fmaf(e00, e11, -(e01 * e01));
+    const float s1 = __builtin_fmaf(e11, q0, -(e01 * q1));
+    const float s2 = __builtin_fmaf(e00, q1, -(e01 * q0));
+    if (s1 >= 0.0f && s2 >= 0.0f && (s1 + s2) <= det) {
+        best.t = t;
+        best.s1 = s1;
+        best.s2 = s2;
+        best.det = det;
+        best.tri = __float_as_uint(rec.a.w);
+        best.nx = n.x;
+        best.ny = n.y;
+        best.nz = n.z;
+        return true;
+    }
+    return false;
+}
+
+// The same test straight from three vertices (brute force variant,
+// GPUTrace_bruteforce.cl:30-42).
+__device__ __forceinline__ void test_vertices(const Ray& r, V3 p1, V3 p2, V3 p3, uint32_t id, Best& best) {
+    const V3 e0 = sub(p2, p1), e1 = sub(p3, p1), e2 = sub(p3, p2);
+    const V3 n = normalize(add(add(cross(e0, e1), cross(e1, e2)), cross(e2, e0)));
+    TriRec rec;
+    rec.a = make_float4(n.x, n.y, n.z, __uint_as_float(id));
+    rec.b = make_float4(p1.x, p1.y, p1.z, dot(e0, e0));
+    rec.c = make_float4(e0.x, e0.y, e0.z, dot(e0, e1));
+    rec.d = make_float4(e1.x, e1.y, e1.z, dot(e1, e1));
+    test_record(r, rec, best);
+}
+
+// isRayToAbovePlane / isRayToBelowPlane / checkRayToBoundingBoxGPU
+// (SharedFunctions.hpp:154-264).
+__device__ __forceinline__ bool above_plane(float p0, float d0, float e0, float& t0, float& t1) {
+    if (d0 > 0.0f) {
+        float ti = (e0 - p0) / d0;
+        if (ti > t1) return false;
+        if (ti > t0) t0 = ti;
+        return true;
+    } else if (d0 < 0.0f) {
+        float ti = (e0 - p0) / d0;
+        if (ti < t0) return false;
+        if (ti < t1) t1 = ti;
+        return true;
+    }
+    return e0 < p0;
+}
+__device__ __forceinline__ bool below_plane(float p0, float d0, float e0, float& t0, float& t1) {
+    if (d0 < 0.0f) {
+        float ti = (e0 - p0) / d0;
+        if (ti > t1) return false;
+        if (ti > t0) t0 = ti;
+        return true;
+    } else if (d0 > 0.0f) {
+        float ti = (e0 - p0) / d0;
+        if (ti < t0) return false;
+        if (ti < t1) t1 = ti;
+        return true;
+    }
+    return e0 > p0;
+}
+
+struct SceneBox {
+    float mn[3], mx[3];
+};
+
+__device__ __forceinline__ bool ray_box(const SceneBox& b, const Ray& r, float& ts, float& te) {
+    float t0 = 0.0f, t1 = kFltMax;
+    bool ok = above_plane(r.o.x, r.d.x, b.mn[0], t0, t1);
+    ok &= below_plane(r.o.x, r.d.x, b.mx[0], t0, t1);
+    ok &= above_plane(r.o.y, r.d.y, b.mn[1], t0, t1);
+    ok &= below_plane(r.o.y, r.d.y, b.mx[1], t0, t1);
+    ok &= above_plane(r.o.z, r.d.z, b.mn[2], t0, t1);
+    ok &= below_plane(r.o.z, r.d.z, b.mx[2], t0, t1);
+    if (ok) {
+        ts = t0;
+        te = t1;
+    }
+    return ok;
+}
+
+// castRay (GPU_common.cl:10-30): d = ((lowerLeft + vF*(x/W)) + hF*(y/H)) - from,
+// not normalized (the normalize() result is discarded at :27).
+struct Camera {
+    float from[3], vf[3], hf[3], ll[3];
+};
+
+__device__ __forceinline__ void cast_ray(const Camera& c, uint32_t x, uint32_t y, uint32_t W, uint32_t H, Ray& r) {
+    const float rr = (float)y / (float)H;
+    const float rc = (float)x / (float)W;
+    r.o = v3(c.from[0], c.from[1], c.from[2]);
+    const V3 ll = v3(c.ll[0], c.ll[1], c.ll[2]);
+    const V3 vf = v3(c.vf[0], c.vf[1], c.vf[2]);
+    const V3 hf = v3(c.hf[0], c.hf[1], c.hf[2]);
+    r.d = sub(add(add(ll, scale(vf, rc)), scale(hf, rr)), r.o);
+}
+
+__device__ __forceinline__ void set_inv(Ray& r) {
+    r.inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+}
+
+}  // namespace hulk

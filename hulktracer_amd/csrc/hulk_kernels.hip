@@ -1,0 +1,787 @@
+// hulk_kernels.hip - gfx950 traversal, shading and compaction kernels.
+//
+// One lane = one ray.  Every traversal variant is a restatement of one
+// reference kernel (cited per function) and must produce the same nearest
+// hit, t and barycentrics as the CPU oracle of that variant, bit for bit.
+//
+// Layout in HBM (built by hulk_api.hip on upload):
+//   nodes  : uint4 per node {flags, above|start, below|count, plane|leaf id};
+//            re-ordered so that nodes of depth < 12 come first in
+//            breadth-first order (the LDS top tree is a prefix), the rest
+//            in depth-first order per subtree.  Root = 0, empty child = 0.
+//   recs   : TriRec (64 B) per leaf reference, leaf-contiguous.
+//   ropes  : RopeNode (16 B) + RopeSlot (48 B) per rope slot.
+// Work distribution: chunks of 64 rays per wave, either statically mapped
+// with an XCD-aware block order or fetched with one atomic per wave
+// (persistent mode).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hulk_abi.h"
+#include "hulk_device.hpp"
+#include "hulk_kernels.hpp"
+
+namespace hulk {
+
+// ------------------------------------------------------------ utilities --
+
+__device__ __forceinline__ uint4 fetch_node(const uint4* __restrict__ g, const uint4* lds, uint32_t m, uint32_t i) {
+    if (i < m) return lds[i];
+    return g[i];
+}
+
+__device__ __forceinline__ void junction(const uint4& n, const Ray& r, float& split, uint32_t& first,
+                                         uint32_t& second) {
+    const uint32_t axis = n.x & 3u;
+    const float plane = __uint_as_float(n.w);
+    const float ro = lane(r.o, axis), rd = lane(r.d, axis), iv = lane(r.inv, axis);
+    split = (plane - ro) * iv;
+    first = n.y;
+    second = n.z;
+    if (ro < plane || (ro == plane && rd <= 0.0f)) {
+        first = n.z;
+        second = n.y;
+    }
+}
+
+struct Cnt {
+    uint32_t nodes, leaves, tris;
+};
+
+template <bool COUNT>
+__device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, const uint4& n, const Ray& r, Best& best,
+                                          Cnt& c) {
+    const uint32_t start = n.y, end = n.y + n.z;
+    if (COUNT) {
+        c.leaves++;
+        c.tris += n.z;
+    }
+    for (uint32_t i = start; i < end; i++) {
+        TriRec rec;
+        rec.a = recs[i].a;
+        rec.b = recs[i].b;
+        rec.c = recs[i].c;
+        rec.d = recs[i].d;
+        test_record(r, rec, best);
+    }
+}
+
+// ------------------------------------------------------- traversals -----
+
+// Full stack: trace_ray of GPUTrace.cl:77-161 (CPU twin KDTreeCPU::
+// searchKDTree, KDTreeCPU.cpp:285-361).  The stack lives in LDS, one
+// column per lane.  Entries keep (node, tStart) only: the tEnd of entry k
+// equals the tStart of entry k-1 (tEnd0 for the bottom entry), because tEnd
+// changes only by a push (to the pushed split) or a pop.
+template <bool COUNT>
+__device__ void trav_fullstack(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk, uint32_t stride,
+                               const Ray& r, float tclip, Best& best, Cnt& c, uint32_t& flags) {
+    float ts = 0.0f, te = kFltMax;
+    if (!ray_box(P.box, r, ts, te)) return;
+    if (tclip < te) te = tclip;
+    const float te0 = te;
+    uint32_t sp = 0, node = 0, guard = 0;
+    do {
+        if (best.t < ts) break;
+        const uint4 n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
+        if (COUNT) c.nodes++;
+        if (n.x & kNodeLeaf) {
+            test_leaf<COUNT>(P.recs, n, r, best, c);
+            node = 0;
+        } else {
+            float split;
+            uint32_t first, second;
+            junction(n, r, split, first, second);
+            if (split > te || split <= 0.0f) {
+                node = first;
+            } else if (split < ts) {
+                node = second;
+            } else {
+                if (second != 0) {
+                    if (sp >= P.stack_depth) {  // deeper than the uploaded tree allows
+                        flags |= kHitFlagOverflow;
+                        break;
+                    }
+                    stk[(2 * sp) * stride] = second;
+                    stk[(2 * sp + 1) * stride] = __float_as_uint(split);
+                    sp++;
+                }
+                node = first;
+                te = split;
+            }
+        }
+        if (node == 0 && sp > 0) {
+            sp--;
+            node = stk[(2 * sp) * stride];
+            ts = __uint_as_float(stk[(2 * sp + 1) * stride]);
+            te = sp > 0 ? __uint_as_float(stk[(2 * sp - 1) * stride]) : te0;
+        }
+        if (++guard > P.guard) {
+            flags |= kHitFlagGuard;
+            break;
+        }
+    } while (node != 0);
+}
+
+// Short stack with restart: trace_ray of GPUTrace_shortStack.cl:6-105 and,
+// with the stack in LDS, GPUTrace_LDSshortStack.cl:7-105.  Circular stack of
+// `depth` entries that drops the oldest on overflow; an empty stack
+// restarts from the root with tmin = tmax.  Frame i of lane L lives at
+// stk[(3 i + k) * stride] (k = tStart, tEnd, node).
+template <bool COUNT>
+__device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk, uint32_t stride,
+                                const Ray& r, float tclip, Best& best, Cnt& c, uint32_t& flags) {
+    float ts = 0.0f, te = kFltMax;
+    if (!ray_box(P.box, r, ts, te)) return;
+    if (tclip < te) te = tclip;
+    const int depth = (int)P.stack_depth;
+    int size = 0, idx = 0;
+    uint32_t node = 0, guard = 0;
+    float tmax = ts, tmin = ts;
+    while (tmax < te) {
+        if (size == 0) {
+            tmin = tmax;
+            tmax = te;
+            node = 0;
+        } else {
+            --size;
+            --idx;
+            if (idx == -1) idx += depth;
+            tmin = __uint_as_float(stk[(3 * idx) * stride]);
+            tmax = __uint_as_float(stk[(3 * idx + 1) * stride]);
+            node = stk[(3 * idx + 2) * stride];
+        }
+        uint4 n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
+        if (COUNT) c.nodes++;
+        while (n.x & kNodeJunction) {
+            float split;
+            uint32_t first, second;
+            junction(n, r, split, first, second);
+            if (split >= tmax || split <= 0.0f) {
+                node = first;
+            } else if (split <= tmin) {
+                node = second;
+            } else {
+                stk[(3 * idx) * stride] = __float_as_uint(split);
+                stk[(3 * idx + 1) * stride] = __float_as_uint(tmax);
+                stk[(3 * idx + 2) * stride] = second;
+                idx = idx + 1 == depth ? 0 : idx + 1;
+                size = size + 1 > depth ? depth : size + 1;
+                node = first;
+                tmax = split;
+            }
+            n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
+            if (COUNT) c.nodes++;
+            if (node == 0) break;
+        }
+        if (n.x & kNodeLeaf) {
+            test_leaf<COUNT>(P.recs, n, r, best, c);
+            if (best.t < tmax) break;
+        }
+        if (++guard > P.guard) {
+            flags |= kHitFlagGuard;
+            break;
+        }
+    }
+}
+
+// The same short stack kept in a private array (registers / scratch).
+template <bool COUNT>
+__device__ void trav_shortstack_private(const TraceParams& P, const uint4* lds_nodes, const Ray& r, float tclip,
+                                        Best& best, Cnt& c, uint32_t& flags) {
+    float ts = 0.0f, te = kFltMax;
+    if (!ray_box(P.box, r, ts, te)) return;
+    if (tclip < te) te = tclip;
+    const int depth = (int)P.stack_depth;
+    float f_ts[kMaxShortStack], f_te[kMaxShortStack];
+    uint32_t f_node[kMaxShortStack];
+    int size = 0, idx = 0;
+    uint32_t node = 0, guard = 0;
+    float tmax = ts, tmin = ts;
+    while (tmax < te) {
+        if (size == 0) {
+            tmin = tmax;
+            tmax = te;
+            node = 0;
+        } else {
+            --size;
+            --idx;
+            if (idx == -1) idx += depth;
+            tmin = f_ts[idx];
+            tmax = f_te[idx];
+            node = f_node[idx];
+        }
+        uint4 n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
+        if (COUNT) c.nodes++;
+        while (n.x & kNodeJunction) {
+            float split;
+            uint32_t first, second;
+            junction(n, r, split, first, second);
+            if (split >= tmax || split <= 0.0f) {
+                node = first;
+            } else if (split <= tmin) {
+                node = second;
+            } else {
+                f_ts[idx] = split;
+                f_te[idx] = tmax;
+                f_node[idx] = second;
+                idx = idx + 1 == depth ? 0 : idx + 1;
+                size = size + 1 > depth ? depth : size + 1;
+                node = first;
+                tmax = split;
+            }
+            n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
+            if (COUNT) c.nodes++;
+            if (node == 0) break;
+        }
+        if (n.x & kNodeLeaf) {
+            test_leaf<COUNT>(P.recs, n, r, best, c);
+            if (best.t < tmax) break;
+        }
+        if (++guard > P.guard) {
+            flags |= kHitFlagGuard;
+            break;
+        }
+    }
+}
+
+// kd-restart: trace_ray of GPUTrace_stackless.cl:182-296 with every leaf
+// resident.  ThreadContext starts zeroed, so tmin = tmax = 0 (the box entry
+// is not used); each leaf is followed by a restart from the root; there is
+// no early exit after a hit.
+template <bool COUNT>
+__device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const Ray& r, float tclip, Best& best,
+                             Cnt& c, uint32_t& flags) {
+    float ts = 0.0f, te = kFltMax;
+    if (!ray_box(P.box, r, ts, te)) return;
+    if (tclip < te) te = tclip;
+    float tmax = 0.0f, tmin = 0.0f;
+    if (best.t < tmax) return;
+    uint32_t guard = 0;
+    while (tmax < te) {
+        tmin = tmax;
+        tmax = te;
+        uint32_t node = 0;
+        uint4 n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, 0);
+        if (COUNT) c.nodes++;
+        while (n.x & kNodeJunction) {
+            float split;
+            uint32_t first, second;
+            junction(n, r, split, first, second);
+            if (split >= tmax || split <= 0.0f) {
+                node = first;
+            } else if (split <= tmin) {
+                node = second;
+            } else {
+                node = first;
+                tmax = split;
+            }
+            n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
+            if (COUNT) c.nodes++;
+            if (node == 0) break;
+        }
+        if (n.x & kNodeLeaf) test_leaf<COUNT>(P.recs, n, r, best, c);
+        if (++guard > P.guard) {
+            flags |= kHitFlagGuard;
+            break;
+        }
+    }
+}
+
+// Stackless rope traversal on the unchanged tree (no reference kernel; the
+// reference's "stackless" kernel is kd-restart).  Rope nodes: junction
+// {flags, above slot, below slot, plane}, leaf {flags, start, count, id};
+// empty children are empty leaves with their own cell and ropes.
+// Rules: descend from a slot at time t taking first if split <= 0 or
+// split > t, else second; test the leaf; leave through the face of minimum
+// slab time (lowest axis on ties); stop when best.t <= t_out, t_out >= te,
+// or there is no rope; else t = max(t, t_out) and follow the rope.
+template <bool COUNT>
+__device__ void trav_ropes(const TraceParams& P, const Ray& r, float tclip, Best& best, Cnt& c, uint32_t& flags) {
+    float ts = 0.0f, te = kFltMax;
+    if (!ray_box(P.box, r, ts, te)) return;
+    if (tclip < te) te = tclip;
+    float t = ts;
+    uint32_t slot = 0, guard = 0;
+    for (;;) {
+        uint4 n = P.rope_nodes[slot];
+        if (COUNT) c.nodes++;
+        while (n.x & kNodeJunction) {
+            float split;
+            uint32_t first, second;
+            junction(n, r, split, first, second);
+            slot = (split <= 0.0f || split > t) ? first : second;
+            n = P.rope_nodes[slot];
+            if (COUNT) c.nodes++;
+        }
+        test_leaf<COUNT>(P.recs, n, r, best, c);
+        const RopeSlot& rs = P.rope_slots[slot];
+        const float4 lo = rs.lo, hi = rs.hi;
+        const float mn[3] = {lo.x, lo.y, lo.z};
+        const float mx[3] = {hi.x, hi.y, hi.z};
+        const float o[3] = {r.o.x, r.o.y, r.o.z};
+        const float d[3] = {r.d.x, r.d.y, r.d.z};
+        const float iv[3] = {r.inv.x, r.inv.y, r.inv.z};
+        float tout = kFltMax;
+        int face = -1;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float ta;
+            int f;
+            if (d[a] > 0.0f) {
+                ta = (mx[a] - o[a]) * iv[a];
+                f = 2 * a + 1;
+            } else if (d[a] < 0.0f) {
+                ta = (mn[a] - o[a]) * iv[a];
+                f = 2 * a;
+            } else {
+                continue;
+            }
+            if (ta < tout) {
+                tout = ta;
+                face = f;
+            }
+        }
+        if (best.t <= tout || tout >= te || face < 0) break;
+        const uint32_t nxt = rs.rope[face];
+        if (nxt == kRopeNone) break;
+        if (tout > t) t = tout;
+        slot = nxt;
+        if (++guard > P.guard) {
+            flags |= kHitFlagGuard;
+            break;
+        }
+    }
+}
+
+template <bool COUNT>
+__device__ void trav_brute(const TraceParams& P, const Ray& r, Best& best, Cnt& c) {
+    for (uint32_t i = 0; i < P.n_tris; i++) {
+        const uint4 t = P.tris[i];
+        const float4 a = P.verts[t.x], b = P.verts[t.y], cc = P.verts[t.z];
+        test_vertices(r, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), i, best);
+    }
+    if (COUNT) c.tris += P.n_tris;
+}
+
+template <int VARIANT, bool COUNT>
+__device__ __forceinline__ void trace_one(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk,
+                                          uint32_t stride, const Ray& r, float tclip, Best& best, Cnt& c,
+                                          uint32_t& flags) {
+    if (VARIANT == HULK_VARIANT_FULLSTACK) trav_fullstack<COUNT>(P, lds_nodes, stk, stride, r, tclip, best, c, flags);
+    else if (VARIANT == HULK_VARIANT_SHORTSTACK) trav_shortstack_private<COUNT>(P, lds_nodes, r, tclip, best, c, flags);
+    else if (VARIANT == HULK_VARIANT_LDSSTACK) trav_shortstack<COUNT>(P, lds_nodes, stk, stride, r, tclip, best, c, flags);
+    else if (VARIANT == HULK_VARIANT_RESTART) trav_restart<COUNT>(P, lds_nodes, r, tclip, best, c, flags);
+    else if (VARIANT == HULK_VARIANT_ROPES) trav_ropes<COUNT>(P, r, tclip, best, c, flags);
+    else trav_brute<COUNT>(P, r, best, c);
+}
+
+// ---------------------------------------------------------- trace kernel --
+
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Generic trace kernel.  Ray source: the camera (one ray per pixel) or a
+// ray buffer (optionally a compacted queue whose length is read on the
+// device).  Output: a full hit record, or only t scattered to t_slot[i].
+template <int VARIANT, bool COUNT>
+__global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
+    extern __shared__ uint4 smem[];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t block = blockDim.x;
+    uint4* lds_nodes = smem;
+    uint32_t* stk = reinterpret_cast<uint32_t*>(smem + P.lds_node_count) + tid;
+    const uint32_t stride = block;
+
+    // Stage the top tree levels in LDS.
+    for (uint32_t i = tid; i < P.lds_node_count; i += block) lds_nodes[i] = P.nodes[i];
+    if (P.lds_node_count) __syncthreads();
+
+    const uint32_t n_rays = P.n_dev ? *P.n_dev : P.n;
+    const uint32_t wave = tid >> 6, lane_id = tid & 63u;
+    const uint32_t waves_per_block = block >> 6;
+    uint64_t cn = 0, cl = 0, ct = 0, cfb = 0, chits = 0, crays = 0;
+
+    // Persistent: one atomic per wave fetches the next 64 rays.  Static:
+    // XCD-aware chunk order (blocks are dispatched round-robin over the 8
+    // XCDs, so consecutive chunks go to blocks on the same XCD).
+    uint32_t chunk;
+    const uint32_t nblocks = gridDim.x;
+    if (P.persistent) {
+        uint32_t base = 0;
+        if (lane_id == 0) base = atomicAdd(P.work_counter, 64u);
+        chunk = __shfl(base, 0, 64) >> 6;
+    } else {
+        const uint32_t b = blockIdx.x;
+        const uint32_t per_xcd = (nblocks + 7u) >> 3;
+        const uint32_t xcd = b & 7u, k = b >> 3;
+        uint32_t mapped = xcd * per_xcd + k;
+        // tail: when nblocks is not a multiple of 8 some mapped ids overflow
+        if ((nblocks & 7u) != 0) mapped = b;
+        chunk = mapped * waves_per_block + wave;
+    }
+    while (chunk * 64u < n_rays) {
+        const uint32_t i = chunk * 64u + lane_id;
+        if (i < n_rays) {
+            Ray r;
+            if (P.source == kSourceCamera) {
+                const uint32_t y = i / P.W, x = i - y * P.W;
+                cast_ray(P.cam, x, y, P.W, P.H, r);
+            } else {
+                const hulk_ray ray = P.rays[i];
+                r.o = v3(ray.start.x, ray.start.y, ray.start.z);
+                r.d = v3(ray.dir.x, ray.dir.y, ray.dir.z);
+            }
+            set_inv(r);
+            Best best;
+            best_init(best);
+            Cnt c = {0, 0, 0};
+            uint32_t flags = 0;
+            const float tb = P.tbound ? P.tbound[i] : kFltMax;
+            if (tb < kFltMax) {
+                const float tclip = tb * (1.0f + 1.0f / 1024.0f);
+                trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, tclip, best, c, flags);
+                if (!(best.tri != kNoHit && best.t < tb)) {
+                    best_init(best);
+                    flags |= HULK_HITF_BOUNDED_FALLBACK;
+                    trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, kFltMax, best, c, flags);
+                    if (COUNT) cfb++;
+                }
+            } else {
+                trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, kFltMax, best, c, flags);
+            }
+            if (P.t_out) {
+                P.t_out[P.t_slot ? P.t_slot[i] : i] = best.t;
+            } else {
+                float u = 0.0f, v = 0.0f;
+                if (best.tri != kNoHit) {
+                    u = best.s1 / best.det;
+                    v = best.s2 / best.det;
+                }
+                float4* h = reinterpret_cast<float4*>(P.hits + i);
+                h[0] = make_float4(best.t, u, v, __uint_as_float(best.tri));
+                h[1] = make_float4(best.nx, best.ny, best.nz, __uint_as_float(flags));
+            }
+            if (COUNT) {
+                cn += c.nodes;
+                cl += c.leaves;
+                ct += c.tris;
+                chits += best.tri != kNoHit ? 1 : 0;
+                crays += 1;
+            }
+        }
+        if (P.persistent) {
+            uint32_t base = 0;
+            if (lane_id == 0) base = atomicAdd(P.work_counter, 64u);
+            chunk = __shfl(base, 0, 64) >> 6;
+        } else {
+            break;
+        }
+    }
+    if (COUNT) {
+        cn = wave_sum(cn);
+        cl = wave_sum(cl);
+        ct = wave_sum(ct);
+        cfb = wave_sum(cfb);
+        chits = wave_sum(chits);
+        crays = wave_sum(crays);
+        if (lane_id == 0) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[0]), (unsigned long long)crays);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[1]), (unsigned long long)cn);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[2]), (unsigned long long)cl);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[3]), (unsigned long long)ct);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[4]), (unsigned long long)chits);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[5]), (unsigned long long)cfb);
+        }
+    }
+}
+
+// ------------------------------------------------------ record builder --
+
+// Per-leaf-reference records: the normal, edges and Gram terms of
+// checkRayToPoly (SharedFunctions.hpp:53-85), same operations and order.
+__global__ void k_build_records(const float4* __restrict__ verts, const uint4* __restrict__ tris,
+                                const uint32_t* __restrict__ lp, uint32_t n, TriRec* __restrict__ recs) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = lp[i];
+    const uint4 t = tris[id];
+    const float4 a = verts[t.x], b = verts[t.y], c = verts[t.z];
+    const V3 p1 = v3(a.x, a.y, a.z), p2 = v3(b.x, b.y, b.z), p3 = v3(c.x, c.y, c.z);
+    const V3 e0 = sub(p2, p1), e1 = sub(p3, p1), e2 = sub(p3, p2);
+    const V3 nn = normalize(add(add(cross(e0, e1), cross(e1, e2)), cross(e2, e0)));
+    TriRec r;
+    r.a = make_float4(nn.x, nn.y, nn.z, __uint_as_float(id));
+    r.b = make_float4(p1.x, p1.y, p1.z, dot(e0, e0));
+    r.c = make_float4(e0.x, e0.y, e0.z, dot(e0, e1));
+    r.d = make_float4(e1.x, e1.y, e1.z, dot(e1, e1));
+    recs[i] = r;
+}
+
+// ------------------------------------------------- secondary generation --
+
+// Wave-aggregated append: one atomic per wave, order within the wave by
+// lane (mbcnt over the ballot).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane_id = threadIdx.x & 63u;
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    uint32_t base = 0;
+    const uint32_t leader = mask ? (uint32_t)__builtin_ctzll(mask) : 0u;
+    if (mask && lane_id == leader) base = atomicAdd(counter, (uint32_t)__builtin_popcountll(mask));
+    base = __shfl(base, (int)leader, 64);
+    return base + below;
+}
+
+// For every ray of generation g with a hit: one shadow ray per point light
+// (origin P + 0.001*L, direction L = normalize(pos - P)), written to the
+// shadow queue with slot = ray * n_point + k; and, for a mirror triangle
+// while g < max_bounce, the reflected ray d' = dn - (2 (dn.n)) n (dn =
+// normalize(d)), origin P + 0.001*d', appended to the next generation.
+__global__ void k_gen_secondary(GenParams G) {
+    const uint32_t n = G.n_dev ? *G.n_dev : G.n;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < n;
+    hulk_hit h;
+    Ray r;
+    uint32_t pixel = 0;
+    float w = 1.0f;
+    if (active) {
+        h = G.hits[i];
+        if (G.camera_rays) {
+            const uint32_t y = i / G.W, x = i - y * G.W;
+            cast_ray(G.cam, x, y, G.W, G.H, r);
+            pixel = i;
+        } else {
+            const hulk_ray ray = G.rays[i];
+            r.o = v3(ray.start.x, ray.start.y, ray.start.z);
+            r.d = v3(ray.dir.x, ray.dir.y, ray.dir.z);
+            pixel = G.pixel_in[i];
+            w = G.weight_in[i];
+        }
+    }
+    const bool hit = active && h.tri != kNoHit;
+    V3 P = v3(0, 0, 0);
+    if (hit) P = add(r.o, scale(r.d, h.t));
+    for (uint32_t k = 0; k < G.n_point; k++) {
+        const float4 lp = G.point_pos[k];
+        const uint32_t j = wave_append(G.shadow_count, hit);
+        if (hit) {
+            V3 L = sub(v3(lp.x, lp.y, lp.z), P);
+            const float dd = dot(L, L);
+            L = scale(L, 1.0f / __builtin_sqrtf(dd));
+            hulk_ray sr;
+            const V3 so = add(P, scale(L, 0.001f));
+            sr.start = hulk_float4{so.x, so.y, so.z, 0.0f};
+            sr.dir = hulk_float4{L.x, L.y, L.z, 0.0f};
+            G.shadow_rays[j] = sr;
+            G.shadow_slot[j] = i * G.n_point + k;
+        }
+    }
+    if (G.next_rays) {
+        const bool mirror = hit && G.mirror && G.mirror[h.tri];
+        const uint32_t j = wave_append(G.next_count, mirror);
+        if (mirror) {
+            const V3 nn = v3(h.nx, h.ny, h.nz);
+            const V3 dn = normalize(r.d);
+            const float kk = 2.0f * dot(dn, nn);
+            const V3 rr = sub(dn, scale(nn, kk));
+            const V3 o = add(P, scale(rr, 0.001f));
+            hulk_ray nr;
+            nr.start = hulk_float4{o.x, o.y, o.z, 0.0f};
+            nr.dir = hulk_float4{rr.x, rr.y, rr.z, 0.0f};
+            G.next_rays[j] = nr;
+            G.pixel_out[j] = pixel;
+            G.weight_out[j] = w * 0.8f;
+        }
+    }
+}
+
+// ------------------------------------------------------------ shading --
+
+// Lambert shading of generation g (fused semantics, GPUTrace.cl:193-228):
+// c = 0.1 + sum_k color_k * max(I_k, 0), I = n.dir (directional) or n.L
+// halved when dist > t_shadow (point).  acc[pixel] += c * keep, keep = w*0.2
+// for a mirror hit that spawned a reflection, else w.
+__global__ void k_shade_accumulate(ShadeParams S) {
+    const uint32_t n = S.n_dev ? *S.n_dev : S.n;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const hulk_hit h = S.hits[i];
+    if (h.tri == kNoHit) return;
+    Ray r;
+    uint32_t pixel;
+    float w;
+    if (S.camera_rays) {
+        const uint32_t y = i / S.W, x = i - y * S.W;
+        cast_ray(S.cam, x, y, S.W, S.H, r);
+        pixel = i;
+        w = 1.0f;
+    } else {
+        const hulk_ray ray = S.rays[i];
+        r.o = v3(ray.start.x, ray.start.y, ray.start.z);
+        r.d = v3(ray.dir.x, ray.dir.y, ray.dir.z);
+        pixel = S.pixel_in[i];
+        w = S.weight_in[i];
+    }
+    const V3 P = add(r.o, scale(r.d, h.t));
+    const V3 nn = v3(h.nx, h.ny, h.nz);
+    float cr = 0.1f, cg = 0.1f, cb = 0.1f;
+    uint32_t kp = 0;
+    for (uint32_t k = 0; k < S.n_lights; k++) {
+        const float4 col = S.lights[2 * k];
+        const float4 dat = S.lights[2 * k + 1];
+        float intensity = 0.0f;
+        if (col.w == 0.0f) intensity = dot(nn, v3(dat.x, dat.y, dat.z));
+        if (col.w == 1.0f) {
+            V3 L = sub(v3(dat.x, dat.y, dat.z), P);
+            const float dd = dot(L, L);
+            const float dist = __builtin_sqrtf(dd);
+            L = scale(L, 1.0f / __builtin_sqrtf(dd));
+            intensity = dot(nn, L);
+            const float tsh = S.shadow_t[i * S.n_point + kp];
+            if (dist > tsh) intensity *= 0.5f;
+            kp++;
+        }
+        const float m = intensity > 0.0f ? intensity : 0.0f;
+        cr += col.x * m;
+        cg += col.y * m;
+        cb += col.z * m;
+    }
+    const bool mirror = S.mirror && S.spawned && S.mirror[h.tri];
+    const float keep = mirror ? w * 0.2f : w;
+    float* acc = S.acc + 3 * (size_t)pixel;
+    acc[0] += cr * keep;
+    acc[1] += cg * keep;
+    acc[2] += cb * keep;
+}
+
+// FrameBuffer::replacePixel (FrameBuffer.hpp:196-217): min(1, c) * 255
+// rounded to nearest even, packed R | G << 8 | B << 16.
+__device__ __forceinline__ uint32_t to_u8(float c) {
+    c = c > 1.0f ? 1.0f : c;
+    c = c > 0.0f ? c : 0.0f;
+    return (uint32_t)__builtin_rintf(c * 255.0f);
+}
+
+__global__ void k_pack(const float* __restrict__ acc, uint32_t n, uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = to_u8(acc[3 * i]) | (to_u8(acc[3 * i + 1]) << 8) | (to_u8(acc[3 * i + 2]) << 16);
+}
+
+__global__ void k_fill_f32(float* p, size_t n, float v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ShadePicture, shadePicture.cl:73-121: base 0.1 for every pixel; a hit
+// (t != FLT_MAX and t > 0) adds, per light, n.dir (directional) or
+// (n.L)/2 (point, no shadow ray).
+__global__ void k_shade_picture(ShadeParams S, uint32_t* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.n) return;
+    const hulk_hit h = S.hits[i];
+    float cr = 0.1f, cg = 0.1f, cb = 0.1f;
+    if (h.t != kFltMax && h.t > 0.0f) {
+        Ray r;
+        const uint32_t y = i / S.W, x = i - y * S.W;
+        cast_ray(S.cam, x, y, S.W, S.H, r);
+        const V3 P = add(r.o, scale(r.d, h.t));
+        const V3 nn = v3(h.nx, h.ny, h.nz);
+        for (uint32_t k = 0; k < S.n_lights; k++) {
+            const float4 col = S.lights[2 * k];
+            const float4 dat = S.lights[2 * k + 1];
+            float intensity = 0.0f;
+            if (col.w == 1.0f) {
+                V3 L = sub(v3(dat.x, dat.y, dat.z), P);
+                L = scale(L, 1.0f / __builtin_sqrtf(dot(L, L)));
+                intensity = dot(nn, L) / 2.0f;
+            }
+            if (col.w == 0.0f) intensity = dot(nn, v3(dat.x, dat.y, dat.z));
+            const float m = intensity > 0.0f ? intensity : 0.0f;
+            cr += col.x * m;
+            cg += col.y * m;
+            cb += col.z * m;
+        }
+    }
+    out[i] = to_u8(cr) | (to_u8(cg) << 8) | (to_u8(cb) << 16);
+}
+
+// ---------------------------------------------------------- launchers --
+
+template <int V, bool C>
+static hipError_t launch_trace_t(const TraceParams& P, uint32_t grid, uint32_t block, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_trace<V, C>), dim3(grid), dim3(block), lds, s, P);
+    return hipGetLastError();
+}
+
+template <int V>
+static hipError_t launch_trace_v(const TraceParams& P, bool count, uint32_t grid, uint32_t block, size_t lds,
+                                 hipStream_t s) {
+    return count ? launch_trace_t<V, true>(P, grid, block, lds, s) : launch_trace_t<V, false>(P, grid, block, lds, s);
+}
+
+hipError_t launch_trace(int variant, const TraceParams& P, bool count, uint32_t grid, uint32_t block, size_t lds,
+                        hipStream_t s) {
+    switch (variant) {
+        case HULK_VARIANT_BRUTEFORCE: return launch_trace_v<HULK_VARIANT_BRUTEFORCE>(P, count, grid, block, lds, s);
+        case HULK_VARIANT_FULLSTACK: return launch_trace_v<HULK_VARIANT_FULLSTACK>(P, count, grid, block, lds, s);
+        case HULK_VARIANT_SHORTSTACK: return launch_trace_v<HULK_VARIANT_SHORTSTACK>(P, count, grid, block, lds, s);
+        case HULK_VARIANT_LDSSTACK: return launch_trace_v<HULK_VARIANT_LDSSTACK>(P, count, grid, block, lds, s);
+        case HULK_VARIANT_RESTART: return launch_trace_v<HULK_VARIANT_RESTART>(P, count, grid, block, lds, s);
+        case HULK_VARIANT_ROPES: return launch_trace_v<HULK_VARIANT_ROPES>(P, count, grid, block, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_build_records(const float4* verts, const uint4* tris, const uint32_t* lp, uint32_t n,
+                                TriRec* recs, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_build_records, dim3((n + 255) / 256), dim3(256), 0, s, verts, tris, lp, n, recs);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_secondary(const GenParams& G, uint32_t n_max, hipStream_t s) {
+    if (n_max == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gen_secondary, dim3((n_max + 255) / 256), dim3(256), 0, s, G);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_accumulate(const ShadeParams& S, uint32_t n_max, hipStream_t s) {
+    if (n_max == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shade_accumulate, dim3((n_max + 255) / 256), dim3(256), 0, s, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_picture(const ShadeParams& S, uint32_t* out, hipStream_t s) {
+    if (S.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shade_picture, dim3((S.n + 255) / 256), dim3(256), 0, s, S, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const float* acc, uint32_t n, uint32_t* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, s, acc, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace hulk

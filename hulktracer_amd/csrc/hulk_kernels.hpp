@@ -1,0 +1,109 @@
+// hulk_kernels.hpp - parameter blocks and launchers shared by the kernels
+// (hulk_kernels.hip) and the C-ABI host code (hulk_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hulk_abi.h"
+#include "hulk_device.hpp"
+
+namespace hulk {
+
+constexpr uint32_t kMaxShortStack = 16;
+constexpr uint32_t kMaxFullStack = 64;
+constexpr uint32_t kSourceCamera = 0;
+constexpr uint32_t kSourceRays = 1;
+constexpr uint32_t kHitFlagOverflow = 1u << 30;  // full stack deeper than the uploaded tree depth
+constexpr uint32_t kHitFlagGuard = 1u << 31;     // iteration guard reached (malformed tree)
+
+// Rope slot: cell box + six ropes (face 2*axis + 0 = min, + 1 = max).
+struct RopeSlot {
+    float4 lo;  // cell min (w unused)
+    float4 hi;  // cell max
+    uint32_t rope[6];
+    uint32_t pad[2];
+};
+
+struct TraceParams {
+    const uint4* nodes;
+    uint32_t lds_node_count;
+    uint32_t stack_depth;
+    const TriRec* recs;
+    const float4* verts;
+    const uint4* tris;
+    uint32_t n_tris;
+    const uint4* rope_nodes;
+    const RopeSlot* rope_slots;
+    SceneBox box;
+    uint32_t guard;
+    // ray source
+    uint32_t source;
+    Camera cam;
+    uint32_t W, H;
+    const hulk_ray* rays;
+    const uint32_t* n_dev;
+    uint32_t n;
+    const float* tbound;
+    // outputs
+    hulk_hit* hits;
+    float* t_out;
+    const uint32_t* t_slot;
+    unsigned long long* counters;  // rays, nodes, leaves, tris, hits, fallbacks
+    uint32_t* work_counter;
+    uint32_t persistent;
+};
+
+struct GenParams {
+    const hulk_hit* hits;
+    uint32_t camera_rays;
+    Camera cam;
+    uint32_t W, H;
+    const hulk_ray* rays;
+    const uint32_t* pixel_in;
+    const float* weight_in;
+    const uint32_t* n_dev;
+    uint32_t n;
+    const float4* point_pos;
+    uint32_t n_point;
+    hulk_ray* shadow_rays;
+    uint32_t* shadow_slot;
+    uint32_t* shadow_count;
+    const uint8_t* mirror;
+    hulk_ray* next_rays;
+    uint32_t* pixel_out;
+    float* weight_out;
+    uint32_t* next_count;
+};
+
+struct ShadeParams {
+    const hulk_hit* hits;
+    uint32_t camera_rays;
+    Camera cam;
+    uint32_t W, H;
+    const hulk_ray* rays;
+    const uint32_t* pixel_in;
+    const float* weight_in;
+    const uint32_t* n_dev;
+    uint32_t n;
+    const float4* lights;  // 2 float4 per light
+    uint32_t n_lights;
+    uint32_t n_point;
+    const float* shadow_t;
+    const uint8_t* mirror;
+    uint32_t spawned;
+    float* acc;
+};
+
+hipError_t launch_trace(int variant, const TraceParams& P, bool count, uint32_t grid, uint32_t block, size_t lds,
+                        hipStream_t s);
+hipError_t launch_build_records(const float4* verts, const uint4* tris, const uint32_t* lp, uint32_t n,
+                                TriRec* recs, hipStream_t s);
+hipError_t launch_gen_secondary(const GenParams& G, uint32_t n_max, hipStream_t s);
+hipError_t launch_shade_accumulate(const ShadeParams& S, uint32_t n_max, hipStream_t s);
+hipError_t launch_shade_picture(const ShadeParams& S, uint32_t* out, hipStream_t s);
+hipError_t launch_pack(const float* acc, uint32_t n, uint32_t* out, hipStream_t s);
+hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+
+}  // namespace hulk

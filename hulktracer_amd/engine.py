@@ -1,0 +1,339 @@
+"""Host-side mirror of the HulkEngine interfaces on the traverse-and-shade path.
+
+Names follow the reference so a caller of the reference finds the same
+seams:
+  Scene                 - Scene arrays (HulkEngine/public/Scene.hpp:18-96)
+  KDTreeCPU             - buildKDTree / getNodeArray / getPolygonArray
+                          (HulkEngine/src/KDTreeCPU.hpp:53-72)
+  GPUTrace              - one libhulk context (the OpenCL objects of
+                          GLVisitorGPUTrace, GLVisitorGPUTrace.hpp:73-109)
+  GLVisitorGPUTrace     - the renderer plugin: initializeVisitor / buildTree /
+                          updateCamera / traceScene / launchTraceIterations
+                          (GLVisitorGPUTrace.hpp:22-35,72)
+  Settings              - --key=value flags with the reference defaults
+                          (HulkEngine/src/Settings.cpp:11-53)
+Every compute call goes through libhulk.so; nothing here traces rays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import HulkError, HIT_DTYPE, RAY_DTYPE, ptr, lib
+
+
+def _check(ctx_handle, code: int) -> None:
+    if code != 0:
+        msg = lib().hulk_last_error(ctx_handle) if ctx_handle else b""
+        raise HulkError(code, (msg or b"").decode())
+
+
+# ------------------------------------------------------------- settings --
+
+class Settings:
+    """--key=value flags (Settings.cpp:11-53).  Unknown keys raise, as the
+    reference aborts on them."""
+
+    DEFAULTS = {
+        "--renderType": "GPUTrace", "--fixedTic": "0", "--singleFrame": "0", "--saveImages": "0",
+        "--dumpStats": "0", "--treeType": "SAHNlogN", "--kernelType": "GPUTrace_stackless",
+        "--saveBin": "0", "--p2p": "0",
+        # MI355X knobs of this library (autotuner space)
+        "--blockSize": "256", "--shortStackDepth": "6", "--ldsTopLevels": "0", "--persistent": "0",
+        "--shadeMode": "fused",
+    }
+
+    def __init__(self, argv: list[str] | None = None):
+        self.values = dict(self.DEFAULTS)
+        for a in argv or []:
+            if "=" not in a or a.split("=", 1)[0] not in self.DEFAULTS:
+                raise ValueError(f"unknown setting {a!r}")
+            k, v = a.split("=", 1)
+            self.values[k] = v
+
+    def __getitem__(self, k: str) -> str:
+        return self.values[k]
+
+
+# ---------------------------------------------------------------- scene --
+
+@dataclass
+class Scene:
+    """Flat scene arrays: vertices (V,4) float32 with w = 1, triangles (T,4)
+    uint32 (TriangleIndices), lights (L,8) float32 (color.xyz + type,
+    data.xyzw), scene box (8,) float32 (min.xyzw, max.xyzw)."""
+
+    vertices: np.ndarray
+    triangles: np.ndarray
+    lights: np.ndarray
+    box: np.ndarray
+    mirror: np.ndarray | None = None
+    kind: int = 0
+    seed: int = 0
+    fov: float = 45.0
+    name: str = "scene"
+    _handle: int | None = None
+
+    @classmethod
+    def procedural(cls, kind: int, seed: int = 1, param: int = 0, frame: int = 0) -> "Scene":
+        h = C.c_void_p()
+        _check(None, lib().hulk_scene_create(kind, seed, param, frame, C.byref(h)))
+        try:
+            nv = lib().hulk_scene_vertex_count(h)
+            nt = lib().hulk_scene_triangle_count(h)
+            nl = lib().hulk_scene_light_count(h)
+            verts = np.ctypeslib.as_array((C.c_float * (4 * nv)).from_address(lib().hulk_scene_vertices(h))).reshape(nv, 4).copy()
+            tris = np.ctypeslib.as_array((C.c_uint32 * (4 * nt)).from_address(lib().hulk_scene_triangles(h))).reshape(nt, 4).copy()
+            lights = np.ctypeslib.as_array((C.c_float * (8 * nl)).from_address(lib().hulk_scene_lights(h))).reshape(nl, 8).copy()
+            mirror = np.ctypeslib.as_array((C.c_uint8 * nt).from_address(lib().hulk_scene_mirror_mask(h))).copy()
+            box = np.zeros(8, np.float32)
+            _check(None, lib().hulk_scene_get_box(h, ptr(box)))
+        finally:
+            lib().hulk_scene_free(h)
+        s = cls(verts, tris, lights, box, mirror, kind, seed)
+        s._frame = frame
+        return s
+
+    @classmethod
+    def from_arrays(cls, vertices, triangles, lights=None) -> "Scene":
+        v = np.ascontiguousarray(vertices, np.float32)
+        if v.shape[1] == 3:
+            v = np.concatenate([v, np.ones((len(v), 1), np.float32)], 1)
+        t = np.ascontiguousarray(triangles, np.uint32)
+        if t.shape[1] == 3:
+            t = np.concatenate([t, np.zeros((len(t), 1), np.uint32)], 1)
+        lt = np.zeros((0, 8), np.float32) if lights is None else np.ascontiguousarray(lights, np.float32)
+        return cls(v, np.ascontiguousarray(t), lt, scene_box(v))
+
+    def camera(self, frame: int = 0, aspect: float = 1.0) -> np.ndarray:
+        """CameraPos (64 B) of a procedural scene's pose / frame."""
+        if not self.kind:
+            raise ValueError("camera poses exist only for procedural scenes; use make_camera")
+        cam = np.zeros(16, np.float32)
+        _check(None, lib().hulk_scene_kind_camera(self.kind, frame, aspect, ptr(cam)))
+        return cam
+
+    @property
+    def n_tris(self) -> int:
+        return int(self.triangles.shape[0])
+
+
+def scene_box(vertices: np.ndarray) -> np.ndarray:
+    """BoundingBoxVisitor box (BoundingBoxVisitor.cpp:27-79)."""
+    v = np.ascontiguousarray(vertices, np.float32)
+    box = np.zeros(8, np.float32)
+    _check(None, lib().hulk_scene_box(ptr(v), len(v), ptr(box)))
+    return box
+
+
+def make_camera(from_, axis_angle=(0.0, 1.0, 0.0, 0.0), fov: float = 45.0, aspect: float = 1.0) -> np.ndarray:
+    """updateCamera (GLVisitorCPUTrace.cpp:105-148) -> CameraPos as (16,) float32."""
+    f = np.asarray(from_, np.float32)
+    aa = np.asarray(axis_angle, np.float32)
+    cam = np.zeros(16, np.float32)
+    _check(None, lib().hulk_make_camera(ptr(f), ptr(aa), fov, aspect, ptr(cam)))
+    return cam
+
+
+def make_camera_lookat(from_, target, up=(0.0, 1.0, 0.0), fov: float = 45.0, aspect: float = 1.0) -> np.ndarray:
+    f, t, u = (np.asarray(x, np.float32) for x in (from_, target, up))
+    cam = np.zeros(16, np.float32)
+    _check(None, lib().hulk_make_camera_lookat(ptr(f), ptr(t), ptr(u), fov, aspect, ptr(cam)))
+    return cam
+
+
+# -------------------------------------------------------------- kd tree --
+
+class KDTreeCPU:
+    """SAH-NlogN KD tree (KDTreeCPU::buildKDTree, KDTreeCPU.cpp:61-94)."""
+
+    def __init__(self):
+        self.m_treeNodes = np.zeros((0, 4), np.uint32)
+        self.m_leafPolygons = np.zeros(0, np.uint32)
+
+    def buildKDTree(self, scene: Scene) -> int:
+        h = C.c_void_p()
+        _check(None, lib().hulk_kdtree_build(ptr(scene.vertices), len(scene.vertices), ptr(scene.triangles),
+                                             scene.n_tris, ptr(scene.box), C.byref(h)))
+        try:
+            nn = lib().hulk_kdtree_node_count(h)
+            nl = lib().hulk_kdtree_leaf_poly_count(h)
+            self.m_treeNodes = np.ctypeslib.as_array(
+                (C.c_uint32 * (4 * nn)).from_address(lib().hulk_kdtree_nodes(h))).reshape(nn, 4).copy()
+            self.m_leafPolygons = (np.ctypeslib.as_array(
+                (C.c_uint32 * nl).from_address(lib().hulk_kdtree_leaf_polys(h))).copy()
+                if nl else np.zeros(0, np.uint32))
+        finally:
+            lib().hulk_kdtree_free(h)
+        return 0
+
+    def getNodeArray(self) -> np.ndarray:
+        return self.m_treeNodes
+
+    def getPolygonArray(self) -> np.ndarray:
+        return self.m_leafPolygons
+
+
+# -------------------------------------------------------------- context --
+
+class GPUTrace:
+    """One libhulk context (one HIP stream, device buffers owned here)."""
+
+    def __init__(self, variant: int | str = L.FULLSTACK, block_size: int = 256, short_stack_depth: int = 6,
+                 lds_top_levels: int = 0, persistent: bool = False, device: int = 0, stream: int | None = None):
+        if isinstance(variant, str):
+            variant = L.VARIANTS[variant]
+        self.opts = L.Opts(device, variant, block_size, short_stack_depth, lds_top_levels, int(persistent),
+                           stream)
+        self.h = C.c_void_p()
+        _check(None, lib().hulk_create(C.byref(self.h), C.byref(self.opts)))
+
+    def close(self):
+        if self.h:
+            lib().hulk_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, code):
+        _check(self.h, code)
+
+    def set_options(self, **kw):
+        for k, v in kw.items():
+            if k == "variant" and isinstance(v, str):
+                v = L.VARIANTS[v]
+            setattr(self.opts, k, int(v))
+        self._c(lib().hulk_set_options(self.h, C.byref(self.opts)))
+
+    @property
+    def stream(self) -> int:
+        return lib().hulk_stream(self.h)
+
+    def upload_scene(self, scene: Scene):
+        self._c(lib().hulk_upload_scene(self.h, ptr(scene.vertices), len(scene.vertices), ptr(scene.triangles),
+                                        scene.n_tris, ptr(scene.box)))
+        if scene.lights is not None:
+            self.set_lights(scene.lights)
+
+    def upload_kdtree(self, nodes: np.ndarray, leaf_polys: np.ndarray):
+        nodes = np.ascontiguousarray(nodes, np.uint32)
+        lp = np.ascontiguousarray(leaf_polys, np.uint32)
+        self._c(lib().hulk_upload_kdtree(self.h, ptr(nodes), len(nodes), ptr(lp) if len(lp) else None, len(lp)))
+
+    def set_lights(self, lights: np.ndarray):
+        lt = np.ascontiguousarray(lights, np.float32)
+        self._c(lib().hulk_set_lights(self.h, ptr(lt) if len(lt) else None, len(lt)))
+
+    def trace_camera(self, cam: np.ndarray, W: int, H: int, tbound: np.ndarray | None = None,
+                     count: bool = False) -> np.ndarray:
+        hits = np.zeros(W * H, HIT_DTYPE)
+        tb = None if tbound is None else np.ascontiguousarray(tbound, np.float32)
+        self._c(lib().hulk_trace_camera(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, ptr(tb),
+                                        ptr(hits), L.FLAG_COUNT if count else 0))
+        return hits
+
+    def trace_rays(self, rays: np.ndarray, tbound: np.ndarray | None = None, count: bool = False) -> np.ndarray:
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros(len(r), HIT_DTYPE)
+        tb = None if tbound is None else np.ascontiguousarray(tbound, np.float32)
+        self._c(lib().hulk_trace_rays(self.h, ptr(r), len(r), ptr(tb), ptr(hits), L.FLAG_COUNT if count else 0))
+        return hits
+
+    def shade(self, cam: np.ndarray, W: int, H: int, hits: np.ndarray, mode: int = L.SHADE_FUSED) -> np.ndarray:
+        out = np.zeros(W * H, np.uint32)
+        self._c(lib().hulk_shade(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, ptr(hits), ptr(out),
+                                 mode, 0))
+        return out
+
+    def render_frame(self, cam: np.ndarray, W: int, H: int, max_bounce: int = 0, mirror: np.ndarray | None = None,
+                     tbound: np.ndarray | None = None, count: bool = False) -> tuple[np.ndarray, np.ndarray]:
+        out = np.zeros(W * H, np.uint32)
+        hits = np.zeros(W * H, HIT_DTYPE)
+        m = None if mirror is None else np.ascontiguousarray(mirror, np.uint8)
+        tb = None if tbound is None else np.ascontiguousarray(tbound, np.float32)
+        self._c(lib().hulk_render_frame(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, max_bounce,
+                                        ptr(m), ptr(tb), ptr(out), ptr(hits), L.FLAG_COUNT if count else 0))
+        return out, hits
+
+    # device-pointer forms (inputs resident in HBM, e.g. torch tensors)
+    def trace_camera_dev(self, cam: np.ndarray, W: int, H: int, hits_dev: int, tbound_dev: int | None = None,
+                         flags: int = 0):
+        self._c(lib().hulk_trace_camera(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H,
+                                        ptr(tbound_dev), ptr(hits_dev), flags | L.FLAG_DEVICE_PTRS))
+
+    def render_frame_dev(self, cam: np.ndarray, W: int, H: int, max_bounce: int, mirror_dev: int | None,
+                         rgba_dev: int, hits_dev: int | None = None, tbound_dev: int | None = None, flags: int = 0):
+        self._c(lib().hulk_render_frame(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, max_bounce,
+                                        ptr(mirror_dev), ptr(tbound_dev), ptr(rgba_dev), ptr(hits_dev),
+                                        flags | L.FLAG_DEVICE_PTRS))
+
+    def sync(self):
+        self._c(lib().hulk_sync(self.h))
+
+    def counters(self) -> dict:
+        c = L.Counters()
+        self._c(lib().hulk_get_counters(self.h, C.byref(c)))
+        return {f: getattr(c, f) for f, _ in L.Counters._fields_}
+
+    def reset_counters(self):
+        self._c(lib().hulk_reset_counters(self.h))
+
+
+# ------------------------------------------------------- renderer plugin --
+
+class GLVisitorGPUTrace:
+    """Headless renderer plugin with the reference seam (GLVisitorGPUTrace.hpp:
+    22-35,72): initializeVisitor once, then per frame traverseBegin =
+    buildTree -> updateCamera -> traceScene (GLVisitorGPUTrace.cpp:598-631).
+    Unlike the reference (static firstTime, :258-266) the tree is rebuilt on
+    every frame, as the CPU path does (GLVisitorCPUTrace.cpp:52-55)."""
+
+    def __init__(self, scene: Scene, settings: Settings | None = None, width: int = 256, height: int = 256):
+        self.m_pScene = scene
+        self.settings = settings or Settings()
+        self.m_resx, self.m_resy = width, height
+        self.m_pKdtree: KDTreeCPU | None = None
+        self.ctx: GPUTrace | None = None
+        self.m_pCamPos: np.ndarray | None = None
+        self.frame = 0
+        self.framebuffer: np.ndarray | None = None
+        self.hits: np.ndarray | None = None
+
+    def initializeVisitor(self):
+        s = self.settings
+        self.m_pKdtree = KDTreeCPU()
+        self.ctx = GPUTrace(variant=s["--kernelType"], block_size=int(s["--blockSize"]),
+                            short_stack_depth=int(s["--shortStackDepth"]), lds_top_levels=int(s["--ldsTopLevels"]),
+                            persistent=s["--persistent"] == "1")
+
+    def buildTree(self):
+        self.m_pKdtree.buildKDTree(self.m_pScene)
+        self.ctx.upload_scene(self.m_pScene)
+        self.ctx.upload_kdtree(self.m_pKdtree.getNodeArray(), self.m_pKdtree.getPolygonArray())
+
+    def updateCamera(self):
+        self.m_pCamPos = self.m_pScene.camera(self.frame, self.m_resy / self.m_resx)
+
+    def launchTraceIterations(self):
+        # every leaf is resident, so one launch finishes every ray
+        self.hits = self.ctx.trace_camera(self.m_pCamPos, self.m_resx, self.m_resy)
+
+    def traceScene(self) -> np.ndarray:
+        self.launchTraceIterations()
+        mode = L.SHADE_PICTURE if self.settings["--shadeMode"] == "picture" else L.SHADE_FUSED
+        self.framebuffer = self.ctx.shade(self.m_pCamPos, self.m_resx, self.m_resy, self.hits, mode)
+        return self.framebuffer
+
+    def traverseBegin(self, node=None) -> bool:
+        self.buildTree()
+        self.updateCamera()
+        self.traceScene()
+        self.frame += 1
+        return False

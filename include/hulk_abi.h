@@ -1,0 +1,250 @@
+/*
+ * hulk_abi.h - C ABI of the MI355X (gfx950) traverse-and-shade library
+ * (libhulk.so).
+ *
+ * This header is the drop-in boundary for hulkTracer's GPU trace stage.
+ * It replaces the OpenCL seam that `GLVisitorGPUTrace` drives:
+ *   - kernel args of `gpu_trace` (HulkEngine/src/GPUTrace_stackless.cl:298-309,
+ *     bound at HulkEngine/src/GLVisitorGPUTrace.cpp:342-378),
+ *   - `ShadePicture` (HulkEngine/src/shadePicture.cl:7-11, bound at
+ *     GLVisitorGPUTrace.cpp:385-398),
+ *   - `SetInitialValues` (HulkEngine/src/GPU_setValues.cl:4-5),
+ *   - the host frame sequence `traceScene` / `launchTraceIterations`
+ *     (GLVisitorGPUTrace.cpp:296-489).
+ *
+ * Conventions
+ *   - Every entry point returns a hulk_status (0 = ok).  The codes extend the
+ *     reference's EHulkResult (HulkEngine/public/hulk_assert.h:42-50) with HIP
+ *     errors; hulk_last_error() gives a message.  Nothing fails silently.
+ *   - The caller owns host buffers; the library owns device buffers.  Output
+ *     pointers are host pointers unless HULK_FLAG_DEVICE_PTRS is set, in which
+ *     case every pointer argument of that call is a device pointer.
+ *   - One context = one HIP stream.  Calls on one context are serialized on
+ *     that stream; a context must not be used from two threads at once.
+ *     Contexts are independent; the library keeps no global mutable state.
+ *   - Structs below are binary-identical to the reference's shared structs
+ *     (HulkEngine/public/SharedStructs.hpp), so the output of the reference's
+ *     KDTreeCPU passes through unchanged.
+ */
+#ifndef HULK_ABI_H
+#define HULK_ABI_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- types -- */
+
+/* float4 vertex / vector; vertices carry w = 1 (Scene.hpp:68-70). */
+typedef struct hulk_float4 { float x, y, z, w; } hulk_float4;
+
+/* TriangleIndices (SharedStructs.hpp:37-41): three vertex indices + padding. */
+typedef struct hulk_tri { uint32_t idx[4]; } hulk_tri;
+
+/* Box (SharedStructs.hpp:65-69). */
+typedef struct hulk_box { hulk_float4 min, max; } hulk_box;
+
+/* KDTreeNode (SharedStructs.hpp:113-137): flags bits 0-1 axis
+ * (0 = YZ plane / x, 1 = XZ / y, 2 = XY / z), 4 = leaf, 8 = junction.
+ * Root = node 0; child index 0 = empty child. */
+typedef struct hulk_kd_node {
+    uint32_t flags;
+    union {
+        struct { uint32_t start, count, leaf_id; } leaf;
+        struct { uint32_t above, below; float plane; } junction;
+    } u;
+} hulk_kd_node;
+
+#define HULK_NODE_AXIS_MASK 3u
+#define HULK_NODE_LEAF      4u
+#define HULK_NODE_JUNCTION  8u
+
+/* CameraPos (SharedStructs.hpp:99-105), field order kept. */
+typedef struct hulk_camera {
+    hulk_float4 from;
+    hulk_float4 vertical_factor;    /* multiplies x / W (GPU_common.cl:24) */
+    hulk_float4 horizontal_factor;  /* multiplies y / H (GPU_common.cl:25) */
+    hulk_float4 lower_left;
+} hulk_camera;
+
+/* Light (SharedStructs.hpp:53-63): type in color.w (0 = directional,
+ * 1 = point); data = direction or position. */
+typedef struct hulk_light { hulk_float4 color; hulk_float4 data; } hulk_light;
+
+#define HULK_LIGHT_DIRECTIONAL 0.0f
+#define HULK_LIGHT_POINT       1.0f
+
+/* Ray (SharedStructs.hpp:71-76).  Directions need not be normalized. */
+typedef struct hulk_ray { hulk_float4 start; hulk_float4 dir; } hulk_ray;
+
+/* Hit record.  The reference writes float4(normal.xyz, t)
+ * (GPUTrace_stackless.cl:339-341); this record adds the global triangle id
+ * (m_leafPolygons[start+i], KDTreeCPU.cpp:267) and the barycentrics
+ * u = s1/det, v = s2/det of the triangle test (SharedFunctions.hpp:88-91).
+ * Miss: t = FLT_MAX, tri = HULK_NO_HIT. */
+typedef struct hulk_hit {
+    float t, u, v;
+    uint32_t tri;
+    float nx, ny, nz;
+    uint32_t flags;   /* HULK_HITF_* */
+} hulk_hit;
+
+#define HULK_NO_HIT 0xFFFFFFFFu
+#define HULK_HITF_BOUNDED_FALLBACK 1u  /* bounded pass missed; re-run unbounded */
+
+/* ------------------------------------------------------------- options -- */
+
+/* Traversal variants.  Names follow the reference --kernelType values
+ * (HulkEngine/src/Settings.cpp:50). */
+typedef enum hulk_variant {
+    HULK_VARIANT_BRUTEFORCE = 0,  /* GPUTrace_bruteforce.cl:27-42          */
+    HULK_VARIANT_FULLSTACK  = 1,  /* GPUTrace.cl:6-90                       */
+    HULK_VARIANT_SHORTSTACK = 2,  /* GPUTrace_shortStack.cl:6-105 (regs)    */
+    HULK_VARIANT_LDSSTACK   = 3,  /* GPUTrace_LDSshortStack.cl:7-105 (LDS)  */
+    HULK_VARIANT_RESTART    = 4,  /* GPUTrace_stackless.cl:6-120 kd-restart */
+    HULK_VARIANT_ROPES      = 5,  /* stackless rope traversal (new)         */
+    HULK_VARIANT_COUNT      = 6
+} hulk_variant;
+
+typedef struct hulk_opts {
+    int32_t  device;            /* HIP device ordinal                        */
+    int32_t  variant;           /* hulk_variant                              */
+    uint32_t block_size;        /* threads per workgroup: 64..1024, mult. 64 */
+    uint32_t short_stack_depth; /* 1..16, short-stack / LDS-stack variants   */
+    uint32_t lds_top_levels;    /* 0..12: top tree levels staged in LDS      */
+    uint32_t persistent;        /* 1 = persistent waves with atomic fetch    */
+    void*    stream;            /* hipStream_t to use; NULL = own stream     */
+} hulk_opts;
+
+/* Call flags. */
+#define HULK_FLAG_DEVICE_PTRS  1u   /* all pointer args are device pointers */
+#define HULK_FLAG_ANYHIT       2u   /* shadow mode: stop at first t < tmax   */
+#define HULK_FLAG_COUNT        4u   /* accumulate node / triangle counters   */
+#define HULK_FLAG_NO_SYNC      8u   /* do not synchronize the stream on exit */
+
+/* Shading modes for hulk_shade / hulk_render. */
+#define HULK_SHADE_FUSED    0u  /* GPUTrace.cl:193-228: shadow rays, miss = 0      */
+#define HULK_SHADE_PICTURE  1u  /* shadePicture.cl:14-54: no shadow, miss = 0.1    */
+
+typedef enum hulk_status {
+    HULK_OK = 0,
+    HULK_E_ALLOC = 1,         /* e_allocFail    */
+    HULK_E_FILE_OPEN = 2,     /* e_fileOpenFail */
+    HULK_E_FILE_CLOSE = 3,    /* e_fileCloseFail*/
+    HULK_E_ILLEGAL_INPUT = 4, /* e_illegalInput */
+    HULK_E_NOT_FOUND = 5,     /* e_nodeNotFound */
+    HULK_E_HIP = 100,         /* HIP runtime error, message via hulk_last_error */
+    HULK_E_NO_SCENE = 101,
+    HULK_E_NO_TREE = 102,
+    HULK_E_UNSUPPORTED = 103
+} hulk_status;
+
+typedef struct hulk_ctx hulk_ctx;
+
+/* Counters of the last counted call (HULK_FLAG_COUNT).  Definitions follow
+ * the reference Statistics counters (HulkEngine/src/Statistics.hpp:58-60,
+ * 112-120), counted in every variant. */
+typedef struct hulk_counters {
+    uint64_t rays;
+    uint64_t node_visits;   /* node fetches                                 */
+    uint64_t leaf_visits;
+    uint64_t tri_tests;
+    uint64_t hits;
+    uint64_t fallbacks;     /* bounded rays re-run unbounded                */
+    float    last_kernel_ms;/* device time of the last trace kernel (events)*/
+    float    last_shade_ms;
+} hulk_counters;
+
+/* ------------------------------------------------------- context calls -- */
+
+/* Replaces GLVisitorGPUTrace::initializeVisitor (GLVisitorGPUTrace.cpp:94-161). */
+int hulk_create(hulk_ctx** out, const hulk_opts* opts);
+void hulk_destroy(hulk_ctx* ctx);
+const char* hulk_last_error(const hulk_ctx* ctx);
+int hulk_set_options(hulk_ctx* ctx, const hulk_opts* opts);
+int hulk_get_options(const hulk_ctx* ctx, hulk_opts* opts);
+void* hulk_stream(hulk_ctx* ctx);
+int hulk_sync(hulk_ctx* ctx);
+
+/* Scene arrays (Scene::m_vertices / m_vertexIndices / m_sceneBox,
+ * Scene.hpp:66-76).  Replaces the vertex/index uploads of
+ * GLVisitorGPUTrace::buildTree (GLVisitorGPUTrace.cpp:256-294). */
+int hulk_upload_scene(hulk_ctx* ctx, const hulk_float4* verts, uint32_t n_verts,
+                      const hulk_tri* tris, uint32_t n_tris, const hulk_box* scene_box);
+
+/* KD tree as produced by KDTreeCPU::buildKDTree (m_treeNodes, m_leafPolygons;
+ * KDTreeCPU.hpp:128-132).  The library derives its own device layout
+ * (per-leaf triangle records, LDS top tree, ropes).  Replaces
+ * buildPerLeafData + the node upload (GLVisitorGPUTrace.cpp:163-294). */
+int hulk_upload_kdtree(hulk_ctx* ctx, const hulk_kd_node* nodes, uint32_t n_nodes,
+                       const uint32_t* leaf_polys, uint32_t n_leaf_polys);
+
+/* Lights (Scene::m_lights); replaces the light write of traceScene
+ * (GLVisitorGPUTrace.cpp:302-305). */
+int hulk_set_lights(hulk_ctx* ctx, const hulk_light* lights, uint32_t n_lights);
+
+/* Nearest-hit primary rays of a pinhole camera, one per pixel, index
+ * y*W + x.  Replaces SetInitialValues + gpu_trace (+ relaunch loop)
+ * (GLVisitorGPUTrace.cpp:320-382, 412-489).  tbound (nullable, W*H floats):
+ * learned bound; rays without a hit below it are re-run unbounded so the
+ * result equals the unbounded result. */
+int hulk_trace_camera(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height,
+                      const float* tbound, hulk_hit* hits, uint32_t flags);
+
+/* Nearest-hit (or any-hit with HULK_FLAG_ANYHIT) for arbitrary rays. */
+int hulk_trace_rays(hulk_ctx* ctx, const hulk_ray* rays, uint32_t n_rays,
+                    const float* tbound, hulk_hit* hits, uint32_t flags);
+
+/* Shade a hit buffer produced by hulk_trace_camera with the same camera:
+ * Lambert + ambient 0.1, point-light shadow rays from P + 0.001*L
+ * (HULK_SHADE_FUSED, GPUTrace.cl:193-228) or the live ShadePicture rule
+ * (HULK_SHADE_PICTURE, shadePicture.cl:14-54).  Output packed
+ * R | G<<8 | B<<16 per pixel (FrameBuffer.hpp:196-217). */
+int hulk_shade(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height,
+               const hulk_hit* hits, uint32_t* rgba8, uint32_t shade_mode, uint32_t flags);
+
+/* Whole frame: primary trace + shadow rays + up to max_bounce mirror
+ * generations with per-generation ray compaction, then shading.
+ * mirror_mask (nullable, one byte per triangle, 1 = mirror).  Outputs the
+ * packed image and (nullable) the primary hit buffer. */
+int hulk_render_frame(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height,
+                      uint32_t max_bounce, const uint8_t* mirror_mask,
+                      const float* tbound, uint32_t* rgba8, hulk_hit* primary_hits,
+                      uint32_t flags);
+
+int hulk_get_counters(const hulk_ctx* ctx, hulk_counters* out);
+int hulk_reset_counters(hulk_ctx* ctx);
+
+/* ------------------------------------------ CPU KD tree (input producer) -- */
+
+/* SAH-NlogN KD tree, the reference's default --treeType
+ * (KDTreeCPU::buildKDTree, KDTreeCPU.cpp:61-94; KDTreeSAH.cpp:247-521).
+ * Host code; produces the node and leaf-reference arrays unchanged. */
+typedef struct hulk_kdtree hulk_kdtree;
+int hulk_kdtree_build(const hulk_float4* verts, uint32_t n_verts, const hulk_tri* tris,
+                      uint32_t n_tris, const hulk_box* scene_box, hulk_kdtree** out);
+uint32_t hulk_kdtree_node_count(const hulk_kdtree* t);
+uint32_t hulk_kdtree_leaf_poly_count(const hulk_kdtree* t);
+const hulk_kd_node* hulk_kdtree_nodes(const hulk_kdtree* t);
+const uint32_t* hulk_kdtree_leaf_polys(const hulk_kdtree* t);
+void hulk_kdtree_free(hulk_kdtree* t);
+
+/* Scene box as BoundingBoxVisitor computes it (BoundingBoxVisitor.cpp:27-79):
+ * per-lane min / max over all vertices. */
+int hulk_scene_box(const hulk_float4* verts, uint32_t n_verts, hulk_box* out);
+
+/* Camera as GLVisitorCPUTrace::updateCamera builds it
+ * (GLVisitorCPUTrace.cpp:105-148) from a position, a rotation matrix
+ * (SPMatrix::rotateMatrix(x, y, z, angle), SPMatrix.cpp:54-84, angle in
+ * radians as passed to rotateMatrix) and fov (degrees) / aspect.  cos, sin
+ * and tan are evaluated in double and rounded to float. */
+int hulk_make_camera(const float from[3], const float axis_angle[4], float fov_deg,
+                     float aspect, hulk_camera* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HULK_ABI_H */

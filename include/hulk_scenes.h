@@ -1,0 +1,55 @@
+/*
+ * hulk_scenes.h - seeded procedural scenes for the headless frame driver.
+ *
+ * The reference fills Scene::m_vertices / m_vertexIndices / m_lights from
+ * its BART and OBJ loaders (LoaderInterface, HulkEngine/public/
+ * LoaderInterface.hpp:18-27).  These calls have no counterpart in the
+ * reference boundary: they synthesize the benchmark scenes (random
+ * triangles; heightfield + spheres; animated tori) deterministically from
+ * a seed with integer hashing and double arithmetic rounded to float, so
+ * the same bytes come out on every x86-64 host.
+ */
+#ifndef HULK_SCENES_H
+#define HULK_SCENES_H
+
+#include "hulk_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hulk_scene_kind {
+    HULK_SCENE_RANDOM_TRIS = 1,     /* n random triangles in the unit cube, edges <= 0.1   */
+    HULK_SCENE_TERRAIN_SPHERES = 2, /* 256x256-vertex value-noise heightfield + 200 spheres */
+    HULK_SCENE_ANIMATED_TORI = 3    /* 50 tori of 20,000 triangles + ground plane, per frame */
+} hulk_scene_kind;
+
+typedef struct hulk_scene hulk_scene;
+
+/* param: triangle count for RANDOM_TRIS (0 = 1000); sphere count for
+ * TERRAIN_SPHERES (0 = 200); mesh count for ANIMATED_TORI (0 = 50).
+ * frame: animation frame (ANIMATED_TORI) or camera pose (others). */
+int hulk_scene_create(int kind, uint64_t seed, uint32_t param, uint32_t frame, hulk_scene** out);
+void hulk_scene_free(hulk_scene* s);
+uint32_t hulk_scene_vertex_count(const hulk_scene* s);
+uint32_t hulk_scene_triangle_count(const hulk_scene* s);
+uint32_t hulk_scene_light_count(const hulk_scene* s);
+const hulk_float4* hulk_scene_vertices(const hulk_scene* s);
+const hulk_tri* hulk_scene_triangles(const hulk_scene* s);
+const hulk_light* hulk_scene_lights(const hulk_scene* s);
+const uint8_t* hulk_scene_mirror_mask(const hulk_scene* s);  /* one byte per triangle */
+int hulk_scene_get_box(const hulk_scene* s, hulk_box* out);
+/* Camera of pose / frame `frame` for an image of aspect H / W = aspect. */
+int hulk_scene_camera(const hulk_scene* s, uint32_t frame, float aspect, hulk_camera* out);
+/* The same camera from the scene kind alone. */
+int hulk_scene_kind_camera(int kind, uint32_t frame, float aspect, hulk_camera* out);
+
+/* Look-at camera: basis in double rounded to float, then the float steps of
+ * GLVisitorCPUTrace::updateCamera (GLVisitorCPUTrace.cpp:136-145). */
+int hulk_make_camera_lookat(const float from[3], const float target[3], const float up[3],
+                            float fov_deg, float aspect, hulk_camera* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

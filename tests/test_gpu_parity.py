@@ -1,0 +1,170 @@
+"""GPU parity: every traversal variant through the C ABI against the CPU
+oracle of the same variant, bit for bit (triangle id, t, u, v, normal).
+
+Bar: integer / index results and every float of the hit record identical to
+the oracle (the kernels restate the reference arithmetic with the same
+operation order and no FMA contraction, so the tolerance is zero ulp).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import hulktracer_amd as H
+import oracle as O
+from tests.helpers import describe, gpu_ctx, hit_mismatch, oracle_scene, random_rays, scene_and_tree
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [H.FULLSTACK, H.SHORTSTACK, H.LDSSTACK, H.RESTART, H.ROPES]
+ORACLE_OF = {H.BRUTEFORCE: O.V_BRUTE, H.FULLSTACK: O.V_FULLSTACK, H.SHORTSTACK: O.V_SHORTSTACK,
+             H.LDSSTACK: O.V_LDSSTACK, H.RESTART: O.V_RESTART, H.ROPES: O.V_ROPES}
+
+
+def _check(got, ref, what):
+    bad = hit_mismatch(got, ref)
+    assert len(bad) == 0, f"{what}: {len(bad)} of {len(ref)} rays differ\n{describe(got, ref, bad)}"
+
+
+@pytest.mark.parametrize("variant", [H.BRUTEFORCE] + VARIANTS)
+def test_primary_parity_random_tris(variant):
+    """Config 1 scene (1,000 random triangles), 256x256 primary rays."""
+    s, nodes, lp = scene_and_tree(1, 1)
+    cam = s.camera(0, 1.0)
+    ctx = gpu_ctx(s, nodes, lp, variant)
+    got = ctx.trace_camera(cam, 256, 256)
+    ref = oracle_scene(1, 1).trace_camera(ORACLE_OF[variant], cam, 256, 256, depth=6)
+    _check(got, ref, f"variant {variant}")
+    assert (got["tri"] != H.NO_HIT).sum() > 1000
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("pose", [0, 13])
+def test_primary_parity_terrain(variant, pose):
+    """Terrain + spheres scene (163,650 triangles), 384x256 primary rays."""
+    s, nodes, lp = scene_and_tree(2, 1)
+    cam = s.camera(pose, 256 / 384)
+    ctx = gpu_ctx(s, nodes, lp, variant)
+    got = ctx.trace_camera(cam, 384, 256)
+    ref = oracle_scene(2, 1).trace_camera(ORACLE_OF[variant], cam, 384, 256, depth=6)
+    _check(got, ref, f"variant {variant} pose {pose}")
+
+
+@pytest.mark.parametrize("variant", [H.BRUTEFORCE] + VARIANTS)
+def test_arbitrary_rays_parity(variant):
+    """Rays from inside and outside the box, zero and axis-aligned direction
+    components, origins at the box centre."""
+    s, nodes, lp = scene_and_tree(1, 3, 300)
+    rays = random_rays(20000, s.box, seed=variant)
+    ctx = gpu_ctx(s, nodes, lp, variant)
+    got = ctx.trace_rays(rays)
+    ref = oracle_scene(1, 3, 300).trace_rays(ORACLE_OF[variant], rays, depth=6)
+    _check(got, ref, f"variant {variant}")
+
+
+@pytest.mark.parametrize("depth", [1, 4, 16])
+@pytest.mark.parametrize("variant", [H.SHORTSTACK, H.LDSSTACK])
+def test_short_stack_depths(variant, depth):
+    s, nodes, lp = scene_and_tree(2, 1)
+    cam = s.camera(5, 1.0)
+    ctx = gpu_ctx(s, nodes, lp, variant, short_stack_depth=depth)
+    got = ctx.trace_camera(cam, 192, 192)
+    ref = oracle_scene(2, 1).trace_camera(ORACLE_OF[variant], cam, 192, 192, depth=depth)
+    _check(got, ref, f"variant {variant} depth {depth}")
+
+
+@pytest.mark.parametrize("opts", [dict(lds_top_levels=6), dict(lds_top_levels=10, block_size=128),
+                                  dict(persistent=True), dict(persistent=True, lds_top_levels=8, block_size=128),
+                                  dict(block_size=64)])
+@pytest.mark.parametrize("variant", [H.FULLSTACK, H.LDSSTACK, H.RESTART])
+def test_launch_options_parity(variant, opts):
+    """LDS top tree, persistent waves and block sizes change only scheduling."""
+    s, nodes, lp = scene_and_tree(2, 1)
+    cam = s.camera(9, 1.0)
+    ctx = gpu_ctx(s, nodes, lp, variant, **opts)
+    got = ctx.trace_camera(cam, 256, 256)
+    ref = oracle_scene(2, 1).trace_camera(ORACLE_OF[variant], cam, 256, 256, depth=6)
+    _check(got, ref, f"variant {variant} {opts}")
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("variant", [H.FULLSTACK, H.RESTART])
+def test_shade_parity(variant, mode):
+    """Fused shading with point-light shadow rays (mode 0) and the live
+    ShadePicture rule (mode 1): packed pixels identical to the oracle."""
+    for kind in (1, 2):
+        s, nodes, lp = scene_and_tree(kind, 1)
+        cam = s.camera(3, 1.0)
+        ctx = gpu_ctx(s, nodes, lp, variant)
+        hits = ctx.trace_camera(cam, 256, 256)
+        img = ctx.shade(cam, 256, 256, hits, mode)
+        osc = oracle_scene(kind, 1)
+        rh = osc.trace_camera(ORACLE_OF[variant], cam, 256, 256)
+        rimg = osc.shade(ORACLE_OF[variant], cam, 256, 256, rh, mode=mode)
+        diff = np.abs(img.view(np.uint8).astype(int) - rimg.view(np.uint8).astype(int))
+        assert diff.max() == 0, f"scene {kind}: {int((diff > 0).sum())} channels differ, max {diff.max()} LSB"
+
+
+def test_render_bounces_parity():
+    """Animated tori (5 meshes, 100,002 triangles): primary + shadow + 2
+    mirror generations with per-generation compaction."""
+    s, nodes, lp = scene_and_tree(3, 7, 5, 4)
+    cam = s.camera(4, 1.0)
+    ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK)
+    img, hits = ctx.render_frame(cam, 160, 160, max_bounce=2, mirror=s.mirror)
+    rimg, counts = oracle_scene(3, 7, 5, 4).render(O.V_FULLSTACK, cam, 160, 160, max_bounce=2)
+    diff = np.abs(img.view(np.uint8).astype(int) - rimg.view(np.uint8).astype(int))
+    assert diff.max() == 0, f"{int((diff > 0).sum())} channels differ, max {diff.max()} LSB"
+    assert int(s.mirror.sum()) > 0
+
+
+@pytest.mark.parametrize("variant", [H.FULLSTACK, H.SHORTSTACK, H.RESTART])
+def test_bounded_trace_exact(variant):
+    """Learned-bound traversal: bounds below, at and above the true hit
+    distance (and for misses) give exactly the unbounded result; rays whose
+    bounded pass missed are flagged as fallbacks."""
+    s, nodes, lp = scene_and_tree(2, 1)
+    cam = s.camera(11, 1.0)
+    ctx = gpu_ctx(s, nodes, lp, variant)
+    plain = ctx.trace_camera(cam, 256, 256)
+    rng = np.random.default_rng(0)
+    t = np.where(plain["tri"] != H.NO_HIT, plain["t"], 2.0).astype(np.float32)
+    bound = (t * rng.choice([0.5, 0.999, 1.0, 1.01, 1.5, 4.0], size=t.shape)).astype(np.float32)
+    got = ctx.trace_camera(cam, 256, 256, tbound=bound)
+    _check(got, plain, f"variant {variant} bounded vs plain")
+    ref = oracle_scene(2, 1).trace_camera(ORACLE_OF[variant], cam, 256, 256, tbound=bound)
+    _check(got, ref, f"variant {variant} bounded vs oracle")
+    fb = (got["flags"] & 1) != 0
+    assert np.array_equal(fb, (ref["flags"] & 1) != 0)
+    assert fb.sum() > 0 and (~fb).sum() > 0
+
+
+def test_counters_match_oracle():
+    """Node visits / leaf visits / triangle tests counted on the GPU equal
+    the oracle's counts for the full-stack variant."""
+    s, nodes, lp = scene_and_tree(2, 1)
+    cam = s.camera(2, 1.0)
+    ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK)
+    ctx.reset_counters()
+    ctx.trace_camera(cam, 128, 128, count=True)
+    c = ctx.counters()
+    _, cnt = oracle_scene(2, 1).trace_camera(O.V_FULLSTACK, cam, 128, 128, counts=True)
+    assert c["rays"] == 128 * 128
+    assert c["node_visits"] == int(cnt[:, 0].sum())
+    assert c["leaf_visits"] == int(cnt[:, 1].sum())
+    assert c["tri_tests"] == int(cnt[:, 2].sum())
+
+
+def test_full_size_terrain_1024():
+    """Full size of the headline workload: terrain scene, 1024x1024 primary
+    rays, full-stack variant, every ray against the oracle; plus the
+    size-independent property that variants agree except at ties."""
+    s, nodes, lp = scene_and_tree(2, 1)
+    cam = s.camera(0, 1.0)
+    ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK)
+    got = ctx.trace_camera(cam, 1024, 1024)
+    ref = oracle_scene(2, 1).trace_camera(O.V_FULLSTACK, cam, 1024, 1024)
+    _check(got, ref, "1024x1024 full stack")
+    ctx.set_options(variant=H.SHORTSTACK)
+    ss = ctx.trace_camera(cam, 1024, 1024)
+    assert (ss["tri"] != got["tri"]).sum() <= 16, "short stack and full stack disagree beyond ties"
