@@ -67,6 +67,16 @@ class Counters(C.Structure):
                 ("last_kernel_ms", C.c_float), ("last_shade_ms", C.c_float)]
 
 
+class FrameDesc(C.Structure):
+    _fields_ = [("cam", C.c_float * 16), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("max_bounce", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("variant", C.c_int32), ("block_size", C.c_uint32), ("short_stack_depth", C.c_uint32),
+                ("lds_top_levels", C.c_uint32), ("persistent", C.c_uint32), ("ms", C.c_float)]
+
+
 EXPORTS = {
     # name: (restype, argtypes)
     "hulk_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(Opts)]),
@@ -88,6 +98,29 @@ EXPORTS = {
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "hulk_get_counters": (C.c_int, [C.c_void_p, C.POINTER(Counters)]),
     "hulk_reset_counters": (C.c_int, [C.c_void_p]),
+    "hulk_batch_create": (C.c_int, [C.c_void_p, C.POINTER(FrameDesc), C.c_uint32, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "hulk_batch_launch": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "hulk_batch_ray_count": (C.c_uint32, [C.c_void_p]),
+    "hulk_batch_destroy": (None, [C.c_void_p]),
+    "hulk_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                C.POINTER(TuneEntry), C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(Opts)]),
+    "hulk_mlp_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "hulk_mlp_destroy": (None, [C.c_void_p]),
+    "hulk_mlp_last_error": (C.c_char_p, [C.c_void_p]),
+    "hulk_mlp_param_count": (C.c_uint32, []),
+    "hulk_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hulk_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hulk_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "hulk_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
+                                     C.c_void_p]),
+    "hulk_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
+                                      C.POINTER(C.c_float)]),
+    "hulk_mlp_bound_camera": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
+                                        C.c_void_p, C.c_uint32]),
+    "hulk_ctx_scene_box": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hulk_camera_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                   C.c_uint32]),
     "hulk_kdtree_build": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                     C.POINTER(C.c_void_p)]),
     "hulk_kdtree_node_count": (C.c_uint32, [C.c_void_p]),

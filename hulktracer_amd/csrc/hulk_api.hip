@@ -32,6 +32,13 @@ struct DevBuf {
 
 }  // namespace
 
+// Buffers of one frame pipeline (primary hits, shadow queue, mirror queues,
+// accumulation).  The context owns one set; every batch owns its own, so a
+// captured graph never sees a buffer that a later call reallocates.
+struct Work {
+    DevBuf hits, hits2, hits3, shadow_rays, shadow_slot, shadow_t, qrays[2], qpix[2], qw[2], acc, ctr;
+};
+
 struct hulk_ctx {
     hulk_opts opts{};
     hipStream_t stream = nullptr;
@@ -61,11 +68,10 @@ struct hulk_ctx {
     DevBuf d_lights, d_point;
     uint32_t n_point = 0;
 
-    // work buffers
-    DevBuf rays, hits, hits2, tbound, shadow_rays, shadow_slot, shadow_t, qrays[2], qpix[2], qw[2], acc, rgba,
-        mirror, ctr, stats;
+    // staging + work buffers of single-frame calls
+    DevBuf rays, tbound, rgba, mirror, stats, hits_io;
+    Work work;
     unsigned long long* host_stats = nullptr;  // pinned
-    uint32_t* host_ctr = nullptr;              // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     hulk_counters cnt{};
 };
@@ -221,9 +227,9 @@ int ensure_ropes(hulk_ctx* c) {
 
 // Fill the tree / geometry part of a TraceParams and the LDS budget for the
 // context's options.
-int trace_setup(hulk_ctx* c, TraceParams& P, uint32_t& block, size_t& lds) {
+int trace_setup(hulk_ctx* c, const hulk_opts& o, TraceParams& P, uint32_t& block, size_t& lds) {
     if (!c->have_scene) return fail(c, HULK_E_NO_SCENE, "no scene uploaded");
-    const int v = c->opts.variant;
+    const int v = o.variant;
     if (v != HULK_VARIANT_BRUTEFORCE && !c->have_tree) return fail(c, HULK_E_NO_TREE, "no KD tree uploaded");
     if (v == HULK_VARIANT_ROPES) {
         int r = ensure_ropes(c);
@@ -239,45 +245,48 @@ int trace_setup(hulk_ctx* c, TraceParams& P, uint32_t& block, size_t& lds) {
     P.rope_slots = static_cast<const RopeSlot*>(c->rope_slots.p);
     P.box = c->box;
     P.guard = 4u * (c->n_nodes + c->n_slots) + 4096u;
-    block = c->opts.block_size;
+    block = o.block_size;
     uint32_t lds_nodes = 0;
     if (v != HULK_VARIANT_BRUTEFORCE && v != HULK_VARIANT_ROPES)
-        lds_nodes = c->level_prefix[std::min(c->opts.lds_top_levels, kTopLevelsMax)];
+        lds_nodes = c->level_prefix[std::min(o.lds_top_levels, kTopLevelsMax)];
     P.lds_node_count = lds_nodes;
     size_t stack_words = 0;
     if (v == HULK_VARIANT_FULLSTACK) {
         P.stack_depth = std::max<uint32_t>(c->max_jdepth, 1u);
         stack_words = 2u * P.stack_depth;
     } else if (v == HULK_VARIANT_SHORTSTACK || v == HULK_VARIANT_LDSSTACK) {
-        P.stack_depth = c->opts.short_stack_depth;
+        P.stack_depth = o.short_stack_depth;
         if (v == HULK_VARIANT_LDSSTACK) stack_words = 3u * P.stack_depth;
     }
     lds = (size_t)lds_nodes * 16u + stack_words * 4u * block;
     if (lds > c->max_lds) {
         char buf[160];
-        std::snprintf(buf, sizeof(buf), "LDS need %zu B exceeds the per-block limit %zu B (lower block_size or lds_top_levels)",
-                      lds, c->max_lds);
+        std::snprintf(buf, sizeof(buf),
+                      "LDS need %zu B exceeds the per-block limit %zu B (lower block_size or lds_top_levels)", lds,
+                      c->max_lds);
         return fail(c, HULK_E_ILLEGAL_INPUT, buf);
     }
     P.counters = static_cast<unsigned long long*>(c->stats.p);
-    P.work_counter = static_cast<uint32_t*>(c->ctr.p);
     return HULK_OK;
 }
 
-int run_trace(hulk_ctx* c, TraceParams& P, uint32_t n_max, uint32_t block, size_t lds, bool count, bool persistent,
-              bool timed) {
+// Launch one trace.  Persistent launches zero their work counter with a
+// fill kernel first (graph-capture safe: no memset nodes).
+int run_trace(hulk_ctx* c, int variant, TraceParams& P, uint32_t n_max, uint32_t block, size_t lds, bool count,
+              bool persistent, uint32_t* work_counter, bool timed) {
+    if (n_max == 0) return HULK_OK;
     P.persistent = persistent ? 1u : 0u;
+    P.work_counter = work_counter;
     uint32_t grid;
     if (persistent) {
-        HIP_TRY(c, launch_fill_u32(P.work_counter, 1, 0u, c->stream));
+        HIP_TRY(c, launch_fill_u32(work_counter, 1, 0u, c->stream));
         const uint32_t per_cu = std::max<uint32_t>(1u, 2048u / block);
         grid = std::max<uint32_t>(1u, std::min<uint32_t>((n_max + block - 1) / block, c->num_cus * per_cu));
     } else {
         grid = std::max<uint32_t>(1u, (n_max + block - 1) / block);
     }
-    if (n_max == 0) return HULK_OK;
     if (timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-    HIP_TRY(c, launch_trace(c->opts.variant, P, count, grid, block, lds, c->stream));
+    HIP_TRY(c, launch_trace(variant, P, count, grid, block, lds, c->stream));
     if (timed) HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return HULK_OK;
 }
@@ -311,7 +320,194 @@ int finish(hulk_ctx* c, uint32_t flags, bool timed) {
     return HULK_OK;
 }
 
+void release_work(Work& w) {
+    DevBuf* bufs[] = {&w.hits, &w.hits2, &w.hits3, &w.shadow_rays, &w.shadow_slot, &w.shadow_t, &w.qrays[0],
+                      &w.qrays[1], &w.qpix[0], &w.qpix[1], &w.qw[0], &w.qw[1], &w.acc, &w.ctr};
+    for (DevBuf* b : bufs) release(*b);
+}
+
+// Size a work set for n primary rays, np point lights, with or without
+// mirror generations.
+int ensure_work(hulk_ctx* c, Work& w, uint32_t n, uint32_t np, bool bounces) {
+    int r;
+    const size_t m = std::max<uint32_t>(np, 1);
+    if ((r = ensure(c, w.ctr, 64))) return r;
+    if ((r = ensure(c, w.hits, (size_t)n * sizeof(hulk_hit)))) return r;
+    if ((r = ensure(c, w.acc, (size_t)n * 12))) return r;
+    if ((r = ensure(c, w.shadow_rays, (size_t)n * m * sizeof(hulk_ray)))) return r;
+    if ((r = ensure(c, w.shadow_slot, (size_t)n * m * 4))) return r;
+    if ((r = ensure(c, w.shadow_t, (size_t)n * m * 4))) return r;
+    if (bounces) {
+        if ((r = ensure(c, w.hits2, (size_t)n * sizeof(hulk_hit)))) return r;
+        if ((r = ensure(c, w.hits3, (size_t)n * sizeof(hulk_hit)))) return r;
+        for (int k = 0; k < 2; k++) {
+            if ((r = ensure(c, w.qrays[k], (size_t)n * sizeof(hulk_ray)))) return r;
+            if ((r = ensure(c, w.qpix[k], (size_t)n * 4))) return r;
+            if ((r = ensure(c, w.qw[k], (size_t)n * 4))) return r;
+        }
+    }
+    return HULK_OK;
+}
+
+// Primary rays of one camera (cam / W / H) or of a device frame table.
+struct FrameSet {
+    const FrameDev* frames = nullptr;
+    uint32_t n_frames = 0;
+    Camera cam{};
+    uint32_t W = 0, H = 0;
+    uint32_t n = 0;
+};
+
+int trace_primary(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, const float* tbound, hulk_hit* out,
+                  bool count, bool timed) {
+    TraceParams P;
+    uint32_t block;
+    size_t lds;
+    int r = trace_setup(c, o, P, block, lds);
+    if (r) return r;
+    if (fs.frames) {
+        P.source = kSourceFrames;
+        P.frames = fs.frames;
+        P.n_frames = fs.n_frames;
+    } else {
+        P.source = kSourceCamera;
+        P.cam = fs.cam;
+        P.W = fs.W;
+        P.H = fs.H;
+    }
+    P.n = fs.n;
+    P.tbound = tbound;
+    P.hits = out;
+    return run_trace(c, o.variant, P, fs.n, block, lds, count, o.persistent != 0,
+                     static_cast<uint32_t*>(w.ctr.p), timed);
+}
+
+// Shading of a primary hit buffer: per generation g, point-light shadow
+// rays (compacted queue, nearest hit, t scattered per (ray, light)), Lambert
+// accumulation, and mirror rays (compacted queue) traced as generation
+// g + 1.  No allocation, no synchronization: safe to capture in a graph.
+int shade_pipeline(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, const hulk_hit* hits0,
+                   uint32_t max_bounce, const uint8_t* mirror_dev, uint32_t* rgba_dev, bool count) {
+    const uint32_t n = fs.n;
+    const uint32_t np = c->n_point;
+    int r;
+    TraceParams P;
+    uint32_t block;
+    size_t lds;
+    if ((r = trace_setup(c, o, P, block, lds))) return r;
+    uint32_t* ctr = static_cast<uint32_t*>(w.ctr.p);  // [0] work, [1] shadow count, [2..3] queue counts
+    HIP_TRY(c, launch_fill_f32(static_cast<float*>(w.acc.p), (size_t)n * 3, 0.0f, c->stream));
+    const hulk_hit* cur_hits = hits0;
+    const hulk_ray* cur_rays = nullptr;
+    const uint32_t* cur_pix = nullptr;
+    const float* cur_w = nullptr;
+    const uint32_t* cur_n = nullptr;
+    for (uint32_t g = 0; g <= max_bounce; g++) {
+        const bool spawn = g < max_bounce && mirror_dev;
+        const int qo = g & 1;  // queue written by this generation
+        HIP_TRY(c, launch_fill_u32(ctr + 1, 1, 0u, c->stream));
+        if (spawn) HIP_TRY(c, launch_fill_u32(ctr + 2 + qo, 1, 0u, c->stream));
+        GenParams G;
+        std::memset(&G, 0, sizeof(G));
+        G.hits = cur_hits;
+        G.camera_rays = g == 0;
+        G.cam = fs.cam;
+        G.W = fs.W;
+        G.H = fs.H;
+        G.frames = fs.frames;
+        G.n_frames = fs.n_frames;
+        G.gen = g;
+        G.rays = cur_rays;
+        G.pixel_in = cur_pix;
+        G.weight_in = cur_w;
+        G.n_dev = g == 0 ? nullptr : cur_n;
+        G.n = n;
+        G.point_pos = static_cast<const float4*>(c->d_point.p);
+        G.n_point = np;
+        G.shadow_rays = static_cast<hulk_ray*>(w.shadow_rays.p);
+        G.shadow_slot = static_cast<uint32_t*>(w.shadow_slot.p);
+        G.shadow_count = ctr + 1;
+        G.mirror = mirror_dev;
+        if (spawn) {
+            G.next_rays = static_cast<hulk_ray*>(w.qrays[qo].p);
+            G.pixel_out = static_cast<uint32_t*>(w.qpix[qo].p);
+            G.weight_out = static_cast<float*>(w.qw[qo].p);
+            G.next_count = ctr + 2 + qo;
+        }
+        HIP_TRY(c, launch_gen_secondary(G, n, c->stream));
+        if (np) {
+            TraceParams S = P;
+            S.source = kSourceRays;
+            S.rays = static_cast<const hulk_ray*>(w.shadow_rays.p);
+            S.n_dev = ctr + 1;
+            S.n = n * np;
+            S.t_out = static_cast<float*>(w.shadow_t.p);
+            S.t_slot = static_cast<const uint32_t*>(w.shadow_slot.p);
+            if ((r = run_trace(c, o.variant, S, n * np, block, lds, count, true, ctr, false))) return r;
+        }
+        ShadeParams SP;
+        std::memset(&SP, 0, sizeof(SP));
+        SP.hits = cur_hits;
+        SP.camera_rays = g == 0;
+        SP.cam = fs.cam;
+        SP.W = fs.W;
+        SP.H = fs.H;
+        SP.frames = fs.frames;
+        SP.n_frames = fs.n_frames;
+        SP.gen = g;
+        SP.rays = cur_rays;
+        SP.pixel_in = cur_pix;
+        SP.weight_in = cur_w;
+        SP.n_dev = g == 0 ? nullptr : cur_n;
+        SP.n = n;
+        SP.lights = static_cast<const float4*>(c->d_lights.p);
+        SP.n_lights = static_cast<uint32_t>(c->lights.size());
+        SP.n_point = np;
+        SP.shadow_t = static_cast<const float*>(w.shadow_t.p);
+        SP.mirror = mirror_dev;
+        SP.spawned = spawn ? 1u : 0u;
+        SP.acc = static_cast<float*>(w.acc.p);
+        HIP_TRY(c, launch_shade_accumulate(SP, n, c->stream));
+        if (!spawn) break;
+        TraceParams Q = P;
+        Q.source = kSourceRays;
+        Q.rays = static_cast<const hulk_ray*>(w.qrays[qo].p);
+        Q.n_dev = ctr + 2 + qo;
+        Q.n = n;
+        hulk_hit* nh = static_cast<hulk_hit*>((g & 1) ? w.hits3.p : w.hits2.p);
+        Q.hits = nh;
+        if ((r = run_trace(c, o.variant, Q, n, block, lds, count, true, ctr, false))) return r;
+        cur_hits = nh;
+        cur_rays = static_cast<const hulk_ray*>(w.qrays[qo].p);
+        cur_pix = static_cast<const uint32_t*>(w.qpix[qo].p);
+        cur_w = static_cast<const float*>(w.qw[qo].p);
+        cur_n = ctr + 2 + qo;
+    }
+    HIP_TRY(c, launch_pack(static_cast<const float*>(w.acc.p), n, rgba_dev, c->stream));
+    return HULK_OK;
+}
+
+// Resolve an input buffer: device pointers are used as is; host pointers
+// are staged through a context buffer.
+template <typename T>
+int stage_in(hulk_ctx* c, const T* src, size_t count, bool device_ptrs, DevBuf& buf, const T** dst) {
+    if (!src) {
+        *dst = nullptr;
+        return HULK_OK;
+    }
+    if (device_ptrs) {
+        *dst = src;
+        return HULK_OK;
+    }
+    int r = ensure(c, buf, count * sizeof(T));
+    if (r) return r;
+    HIP_TRY(c, hipMemcpyAsync(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    *dst = static_cast<const T*>(buf.p);
+    return HULK_OK;
+}
+
 }  // namespace
+
 
 // ------------------------------------------------------------------ ABI --
 
@@ -355,9 +551,8 @@ extern "C" int hulk_create(hulk_ctx** out, const hulk_opts* opts) {
         c->own_stream = true;
     }
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreate(&c->ev2) != hipSuccess || ensure(c, c->ctr, 64) || ensure(c, c->stats, 128) ||
-        hipHostMalloc(reinterpret_cast<void**>(&c->host_stats), 128, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&c->host_ctr), 64, hipHostMallocDefault) != hipSuccess) {
+        hipEventCreate(&c->ev2) != hipSuccess || ensure(c, c->work.ctr, 64) || ensure(c, c->stats, 128) ||
+        hipHostMalloc(reinterpret_cast<void**>(&c->host_stats), 128, hipHostMallocDefault) != hipSuccess) {
         hulk_destroy(c);
         return HULK_E_HIP;
     }
@@ -369,12 +564,10 @@ extern "C" void hulk_destroy(hulk_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->verts, &c->tris, &c->nodes, &c->lp, &c->recs, &c->rope_nodes, &c->rope_slots,
-                      &c->d_lights, &c->d_point, &c->rays, &c->hits, &c->hits2, &c->tbound, &c->shadow_rays,
-                      &c->shadow_slot, &c->shadow_t, &c->qrays[0], &c->qrays[1], &c->qpix[0], &c->qpix[1],
-                      &c->qw[0], &c->qw[1], &c->acc, &c->rgba, &c->mirror, &c->ctr, &c->stats};
+                      &c->d_lights, &c->d_point, &c->rays, &c->tbound, &c->rgba, &c->mirror, &c->stats, &c->hits_io};
     for (DevBuf* b : bufs) release(*b);
+    release_work(c->work);
     if (c->host_stats) (void)hipHostFree(c->host_stats);
-    if (c->host_ctr) (void)hipHostFree(c->host_ctr);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
@@ -556,198 +749,67 @@ extern "C" int hulk_set_lights(hulk_ctx* c, const hulk_light* lights, uint32_t n
     return HULK_OK;
 }
 
-namespace {
 
-// Resolve an in/out buffer: device pointers are used as is; host pointers
-// are staged through a context buffer.
-template <typename T>
-int stage_in(hulk_ctx* c, const T* src, size_t count, bool device_ptrs, DevBuf& buf, const T** dst) {
-    if (!src) {
-        *dst = nullptr;
-        return HULK_OK;
-    }
-    if (device_ptrs) {
-        *dst = src;
-        return HULK_OK;
-    }
-    int r = ensure(c, buf, count * sizeof(T));
-    if (r) return r;
-    HIP_TRY(c, hipMemcpyAsync(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
-    *dst = static_cast<const T*>(buf.p);
-    return HULK_OK;
-}
-
-int trace_impl(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H, const hulk_ray* rays, uint32_t n_rays,
-               const float* tbound, hulk_hit* hits, uint32_t flags) {
-    TraceParams P;
-    uint32_t block;
-    size_t lds;
-    int r = trace_setup(c, P, block, lds);
-    if (r) return r;
-    const bool dev = (flags & HULK_FLAG_DEVICE_PTRS) != 0;
-    const bool count = (flags & HULK_FLAG_COUNT) != 0;
-    const uint32_t n = cam ? W * H : n_rays;
-    if (cam) {
-        P.source = kSourceCamera;
-        P.cam = to_cam(*cam);
-        P.W = W;
-        P.H = H;
-    } else {
-        P.source = kSourceRays;
-        if ((r = stage_in(c, rays, n, dev, c->rays, &P.rays))) return r;
-    }
-    P.n = n;
-    if ((r = stage_in(c, tbound, n, dev, c->tbound, &P.tbound))) return r;
-    hulk_hit* out = hits;
-    if (!dev) {
-        if ((r = ensure(c, c->hits, (size_t)n * sizeof(hulk_hit)))) return r;
-        out = static_cast<hulk_hit*>(c->hits.p);
-    }
-    P.hits = out;
-    if (count && (r = zero_stats(c))) return r;
-    if ((r = run_trace(c, P, n, block, lds, count, c->opts.persistent != 0, true))) return r;
-    if (!dev && n) HIP_TRY(c, hipMemcpyAsync(hits, out, (size_t)n * sizeof(hulk_hit), hipMemcpyDeviceToHost, c->stream));
-    if (count && (r = read_stats(c))) return r;
-    return finish(c, flags, true);
-}
-
-}  // namespace
+// ------------------------------------------------------- trace entry points --
 
 extern "C" int hulk_trace_camera(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H, const float* tbound,
                                  hulk_hit* hits, uint32_t flags) {
     if (!c || !cam || !hits) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
     if ((uint64_t)W * H > 0xFFFFFFC0ull) return fail(c, HULK_E_ILLEGAL_INPUT, "image too large");
-    return trace_impl(c, cam, W, H, nullptr, 0, tbound, hits, flags);
+    const bool dev = (flags & HULK_FLAG_DEVICE_PTRS) != 0;
+    const bool count = (flags & HULK_FLAG_COUNT) != 0;
+    const uint32_t n = W * H;
+    int r;
+    FrameSet fs;
+    fs.cam = to_cam(*cam);
+    fs.W = W;
+    fs.H = H;
+    fs.n = n;
+    if ((r = ensure(c, c->work.ctr, 64))) return r;
+    const float* tb;
+    if ((r = stage_in(c, tbound, n, dev, c->tbound, &tb))) return r;
+    hulk_hit* out = hits;
+    if (!dev) {
+        if ((r = ensure(c, c->hits_io, (size_t)n * sizeof(hulk_hit)))) return r;
+        out = static_cast<hulk_hit*>(c->hits_io.p);
+    }
+    if (count && (r = zero_stats(c))) return r;
+    if ((r = trace_primary(c, c->opts, fs, c->work, tb, out, count, true))) return r;
+    if (!dev && n) HIP_TRY(c, hipMemcpyAsync(hits, out, (size_t)n * sizeof(hulk_hit), hipMemcpyDeviceToHost, c->stream));
+    if (count && (r = read_stats(c))) return r;
+    return finish(c, flags, true);
 }
 
 extern "C" int hulk_trace_rays(hulk_ctx* c, const hulk_ray* rays, uint32_t n, const float* tbound, hulk_hit* hits,
                                uint32_t flags) {
     if (!c || (n && (!rays || !hits))) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
     if (n > 0xFFFFFFC0u) return fail(c, HULK_E_ILLEGAL_INPUT, "too many rays");
-    return trace_impl(c, nullptr, 0, 0, rays, n, tbound, hits, flags);
-}
-
-namespace {
-
-// Frame pipeline on device buffers: primary hits in `hits0` (already
-// traced), then per generation: shadow rays -> trace -> shade; mirror rays
-// -> trace next generation.  Accumulates into acc and packs into rgba.
-int shade_pipeline(hulk_ctx* c, const hulk_camera& cam, uint32_t W, uint32_t H, const hulk_hit* hits0,
-                   uint32_t max_bounce, const uint8_t* mirror_dev, uint32_t* rgba_dev, bool count) {
-    const uint32_t n = W * H;
-    int r;
-    const uint32_t np = c->n_point;
-    if ((r = ensure(c, c->acc, (size_t)n * 12))) return r;
-    if ((r = ensure(c, c->shadow_rays, (size_t)n * std::max<uint32_t>(np, 1) * sizeof(hulk_ray)))) return r;
-    if ((r = ensure(c, c->shadow_slot, (size_t)n * std::max<uint32_t>(np, 1) * 4))) return r;
-    if ((r = ensure(c, c->shadow_t, (size_t)n * std::max<uint32_t>(np, 1) * 4))) return r;
-    if (max_bounce) {
-        for (int k = 0; k < 2; k++) {
-            if ((r = ensure(c, c->qrays[k], (size_t)n * sizeof(hulk_ray)))) return r;
-            if ((r = ensure(c, c->qpix[k], (size_t)n * 4))) return r;
-            if ((r = ensure(c, c->qw[k], (size_t)n * 4))) return r;
-        }
-        if ((r = ensure(c, c->hits2, (size_t)n * sizeof(hulk_hit)))) return r;
-    }
-    uint32_t* ctr = static_cast<uint32_t*>(c->ctr.p);  // [0] work, [1] shadow count, [2..3] queue counts
-    HIP_TRY(c, launch_fill_f32(static_cast<float*>(c->acc.p), (size_t)n * 3, 0.0f, c->stream));
-    Camera kc = to_cam(cam);
+    const bool dev = (flags & HULK_FLAG_DEVICE_PTRS) != 0;
+    const bool count = (flags & HULK_FLAG_COUNT) != 0;
     TraceParams P;
     uint32_t block;
     size_t lds;
-    if ((r = trace_setup(c, P, block, lds))) return r;
-
-    const hulk_hit* cur_hits = hits0;
-    const hulk_ray* cur_rays = nullptr;
-    const uint32_t* cur_pix = nullptr;
-    const float* cur_w = nullptr;
-    const uint32_t* cur_n = nullptr;
-    for (uint32_t g = 0; g <= max_bounce; g++) {
-        const bool spawn = g < max_bounce && mirror_dev;
-        HIP_TRY(c, launch_fill_u32(ctr + 1, 1, 0u, c->stream));
-        const int qo = g & 1;  // output queue of this generation
-        if (spawn) HIP_TRY(c, launch_fill_u32(ctr + 2 + qo, 1, 0u, c->stream));
-        GenParams G;
-        std::memset(&G, 0, sizeof(G));
-        G.hits = cur_hits;
-        G.camera_rays = g == 0;
-        G.cam = kc;
-        G.W = W;
-        G.H = H;
-        G.rays = cur_rays;
-        G.pixel_in = cur_pix;
-        G.weight_in = cur_w;
-        G.n_dev = g == 0 ? nullptr : cur_n;
-        G.n = n;
-        G.point_pos = static_cast<const float4*>(c->d_point.p);
-        G.n_point = np;
-        G.shadow_rays = static_cast<hulk_ray*>(c->shadow_rays.p);
-        G.shadow_slot = static_cast<uint32_t*>(c->shadow_slot.p);
-        G.shadow_count = ctr + 1;
-        G.mirror = mirror_dev;
-        if (spawn) {
-            G.next_rays = static_cast<hulk_ray*>(c->qrays[qo].p);
-            G.pixel_out = static_cast<uint32_t*>(c->qpix[qo].p);
-            G.weight_out = static_cast<float*>(c->qw[qo].p);
-            G.next_count = ctr + 2 + qo;
-        }
-        HIP_TRY(c, launch_gen_secondary(G, n, c->stream));
-        // shadow rays: nearest hit, t scattered to the (ray, light) slot
-        if (np) {
-            TraceParams S = P;
-            S.source = kSourceRays;
-            S.rays = static_cast<const hulk_ray*>(c->shadow_rays.p);
-            S.n_dev = ctr + 1;
-            S.n = n * np;
-            S.t_out = static_cast<float*>(c->shadow_t.p);
-            S.t_slot = static_cast<const uint32_t*>(c->shadow_slot.p);
-            // the work counter of a persistent launch is ctr[0]
-            if ((r = run_trace(c, S, n * np, block, lds, count, true, false))) return r;
-        }
-        ShadeParams SP;
-        std::memset(&SP, 0, sizeof(SP));
-        SP.hits = cur_hits;
-        SP.camera_rays = g == 0;
-        SP.cam = kc;
-        SP.W = W;
-        SP.H = H;
-        SP.rays = cur_rays;
-        SP.pixel_in = cur_pix;
-        SP.weight_in = cur_w;
-        SP.n_dev = g == 0 ? nullptr : cur_n;
-        SP.n = n;
-        SP.lights = static_cast<const float4*>(c->d_lights.p);
-        SP.n_lights = static_cast<uint32_t>(c->lights.size());
-        SP.n_point = np;
-        SP.shadow_t = static_cast<const float*>(c->shadow_t.p);
-        SP.mirror = mirror_dev;
-        SP.spawned = spawn ? 1u : 0u;
-        SP.acc = static_cast<float*>(c->acc.p);
-        HIP_TRY(c, launch_shade_accumulate(SP, n, c->stream));
-        if (!spawn) break;
-        // trace the next generation
-        TraceParams Q = P;
-        Q.source = kSourceRays;
-        Q.rays = static_cast<const hulk_ray*>(c->qrays[qo].p);
-        Q.n_dev = ctr + 2 + qo;
-        Q.n = n;
-        // generation buffers alternate hits2, hits, hits2, ...; the primary
-        // hits (possibly c->hits) are dead once generation 0 is shaded
-        if (g > 0 && (r = ensure(c, c->hits, (size_t)n * sizeof(hulk_hit)))) return r;
-        hulk_hit* nh = static_cast<hulk_hit*>((g & 1) ? c->hits.p : c->hits2.p);
-        Q.hits = nh;
-        if ((r = run_trace(c, Q, n, block, lds, count, true, false))) return r;
-        cur_hits = nh;
-        cur_rays = static_cast<const hulk_ray*>(c->qrays[qo].p);
-        cur_pix = static_cast<const uint32_t*>(c->qpix[qo].p);
-        cur_w = static_cast<const float*>(c->qw[qo].p);
-        cur_n = ctr + 2 + qo;
+    int r = trace_setup(c, c->opts, P, block, lds);
+    if (r) return r;
+    if ((r = ensure(c, c->work.ctr, 64))) return r;
+    P.source = kSourceRays;
+    if ((r = stage_in(c, rays, n, dev, c->rays, &P.rays))) return r;
+    P.n = n;
+    if ((r = stage_in(c, tbound, n, dev, c->tbound, &P.tbound))) return r;
+    hulk_hit* out = hits;
+    if (!dev) {
+        if ((r = ensure(c, c->hits_io, (size_t)std::max<uint32_t>(n, 1) * sizeof(hulk_hit)))) return r;
+        out = static_cast<hulk_hit*>(c->hits_io.p);
     }
-    HIP_TRY(c, launch_pack(static_cast<const float*>(c->acc.p), n, rgba_dev, c->stream));
-    return HULK_OK;
+    P.hits = out;
+    if (count && (r = zero_stats(c))) return r;
+    if ((r = run_trace(c, c->opts.variant, P, n, block, lds, count, c->opts.persistent != 0,
+                       static_cast<uint32_t*>(c->work.ctr.p), true)))
+        return r;
+    if (!dev && n) HIP_TRY(c, hipMemcpyAsync(hits, out, (size_t)n * sizeof(hulk_hit), hipMemcpyDeviceToHost, c->stream));
+    if (count && (r = read_stats(c))) return r;
+    return finish(c, flags, n > 0);
 }
-
-}  // namespace
 
 extern "C" int hulk_shade(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H, const hulk_hit* hits,
                           uint32_t* rgba8, uint32_t mode, uint32_t flags) {
@@ -757,8 +819,14 @@ extern "C" int hulk_shade(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint3
     const bool count = (flags & HULK_FLAG_COUNT) != 0;
     const uint32_t n = W * H;
     int r;
+    FrameSet fs;
+    fs.cam = to_cam(*cam);
+    fs.W = W;
+    fs.H = H;
+    fs.n = n;
+    if ((r = ensure_work(c, c->work, n, c->n_point, false))) return r;
     const hulk_hit* dh;
-    if ((r = stage_in(c, hits, n, dev, c->hits2, &dh))) return r;
+    if ((r = stage_in(c, hits, n, dev, c->hits_io, &dh))) return r;
     uint32_t* out = rgba8;
     if (!dev) {
         if ((r = ensure(c, c->rgba, (size_t)n * 4))) return r;
@@ -771,7 +839,7 @@ extern "C" int hulk_shade(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint3
         std::memset(&SP, 0, sizeof(SP));
         SP.hits = dh;
         SP.camera_rays = 1;
-        SP.cam = to_cam(*cam);
+        SP.cam = fs.cam;
         SP.W = W;
         SP.H = H;
         SP.n = n;
@@ -779,13 +847,7 @@ extern "C" int hulk_shade(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint3
         SP.n_lights = static_cast<uint32_t>(c->lights.size());
         HIP_TRY(c, launch_shade_picture(SP, out, c->stream));
     } else {
-        // the primary hits must not alias the generation buffers
-        if (!dev) {
-            if ((r = ensure(c, c->hits, (size_t)n * sizeof(hulk_hit)))) return r;
-            HIP_TRY(c, hipMemcpyAsync(c->hits.p, dh, (size_t)n * sizeof(hulk_hit), hipMemcpyDeviceToDevice, c->stream));
-            dh = static_cast<const hulk_hit*>(c->hits.p);
-        }
-        if ((r = shade_pipeline(c, *cam, W, H, dh, 0, nullptr, out, count))) return r;
+        if ((r = shade_pipeline(c, c->opts, fs, c->work, dh, 0, nullptr, out, count))) return r;
     }
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     if (!dev) HIP_TRY(c, hipMemcpyAsync(rgba8, out, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -803,43 +865,31 @@ extern "C" int hulk_render_frame(hulk_ctx* c, const hulk_camera* cam, uint32_t W
                                  hulk_hit* primary_hits, uint32_t flags) {
     if (!c || !cam || !rgba8) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
     if (max_bounce > 8) return fail(c, HULK_E_ILLEGAL_INPUT, "max_bounce must be <= 8");
+    if ((uint64_t)W * H > 0xFFFFFFC0ull) return fail(c, HULK_E_ILLEGAL_INPUT, "image too large");
     const bool dev = (flags & HULK_FLAG_DEVICE_PTRS) != 0;
     const bool count = (flags & HULK_FLAG_COUNT) != 0;
     const uint32_t n = W * H;
     int r;
-    TraceParams P;
-    uint32_t block;
-    size_t lds;
-    if ((r = trace_setup(c, P, block, lds))) return r;
+    FrameSet fs;
+    fs.cam = to_cam(*cam);
+    fs.W = W;
+    fs.H = H;
+    fs.n = n;
+    const bool bounces = max_bounce > 0 && mirror_mask;
+    if ((r = ensure_work(c, c->work, n, c->n_point, bounces))) return r;
     const uint8_t* dm = nullptr;
-    if (mirror_mask && max_bounce) {
-        if ((r = stage_in(c, mirror_mask, c->n_tris, dev, c->mirror, &dm))) return r;
-    }
-    const float* dt = nullptr;
+    if (bounces && (r = stage_in(c, mirror_mask, c->n_tris, dev, c->mirror, &dm))) return r;
+    const float* dt;
     if ((r = stage_in(c, tbound, n, dev, c->tbound, &dt))) return r;
-    hulk_hit* h0 = nullptr;
-    if (dev && primary_hits) {
-        h0 = primary_hits;
-    } else {
-        if ((r = ensure(c, c->hits, (size_t)n * sizeof(hulk_hit)))) return r;
-        h0 = static_cast<hulk_hit*>(c->hits.p);
-    }
+    hulk_hit* h0 = (dev && primary_hits) ? primary_hits : static_cast<hulk_hit*>(c->work.hits.p);
     uint32_t* out = rgba8;
     if (!dev) {
         if ((r = ensure(c, c->rgba, (size_t)n * 4))) return r;
         out = static_cast<uint32_t*>(c->rgba.p);
     }
     if (count && (r = zero_stats(c))) return r;
-    HIP_TRY(c, hipEventRecord(c->ev2, c->stream));
-    P.source = kSourceCamera;
-    P.cam = to_cam(*cam);
-    P.W = W;
-    P.H = H;
-    P.n = n;
-    P.tbound = dt;
-    P.hits = h0;
-    if ((r = run_trace(c, P, n, block, lds, count, c->opts.persistent != 0, true))) return r;
-    if ((r = shade_pipeline(c, *cam, W, H, h0, max_bounce, dm, out, count))) return r;
+    if ((r = trace_primary(c, c->opts, fs, c->work, dt, h0, count, true))) return r;
+    if ((r = shade_pipeline(c, c->opts, fs, c->work, h0, bounces ? max_bounce : 0, dm, out, count))) return r;
     if (!dev) {
         HIP_TRY(c, hipMemcpyAsync(rgba8, out, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
         if (primary_hits)
@@ -847,6 +897,24 @@ extern "C" int hulk_render_frame(hulk_ctx* c, const hulk_camera* cam, uint32_t W
     }
     if (count && (r = read_stats(c))) return r;
     return finish(c, flags, true);
+}
+
+extern "C" int hulk_camera_rays(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H, hulk_ray* rays_dev,
+                                float* texit_dev, uint32_t flags) {
+    if (!c || !cam || !rays_dev) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    if (!c->have_scene) return fail(c, HULK_E_NO_SCENE, "no scene uploaded");
+    HIP_TRY(c, launch_camera_rays(to_cam(*cam), W, H, c->box, rays_dev, texit_dev, c->stream));
+    return finish(c, flags, false);
+}
+
+extern "C" int hulk_ctx_scene_box(const hulk_ctx* c, float out[6]) {
+    if (!c || !out) return HULK_E_ILLEGAL_INPUT;
+    if (!c->have_scene) return HULK_E_NO_SCENE;
+    for (int i = 0; i < 3; i++) {
+        out[i] = c->box.mn[i];
+        out[3 + i] = c->box.mx[i];
+    }
+    return HULK_OK;
 }
 
 extern "C" int hulk_get_counters(const hulk_ctx* c, hulk_counters* out) {
@@ -858,5 +926,278 @@ extern "C" int hulk_get_counters(const hulk_ctx* c, hulk_counters* out) {
 extern "C" int hulk_reset_counters(hulk_ctx* c) {
     if (!c) return HULK_E_ILLEGAL_INPUT;
     c->cnt = hulk_counters{};
+    return HULK_OK;
+}
+
+// ------------------------------------------------------------------ batch --
+
+struct hulk_batch {
+    hulk_ctx* c = nullptr;
+    hulk_opts opts{};
+    std::vector<FrameDev> frames;
+    DevBuf d_frames, d_mirror;
+    Work w;
+    uint32_t n = 0, max_bounce = 0;
+    uint32_t* rgba = nullptr;
+    const float* tbound = nullptr;
+    bool graph_ready = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int batch_record(hulk_batch* b, bool count) {
+    hulk_ctx* c = b->c;
+    FrameSet fs;
+    fs.frames = static_cast<const FrameDev*>(b->d_frames.p);
+    fs.n_frames = static_cast<uint32_t>(b->frames.size());
+    fs.n = b->n;
+    int r;
+    hulk_hit* h0 = static_cast<hulk_hit*>(b->w.hits.p);
+    if ((r = trace_primary(c, b->opts, fs, b->w, b->tbound, h0, count, false))) return r;
+    const uint8_t* dm = b->max_bounce ? static_cast<const uint8_t*>(b->d_mirror.p) : nullptr;
+    return shade_pipeline(c, b->opts, fs, b->w, h0, dm ? b->max_bounce : 0, dm, b->rgba, count);
+}
+
+}  // namespace
+
+extern "C" int hulk_batch_create(hulk_ctx* c, const hulk_frame_desc* frames, uint32_t n_frames,
+                                 const uint8_t* mirror_mask, const float* tbound_dev, uint32_t* rgba_dev,
+                                 uint32_t use_graph, hulk_batch** out) {
+    if (!c || !frames || !n_frames || !rgba_dev || !out) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    *out = nullptr;
+    hulk_batch* b = new (std::nothrow) hulk_batch;
+    if (!b) return HULK_E_ALLOC;
+    b->c = c;
+    b->opts = c->opts;
+    uint64_t off = 0;
+    for (uint32_t f = 0; f < n_frames; f++) {
+        const hulk_frame_desc& d = frames[f];
+        if (!d.width || !d.height || d.max_bounce > 8) {
+            delete b;
+            return fail(c, HULK_E_ILLEGAL_INPUT, "frame needs width, height > 0 and max_bounce <= 8");
+        }
+        FrameDev fd;
+        fd.cam = to_cam(d.cam);
+        fd.W = d.width;
+        fd.H = d.height;
+        fd.offset = static_cast<uint32_t>(off);
+        fd.max_bounce = d.max_bounce;
+        b->frames.push_back(fd);
+        b->max_bounce = std::max(b->max_bounce, d.max_bounce);
+        off += (uint64_t)d.width * d.height;
+        if (off > 0xFFFFFFC0ull) {
+            delete b;
+            return fail(c, HULK_E_ILLEGAL_INPUT, "batch has too many rays");
+        }
+    }
+    b->n = static_cast<uint32_t>(off);
+    b->rgba = rgba_dev;
+    b->tbound = tbound_dev;
+    int r;
+    const bool bounces = b->max_bounce > 0 && mirror_mask;
+    if (!bounces) b->max_bounce = 0;
+    if ((r = ensure(c, b->d_frames, b->frames.size() * sizeof(FrameDev))) ||
+        (r = ensure_work(c, b->w, b->n, c->n_point, bounces))) {
+        hulk_batch_destroy(b);
+        return r;
+    }
+    hipError_t e = hipMemcpyAsync(b->d_frames.p, b->frames.data(), b->frames.size() * sizeof(FrameDev),
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && bounces) {
+        if ((r = ensure(c, b->d_mirror, std::max<uint32_t>(c->n_tris, 1)))) {
+            hulk_batch_destroy(b);
+            return r;
+        }
+        e = hipMemcpyAsync(b->d_mirror.p, mirror_mask, c->n_tris, hipMemcpyHostToDevice, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        hulk_batch_destroy(b);
+        return hip_fail(c, e, "batch upload");
+    }
+    // one eager run: builds ropes if needed and checks the launch shapes
+    if ((r = batch_record(b, false))) {
+        hulk_batch_destroy(b);
+        return r;
+    }
+    if (use_graph) {
+        e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            r = batch_record(b, false);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+            if (r) {
+                if (g) (void)hipGraphDestroy(g);
+                hulk_batch_destroy(b);
+                return r;
+            }
+            e = e2;
+            b->graph = g;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hulk_batch_destroy(b);
+            return hip_fail(c, e, "graph capture");
+        }
+        b->graph_ready = true;
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        hulk_batch_destroy(b);
+        return hip_fail(c, e, "batch warm-up");
+    }
+    *out = b;
+    return HULK_OK;
+}
+
+extern "C" int hulk_batch_launch(hulk_batch* b, uint32_t flags) {
+    if (!b) return HULK_E_ILLEGAL_INPUT;
+    hulk_ctx* c = b->c;
+    const bool count = (flags & HULK_FLAG_COUNT) != 0;
+    int r;
+    if (count && (r = zero_stats(c))) return r;
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    if (b->graph_ready && !count) {
+        HIP_TRY(c, hipGraphLaunch(b->exec, c->stream));
+    } else if ((r = batch_record(b, count))) {
+        return r;
+    }
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    if (count && (r = read_stats(c))) return r;
+    return finish(c, flags, true);
+}
+
+extern "C" uint32_t hulk_batch_ray_count(const hulk_batch* b) { return b ? b->n : 0; }
+
+extern "C" void hulk_batch_destroy(hulk_batch* b) {
+    if (!b) return;
+    if (b->c && b->c->stream) (void)hipStreamSynchronize(b->c->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    release(b->d_frames);
+    release(b->d_mirror);
+    release_work(b->w);
+    delete b;
+}
+
+// --------------------------------------------------------------- autotune --
+
+namespace {
+
+struct Cand {
+    int32_t variant;
+    uint32_t block, depth, lds, persistent;
+};
+
+}  // namespace
+
+// Per-scene search over traversal variant, block size, short-stack depth,
+// LDS top-tree depth and persistent waves, timing the primary trace of the
+// given camera with HIP events: stage 1 times every allowed variant at its
+// default shape, stage 2 sweeps the shape of the two fastest variants.
+// The fastest candidate becomes the context's options.
+extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H, uint32_t reps,
+                             uint32_t variant_mask, hulk_tune_entry* table, uint32_t cap, uint32_t* n_out,
+                             hulk_opts* best_out) {
+    if (!c || !cam || !W || !H) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    if (!variant_mask) variant_mask = (1u << HULK_VARIANT_FULLSTACK) | (1u << HULK_VARIANT_SHORTSTACK) |
+                                      (1u << HULK_VARIANT_LDSSTACK) | (1u << HULK_VARIANT_RESTART) |
+                                      (1u << HULK_VARIANT_ROPES);
+    reps = std::max<uint32_t>(reps, 1);
+    const uint32_t n = W * H;
+    FrameSet fs;
+    fs.cam = to_cam(*cam);
+    fs.W = W;
+    fs.H = H;
+    fs.n = n;
+    int r;
+    if ((r = ensure_work(c, c->work, n, c->n_point, false))) return r;
+    hulk_hit* out = static_cast<hulk_hit*>(c->work.hits.p);
+    std::vector<std::pair<Cand, float>> done;
+    auto measure = [&](const Cand& k, float& ms) -> int {
+        hulk_opts o = c->opts;
+        o.variant = k.variant;
+        o.block_size = k.block;
+        o.short_stack_depth = k.depth;
+        o.lds_top_levels = k.lds;
+        o.persistent = k.persistent;
+        TraceParams P;
+        uint32_t block;
+        size_t lds;
+        if (check_opts(c, o) || trace_setup(c, o, P, block, lds)) return 1;  // shape not allowed: skip
+        int rr;
+        if ((rr = trace_primary(c, o, fs, c->work, nullptr, out, false, false))) return rr;
+        HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+        for (uint32_t i = 0; i < reps; i++)
+            if ((rr = trace_primary(c, o, fs, c->work, nullptr, out, false, false))) return rr;
+        HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        ms /= reps;
+        done.push_back({k, ms});
+        return 0;
+    };
+    // stage 1
+    std::vector<std::pair<float, Cand>> firsts;
+    for (int v = HULK_VARIANT_FULLSTACK; v < HULK_VARIANT_COUNT; v++) {
+        if (!(variant_mask & (1u << v))) continue;
+        Cand k{v, 256, 6, 0, 0};
+        float ms;
+        r = measure(k, ms);
+        if (r > 1) return r;
+        if (r == 0) firsts.push_back({ms, k});
+    }
+    if (firsts.empty()) return fail(c, HULK_E_ILLEGAL_INPUT, "no variant in the mask can run");
+    std::sort(firsts.begin(), firsts.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    // stage 2
+    const uint32_t blocks[] = {64, 128, 256};
+    const uint32_t depths[] = {4, 6, 8, 12, 16};
+    const uint32_t ldss[] = {0, 6, 8, 10, 12};
+    for (size_t t = 0; t < std::min<size_t>(2, firsts.size()); t++) {
+        const int v = firsts[t].second.variant;
+        const bool shortv = v == HULK_VARIANT_SHORTSTACK || v == HULK_VARIANT_LDSSTACK;
+        const bool ldsv = v != HULK_VARIANT_ROPES;
+        for (uint32_t bs : blocks)
+            for (uint32_t dp : depths) {
+                if (!shortv && dp != 6) continue;
+                for (uint32_t ld : ldss) {
+                    if (!ldsv && ld) continue;
+                    for (uint32_t ps = 0; ps < 2; ps++) {
+                        Cand k{v, bs, dp, ld, ps};
+                        if (bs == 256 && dp == 6 && ld == 0 && ps == 0) continue;  // stage 1
+                        // persistent + LDS top tree pays off; skip the
+                        // non-persistent LDS shapes beyond depth 8
+                        if (!ps && ld > 8) continue;
+                        float ms;
+                        r = measure(k, ms);
+                        if (r > 1) return r;
+                    }
+                }
+            }
+    }
+    size_t bi = 0;
+    for (size_t i = 1; i < done.size(); i++)
+        if (done[i].second < done[bi].second) bi = i;
+    const Cand& b = done[bi].first;
+    c->opts.variant = b.variant;
+    c->opts.block_size = b.block;
+    c->opts.short_stack_depth = b.depth;
+    c->opts.lds_top_levels = b.lds;
+    c->opts.persistent = b.persistent;
+    if (best_out) *best_out = c->opts;
+    if (table) {
+        for (size_t i = 0; i < done.size() && i < cap; i++) {
+            table[i].variant = done[i].first.variant;
+            table[i].block_size = done[i].first.block;
+            table[i].short_stack_depth = done[i].first.depth;
+            table[i].lds_top_levels = done[i].first.lds;
+            table[i].persistent = done[i].first.persistent;
+            table[i].ms = done[i].second;
+        }
+    }
+    if (n_out) *n_out = static_cast<uint32_t>(done.size());
     return HULK_OK;
 }

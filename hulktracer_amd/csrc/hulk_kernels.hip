@@ -48,6 +48,26 @@ struct Cnt {
     uint32_t nodes, leaves, tris;
 };
 
+// Frame of global ray / pixel index i in a batch (offsets ascending).
+__device__ __forceinline__ uint32_t find_frame(const FrameDev* F, uint32_t n, uint32_t i) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (F[mid].offset <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t frame_ray(const FrameDev* F, uint32_t n, uint32_t i, Ray& r) {
+    const uint32_t f = find_frame(F, n, i);
+    const FrameDev& fr = F[f];
+    const uint32_t local = i - fr.offset;
+    const uint32_t y = local / fr.W, x = local - y * fr.W;
+    cast_ray(fr.cam, x, y, fr.W, fr.H, r);
+    return f;
+}
+
 template <bool COUNT>
 __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, const uint4& n, const Ray& r, Best& best,
                                           Cnt& c) {
@@ -430,6 +450,8 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
             if (P.source == kSourceCamera) {
                 const uint32_t y = i / P.W, x = i - y * P.W;
                 cast_ray(P.cam, x, y, P.W, P.H, r);
+            } else if (P.source == kSourceFrames) {
+                frame_ray(P.frames, P.n_frames, i, r);
             } else {
                 const hulk_ray ray = P.rays[i];
                 r.o = v3(ray.start.x, ray.start.y, ray.start.z);
@@ -441,8 +463,14 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
             Cnt c = {0, 0, 0};
             uint32_t flags = 0;
             const float tb = P.tbound ? P.tbound[i] : kFltMax;
-            if (tb < kFltMax) {
-                const float tclip = tb * (1.0f + 1.0f / 1024.0f);
+            // Bounded pass: box exit clipped to tb * (1 + 2^-10).  When the
+            // clip does not shorten the interval the pass is the unbounded
+            // traversal; otherwise only a hit below tb is kept and every other
+            // ray runs again unbounded, so the result equals the plain one.
+            float bts = 0.0f, bte = 0.0f;
+            const float tclip = tb * (1.0f + 1.0f / 1024.0f);
+            const bool clipped = tb < kFltMax && ray_box(P.box, r, bts, bte) && tclip < bte;
+            if (clipped) {
                 trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, tclip, best, c, flags);
                 if (!(best.tri != kNoHit && best.t < tb)) {
                     best_init(best);
@@ -549,9 +577,14 @@ __global__ void k_gen_secondary(GenParams G) {
     Ray r;
     uint32_t pixel = 0;
     float w = 1.0f;
+    bool may_spawn = true;
     if (active) {
         h = G.hits[i];
-        if (G.camera_rays) {
+        if (G.camera_rays && G.frames) {
+            const uint32_t f = frame_ray(G.frames, G.n_frames, i, r);
+            pixel = i;
+            may_spawn = G.frames[f].max_bounce > G.gen;
+        } else if (G.camera_rays) {
             const uint32_t y = i / G.W, x = i - y * G.W;
             cast_ray(G.cam, x, y, G.W, G.H, r);
             pixel = i;
@@ -561,6 +594,7 @@ __global__ void k_gen_secondary(GenParams G) {
             r.d = v3(ray.dir.x, ray.dir.y, ray.dir.z);
             pixel = G.pixel_in[i];
             w = G.weight_in[i];
+            if (G.frames) may_spawn = G.frames[find_frame(G.frames, G.n_frames, pixel)].max_bounce > G.gen;
         }
     }
     const bool hit = active && h.tri != kNoHit;
@@ -582,7 +616,7 @@ __global__ void k_gen_secondary(GenParams G) {
         }
     }
     if (G.next_rays) {
-        const bool mirror = hit && G.mirror && G.mirror[h.tri];
+        const bool mirror = hit && may_spawn && G.mirror && G.mirror[h.tri];
         const uint32_t j = wave_append(G.next_count, mirror);
         if (mirror) {
             const V3 nn = v3(h.nx, h.ny, h.nz);
@@ -615,7 +649,11 @@ __global__ void k_shade_accumulate(ShadeParams S) {
     Ray r;
     uint32_t pixel;
     float w;
-    if (S.camera_rays) {
+    if (S.camera_rays && S.frames) {
+        frame_ray(S.frames, S.n_frames, i, r);
+        pixel = i;
+        w = 1.0f;
+    } else if (S.camera_rays) {
         const uint32_t y = i / S.W, x = i - y * S.W;
         cast_ray(S.cam, x, y, S.W, S.H, r);
         pixel = i;
@@ -651,7 +689,9 @@ __global__ void k_shade_accumulate(ShadeParams S) {
         cg += col.y * m;
         cb += col.z * m;
     }
-    const bool mirror = S.mirror && S.spawned && S.mirror[h.tri];
+    const bool spawned = S.frames ? S.frames[find_frame(S.frames, S.n_frames, pixel)].max_bounce > S.gen
+                                  : S.spawned != 0;
+    const bool mirror = S.mirror && spawned && S.mirror[h.tri];
     const float keep = mirror ? w * 0.2f : w;
     float* acc = S.acc + 3 * (size_t)pixel;
     acc[0] += cr * keep;
@@ -715,6 +755,25 @@ __global__ void k_shade_picture(ShadeParams S, uint32_t* out) {
     out[i] = to_u8(cr) | (to_u8(cg) << 8) | (to_u8(cb) << 16);
 }
 
+// Primary rays of a camera as ray records, plus the scene-box exit time
+// (0 when the ray misses the box): training input of the learned bound.
+__global__ void k_camera_rays(Camera cam, uint32_t W, uint32_t H, SceneBox box, hulk_ray* rays, float* texit) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * H) return;
+    const uint32_t y = i / W, x = i - y * W;
+    Ray r;
+    cast_ray(cam, x, y, W, H, r);
+    hulk_ray q;
+    q.start = hulk_float4{r.o.x, r.o.y, r.o.z, 0.0f};
+    q.dir = hulk_float4{r.d.x, r.d.y, r.d.z, 0.0f};
+    rays[i] = q;
+    if (texit) {
+        float ts = 0.0f, te = 0.0f;
+        if (!ray_box(box, r, ts, te)) te = 0.0f;
+        texit[i] = te;
+    }
+}
+
 // ---------------------------------------------------------- launchers --
 
 template <int V, bool C>
@@ -770,6 +829,14 @@ hipError_t launch_shade_picture(const ShadeParams& S, uint32_t* out, hipStream_t
 hipError_t launch_pack(const float* acc, uint32_t n, uint32_t* out, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, s, acc, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_camera_rays(const Camera& cam, uint32_t W, uint32_t H, const SceneBox& box, hulk_ray* rays,
+                              float* texit, hipStream_t s) {
+    const uint32_t n = W * H;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_camera_rays, dim3((n + 255) / 256), dim3(256), 0, s, cam, W, H, box, rays, texit);
     return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@ constexpr uint32_t kMaxShortStack = 16;
 constexpr uint32_t kMaxFullStack = 64;
 constexpr uint32_t kSourceCamera = 0;
 constexpr uint32_t kSourceRays = 1;
+constexpr uint32_t kSourceFrames = 2;  // batched primary rays of a device frame table
 constexpr uint32_t kHitFlagOverflow = 1u << 30;  // full stack deeper than the uploaded tree depth
 constexpr uint32_t kHitFlagGuard = 1u << 31;     // iteration guard reached (malformed tree)
 
@@ -23,6 +24,13 @@ struct RopeSlot {
     float4 hi;  // cell max
     uint32_t rope[6];
     uint32_t pad[2];
+};
+
+// One frame of a batch: rays offset .. offset + W*H - 1 of the batch, pixel
+// index y*W + x inside the frame.
+struct FrameDev {
+    Camera cam;
+    uint32_t W, H, offset, max_bounce;
 };
 
 struct TraceParams {
@@ -42,6 +50,8 @@ struct TraceParams {
     Camera cam;
     uint32_t W, H;
     const hulk_ray* rays;
+    const FrameDev* frames;
+    uint32_t n_frames;
     const uint32_t* n_dev;
     uint32_t n;
     const float* tbound;
@@ -59,6 +69,8 @@ struct GenParams {
     uint32_t camera_rays;
     Camera cam;
     uint32_t W, H;
+    const FrameDev* frames;  // batch: camera per frame, spawn if max_bounce > gen
+    uint32_t n_frames, gen;
     const hulk_ray* rays;
     const uint32_t* pixel_in;
     const float* weight_in;
@@ -81,6 +93,8 @@ struct ShadeParams {
     uint32_t camera_rays;
     Camera cam;
     uint32_t W, H;
+    const FrameDev* frames;
+    uint32_t n_frames, gen;
     const hulk_ray* rays;
     const uint32_t* pixel_in;
     const float* weight_in;
@@ -104,6 +118,8 @@ hipError_t launch_shade_accumulate(const ShadeParams& S, uint32_t n_max, hipStre
 hipError_t launch_shade_picture(const ShadeParams& S, uint32_t* out, hipStream_t s);
 hipError_t launch_pack(const float* acc, uint32_t n, uint32_t* out, hipStream_t s);
 hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+hipError_t launch_camera_rays(const Camera& cam, uint32_t W, uint32_t H, const SceneBox& box, hulk_ray* rays,
+                              float* texit, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 }  // namespace hulk

@@ -10,6 +10,7 @@
 //   verticalFactor = (left*hf) * -2, horizontalFactor = (down*vf) * -2.
 // Trigonometry here is evaluated in double with a fixed polynomial so the
 // result does not depend on the host libm.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -144,7 +145,11 @@ void gen_random_tris(hulk_scene* s, uint32_t n) {
         }
         add_tri(s, 3 * i, 3 * i + 1, 3 * i + 2);
     }
-    s->mirror.assign(n, 0);
+    // mirror flags (used only by renders with mirror generations): p = 0.3
+    // per triangle from a separate stream, so the geometry does not depend on it
+    Rng mr(s->seed ^ 0x5DEECE66Dull);
+    s->mirror.resize(n);
+    for (uint32_t i = 0; i < n; i++) s->mirror[i] = mr.uniform() < 0.3 ? 1 : 0;
     s->lights.push_back({{0.9f, 0.9f, 0.9f, HULK_LIGHT_POINT}, {2.0f, 3.0f, 2.0f, 1.0f}});
 }
 
@@ -195,7 +200,12 @@ void gen_terrain_spheres(hulk_scene* s, uint32_t n_spheres) {
         double y = terrain_height(s->seed, x, z) + 0.8 * r;
         add_uv_sphere(s, {x, y, z}, r, 8, 12);
     }
+    // mirror spheres with p = 0.3 (separate stream); the terrain is diffuse
+    Rng mr(s->seed ^ 0x5DEECE66Dull);
     s->mirror.assign(s->tris.size(), 0);
+    const size_t terrain = (size_t)2 * (N - 1) * (N - 1), per = (s->tris.size() - terrain) / std::max<uint32_t>(n_spheres, 1);
+    for (uint32_t k = 0; k < n_spheres; k++)
+        if (mr.uniform() < 0.3) std::memset(&s->mirror[terrain + k * per], 1, per);
     s->lights.push_back({{0.9f, 0.85f, 0.8f, HULK_LIGHT_POINT}, {0.3f, 1.5f, 0.2f, 1.0f}});
     V3 d = normalize({0.3, 0.8, 0.5});
     s->lights.push_back({{0.2f, 0.2f, 0.25f, HULK_LIGHT_DIRECTIONAL}, f4(d, 0.0f)});
@@ -360,8 +370,11 @@ extern "C" int hulk_scene_kind_camera(int kind, uint32_t frame, float aspect, hu
             return hulk_make_camera(from, aa, 45.0f, aspect, out);
         }
         case HULK_SCENE_TERRAIN_SPHERES: {
-            // 20 poses on an arc around the terrain, looking at its centre
-            const double th = -0.5 * kPi + 0.5 * kPi * (frame % 20) / 19.0;
+            // 20 poses on an arc around the terrain, looking at its centre:
+            // poses 0-9 at angles k/10 of the arc, poses 10-19 half-way
+            // between them (the second pass of the camera path)
+            const uint32_t k = frame % 20;
+            const double th = -0.5 * kPi + 0.5 * kPi * ((k % 10) + 0.5 * (k / 10)) / 10.0;
             float from[3] = {static_cast<float>(0.5 + 0.75 * dcos(th)), 0.55f,
                              static_cast<float>(0.5 + 0.75 * dsin(th) + 0.0)};
             float target[3] = {0.5f, 0.08f, 0.5f};

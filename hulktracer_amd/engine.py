@@ -277,6 +277,34 @@ class GPUTrace:
     def sync(self):
         self._c(lib().hulk_sync(self.h))
 
+    def batch(self, frames, mirror: np.ndarray | None, rgba_dev: int, tbound_dev: int | None = None,
+              use_graph: bool = True) -> "Batch":
+        """frames: list of (camera (16,) float32, width, height, max_bounce)."""
+        arr = (L.FrameDesc * len(frames))()
+        for i, (cam, w, h, mb) in enumerate(frames):
+            arr[i].cam[:] = [float(x) for x in np.asarray(cam, np.float32)]
+            arr[i].width, arr[i].height, arr[i].max_bounce = int(w), int(h), int(mb)
+        m = None if mirror is None else np.ascontiguousarray(mirror, np.uint8)
+        hb = C.c_void_p()
+        self._c(lib().hulk_batch_create(self.h, arr, len(frames), ptr(m), ptr(tbound_dev), ptr(rgba_dev),
+                                        int(use_graph), C.byref(hb)))
+        return Batch(self, hb)
+
+    def autotune(self, cam: np.ndarray, W: int, H: int, reps: int = 3, variants=None):
+        """Per-scene search (hulk_autotune); returns (best options dict, table)."""
+        mask = 0
+        for v in variants or []:
+            mask |= 1 << (L.VARIANTS[v] if isinstance(v, str) else int(v))
+        cap = 512
+        table = (L.TuneEntry * cap)()
+        n = C.c_uint32()
+        best = L.Opts()
+        self._c(lib().hulk_autotune(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, reps, mask, table,
+                                    cap, C.byref(n), C.byref(best)))
+        self.opts = best
+        rows = [{f: getattr(table[i], f) for f, _ in L.TuneEntry._fields_} for i in range(n.value)]
+        return {f: getattr(best, f) for f, _ in L.Opts._fields_ if f != "stream"}, rows
+
     def counters(self) -> dict:
         c = L.Counters()
         self._c(lib().hulk_get_counters(self.h, C.byref(c)))
@@ -284,6 +312,31 @@ class GPUTrace:
 
     def reset_counters(self):
         self._c(lib().hulk_reset_counters(self.h))
+
+
+class Batch:
+    """A hulk_batch: many frames in one (graph-captured) pipeline."""
+
+    def __init__(self, ctx: GPUTrace, handle: C.c_void_p):
+        self.ctx, self.h = ctx, handle
+
+    @property
+    def rays(self) -> int:
+        return lib().hulk_batch_ray_count(self.h)
+
+    def launch(self, flags: int = 0):
+        _check(self.ctx.h, lib().hulk_batch_launch(self.h, flags))
+
+    def close(self):
+        if self.h:
+            lib().hulk_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------- renderer plugin --

@@ -1,0 +1,161 @@
+"""Learned hit-distance bound: the MLP of hulk_mlp.hip driven from Python.
+
+Training data are (ray, log t) pairs the tracer itself produces: primary
+rays of earlier frames, t = nearest-hit distance, or the scene-box exit for
+rays that miss (so a miss predicts "no useful bound" instead of a bound
+that would force the fallback).  Device arrays are torch tensors (plumbing
+only); every forward, backward and optimizer step runs in libhulk.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import HulkError, lib, ptr
+from .engine import GPUTrace
+
+FLT_MAX = 3.4028234663852886e38
+
+
+class HitDistanceMLP:
+    def __init__(self, ctx: GPUTrace, seed: int = 1, max_batch: int = 16384):
+        self.ctx = ctx
+        self.max_batch = max_batch
+        self.h = C.c_void_p()
+        rc = lib().hulk_mlp_create(ctx.h, seed, max_batch, C.byref(self.h))
+        if rc:
+            raise HulkError(rc, "hulk_mlp_create failed")
+        self.margin = 0.0
+
+    def _c(self, rc):
+        if rc:
+            raise HulkError(rc, (lib().hulk_mlp_last_error(self.h) or b"").decode())
+
+    def close(self):
+        if self.h:
+            lib().hulk_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def param_count() -> int:
+        return lib().hulk_mlp_param_count()
+
+    def get_params(self):
+        p = np.zeros(self.param_count(), np.float32)
+        f = np.zeros(32 * 6, np.float32)
+        self._c(lib().hulk_mlp_get_params(self.h, ptr(p), ptr(f)))
+        return p, f
+
+    def set_params(self, params=None, fourier=None):
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        f = None if fourier is None else np.ascontiguousarray(fourier, np.float32)
+        self._c(lib().hulk_mlp_set_params(self.h, ptr(p), ptr(f)))
+
+    def forward(self, rays_dev, n: int, pred_dev):
+        self._c(lib().hulk_mlp_forward(self.h, ptr(rays_dev), n, ptr(pred_dev)))
+
+    def loss_grad(self, rays_dev, y_dev, n: int):
+        loss = C.c_float()
+        g = np.zeros(self.param_count(), np.float32)
+        self._c(lib().hulk_mlp_loss_grad(self.h, ptr(rays_dev), ptr(y_dev), n, C.byref(loss), ptr(g)))
+        return loss.value, g
+
+    def train_step(self, rays_dev, y_dev, n: int, lr: float, want_loss: bool = False):
+        loss = C.c_float()
+        self._c(lib().hulk_mlp_train_step(self.h, ptr(rays_dev), ptr(y_dev), n, lr,
+                                          C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def bound_camera(self, cam, W: int, H: int, tbound_dev, margin: float | None = None, max_log: float = 60.0,
+                     sync: bool = True):
+        m = self.margin if margin is None else margin
+        self._c(lib().hulk_mlp_bound_camera(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, m, max_log,
+                                            ptr(tbound_dev), 0 if sync else L.FLAG_NO_SYNC))
+
+    # ------------------------------------------------------------ training --
+
+    def fit(self, rays, y, steps: int, batch: int = 16384, lr: float = 2e-3, seed: int = 0, log_every: int = 0):
+        """rays: (N, 8) float32 cuda tensor, y: (N,) float32 cuda tensor."""
+        import torch
+
+        n = rays.shape[0]
+        batch = min(batch, self.max_batch, n - n % 64)
+        g = torch.Generator(device=rays.device)
+        g.manual_seed(seed)
+        losses = []
+        for s in range(steps):
+            idx = torch.randint(0, n, (batch,), device=rays.device, generator=g)
+            rb = rays.index_select(0, idx).contiguous()
+            yb = y.index_select(0, idx).contiguous()
+            cur_lr = lr * 0.5 * (1 + math.cos(math.pi * s / steps))  # cosine decay
+            want = bool(log_every) and (s % log_every == 0 or s == steps - 1)
+            loss = self.train_step(rb, yb, batch, cur_lr, want_loss=want)
+            if want:
+                losses.append((s, loss))
+        return losses
+
+    def calibrate(self, rays, y, quantile: float = 0.95, chunk: int = 1 << 20) -> float:
+        """margin = quantile of (y - y^) on held-out pairs, so that about
+        `quantile` of the rays get a bound at or above their hit."""
+        import torch
+
+        res = []
+        for a in range(0, rays.shape[0], chunk):
+            r = rays[a:a + chunk].contiguous()
+            pred = torch.empty(r.shape[0], dtype=torch.float32, device=r.device)
+            self.forward(r, r.shape[0], pred)
+            res.append(y[a:a + chunk] - pred)
+        res = torch.cat(res)
+        self.margin = float(torch.quantile(res.float()[:1 << 24], quantile).item())
+        return self.margin
+
+
+def camera_rays(ctx: GPUTrace, cam, W: int, H: int):
+    """Primary rays (W*H, 8) and scene-box exit times (W*H,) as cuda tensors."""
+    import torch
+
+    n = W * H
+    rays = torch.empty((n, 8), dtype=torch.float32, device="cuda")
+    texit = torch.empty(n, dtype=torch.float32, device="cuda")
+    rc = lib().hulk_camera_rays(ctx.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, ptr(rays), ptr(texit), 0)
+    if rc:
+        raise HulkError(rc, (lib().hulk_last_error(ctx.h) or b"").decode())
+    return rays, texit
+
+
+def training_pairs(ctx: GPUTrace, cams, W: int, H: int, n_samples: int | None = None, seed: int = 0):
+    """(ray, log t) pairs from the tracer's own nearest hits on `cams`.
+    Misses are labelled with the scene-box exit; rays outside the box are
+    dropped."""
+    import torch
+
+    rays_all, y_all = [], []
+    hits = torch.empty(W * H * 32, dtype=torch.uint8, device="cuda")
+    for cam in cams:
+        rays, texit = camera_rays(ctx, cam, W, H)
+        ctx.trace_camera_dev(cam, W, H, hits.data_ptr())
+        h = hits.view(torch.float32).view(-1, 8)
+        t = h[:, 0]
+        tri = h[:, 3].contiguous().view(torch.int32)
+        hit = tri != -1
+        lab = torch.where(hit, t, texit)
+        keep = lab > 0
+        rays_all.append(rays[keep])
+        y_all.append(torch.log(lab[keep]))
+    rays = torch.cat(rays_all)
+    y = torch.cat(y_all)
+    if n_samples is not None and n_samples < rays.shape[0]:
+        g = torch.Generator(device="cuda")
+        g.manual_seed(seed)
+        idx = torch.randperm(rays.shape[0], device="cuda", generator=g)[:n_samples]
+        rays, y = rays[idx].contiguous(), y[idx].contiguous()
+    return rays.contiguous(), y.contiguous()

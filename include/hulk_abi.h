@@ -218,6 +218,88 @@ int hulk_render_frame(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uin
 int hulk_get_counters(const hulk_ctx* ctx, hulk_counters* out);
 int hulk_reset_counters(hulk_ctx* ctx);
 
+/* ------------------------------------------------------- batched frames -- */
+
+/* One frame of a batch (camera, resolution, mirror generations). */
+typedef struct hulk_frame_desc {
+    hulk_camera cam;
+    uint32_t width, height;
+    uint32_t max_bounce;
+    uint32_t reserved;
+} hulk_frame_desc;
+
+typedef struct hulk_batch hulk_batch;
+
+/* Many frames of uneven size and bounce depth on the context's scene, run
+ * as one pipeline: one primary launch over every frame's rays, then per
+ * generation one compacted shadow-ray launch, one shading pass and one
+ * compacted mirror-ray launch for all frames together.  With use_graph the
+ * whole pipeline is captured once into a HIP graph and each
+ * hulk_batch_launch replays it.  The batch owns its work buffers and keeps
+ * the options the context had at creation.  rgba_dev / tbound_dev are
+ * device pointers; frame f's pixels start at the sum of the earlier
+ * frames' width*height.  Replaces the per-frame traceScene loop
+ * (GLVisitorGPUTrace.cpp:296-410) for offline / multi-view rendering. */
+int hulk_batch_create(hulk_ctx* ctx, const hulk_frame_desc* frames, uint32_t n_frames,
+                      const uint8_t* mirror_mask, const float* tbound_dev, uint32_t* rgba_dev,
+                      uint32_t use_graph, hulk_batch** out);
+int hulk_batch_launch(hulk_batch* batch, uint32_t flags);
+uint32_t hulk_batch_ray_count(const hulk_batch* batch);
+void hulk_batch_destroy(hulk_batch* batch);
+
+/* --------------------------------------------------------------- autotune -- */
+
+typedef struct hulk_tune_entry {
+    int32_t variant;
+    uint32_t block_size, short_stack_depth, lds_top_levels, persistent;
+    float ms;                 /* primary-trace time per launch */
+} hulk_tune_entry;
+
+/* Per-scene autotuner: times the primary trace of `cam` over traversal
+ * variant (variant_mask bit per hulk_variant, 0 = all), block size 64-256,
+ * short-stack depth 4-16 and LDS top-tree depth 0-12, persistent or not,
+ * and makes the fastest the context's options.  No reference counterpart
+ * (the reference picks the kernel by --kernelType). */
+int hulk_autotune(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height, uint32_t reps,
+                  uint32_t variant_mask, hulk_tune_entry* table, uint32_t table_cap, uint32_t* n_entries,
+                  hulk_opts* best);
+
+/* ----------------------------------------------- learned distance bound -- */
+
+/* Small MLP that proposes a per-ray hit-distance bound (no reference
+ * counterpart).  Input: ray origin (normalized by the context's scene box)
+ * and unit direction; 32 x 6 Fourier matrix -> (sin, cos) 64-dim encoding;
+ * three 128-wide ReLU layers; output log t.  BF16 weights and activations,
+ * FP32 accumulation on MFMA; FP32 master weights and Adam.  The bound fed
+ * to the traversal is exp(y + margin); traversal stays exact through the
+ * unbounded fallback of hulk_trace_camera / hulk_render_frame.  Parameter
+ * vector order: W1 (128x64), b1, W2 (128x128), b2, W3 (128x128), b3,
+ * w4 (128), b4.  All ray / target / bound pointers are device pointers. */
+typedef struct hulk_mlp hulk_mlp;
+int hulk_mlp_create(hulk_ctx* ctx, uint64_t seed, uint32_t max_batch, hulk_mlp** out);
+void hulk_mlp_destroy(hulk_mlp* mlp);
+const char* hulk_mlp_last_error(const hulk_mlp* mlp);
+uint32_t hulk_mlp_param_count(void);
+int hulk_mlp_get_params(hulk_mlp* mlp, float* params, float* fourier /* 32*6 */);
+int hulk_mlp_set_params(hulk_mlp* mlp, const float* params, const float* fourier);
+int hulk_mlp_forward(hulk_mlp* mlp, const hulk_ray* rays, uint32_t n, float* pred);
+/* Loss mean((y^ - y)^2) and its gradient (n multiple of 64, <= max_batch). */
+int hulk_mlp_loss_grad(hulk_mlp* mlp, const hulk_ray* rays, const float* target, uint32_t n, float* loss,
+                       float* grad_host);
+int hulk_mlp_train_step(hulk_mlp* mlp, const hulk_ray* rays, const float* target, uint32_t n, float lr,
+                        float* loss);
+/* Per-pixel bound of a camera's primary rays: exp(y + margin), FLT_MAX
+ * where y + margin >= max_log. */
+int hulk_mlp_bound_camera(hulk_mlp* mlp, const hulk_camera* cam, uint32_t width, uint32_t height, float margin,
+                          float max_log, float* tbound, uint32_t flags);
+/* Primary rays of a camera as ray records (device) and, optionally, each
+ * ray's scene-box exit time (0 if it misses the box): training input of
+ * the bound (castRay, GPU_common.cl:10-30; box test SharedFunctions.hpp:245-264). */
+int hulk_camera_rays(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height, hulk_ray* rays,
+                     float* texit, uint32_t flags);
+/* Scene box of a context (min xyz, max xyz). */
+int hulk_ctx_scene_box(const hulk_ctx* ctx, float out[6]);
+
 /* ------------------------------------------ CPU KD tree (input producer) -- */
 
 /* SAH-NlogN KD tree, the reference's default --treeType

@@ -507,11 +507,14 @@ static void trace_one(const scene_t* s, int variant, uint32_t depth, const ray_t
 }
 
 /* Bounded trace with exact fallback: a bounded pass with the box exit
- * clipped to tbound * (1 + 2^-10); a hit with t < tbound is kept, anything
- * else re-runs unbounded. */
+ * clipped to tbound * (1 + 2^-10).  If the clip does not shorten the box
+ * interval the pass is the unbounded traversal; otherwise a hit with
+ * t < tbound is kept and anything else re-runs unbounded (flag bit 0). */
 static void trace_bounded(const scene_t* s, int variant, uint32_t depth, const ray_t* ray, float tbound, hit_t* out, cnt_t* c) {
-    if (!(tbound < FLT_MAX)) { trace_one(s, variant, depth, ray, FLT_MAX, out, c); return; }
     float tclip = tbound * (1.0f + 1.0f / 1024.0f);
+    float bts = 0.0f, bte = 0.0f;
+    int clipped = tbound < FLT_MAX && orc_ray_box(&s->box, ray, &bts, &bte) && tclip < bte;
+    if (!clipped) { trace_one(s, variant, depth, ray, FLT_MAX, out, c); return; }
     trace_one(s, variant, depth, ray, tclip, out, c);
     if (out->tri != NO_HIT && out->t < tbound) return;
     trace_one(s, variant, depth, ray, FLT_MAX, out, c);

@@ -1,0 +1,84 @@
+"""Batched frames (one graph-captured pipeline for many frames of uneven
+size and bounce depth) against per-frame rendering and the oracle, and the
+autotuner's choice against the oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import hulktracer_amd as H
+import oracle as O
+from tests.helpers import gpu_ctx, hit_mismatch, oracle_scene, scene_and_tree
+
+pytestmark = pytest.mark.gpu
+
+
+def _frames(scene, specs):
+    return [(scene.camera(pose, h / w), w, h, mb) for pose, w, h, mb in specs]
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_equals_single_frames(use_graph):
+    import torch
+
+    s, nodes, lp = scene_and_tree(3, 7, 5, 4)
+    ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK, persistent=True)
+    specs = [(0, 64, 64, 0), (3, 128, 96, 2), (5, 96, 128, 1), (7, 192, 192, 3), (9, 64, 128, 2)]
+    frames = _frames(s, specs)
+    total = sum(w * h for _, w, h, _ in frames)
+    rgba = torch.zeros(total, dtype=torch.int32, device="cuda")
+    b = ctx.batch(frames, s.mirror, rgba.data_ptr(), use_graph=use_graph)
+    assert b.rays == total
+    outs = []
+    for rep in range(3):
+        rgba.zero_()                       # eager work between replays
+        b.launch()
+        outs.append(rgba.cpu().numpy().view(np.uint32).copy())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0]), "graph replays differ"
+    off = 0
+    for cam, w, h, mb in frames:
+        single, _ = ctx.render_frame(cam, w, h, max_bounce=mb, mirror=s.mirror)
+        assert np.array_equal(outs[0][off:off + w * h], single)
+        off += w * h
+    b.close()
+
+
+def test_batch_matches_oracle_two_scenes():
+    """Config-4 style: frames of the random-triangle and terrain scenes, one
+    batch per scene, uneven sizes, bounce depths 0-3, against orc_render."""
+    import torch
+
+    for kind in (1, 2):
+        s, nodes, lp = scene_and_tree(kind, 1)
+        ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK)
+        specs = [(1, 64, 64, 0), (4, 128, 64, 3), (12, 96, 96, 1)]
+        frames = _frames(s, specs)
+        total = sum(w * h for _, w, h, _ in frames)
+        rgba = torch.zeros(total, dtype=torch.int32, device="cuda")
+        b = ctx.batch(frames, s.mirror, rgba.data_ptr(), use_graph=True)
+        b.launch()
+        got = rgba.cpu().numpy().view(np.uint32)
+        osc = oracle_scene(kind, 1)
+        off = 0
+        for cam, w, h, mb in frames:
+            ref, _ = osc.render(O.V_FULLSTACK, cam, w, h, max_bounce=mb)
+            assert np.array_equal(got[off:off + w * h], ref), f"scene {kind} frame {w}x{h}"
+            off += w * h
+        b.close()
+
+
+def test_autotune_picks_a_parity_clean_config():
+    s, nodes, lp = scene_and_tree(2, 1)
+    ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK)
+    cam = s.camera(6, 1.0)
+    best, table = ctx.autotune(cam, 256, 256, reps=2)
+    assert len(table) >= 10
+    assert min(r["ms"] for r in table) == pytest.approx(
+        [r["ms"] for r in table if r["variant"] == best["variant"] and r["block_size"] == best["block_size"]
+         and r["short_stack_depth"] == best["short_stack_depth"] and r["lds_top_levels"] == best["lds_top_levels"]
+         and r["persistent"] == best["persistent"]][0])
+    got = ctx.trace_camera(cam, 256, 256)
+    var = {1: O.V_FULLSTACK, 2: O.V_SHORTSTACK, 3: O.V_LDSSTACK, 4: O.V_RESTART, 5: O.V_ROPES}[best["variant"]]
+    ref = oracle_scene(2, 1).trace_camera(var, cam, 256, 256, depth=best["short_stack_depth"])
+    assert len(hit_mismatch(got, ref)) == 0
