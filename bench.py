@@ -1,16 +1,21 @@
 """Benchmark of the traverse-and-shade hot path on one MI355X.
 
-Workload (one step = one frame): the terrain + spheres scene (163,650
-triangles, SAH-NlogN KD tree built once on the CPU), 1024x1024 pinhole
-primary rays, nearest hit, one point-light shadow ray per hit (nearest hit,
-as the reference traces it), Lambert shading into an RGBA8 image.  Inputs
-live in HBM before the timed region; outputs stay in HBM.
+Headline workload (one step): poses 10-19 of the terrain + spheres scene
+(163,650 triangles, SAH-NlogN KD tree built on the CPU), each frame 1024x1024
+pinhole primary rays (nearest hit), one nearest-hit shadow ray per hit and
+point light, Lambert shading into RGBA8.  Inputs are resident in HBM before
+the timed region; outputs stay in HBM.
 
-value = nearest-hit rays per second (primary rays of each frame / frame
-time), whole job.  Also reported: total rays/s incl. shadow rays, every
-traversal variant's primary-trace rate, the roofline of the dominant
-kernel (the primary trace) and the reference CPU path (port) timed on the
-host on the same frame.
+Methods on the same 10 frames:
+  plain    - frame after frame (hulk_render_frame), autotuned variant
+  batched  - the 10 frames in one graph-captured pipeline (hulk_batch)
+  learned  - batched, with per-pixel bounds from the MLP trained on poses
+             0-9 (inference inside the timed region; training is not)
+Every method's primary hits are compared with plain's on every ray.
+value = nearest-hit (primary) rays per second of the fastest method whose
+hits match 100%.  Extras: every variant's rate, the roofline of the
+primary trace kernel, the reference CPU path (port) on the host, and the
+animated 1M-triangle and 64-frame batched workloads.
 """
 from __future__ import annotations
 
@@ -34,160 +39,249 @@ def log(msg: str) -> None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--res", type=int, default=1024)
-    ap.add_argument("--pose", type=int, default=0)
-    ap.add_argument("--variant", default="auto")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run a device copy of this many MiB (HBM traffic counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        log("bench.py measures one GPU only: this build has no multi-GPU path (the kernel runs on one "
-            "GPU's share of the rays; see DESIGN.md for how the path would shard). Run with --gpus 1.")
+        log("bench.py measures one GPU only: this build has no multi-GPU path (rays are split per GPU only "
+            "in the design, see DESIGN.md section 8). Run with --gpus 1.")
         return 2
 
     import numpy as np
     import torch
 
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": nbytes, "launches": 5}))
+        return 0
+
     import hulktracer_amd as H
     from hulktracer_amd import _lib as L
+    from hulktracer_amd import workloads as WL
 
     W = Hh = args.res
     n = W * Hh
     t0 = time.time()
     scene = H.Scene.procedural(L.SCENE_TERRAIN_SPHERES, seed=1)
-    kd = H.KDTreeCPU()
-    tb = time.time()
-    kd.buildKDTree(scene)
-    build_ms = (time.time() - tb) * 1e3
-    nodes, lp = kd.getNodeArray(), kd.getPolygonArray()
-    cam = scene.camera(args.pose, Hh / W)
+    nodes, lp, build_ms = WL.build(scene)
+    cams = [scene.camera(k, Hh / W) for k in range(10, 20)]
+    nf = len(cams)
     log(f"scene {scene.n_tris} tris, {len(nodes)} nodes, {len(lp)} leaf refs, KD build {build_ms:.0f} ms "
         f"(setup {time.time() - t0:.1f}s)")
 
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
+    stream = torch.cuda.Stream(device=dev)  # default priority; library + events on it
     hits = torch.empty(n * 32, dtype=torch.uint8, device=dev)
     rgba = torch.empty(n, dtype=torch.int32, device=dev)
 
-    def make_ctx(variant, **kw):
-        ctx = H.GPUTrace(variant=variant, stream=stream.cuda_stream, **kw)
-        ctx.upload_scene(scene)
-        ctx.upload_kdtree(nodes, lp)
-        return ctx
-
-    # -- per-variant primary trace rate (plain), kernel time from HIP events
-    def time_trace(ctx, reps):
-        for _ in range(2):
-            ctx.trace_camera_dev(cam, W, Hh, hits.data_ptr(), flags=L.FLAG_NO_SYNC)
+    def time_trace(ctx, cam, reps, tb=None):
+        ctx.trace_camera_dev(cam, W, Hh, hits.data_ptr(), tbound_dev=tb, flags=L.FLAG_NO_SYNC)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(reps):
-            ctx.trace_camera_dev(cam, W, Hh, hits.data_ptr(), flags=L.FLAG_NO_SYNC)
+            ctx.trace_camera_dev(cam, W, Hh, hits.data_ptr(), tbound_dev=tb, flags=L.FLAG_NO_SYNC)
         e1.record(stream)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    configs = {
-        "GPUTrace": dict(variant=H.FULLSTACK, block_size=256),
-        "GPUTrace_shortStack": dict(variant=H.SHORTSTACK, block_size=256),
-        "GPUTrace_LDSshortStack": dict(variant=H.LDSSTACK, block_size=256, lds_top_levels=8),
-        "GPUTrace_stackless": dict(variant=H.RESTART, block_size=256),
-        "GPUTrace_ropes": dict(variant=H.ROPES, block_size=256),
-    }
+    # -- every variant at its default shape (primary trace of pose 10)
+    shapes = {"GPUTrace": dict(variant=H.FULLSTACK), "GPUTrace_shortStack": dict(variant=H.SHORTSTACK),
+              "GPUTrace_LDSshortStack": dict(variant=H.LDSSTACK, lds_top_levels=8),
+              "GPUTrace_stackless": dict(variant=H.RESTART), "GPUTrace_ropes": dict(variant=H.ROPES)}
     variants = {}
-    for name, kw in configs.items():
-        ctx = make_ctx(**kw)
-        ms = time_trace(ctx, 10 if kw["variant"] != H.RESTART else 3)
-        variants[name] = {"ms": round(ms, 4), "mrays_s": round(n / ms / 1e3, 1)}
+    for name, kw in shapes.items():
+        ctx = WL.make_ctx(scene, nodes, lp, stream, **kw)
+        ms = time_trace(ctx, cams[0], 5 if kw["variant"] != H.RESTART else 2)
+        variants[name] = {"primary_trace_ms": round(ms, 4), "mrays_s": round(n / ms / 1e3, 1)}
         log(f"  {name:24s} primary trace {ms:8.3f} ms  {n / ms / 1e3:9.1f} Mrays/s")
         ctx.close()
-    best = min(variants, key=lambda k: variants[k]["ms"]) if args.variant == "auto" else args.variant
-    ctx = make_ctx(**configs[best])
 
-    # -- counters for the algorithmic bytes: full-stack node / triangle counts
-    cnt_ctx = make_ctx(variant=H.FULLSTACK)
-    cnt_ctx.reset_counters()
-    cnt_ctx.trace_camera(cam, W, Hh, count=True)
-    c = cnt_ctx.counters()
-    cnt_ctx.close()
-    n_node = c["node_visits"] / c["rays"]
-    n_tri = c["tri_tests"] / c["rays"]
+    # -- autotuned options for this scene
+    ctx = WL.make_ctx(scene, nodes, lp, stream, variant=H.FULLSTACK)
+    tt = time.time()
+    best, table = ctx.autotune(cams[0], 512, 512, reps=2)
+    log(f"autotune {time.time() - tt:.1f}s over {len(table)} shapes -> {best}")
+
+    # -- algorithmic bytes per ray: full-stack node / triangle counts on pose 10
+    cnt = WL.make_ctx(scene, nodes, lp, stream, variant=H.FULLSTACK)
+    cnt.trace_camera(cams[0], W, Hh, count=True)
+    c = cnt.counters()
+    cnt.close()
+    n_node, n_tri = c["node_visits"] / c["rays"], c["tri_tests"] / c["rays"]
     b_ray = 32 + 16 + 16 * n_node + 64 * n_tri
 
-    # -- timed frames: primary + shadow + shade on the chosen variant
-    for _ in range(args.warmup):
-        ctx.render_frame_dev(cam, W, Hh, 0, None, rgba.data_ptr(), hits.data_ptr(), flags=L.FLAG_NO_SYNC)
-    torch.cuda.synchronize()
+    # reference hits of every frame (plain, tuned options)
+    ref_hits = [ctx.trace_camera(cam, W, Hh) for cam in cams]
     ctx.reset_counters()
-    ctx.render_frame_dev(cam, W, Hh, 0, None, rgba.data_ptr(), hits.data_ptr(), flags=L.FLAG_COUNT)
-    fc = ctx.counters()
-    shadow_rays = fc["rays"] - n
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.render_frame_dev(cam, W, Hh, 0, None, rgba.data_ptr(), hits.data_ptr(), flags=L.FLAG_NO_SYNC)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    ms_step = elapsed * 1e3 / args.steps
-    value = n * args.steps / elapsed / 1e6
+    ctx.render_frame(cams[0], W, Hh, count=True)
+    shadow_per_frame = ctx.counters()["shadow_rays"]
 
-    # -- dominant kernel: the primary trace launch, timed live with events
-    k_ms = time_trace(ctx, max(args.steps, 10))
+    methods = {}
+
+    def timed(fn, reps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    def hits_match(h):
+        return float(np.mean([np.array_equal(h[k * n:(k + 1) * n]["tri"], ref_hits[k]["tri"]) and
+                              np.array_equal(h[k * n:(k + 1) * n]["t"].view(np.uint32),
+                                             ref_hits[k]["t"].view(np.uint32)) for k in range(nf)]))
+
+    # plain: frame after frame
+    def plain_step():
+        for cam in cams:
+            ctx.render_frame_dev(cam, W, Hh, 0, None, rgba.data_ptr(), hits.data_ptr(), flags=L.FLAG_NO_SYNC)
+
+    s_plain = timed(plain_step, args.steps)
+    methods["plain"] = {"ms_per_step": round(s_plain * 1e3, 3), "mrays_s": round(nf * n / s_plain / 1e6, 1),
+                        "hit_match": 1.0}
+
+    # batched: one graph for the 10 frames
+    frames = [(cam, W, Hh, 0) for cam in cams]
+    brgba = torch.empty(nf * n, dtype=torch.int32, device=dev)
+    batch = ctx.batch(frames, None, brgba.data_ptr(), use_graph=True)
+    s_batch = timed(lambda: batch.launch(L.FLAG_NO_SYNC), args.steps)
+    methods["batched"] = {"ms_per_step": round(s_batch * 1e3, 3), "mrays_s": round(nf * n / s_batch / 1e6, 1),
+                          "hit_match": hits_match(batch.primary_hits())}
+    batch.close()
+
+    # learned + batched
+    learned_info = None
+    if not args.no_learned:
+        from hulktracer_amd.learned import HitDistanceMLP, training_pairs
+
+        tl = time.time()
+        rays, y = training_pairs(ctx, [scene.camera(k, Hh / W) for k in range(10)], W, Hh, n_samples=4_000_000)
+        mlp = HitDistanceMLP(ctx, seed=1, max_batch=16384)
+        tt = time.time()
+        losses = mlp.fit(rays, y, steps=args.train_steps, batch=16384, lr=2e-3,
+                         log_every=max(args.train_steps // 4, 1))
+        torch.cuda.synchronize()
+        train_s = time.time() - tt
+        margin = mlp.calibrate(rays[: 1 << 20], y[: 1 << 20], 0.95)
+        del rays, y
+        tb = torch.empty(nf * n, dtype=torch.float32, device=dev)
+        lbatch = ctx.batch(frames, None, brgba.data_ptr(), tbound_dev=tb.data_ptr(), use_graph=True)
+
+        def learned_step():
+            for k, cam in enumerate(cams):
+                mlp.bound_camera(cam, W, Hh, tb[k * n:(k + 1) * n], sync=False)
+            lbatch.launch(L.FLAG_NO_SYNC)
+
+        s_learn = timed(learned_step, args.steps)
+        lh = lbatch.primary_hits()
+        fb = float(np.mean((lh["flags"] & 1) != 0))
+        # node visits with and without the bound on pose 10 (counted runs)
+        ctx.reset_counters()
+        ctx.trace_camera(cams[0], W, Hh, count=True)
+        cp = ctx.counters()["node_visits"]
+        ctx.reset_counters()
+        ctx.trace_camera(cams[0], W, Hh, tbound=tb[:n].cpu().numpy(), count=True)
+        cb = ctx.counters()["node_visits"]
+        methods["learned+batched"] = {"ms_per_step": round(s_learn * 1e3, 3),
+                                      "mrays_s": round(nf * n / s_learn / 1e6, 1), "hit_match": hits_match(lh),
+                                      "fallback_rate": round(fb, 4), "node_visits_saved": round(1 - cb / cp, 4)}
+        learned_info = {"train_pairs": 4_000_000, "train_steps": args.train_steps, "train_s": round(train_s, 2),
+                        "data_s": round(tt - tl, 2), "margin": round(margin, 4),
+                        "loss": [[int(a), round(float(b), 4)] for a, b in losses]}
+        lbatch.close()
+        mlp.close()
+    for k, v in methods.items():
+        log(f"method {k:16s} {v}")
+
+    ok = {k: v for k, v in methods.items() if v["hit_match"] == 1.0}
+    best_m = max(ok, key=lambda k: ok[k]["mrays_s"])
+    s_best = methods[best_m]["ms_per_step"] / 1e3
+
+    # -- dominant kernel: primary trace with the tuned options, live events
+    k_ms = time_trace(ctx, cams[0], max(args.steps, 10))
     achieved = n * b_ray / (k_ms * 1e-3) / 1e9
-    log(f"frame {ms_step:.3f} ms, primary trace {k_ms:.3f} ms, B_ray {b_ray:.1f} B "
-        f"(nodes {n_node:.2f}, tris {n_tri:.2f}), {achieved:.0f} GB/s algorithmic")
+    log(f"primary trace {k_ms:.3f} ms, B_ray {b_ray:.1f} B (nodes {n_node:.2f}, tris {n_tri:.2f}), "
+        f"{achieved:.0f} GB/s algorithmic")
+    tuned_name = L.VARIANT_NAMES[best["variant"]]
 
     result = {
-        "metric": "Mrays/s nearest-hit @1024^2 (terrain+spheres 163,650 tris, primary+shadow+shade per frame)",
-        "value": round(value, 2),
+        "metric": "Mrays/s nearest-hit @1024^2, learned-bound+batched vs plain, 100% hit match",
+        "value": round(nf * n / s_best / 1e6, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(s_best * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": f"terrain_spheres_163650tris_{W}x{Hh}_primary+shadow", "variant": best,
-                   "method": "plain", "pose": args.pose, "kd_build_ms": round(build_ms, 1)},
-        "total_rays_per_s_M": round((n + shadow_rays) * args.steps / elapsed / 1e6, 2),
-        "shadow_rays_per_frame": int(shadow_rays),
+        "config": {"workload": f"terrain_spheres_163650tris_poses10-19_{W}x{Hh}_primary+shadow+shade",
+                   "frames_per_step": nf, "method": best_m, "variant": tuned_name,
+                   "options": {k: int(v) for k, v in best.items()}, "kd_build_ms": round(build_ms, 1)},
+        "methods": methods,
+        "speedup_vs_plain": round(methods[best_m]["mrays_s"] / methods["plain"]["mrays_s"], 3),
+        "shadow_rays_per_frame": int(shadow_per_frame),
         "variants_primary_trace": variants,
+        "learned": learned_info,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"k_trace<{configs[best]['variant']},0>", "kernel_ms": round(k_ms, 4),
-                     "bytes_per_ray": round(b_ray, 1), "node_visits_per_ray": round(n_node, 3),
-                     "tri_tests_per_ray": round(n_tri, 3)},
+                     "kernel": f"k_trace<{best['variant']},false>", "kernel_ms": round(k_ms, 4),
+                     "rays_per_launch": n, "bytes_per_ray": round(b_ray, 1),
+                     "node_visits_per_ray": round(n_node, 3), "tri_tests_per_ray": round(n_tri, 3)},
         "cpu_baseline": None,
     }
+    ctx.close()
 
-    # -- CPU baseline: reference CPU path port on the same frame
     if not args.no_cpu:
         import oracle as O
 
         threads = min(16, os.cpu_count() or 1)
         osc = O.OracleScene(scene.vertices, scene.triangles, scene.box, nodes, lp, scene.lights)
-        osc.cpu_trace_scene(cam, W, Hh, 0, 8, threads=threads, out=False)  # cache build + warm
-        frames, rays_total, tc = 0, 0, time.perf_counter()
+        osc.cpu_trace_scene(cams[0], W, Hh, 0, 8, threads=threads, out=False)
+        frames_done, rays_total, tc = 0, 0, time.perf_counter()
         while True:
-            rays_total += osc.cpu_trace_scene(cam, W, Hh, threads=threads, out=False)[0]
-            frames += 1
-            if time.perf_counter() - tc > args.cpu_seconds or frames >= 50:
+            rays_total += osc.cpu_trace_scene(cams[frames_done % nf], W, Hh, threads=threads, out=False)[0]
+            frames_done += 1
+            if time.perf_counter() - tc > args.cpu_seconds or frames_done >= 50:
                 break
         cpu_s = time.perf_counter() - tc
         result["cpu_baseline"] = {
-            "value": round(n * frames / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{frames} full {W}x{Hh} frames (primary+shadow+shade), {cpu_s:.1f}s, "
-                      f"{rays_total / cpu_s / 1e6:.2f} M total rays/s, OpenMP over rows, "
-                      f"host {os.cpu_count()} cpus",
+            "value": round(n * frames_done / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{frames_done} whole {W}x{Hh} frames of poses 10-19 (primary + shadow + shade), "
+                      f"{cpu_s:.1f}s, {rays_total / cpu_s / 1e6:.2f} M total rays/s incl. shadow rays, OpenMP "
+                      f"over rows, host has {os.cpu_count()} cpus",
         }
-        log(f"cpu baseline {result['cpu_baseline']['value']} Mrays/s on {threads} threads")
-    ctx.close()
+        log(f"cpu baseline {result['cpu_baseline']}")
+
+    if not args.no_extras:
+        extras = {}
+        for name, fn in (("animated_1M", lambda: WL.animated(frames=20, stream=stream, log=log)),
+                         ("batched_64_frames", lambda: WL.batched(stream=stream, log=log))):
+            try:
+                extras[name] = fn()
+            except Exception as e:  # an extra must not hide the headline line
+                extras[name] = {"error": repr(e)}
+                log(f"{name} failed: {e!r}")
+        result["extras"] = extras
+
     print(json.dumps(result), flush=True)
     return 0
 

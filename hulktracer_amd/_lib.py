@@ -64,7 +64,7 @@ class Opts(C.Structure):
 class Counters(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("node_visits", C.c_uint64), ("leaf_visits", C.c_uint64),
                 ("tri_tests", C.c_uint64), ("hits", C.c_uint64), ("fallbacks", C.c_uint64),
-                ("last_kernel_ms", C.c_float), ("last_shade_ms", C.c_float)]
+                ("shadow_rays", C.c_uint64), ("last_kernel_ms", C.c_float), ("last_shade_ms", C.c_float)]
 
 
 class FrameDesc(C.Structure):
@@ -103,6 +103,7 @@ EXPORTS = {
     "hulk_batch_launch": (C.c_int, [C.c_void_p, C.c_uint32]),
     "hulk_batch_ray_count": (C.c_uint32, [C.c_void_p]),
     "hulk_batch_destroy": (None, [C.c_void_p]),
+    "hulk_batch_primary_hits": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "hulk_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.POINTER(TuneEntry), C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(Opts)]),
     "hulk_mlp_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),

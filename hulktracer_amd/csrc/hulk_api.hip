@@ -306,6 +306,7 @@ int read_stats(hulk_ctx* c) {
     c->cnt.tri_tests += c->host_stats[3];
     c->cnt.hits += c->host_stats[4];
     c->cnt.fallbacks += c->host_stats[5];
+    c->cnt.shadow_rays += c->host_stats[6];
     return HULK_OK;
 }
 
@@ -1071,6 +1072,14 @@ extern "C" int hulk_batch_launch(hulk_batch* b, uint32_t flags) {
 }
 
 extern "C" uint32_t hulk_batch_ray_count(const hulk_batch* b) { return b ? b->n : 0; }
+
+extern "C" int hulk_batch_primary_hits(hulk_batch* b, hulk_hit* out, uint32_t flags) {
+    if (!b || !out) return HULK_E_ILLEGAL_INPUT;
+    hulk_ctx* c = b->c;
+    const hipMemcpyKind kind = (flags & HULK_FLAG_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    HIP_TRY(c, hipMemcpyAsync(out, b->w.hits.p, (size_t)b->n * sizeof(hulk_hit), kind, c->stream));
+    return finish(c, flags, false);
+}
 
 extern "C" void hulk_batch_destroy(hulk_batch* b) {
     if (!b) return;

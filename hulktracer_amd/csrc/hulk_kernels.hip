@@ -59,6 +59,33 @@ __device__ __forceinline__ uint32_t find_frame(const FrameDev* F, uint32_t n, ui
     return lo;
 }
 
+// Ray i of a camera launch -> pixel (x, y).  Images whose sides are
+// multiples of 8 are walked in 8x8 tiles, so each wave traces one compact
+// tile (coherent rays, as the reference's 8x8 work-groups,
+// GLVisitorGPUTrace.cpp:18); other sizes go row by row.
+__device__ __forceinline__ void tile_pixel(uint32_t i, uint32_t W, uint32_t H, uint32_t& x, uint32_t& y) {
+    if (((W | H) & 7u) == 0) {
+        const uint32_t tile = i >> 6, k = i & 63u, tiles_x = W >> 3;
+        const uint32_t ty = tile / tiles_x, tx = tile - ty * tiles_x;
+        x = (tx << 3) | (k & 7u);
+        y = (ty << 3) | (k >> 3);
+    } else {
+        y = i / W;
+        x = i - y * W;
+    }
+}
+
+// Batched primary ray i: pixel index (frame offset + y*W + x) returned,
+// 8x8 tiles inside a frame when its sides are multiples of 8.
+__device__ __forceinline__ uint32_t frame_ray_tiled(const FrameDev* F, uint32_t n, uint32_t i, Ray& r) {
+    const uint32_t f = find_frame(F, n, i);
+    const FrameDev& fr = F[f];
+    uint32_t x, y;
+    tile_pixel(i - fr.offset, fr.W, fr.H, x, y);
+    cast_ray(fr.cam, x, y, fr.W, fr.H, r);
+    return fr.offset + y * fr.W + x;
+}
+
 __device__ __forceinline__ uint32_t frame_ray(const FrameDev* F, uint32_t n, uint32_t i, Ray& r) {
     const uint32_t f = find_frame(F, n, i);
     const FrameDev& fr = F[f];
@@ -447,11 +474,14 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
         const uint32_t i = chunk * 64u + lane_id;
         if (i < n_rays) {
             Ray r;
+            uint32_t o = i;  // output / bound index: the pixel index for camera rays
             if (P.source == kSourceCamera) {
-                const uint32_t y = i / P.W, x = i - y * P.W;
+                uint32_t x, y;
+                tile_pixel(i, P.W, P.H, x, y);
+                o = y * P.W + x;
                 cast_ray(P.cam, x, y, P.W, P.H, r);
             } else if (P.source == kSourceFrames) {
-                frame_ray(P.frames, P.n_frames, i, r);
+                o = frame_ray_tiled(P.frames, P.n_frames, i, r);
             } else {
                 const hulk_ray ray = P.rays[i];
                 r.o = v3(ray.start.x, ray.start.y, ray.start.z);
@@ -462,7 +492,7 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
             best_init(best);
             Cnt c = {0, 0, 0};
             uint32_t flags = 0;
-            const float tb = P.tbound ? P.tbound[i] : kFltMax;
+            const float tb = P.tbound ? P.tbound[o] : kFltMax;
             // Bounded pass: box exit clipped to tb * (1 + 2^-10).  When the
             // clip does not shorten the interval the pass is the unbounded
             // traversal; otherwise only a hit below tb is kept and every other
@@ -482,14 +512,14 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
                 trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, kFltMax, best, c, flags);
             }
             if (P.t_out) {
-                P.t_out[P.t_slot ? P.t_slot[i] : i] = best.t;
+                P.t_out[P.t_slot ? P.t_slot[i] : o] = best.t;
             } else {
                 float u = 0.0f, v = 0.0f;
                 if (best.tri != kNoHit) {
                     u = best.s1 / best.det;
                     v = best.s2 / best.det;
                 }
-                float4* h = reinterpret_cast<float4*>(P.hits + i);
+                float4* h = reinterpret_cast<float4*>(P.hits + o);
                 h[0] = make_float4(best.t, u, v, __uint_as_float(best.tri));
                 h[1] = make_float4(best.nx, best.ny, best.nz, __uint_as_float(flags));
             }
@@ -517,7 +547,8 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
         chits = wave_sum(chits);
         crays = wave_sum(crays);
         if (lane_id == 0) {
-            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[0]), (unsigned long long)crays);
+            // rays with a full hit record (primary / mirror) vs shadow rays
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[P.t_out ? 6 : 0]), (unsigned long long)crays);
             atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[1]), (unsigned long long)cn);
             atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[2]), (unsigned long long)cl);
             atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[3]), (unsigned long long)ct);

@@ -327,6 +327,11 @@ class Batch:
     def launch(self, flags: int = 0):
         _check(self.ctx.h, lib().hulk_batch_launch(self.h, flags))
 
+    def primary_hits(self) -> np.ndarray:
+        out = np.zeros(self.rays, HIT_DTYPE)
+        _check(self.ctx.h, lib().hulk_batch_primary_hits(self.h, ptr(out), 0))
+        return out
+
     def close(self):
         if self.h:
             lib().hulk_batch_destroy(self.h)

@@ -148,12 +148,13 @@ typedef struct hulk_ctx hulk_ctx;
  * the reference Statistics counters (HulkEngine/src/Statistics.hpp:58-60,
  * 112-120), counted in every variant. */
 typedef struct hulk_counters {
-    uint64_t rays;
-    uint64_t node_visits;   /* node fetches                                 */
+    uint64_t rays;          /* nearest-hit rays with a hit record (primary, mirror) */
+    uint64_t node_visits;   /* node fetches (all rays)                      */
     uint64_t leaf_visits;
     uint64_t tri_tests;
     uint64_t hits;
     uint64_t fallbacks;     /* bounded rays re-run unbounded                */
+    uint64_t shadow_rays;   /* point-light shadow rays (nearest hit, t only) */
     float    last_kernel_ms;/* device time of the last trace kernel (events)*/
     float    last_shade_ms;
 } hulk_counters;
@@ -245,6 +246,8 @@ int hulk_batch_create(hulk_ctx* ctx, const hulk_frame_desc* frames, uint32_t n_f
                       uint32_t use_graph, hulk_batch** out);
 int hulk_batch_launch(hulk_batch* batch, uint32_t flags);
 uint32_t hulk_batch_ray_count(const hulk_batch* batch);
+/* Copy the primary hit records of the last launch (frames back to back). */
+int hulk_batch_primary_hits(hulk_batch* batch, hulk_hit* out, uint32_t flags);
 void hulk_batch_destroy(hulk_batch* batch);
 
 /* --------------------------------------------------------------- autotune -- */

@@ -394,7 +394,7 @@ static void trav_brute(const scene_t* s, const ray_t* ray, float tclip, best_t* 
 uint32_t orc_build_ropes(const node_t* nodes, uint32_t n_nodes, const box_t* root_box,
                          uint32_t* ropes, box_t* cells, uint32_t* kids) {
     typedef struct { uint32_t slot; box_t cell; uint32_t r[6]; } item;
-    item st[64];
+    item st[256];
     uint32_t sp = 0, slots = n_nodes;
     st[0].slot = 0; st[0].cell = *root_box;
     for (int k = 0; k < 6; k++) st[0].r[k] = ROPE_NONE;
