@@ -21,7 +21,7 @@ def _frames(scene, specs):
 def test_batch_equals_single_frames(use_graph):
     import torch
 
-    s, nodes, lp = scene_and_tree(3, 7, 5, 4)
+    s, nodes, lp = scene_and_tree(3, 6, 5, 4)
     ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK, persistent=True)
     specs = [(0, 64, 64, 0), (3, 128, 96, 2), (5, 96, 128, 1), (7, 192, 192, 3), (9, 64, 128, 2)]
     frames = _frames(s, specs)
