@@ -142,10 +142,11 @@ def main() -> int:
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / reps
 
-    def hits_match(h):
-        return float(np.mean([np.array_equal(h[k * n:(k + 1) * n]["tri"], ref_hits[k]["tri"]) and
+    def hits_match(h, ref=None):
+        ref = ref_hits if ref is None else ref
+        return float(np.mean([np.array_equal(h[k * n:(k + 1) * n]["tri"], ref[k]["tri"]) and
                               np.array_equal(h[k * n:(k + 1) * n]["t"].view(np.uint32),
-                                             ref_hits[k]["t"].view(np.uint32)) for k in range(nf)]))
+                                             ref[k]["t"].view(np.uint32)) for k in range(nf)]))
 
     # plain: frame after frame
     def plain_step():
@@ -165,46 +166,72 @@ def main() -> int:
                           "hit_match": hits_match(batch.primary_hits())}
     batch.close()
 
-    # learned + batched
+    # learned + batched, on the tuned variant and on the reference's live
+    # kernel (kd-restart, GPUTrace_stackless: no exit before the box exit,
+    # so a bound is what lets it stop early)
     learned_info = None
     if not args.no_learned:
-        from hulktracer_amd.learned import HitDistanceMLP, training_pairs
+        from hulktracer_amd.learned import HitDistanceMLP, calibrate_frames, training_pairs
 
         tl = time.time()
-        rays, y = training_pairs(ctx, [scene.camera(k, Hh / W) for k in range(10)], W, Hh, n_samples=4_000_000)
+        train_cams = [scene.camera(k, Hh / W) for k in range(10)]
+        rays, y = training_pairs(ctx, train_cams, W, Hh, n_samples=4_000_000)
         mlp = HitDistanceMLP(ctx, seed=1, max_batch=16384)
         tt = time.time()
         losses = mlp.fit(rays, y, steps=args.train_steps, batch=16384, lr=2e-3,
                          log_every=max(args.train_steps // 4, 1))
         torch.cuda.synchronize()
         train_s = time.time() - tt
-        margin = mlp.calibrate(rays[: 1 << 20], y[: 1 << 20], 0.95)
         del rays, y
+        margin = calibrate_frames(mlp, ctx, train_cams[::3], W, Hh, quantile=0.95, stride=2)
         tb = torch.empty(nf * n, dtype=torch.float32, device=dev)
-        lbatch = ctx.batch(frames, None, brgba.data_ptr(), tbound_dev=tb.data_ptr(), use_graph=True)
 
-        def learned_step():
+        def bounds():
             for k, cam in enumerate(cams):
                 mlp.bound_camera(cam, W, Hh, tb[k * n:(k + 1) * n], sync=False)
-            lbatch.launch(L.FLAG_NO_SYNC)
 
-        s_learn = timed(learned_step, args.steps)
-        lh = lbatch.primary_hits()
-        fb = float(np.mean((lh["flags"] & 1) != 0))
-        # node visits with and without the bound on pose 10 (counted runs)
-        ctx.reset_counters()
-        ctx.trace_camera(cams[0], W, Hh, count=True)
-        cp = ctx.counters()["node_visits"]
-        ctx.reset_counters()
-        ctx.trace_camera(cams[0], W, Hh, tbound=tb[:n].cpu().numpy(), count=True)
-        cb = ctx.counters()["node_visits"]
-        methods["learned+batched"] = {"ms_per_step": round(s_learn * 1e3, 3),
-                                      "mrays_s": round(nf * n / s_learn / 1e6, 1), "hit_match": hits_match(lh),
-                                      "fallback_rate": round(fb, 4), "node_visits_saved": round(1 - cb / cp, 4)}
+        bounds()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        bounds()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        infer_ms = e0.elapsed_time(e1) / nf
         learned_info = {"train_pairs": 4_000_000, "train_steps": args.train_steps, "train_s": round(train_s, 2),
-                        "data_s": round(tt - tl, 2), "margin": round(margin, 4),
+                        "data_s": round(tt - tl, 2), "margin": round(margin, 4), "bound_stride": 2,
+                        "inference_ms_per_frame": round(infer_ms, 4),
+                        "inference_tflops": round(n / 4 * 82176 / (infer_ms * 1e-3) / 1e12, 2),
                         "loss": [[int(a), round(float(b), 4)] for a, b in losses]}
-        lbatch.close()
+        opts_now = {k: int(v) for k, v in best.items() if k != "device"}
+        for tag, vopts in (("", opts_now), ("_stackless", dict(opts_now, variant=H.RESTART))):
+            ctx.set_options(**vopts)
+            ref_v = [ctx.trace_camera(cam, W, Hh) for cam in cams] if tag else ref_hits
+            if tag:  # plain frame-by-frame on the reference's live kernel
+                s_pl = timed(plain_step, max(1, args.steps // 5))
+                methods["plain" + tag] = {"ms_per_step": round(s_pl * 1e3, 3),
+                                         "mrays_s": round(nf * n / s_pl / 1e6, 1), "hit_match": 1.0}
+            lbatch = ctx.batch(frames, None, brgba.data_ptr(), tbound_dev=tb.data_ptr(), use_graph=True)
+
+            def learned_step():
+                bounds()
+                lbatch.launch(L.FLAG_NO_SYNC)
+
+            s_learn = timed(learned_step, args.steps)
+            lh = lbatch.primary_hits()
+            fb = float(np.mean((lh["flags"] & 1) != 0))
+            ctx.reset_counters()
+            ctx.trace_camera(cams[0], W, Hh, count=True)
+            cp = ctx.counters()["node_visits"]
+            ctx.reset_counters()
+            ctx.trace_camera(cams[0], W, Hh, tbound=tb[:n].cpu().numpy(), count=True)
+            cb = ctx.counters()["node_visits"]
+            methods["learned+batched" + tag] = {
+                "ms_per_step": round(s_learn * 1e3, 3), "mrays_s": round(nf * n / s_learn / 1e6, 1),
+                "hit_match": hits_match(lh, ref_v), "fallback_rate": round(fb, 4),
+                "node_visits_saved": round(1 - cb / cp, 4)}
+            lbatch.close()
+        ctx.set_options(**opts_now)
         mlp.close()
     for k, v in methods.items():
         log(f"method {k:16s} {v}")

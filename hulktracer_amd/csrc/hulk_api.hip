@@ -36,7 +36,8 @@ struct DevBuf {
 // accumulation).  The context owns one set; every batch owns its own, so a
 // captured graph never sees a buffer that a later call reallocates.
 struct Work {
-    DevBuf hits, hits2, hits3, shadow_rays, shadow_slot, shadow_t, qrays[2], qpix[2], qw[2], acc, ctr;
+    DevBuf hits, hits2, hits3, shadow_rays, shadow_slot, shadow_t, qrays[2], qpix[2], qw[2], acc, ctr, wcount,
+        woffset;
 };
 
 struct hulk_ctx {
@@ -323,7 +324,8 @@ int finish(hulk_ctx* c, uint32_t flags, bool timed) {
 
 void release_work(Work& w) {
     DevBuf* bufs[] = {&w.hits, &w.hits2, &w.hits3, &w.shadow_rays, &w.shadow_slot, &w.shadow_t, &w.qrays[0],
-                      &w.qrays[1], &w.qpix[0], &w.qpix[1], &w.qw[0], &w.qw[1], &w.acc, &w.ctr};
+                      &w.qrays[1], &w.qpix[0], &w.qpix[1], &w.qw[0], &w.qw[1], &w.acc, &w.ctr, &w.wcount,
+                      &w.woffset};
     for (DevBuf* b : bufs) release(*b);
 }
 
@@ -338,6 +340,9 @@ int ensure_work(hulk_ctx* c, Work& w, uint32_t n, uint32_t np, bool bounces) {
     if ((r = ensure(c, w.shadow_rays, (size_t)n * m * sizeof(hulk_ray)))) return r;
     if ((r = ensure(c, w.shadow_slot, (size_t)n * m * 4))) return r;
     if ((r = ensure(c, w.shadow_t, (size_t)n * m * 4))) return r;
+    const size_t waves = ((size_t)n + 255) / 256 * 4;
+    if ((r = ensure(c, w.wcount, waves * 8))) return r;
+    if ((r = ensure(c, w.woffset, waves * 8))) return r;
     if (bounces) {
         if ((r = ensure(c, w.hits2, (size_t)n * sizeof(hulk_hit)))) return r;
         if ((r = ensure(c, w.hits3, (size_t)n * sizeof(hulk_hit)))) return r;
@@ -405,9 +410,7 @@ int shade_pipeline(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w,
     const uint32_t* cur_n = nullptr;
     for (uint32_t g = 0; g <= max_bounce; g++) {
         const bool spawn = g < max_bounce && mirror_dev;
-        const int qo = g & 1;  // queue written by this generation
-        HIP_TRY(c, launch_fill_u32(ctr + 1, 1, 0u, c->stream));
-        if (spawn) HIP_TRY(c, launch_fill_u32(ctr + 2 + qo, 1, 0u, c->stream));
+        const int qo = g & 1;  // queue written by this generation (its length is set by the scan)
         GenParams G;
         std::memset(&G, 0, sizeof(G));
         G.hits = cur_hits;
@@ -435,6 +438,8 @@ int shade_pipeline(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w,
             G.weight_out = static_cast<float*>(w.qw[qo].p);
             G.next_count = ctr + 2 + qo;
         }
+        G.wave_counts = static_cast<uint32_t*>(w.wcount.p);
+        G.wave_offsets = static_cast<uint32_t*>(w.woffset.p);
         HIP_TRY(c, launch_gen_secondary(G, n, c->stream));
         if (np) {
             TraceParams S = P;
@@ -444,7 +449,8 @@ int shade_pipeline(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w,
             S.n = n * np;
             S.t_out = static_cast<float*>(w.shadow_t.p);
             S.t_slot = static_cast<const uint32_t*>(w.shadow_slot.p);
-            if ((r = run_trace(c, o.variant, S, n * np, block, lds, count, true, ctr, false))) return r;
+            S.stop_at_w = 1;
+            if ((r = run_trace(c, o.variant, S, n * np, block, lds, count, false, ctr, false))) return r;
         }
         ShadeParams SP;
         std::memset(&SP, 0, sizeof(SP));
@@ -477,7 +483,7 @@ int shade_pipeline(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w,
         Q.n = n;
         hulk_hit* nh = static_cast<hulk_hit*>((g & 1) ? w.hits3.p : w.hits2.p);
         Q.hits = nh;
-        if ((r = run_trace(c, o.variant, Q, n, block, lds, count, true, ctr, false))) return r;
+        if ((r = run_trace(c, o.variant, Q, n, block, lds, count, false, ctr, false))) return r;
         cur_hits = nh;
         cur_rays = static_cast<const hulk_ray*>(w.qrays[qo].p);
         cur_pix = static_cast<const uint32_t*>(w.qpix[qo].p);

@@ -95,9 +95,13 @@ __device__ __forceinline__ uint32_t frame_ray(const FrameDev* F, uint32_t n, uin
     return f;
 }
 
+// Leaf loop.  tstop: a shadow ray only needs to know whether its nearest
+// hit lies below the light distance; once best.t < tstop that answer is
+// fixed (best.t only decreases), so the ray stops.  Primary and mirror rays
+// pass tstop = -FLT_MAX, which never stops them.
 template <bool COUNT>
 __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, const uint4& n, const Ray& r, Best& best,
-                                          Cnt& c) {
+                                          Cnt& c, float tstop) {
     const uint32_t start = n.y, end = n.y + n.z;
     if (COUNT) {
         c.leaves++;
@@ -109,7 +113,7 @@ __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, const
         rec.b = recs[i].b;
         rec.c = recs[i].c;
         rec.d = recs[i].d;
-        test_record(r, rec, best);
+        if (test_record(r, rec, best) && best.t < tstop) break;
     }
 }
 
@@ -122,7 +126,7 @@ __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, const
 // changes only by a push (to the pushed split) or a pop.
 template <bool COUNT>
 __device__ void trav_fullstack(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk, uint32_t stride,
-                               const Ray& r, float tclip, Best& best, Cnt& c, uint32_t& flags) {
+                               const Ray& r, float tclip, float tstop, Best& best, Cnt& c, uint32_t& flags) {
     float ts = 0.0f, te = kFltMax;
     if (!ray_box(P.box, r, ts, te)) return;
     if (tclip < te) te = tclip;
@@ -133,7 +137,8 @@ __device__ void trav_fullstack(const TraceParams& P, const uint4* lds_nodes, uin
         const uint4 n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
         if (COUNT) c.nodes++;
         if (n.x & kNodeLeaf) {
-            test_leaf<COUNT>(P.recs, n, r, best, c);
+            test_leaf<COUNT>(P.recs, n, r, best, c, tstop);
+            if (best.t < tstop) break;
             node = 0;
         } else {
             float split;
@@ -177,7 +182,7 @@ __device__ void trav_fullstack(const TraceParams& P, const uint4* lds_nodes, uin
 // stk[(3 i + k) * stride] (k = tStart, tEnd, node).
 template <bool COUNT>
 __device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk, uint32_t stride,
-                                const Ray& r, float tclip, Best& best, Cnt& c, uint32_t& flags) {
+                                const Ray& r, float tclip, float tstop, Best& best, Cnt& c, uint32_t& flags) {
     float ts = 0.0f, te = kFltMax;
     if (!ray_box(P.box, r, ts, te)) return;
     if (tclip < te) te = tclip;
@@ -222,8 +227,8 @@ __device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, ui
             if (node == 0) break;
         }
         if (n.x & kNodeLeaf) {
-            test_leaf<COUNT>(P.recs, n, r, best, c);
-            if (best.t < tmax) break;
+            test_leaf<COUNT>(P.recs, n, r, best, c, tstop);
+            if (best.t < tmax || best.t < tstop) break;
         }
         if (++guard > P.guard) {
             flags |= kHitFlagGuard;
@@ -235,7 +240,7 @@ __device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, ui
 // The same short stack kept in a private array (registers / scratch).
 template <bool COUNT>
 __device__ void trav_shortstack_private(const TraceParams& P, const uint4* lds_nodes, const Ray& r, float tclip,
-                                        Best& best, Cnt& c, uint32_t& flags) {
+                                        float tstop, Best& best, Cnt& c, uint32_t& flags) {
     float ts = 0.0f, te = kFltMax;
     if (!ray_box(P.box, r, ts, te)) return;
     if (tclip < te) te = tclip;
@@ -282,8 +287,8 @@ __device__ void trav_shortstack_private(const TraceParams& P, const uint4* lds_n
             if (node == 0) break;
         }
         if (n.x & kNodeLeaf) {
-            test_leaf<COUNT>(P.recs, n, r, best, c);
-            if (best.t < tmax) break;
+            test_leaf<COUNT>(P.recs, n, r, best, c, tstop);
+            if (best.t < tmax || best.t < tstop) break;
         }
         if (++guard > P.guard) {
             flags |= kHitFlagGuard;
@@ -297,8 +302,8 @@ __device__ void trav_shortstack_private(const TraceParams& P, const uint4* lds_n
 // is not used); each leaf is followed by a restart from the root; there is
 // no early exit after a hit.
 template <bool COUNT>
-__device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const Ray& r, float tclip, Best& best,
-                             Cnt& c, uint32_t& flags) {
+__device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const Ray& r, float tclip, float tstop,
+                             Best& best, Cnt& c, uint32_t& flags) {
     float ts = 0.0f, te = kFltMax;
     if (!ray_box(P.box, r, ts, te)) return;
     if (tclip < te) te = tclip;
@@ -327,7 +332,10 @@ __device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const
             if (COUNT) c.nodes++;
             if (node == 0) break;
         }
-        if (n.x & kNodeLeaf) test_leaf<COUNT>(P.recs, n, r, best, c);
+        if (n.x & kNodeLeaf) {
+            test_leaf<COUNT>(P.recs, n, r, best, c, tstop);
+            if (best.t < tstop) break;
+        }
         if (++guard > P.guard) {
             flags |= kHitFlagGuard;
             break;
@@ -344,7 +352,8 @@ __device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const
 // slab time (lowest axis on ties); stop when best.t <= t_out, t_out >= te,
 // or there is no rope; else t = max(t, t_out) and follow the rope.
 template <bool COUNT>
-__device__ void trav_ropes(const TraceParams& P, const Ray& r, float tclip, Best& best, Cnt& c, uint32_t& flags) {
+__device__ void trav_ropes(const TraceParams& P, const Ray& r, float tclip, float tstop, Best& best, Cnt& c,
+                           uint32_t& flags) {
     float ts = 0.0f, te = kFltMax;
     if (!ray_box(P.box, r, ts, te)) return;
     if (tclip < te) te = tclip;
@@ -361,7 +370,8 @@ __device__ void trav_ropes(const TraceParams& P, const Ray& r, float tclip, Best
             n = P.rope_nodes[slot];
             if (COUNT) c.nodes++;
         }
-        test_leaf<COUNT>(P.recs, n, r, best, c);
+        test_leaf<COUNT>(P.recs, n, r, best, c, tstop);
+        if (best.t < tstop) break;
         const RopeSlot& rs = P.rope_slots[slot];
         const float4 lo = rs.lo, hi = rs.hi;
         const float mn[3] = {lo.x, lo.y, lo.z};
@@ -402,25 +412,29 @@ __device__ void trav_ropes(const TraceParams& P, const Ray& r, float tclip, Best
 }
 
 template <bool COUNT>
-__device__ void trav_brute(const TraceParams& P, const Ray& r, Best& best, Cnt& c) {
+__device__ void trav_brute(const TraceParams& P, const Ray& r, float tstop, Best& best, Cnt& c) {
     for (uint32_t i = 0; i < P.n_tris; i++) {
         const uint4 t = P.tris[i];
         const float4 a = P.verts[t.x], b = P.verts[t.y], cc = P.verts[t.z];
         test_vertices(r, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), i, best);
+        if (best.t < tstop) break;
     }
     if (COUNT) c.tris += P.n_tris;
 }
 
 template <int VARIANT, bool COUNT>
 __device__ __forceinline__ void trace_one(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk,
-                                          uint32_t stride, const Ray& r, float tclip, Best& best, Cnt& c,
-                                          uint32_t& flags) {
-    if (VARIANT == HULK_VARIANT_FULLSTACK) trav_fullstack<COUNT>(P, lds_nodes, stk, stride, r, tclip, best, c, flags);
-    else if (VARIANT == HULK_VARIANT_SHORTSTACK) trav_shortstack_private<COUNT>(P, lds_nodes, r, tclip, best, c, flags);
-    else if (VARIANT == HULK_VARIANT_LDSSTACK) trav_shortstack<COUNT>(P, lds_nodes, stk, stride, r, tclip, best, c, flags);
-    else if (VARIANT == HULK_VARIANT_RESTART) trav_restart<COUNT>(P, lds_nodes, r, tclip, best, c, flags);
-    else if (VARIANT == HULK_VARIANT_ROPES) trav_ropes<COUNT>(P, r, tclip, best, c, flags);
-    else trav_brute<COUNT>(P, r, best, c);
+                                          uint32_t stride, const Ray& r, float tclip, float tstop, Best& best,
+                                          Cnt& c, uint32_t& flags) {
+    if (VARIANT == HULK_VARIANT_FULLSTACK)
+        trav_fullstack<COUNT>(P, lds_nodes, stk, stride, r, tclip, tstop, best, c, flags);
+    else if (VARIANT == HULK_VARIANT_SHORTSTACK)
+        trav_shortstack_private<COUNT>(P, lds_nodes, r, tclip, tstop, best, c, flags);
+    else if (VARIANT == HULK_VARIANT_LDSSTACK)
+        trav_shortstack<COUNT>(P, lds_nodes, stk, stride, r, tclip, tstop, best, c, flags);
+    else if (VARIANT == HULK_VARIANT_RESTART) trav_restart<COUNT>(P, lds_nodes, r, tclip, tstop, best, c, flags);
+    else if (VARIANT == HULK_VARIANT_ROPES) trav_ropes<COUNT>(P, r, tclip, tstop, best, c, flags);
+    else trav_brute<COUNT>(P, r, tstop, best, c);
 }
 
 // ---------------------------------------------------------- trace kernel --
@@ -455,19 +469,25 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
     // Persistent: one atomic per wave fetches the next 64 rays.  Static:
     // XCD-aware chunk order (blocks are dispatched round-robin over the 8
     // XCDs, so consecutive chunks go to blocks on the same XCD).
-    uint32_t chunk;
+    // Persistent waves take kFetch chunks of 64 rays per atomic: device-scope
+    // atomics on one address serialize across the XCDs, so they are kept few.
+    constexpr uint32_t kFetch = 8;
+    uint32_t chunk, chunk_end = 0;
     const uint32_t nblocks = gridDim.x;
     if (P.persistent) {
         uint32_t base = 0;
-        if (lane_id == 0) base = atomicAdd(P.work_counter, 64u);
+        if (lane_id == 0) base = atomicAdd(P.work_counter, 64u * kFetch);
         chunk = __shfl(base, 0, 64) >> 6;
+        chunk_end = chunk + kFetch;
     } else {
         const uint32_t b = blockIdx.x;
         const uint32_t per_xcd = (nblocks + 7u) >> 3;
         const uint32_t xcd = b & 7u, k = b >> 3;
         uint32_t mapped = xcd * per_xcd + k;
-        // tail: when nblocks is not a multiple of 8 some mapped ids overflow
-        if ((nblocks & 7u) != 0) mapped = b;
+        // tail: when nblocks is not a multiple of 8 some mapped ids overflow;
+        // a queue whose length is only known on the device fills its low
+        // chunks, which round-robin dispatch already spreads over the XCDs
+        if ((nblocks & 7u) != 0 || P.n_dev) mapped = b;
         chunk = mapped * waves_per_block + wave;
     }
     while (chunk * 64u < n_rays) {
@@ -475,6 +495,7 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
         if (i < n_rays) {
             Ray r;
             uint32_t o = i;  // output / bound index: the pixel index for camera rays
+            float tstop = -kFltMax;
             if (P.source == kSourceCamera) {
                 uint32_t x, y;
                 tile_pixel(i, P.W, P.H, x, y);
@@ -486,6 +507,8 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
                 const hulk_ray ray = P.rays[i];
                 r.o = v3(ray.start.x, ray.start.y, ray.start.z);
                 r.d = v3(ray.dir.x, ray.dir.y, ray.dir.z);
+                // shadow queue: start.w carries the light distance
+                if (P.stop_at_w) tstop = ray.start.w;
             }
             set_inv(r);
             Best best;
@@ -501,15 +524,15 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
             const float tclip = tb * (1.0f + 1.0f / 1024.0f);
             const bool clipped = tb < kFltMax && ray_box(P.box, r, bts, bte) && tclip < bte;
             if (clipped) {
-                trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, tclip, best, c, flags);
+                trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, tclip, tstop, best, c, flags);
                 if (!(best.tri != kNoHit && best.t < tb)) {
                     best_init(best);
                     flags |= HULK_HITF_BOUNDED_FALLBACK;
-                    trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, kFltMax, best, c, flags);
+                    trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, kFltMax, tstop, best, c, flags);
                     if (COUNT) cfb++;
                 }
             } else {
-                trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, kFltMax, best, c, flags);
+                trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, kFltMax, tstop, best, c, flags);
             }
             if (P.t_out) {
                 P.t_out[P.t_slot ? P.t_slot[i] : o] = best.t;
@@ -532,9 +555,12 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
             }
         }
         if (P.persistent) {
-            uint32_t base = 0;
-            if (lane_id == 0) base = atomicAdd(P.work_counter, 64u);
-            chunk = __shfl(base, 0, 64) >> 6;
+            if (++chunk == chunk_end) {
+                uint32_t base = 0;
+                if (lane_id == 0) base = atomicAdd(P.work_counter, 64u * kFetch);
+                chunk = __shfl(base, 0, 64) >> 6;
+                chunk_end = chunk + kFetch;
+            }
         } else {
             break;
         }
@@ -582,27 +608,23 @@ __global__ void k_build_records(const float4* __restrict__ verts, const uint4* _
 
 // ------------------------------------------------- secondary generation --
 
-// Wave-aggregated append: one atomic per wave, order within the wave by
-// lane (mbcnt over the ballot).
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-    const uint64_t mask = __ballot(pred);
-    const uint32_t lane_id = threadIdx.x & 63u;
-    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-    uint32_t base = 0;
-    const uint32_t leader = mask ? (uint32_t)__builtin_ctzll(mask) : 0u;
-    if (mask && lane_id == leader) base = atomicAdd(counter, (uint32_t)__builtin_popcountll(mask));
-    base = __shfl(base, (int)leader, 64);
-    return base + below;
-}
-
+// Secondary rays of generation g, compacted without atomics in three
+// passes: k_gen_secondary<false> counts per wave (ballots), k_scan turns the
+// wave counts into offsets (and the queue lengths), k_gen_secondary<true>
+// writes each ray at its wave offset + lane rank.  Queue order is therefore
+// ray order, deterministic and coherent.
+//
 // For every ray of generation g with a hit: one shadow ray per point light
-// (origin P + 0.001*L, direction L = normalize(pos - P)), written to the
-// shadow queue with slot = ray * n_point + k; and, for a mirror triangle
-// while g < max_bounce, the reflected ray d' = dn - (2 (dn.n)) n (dn =
-// normalize(d)), origin P + 0.001*d', appended to the next generation.
-__global__ void k_gen_secondary(GenParams G) {
+// (origin P + 0.001*L, direction L = normalize(pos - P), start.w = the light
+// distance), slot = ray * n_point + k; and, for a mirror triangle while the
+// ray's frame allows generation g + 1, the reflected ray
+// d' = dn - (2 (dn.n)) n (dn = normalize(d)), origin P + 0.001*d', appended
+// to the next generation with weight w*0.8.
+template <bool WRITE>
+__global__ void __launch_bounds__(256) k_gen_secondary(GenParams G) {
     const uint32_t n = G.n_dev ? *G.n_dev : G.n;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t wave = i >> 6, lane_id = threadIdx.x & 63u;
     const bool active = i < n;
     hulk_hit h;
     Ray r;
@@ -629,39 +651,88 @@ __global__ void k_gen_secondary(GenParams G) {
         }
     }
     const bool hit = active && h.tri != kNoHit;
-    V3 P = v3(0, 0, 0);
-    if (hit) P = add(r.o, scale(r.d, h.t));
+    const bool mirror = hit && G.next_rays && may_spawn && G.mirror && G.mirror[h.tri];
+    const uint64_t hit_mask = __ballot(hit);
+    const uint64_t mir_mask = __ballot(mirror);
+    if (!WRITE) {
+        if (lane_id == 0) {
+            G.wave_counts[2 * wave] = (uint32_t)__builtin_popcountll(hit_mask) * G.n_point;
+            G.wave_counts[2 * wave + 1] = (uint32_t)__builtin_popcountll(mir_mask);
+        }
+        return;
+    }
+    const uint32_t hit_rank =
+        __builtin_amdgcn_mbcnt_hi((uint32_t)(hit_mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hit_mask, 0u));
+    const uint32_t mir_rank =
+        __builtin_amdgcn_mbcnt_hi((uint32_t)(mir_mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mir_mask, 0u));
+    const uint32_t n_hit = (uint32_t)__builtin_popcountll(hit_mask);
+    if (!hit) return;
+    const V3 P = add(r.o, scale(r.d, h.t));
+    const uint32_t sbase = G.wave_offsets[2 * wave];
     for (uint32_t k = 0; k < G.n_point; k++) {
         const float4 lp = G.point_pos[k];
-        const uint32_t j = wave_append(G.shadow_count, hit);
-        if (hit) {
-            V3 L = sub(v3(lp.x, lp.y, lp.z), P);
-            const float dd = dot(L, L);
-            L = scale(L, 1.0f / __builtin_sqrtf(dd));
-            hulk_ray sr;
-            const V3 so = add(P, scale(L, 0.001f));
-            sr.start = hulk_float4{so.x, so.y, so.z, 0.0f};
-            sr.dir = hulk_float4{L.x, L.y, L.z, 0.0f};
-            G.shadow_rays[j] = sr;
-            G.shadow_slot[j] = i * G.n_point + k;
-        }
+        const uint32_t j = sbase + k * n_hit + hit_rank;
+        V3 L = sub(v3(lp.x, lp.y, lp.z), P);
+        const float dd = dot(L, L);
+        const float dist = __builtin_sqrtf(dd);
+        L = scale(L, 1.0f / __builtin_sqrtf(dd));
+        hulk_ray sr;
+        const V3 so = add(P, scale(L, 0.001f));
+        sr.start = hulk_float4{so.x, so.y, so.z, dist};
+        sr.dir = hulk_float4{L.x, L.y, L.z, 0.0f};
+        G.shadow_rays[j] = sr;
+        G.shadow_slot[j] = i * G.n_point + k;
     }
-    if (G.next_rays) {
-        const bool mirror = hit && may_spawn && G.mirror && G.mirror[h.tri];
-        const uint32_t j = wave_append(G.next_count, mirror);
-        if (mirror) {
-            const V3 nn = v3(h.nx, h.ny, h.nz);
-            const V3 dn = normalize(r.d);
-            const float kk = 2.0f * dot(dn, nn);
-            const V3 rr = sub(dn, scale(nn, kk));
-            const V3 o = add(P, scale(rr, 0.001f));
-            hulk_ray nr;
-            nr.start = hulk_float4{o.x, o.y, o.z, 0.0f};
-            nr.dir = hulk_float4{rr.x, rr.y, rr.z, 0.0f};
-            G.next_rays[j] = nr;
-            G.pixel_out[j] = pixel;
-            G.weight_out[j] = w * 0.8f;
-        }
+    if (mirror) {
+        const uint32_t j = G.wave_offsets[2 * wave + 1] + mir_rank;
+        const V3 nn = v3(h.nx, h.ny, h.nz);
+        const V3 dn = normalize(r.d);
+        const float kk = 2.0f * dot(dn, nn);
+        const V3 rr = sub(dn, scale(nn, kk));
+        const V3 o = add(P, scale(rr, 0.001f));
+        hulk_ray nr;
+        nr.start = hulk_float4{o.x, o.y, o.z, 0.0f};
+        nr.dir = hulk_float4{rr.x, rr.y, rr.z, 0.0f};
+        G.next_rays[j] = nr;
+        G.pixel_out[j] = pixel;
+        G.weight_out[j] = w * 0.8f;
+    }
+}
+
+// Exclusive scan of n pairs (shadow, mirror) of wave counts by one block:
+// each thread sums a contiguous run, the block scans the run sums, then
+// each thread writes its run's offsets.  Totals go to the queue counters.
+__global__ void __launch_bounds__(1024) k_scan_pairs(const uint32_t* counts, uint32_t n, uint32_t* offsets,
+                                                     uint32_t* total0, uint32_t* total1) {
+    __shared__ uint32_t s0[1024], s1[1024];
+    const uint32_t t = threadIdx.x, T = blockDim.x;
+    const uint32_t per = (n + T - 1) / T;
+    const uint32_t a = t * per, b = min(n, a + per);
+    uint32_t c0 = 0, c1 = 0;
+    for (uint32_t k = a; k < b; k++) {
+        c0 += counts[2 * k];
+        c1 += counts[2 * k + 1];
+    }
+    s0[t] = c0;
+    s1[t] = c1;
+    __syncthreads();
+    for (uint32_t off = 1; off < T; off <<= 1) {  // Hillis-Steele inclusive scan
+        const uint32_t v0 = t >= off ? s0[t - off] : 0u, v1 = t >= off ? s1[t - off] : 0u;
+        __syncthreads();
+        s0[t] += v0;
+        s1[t] += v1;
+        __syncthreads();
+    }
+    uint32_t o0 = s0[t] - c0, o1 = s1[t] - c1;
+    for (uint32_t k = a; k < b; k++) {
+        offsets[2 * k] = o0;
+        offsets[2 * k + 1] = o1;
+        o0 += counts[2 * k];
+        o1 += counts[2 * k + 1];
+    }
+    if (t == T - 1) {
+        if (total0) *total0 = s0[t];
+        if (total1) *total1 = s1[t];
     }
 }
 
@@ -841,7 +912,11 @@ hipError_t launch_build_records(const float4* verts, const uint4* tris, const ui
 
 hipError_t launch_gen_secondary(const GenParams& G, uint32_t n_max, hipStream_t s) {
     if (n_max == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gen_secondary, dim3((n_max + 255) / 256), dim3(256), 0, s, G);
+    const uint32_t blocks = (n_max + 255) / 256, waves = blocks * 4;
+    hipLaunchKernelGGL(k_gen_secondary<false>, dim3(blocks), dim3(256), 0, s, G);
+    hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, s, G.wave_counts, waves, G.wave_offsets,
+                       G.shadow_count, G.next_count);
+    hipLaunchKernelGGL(k_gen_secondary<true>, dim3(blocks), dim3(256), 0, s, G);
     return hipGetLastError();
 }
 

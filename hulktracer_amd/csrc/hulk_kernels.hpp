@@ -62,6 +62,7 @@ struct TraceParams {
     unsigned long long* counters;  // rays, nodes, leaves, tris, hits, fallbacks
     uint32_t* work_counter;
     uint32_t persistent;
+    uint32_t stop_at_w;  // shadow queue: stop once best.t < ray.start.w (the light distance)
 };
 
 struct GenParams {
@@ -86,6 +87,8 @@ struct GenParams {
     uint32_t* pixel_out;
     float* weight_out;
     uint32_t* next_count;
+    uint32_t* wave_counts;   // 2 per wave of the count pass
+    uint32_t* wave_offsets;  // 2 per wave, exclusive scan of wave_counts
 };
 
 struct ShadeParams {

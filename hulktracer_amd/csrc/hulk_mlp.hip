@@ -75,132 +75,166 @@ struct FwdParams {
     float* pred;
     float* tbound;
     float margin, max_log;
+    uint32_t stride;                   // bound mode: one evaluation per stride x stride pixels
     uint16_t *encT, *h1T, *h2T, *h3T;  // [feature][ld] (training)
     uint32_t ld;
 };
 
 constexpr uint32_t kLdsRow = kHid + 8;  // bf16 per LDS row (pad against bank conflicts)
+constexpr uint32_t kTiles = 4;          // ray tiles of 16 per wave: 64 rays share each weight fragment
 
-// One hidden layer for the wave's 16 rays: C[neuron][ray] = W . X^T with B
-// fragments from LDS rows (ray-major), bias + relu + bf16 into the next
-// LDS buffer (and the feature-major global copy when training).
-template <uint32_t K>
-__device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ Wb, const float* __restrict__ bias,
-                                             const uint16_t* xin, uint16_t* xout, uint32_t g, uint32_t r,
-                                             uint16_t* outT, uint32_t ld, uint32_t ray, bool valid,
-                                             float* last /* optional: the 32 rounded outputs */) {
-    bf16x8 b[K / 32];
-#pragma unroll
-    for (uint32_t s = 0; s < K / 32; s++) b[s] = ld8(xin + r * kLdsRow + 32 * s + 8 * g);
-#pragma unroll
-    for (uint32_t t = 0; t < 8; t++) {
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (uint32_t s = 0; s < K / 32; s++) acc = mfma(ld8(Wb + (16 * t + r) * K + 32 * s + 8 * g), b[s], acc);
-        uint16_t hv[4];
-#pragma unroll
-        for (uint32_t i = 0; i < 4; i++) {
-            const uint32_t nrn = 16 * t + 4 * g + i;
-            float v = acc[i] + bias[nrn];
-            v = v > 0.0f ? v : 0.0f;
-            hv[i] = f2bf(v);
-            if (last) last[4 * t + i] = bf2f(hv[i]);
-            if (outT && valid) outT[(size_t)nrn * ld + ray] = hv[i];
-        }
-        uint2 w2;
-        w2.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
-        w2.y = (uint32_t)hv[2] | ((uint32_t)hv[3] << 16);
-        *reinterpret_cast<uint2*>(xout + r * kLdsRow + 16 * t + 4 * g) = w2;
-    }
+__device__ __forceinline__ void store4(uint16_t* dst, const uint16_t* hv) {
+    uint2 w2;
+    w2.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
+    w2.y = (uint32_t)hv[2] | ((uint32_t)hv[3] << 16);
+    *reinterpret_cast<uint2*>(dst) = w2;
 }
 
-// Fused forward: one wave (64 threads, one block) handles 16 rays per step.
+// Fused forward: one wave (64 threads, one block) handles 64 rays per step,
+// as 4 column tiles of 16; every weight fragment is loaded once and fed to
+// 4 MFMAs.  Activations go through LDS (ray-major rows); when training the
+// feature-major copies go to HBM.  Bound mode with stride s evaluates the
+// ray of pixel (s*qx, s*qy) and writes its bound to the s x s pixel block.
 __global__ void __launch_bounds__(64) k_mlp_fwd(FwdParams P) {
-    __shared__ __attribute__((aligned(16))) uint16_t buf[2][16 * kLdsRow];
+    __shared__ __attribute__((aligned(16))) uint16_t buf[2][kTiles * 16 * kLdsRow];
     const uint32_t l = threadIdx.x, g = l >> 4, r = l & 15u;
-    const uint32_t groups = (P.n + 15u) / 16u;
+    const uint32_t groups = (P.n + 63u) / 64u;
+    const uint32_t s = P.stride ? P.stride : 1u;
+    const uint32_t Wq = (P.W + s - 1) / s;
     for (uint32_t gi = blockIdx.x; gi < groups; gi += gridDim.x) {
-        const uint32_t ray = gi * 16u + r;
-        const bool valid = ray < P.n;
-        float o[3] = {0, 0, 0}, d[3] = {0, 0, 1};
-        if (valid) {
-            if (P.camera) {
-                Ray rr;
-                const uint32_t y = ray / P.W, x = ray - y * P.W;
-                cast_ray(P.cam, x, y, P.W, P.H, rr);
-                o[0] = rr.o.x; o[1] = rr.o.y; o[2] = rr.o.z;
-                d[0] = rr.d.x; d[1] = rr.d.y; d[2] = rr.d.z;
-            } else {
-                const hulk_ray q = P.rays[ray];
-                o[0] = q.start.x; o[1] = q.start.y; o[2] = q.start.z;
-                d[0] = q.dir.x; d[1] = q.dir.y; d[2] = q.dir.z;
-            }
-        }
-        float x[6];
-        for (int i = 0; i < 3; i++) x[i] = (o[i] - P.c[i]) * P.ih[i];
-        const float dd = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
-        const float inv = 1.0f / __builtin_sqrtf(dd);
-        for (int i = 0; i < 3; i++) x[3 + i] = d[i] * inv;
-        uint16_t es[8], ec[8];
+        uint32_t ray[kTiles];
+        bool valid[kTiles];
+        bf16x8 b1[kTiles][2];
 #pragma unroll
-        for (uint32_t j = 0; j < 8; j++) {
-            const float* F = P.fourier + (8 * g + j) * 6;
-            float z = F[0] * x[0];
-            z = z + F[1] * x[1];
-            z = z + F[2] * x[2];
-            z = z + F[3] * x[3];
-            z = z + F[4] * x[4];
-            z = z + F[5] * x[5];
-            es[j] = f2bf(sinf(z));
-            ec[j] = f2bf(cosf(z));
-            if (P.encT && valid) {
-                P.encT[(size_t)(8 * g + j) * P.ld + ray] = es[j];
-                P.encT[(size_t)(32 + 8 * g + j) * P.ld + ray] = ec[j];
+        for (uint32_t j = 0; j < kTiles; j++) {
+            ray[j] = gi * 64u + 16u * j + r;
+            valid[j] = ray[j] < P.n;
+            float o[3] = {0, 0, 0}, d[3] = {0, 0, 1};
+            if (valid[j]) {
+                if (P.camera) {
+                    Ray rr;
+                    const uint32_t qy = ray[j] / Wq, qx = ray[j] - qy * Wq;
+                    cast_ray(P.cam, qx * s, qy * s, P.W, P.H, rr);
+                    o[0] = rr.o.x; o[1] = rr.o.y; o[2] = rr.o.z;
+                    d[0] = rr.d.x; d[1] = rr.d.y; d[2] = rr.d.z;
+                } else {
+                    const hulk_ray q = P.rays[ray[j]];
+                    o[0] = q.start.x; o[1] = q.start.y; o[2] = q.start.z;
+                    d[0] = q.dir.x; d[1] = q.dir.y; d[2] = q.dir.z;
+                }
             }
+            float x[6];
+            for (int i = 0; i < 3; i++) x[i] = (o[i] - P.c[i]) * P.ih[i];
+            const float dd = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+            const float inv = 1.0f / __builtin_sqrtf(dd);
+            for (int i = 0; i < 3; i++) x[3 + i] = d[i] * inv;
+            uint16_t es[8], ec[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8; k++) {
+                const float* F = P.fourier + (8 * g + k) * 6;
+                float z = F[0] * x[0];
+                z = z + F[1] * x[1];
+                z = z + F[2] * x[2];
+                z = z + F[3] * x[3];
+                z = z + F[4] * x[4];
+                z = z + F[5] * x[5];
+                es[k] = f2bf(__sinf(z));
+                ec[k] = f2bf(__cosf(z));
+                if (P.encT && valid[j]) {
+                    P.encT[(size_t)(8 * g + k) * P.ld + ray[j]] = es[k];
+                    P.encT[(size_t)(32 + 8 * g + k) * P.ld + ray[j]] = ec[k];
+                }
+            }
+            b1[j][0] = pack8(es);
+            b1[j][1] = pack8(ec);
         }
-        // layer 1 straight from registers: k 8g.. (sin part) and 32+8g.. (cos part)
-        const bf16x8 b0 = pack8(es), b1 = pack8(ec);
-        uint16_t* h1 = buf[0];
+        // layer 1 (K = 64) from registers
 #pragma unroll
         for (uint32_t t = 0; t < 8; t++) {
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = mfma(ld8(P.W1 + (16 * t + r) * kIn + 8 * g), b0, acc);
-            acc = mfma(ld8(P.W1 + (16 * t + r) * kIn + 32 + 8 * g), b1, acc);
-            uint16_t hv[4];
+            const bf16x8 a0 = ld8(P.W1 + (16 * t + r) * kIn + 8 * g);
+            const bf16x8 a1 = ld8(P.W1 + (16 * t + r) * kIn + 32 + 8 * g);
 #pragma unroll
-            for (uint32_t i = 0; i < 4; i++) {
-                const uint32_t nrn = 16 * t + 4 * g + i;
-                float v = acc[i] + P.p[ob1 + nrn];
-                v = v > 0.0f ? v : 0.0f;
-                hv[i] = f2bf(v);
-                if (P.h1T && valid) P.h1T[(size_t)nrn * P.ld + ray] = hv[i];
-            }
-            uint2 w2;
-            w2.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
-            w2.y = (uint32_t)hv[2] | ((uint32_t)hv[3] << 16);
-            *reinterpret_cast<uint2*>(h1 + r * kLdsRow + 16 * t + 4 * g) = w2;
-        }
-        __syncthreads();
-        hidden_layer<kHid>(P.W2, P.p + ob2, buf[0], buf[1], g, r, P.h2T, P.ld, ray, valid, nullptr);
-        __syncthreads();
-        float h3[32];
-        hidden_layer<kHid>(P.W3, P.p + ob3, buf[1], buf[0], g, r, P.h3T, P.ld, ray, valid, h3);
-        float part = 0.0f;
+            for (uint32_t j = 0; j < kTiles; j++) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                acc = mfma(a0, b1[j][0], acc);
+                acc = mfma(a1, b1[j][1], acc);
+                uint16_t hv[4];
 #pragma unroll
-        for (uint32_t t = 0; t < 8; t++)
-#pragma unroll
-            for (uint32_t i = 0; i < 4; i++) part = part + bf2f(P.w4[16 * t + 4 * g + i]) * h3[4 * t + i];
-        part += __shfl_xor(part, 16, 64);
-        part += __shfl_xor(part, 32, 64);
-        const float y = part + P.p[ob4];
-        if (g == 0 && valid) {
-            if (P.pred) P.pred[ray] = y;
-            if (P.tbound) {
-                const float e = y + P.margin;
-                P.tbound[ray] = (e < P.max_log) ? expf(e) : kFltMax;
+                for (uint32_t i = 0; i < 4; i++) {
+                    const uint32_t nrn = 16 * t + 4 * g + i;
+                    float v = acc[i] + P.p[ob1 + nrn];
+                    v = v > 0.0f ? v : 0.0f;
+                    hv[i] = f2bf(v);
+                    if (P.h1T && valid[j]) P.h1T[(size_t)nrn * P.ld + ray[j]] = hv[i];
+                }
+                store4(buf[0] + (16 * j + r) * kLdsRow + 16 * t + 4 * g, hv);
             }
         }
         __syncthreads();
+        // layers 2 and 3 (K = 128) through LDS; layer 3 folds the output dot
+        float part[kTiles] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (uint32_t layer = 0; layer < 2; layer++) {
+            const uint16_t* Wb = layer == 0 ? P.W2 : P.W3;
+            const float* bias = P.p + (layer == 0 ? ob2 : ob3);
+            uint16_t* hT = layer == 0 ? P.h2T : P.h3T;
+            const uint16_t* xin = buf[layer];
+            uint16_t* xout = buf[layer ^ 1];
+            bf16x8 b[kTiles][4];
+#pragma unroll
+            for (uint32_t j = 0; j < kTiles; j++)
+#pragma unroll
+                for (uint32_t k = 0; k < 4; k++) b[j][k] = ld8(xin + (16 * j + r) * kLdsRow + 32 * k + 8 * g);
+#pragma unroll
+            for (uint32_t t = 0; t < 8; t++) {
+                f32x4 acc[kTiles];
+#pragma unroll
+                for (uint32_t j = 0; j < kTiles; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (uint32_t k = 0; k < 4; k++) {
+                    const bf16x8 a = ld8(Wb + (16 * t + r) * kHid + 32 * k + 8 * g);
+#pragma unroll
+                    for (uint32_t j = 0; j < kTiles; j++) acc[j] = mfma(a, b[j][k], acc[j]);
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kTiles; j++) {
+                    uint16_t hv[4];
+#pragma unroll
+                    for (uint32_t i = 0; i < 4; i++) {
+                        const uint32_t nrn = 16 * t + 4 * g + i;
+                        float v = acc[j][i] + bias[nrn];
+                        v = v > 0.0f ? v : 0.0f;
+                        hv[i] = f2bf(v);
+                        if (hT && valid[j]) hT[(size_t)nrn * P.ld + ray[j]] = hv[i];
+                        if (layer == 1) part[j] = part[j] + bf2f(P.w4[nrn]) * bf2f(hv[i]);
+                    }
+                    if (layer == 0) store4(xout + (16 * j + r) * kLdsRow + 16 * t + 4 * g, hv);
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kTiles; j++) {
+            float pj = part[j];
+            pj += __shfl_xor(pj, 16, 64);
+            pj += __shfl_xor(pj, 32, 64);
+            const float y = pj + P.p[ob4];
+            if (g == 0 && valid[j]) {
+                if (P.pred) P.pred[ray[j]] = y;
+                if (P.tbound) {
+                    const float e = y + P.margin;
+                    const float tb = (e < P.max_log) ? __expf(e) : kFltMax;
+                    if (P.camera) {
+                        const uint32_t qy = ray[j] / Wq, qx = ray[j] - qy * Wq;
+                        for (uint32_t yy = qy * s; yy < min(P.H, qy * s + s); yy++)
+                            for (uint32_t xx = qx * s; xx < min(P.W, qx * s + s); xx++)
+                                P.tbound[(size_t)yy * P.W + xx] = tb;
+                    } else {
+                        P.tbound[ray[j]] = tb;
+                    }
+                }
+            }
+        }
     }
 }
 
@@ -420,7 +454,7 @@ FwdParams fwd_params(hulk_mlp* m) {
 }
 
 int launch_fwd(hulk_mlp* m, const FwdParams& P) {
-    const uint32_t groups = (P.n + 15u) / 16u;
+    const uint32_t groups = (P.n + 63u) / 64u;
     const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(groups, 8192u));
     hipLaunchKernelGGL(k_mlp_fwd, dim3(grid), dim3(64), 0, m->stream, P);
     MLP_TRY(m, hipGetLastError());
@@ -643,7 +677,7 @@ extern "C" int hulk_mlp_train_step(hulk_mlp* m, const hulk_ray* rays_dev, const 
 }
 
 extern "C" int hulk_mlp_bound_camera(hulk_mlp* m, const hulk_camera* cam, uint32_t W, uint32_t H, float margin,
-                                     float max_log, float* tbound_dev, uint32_t flags) {
+                                     float max_log, uint32_t stride, float* tbound_dev, uint32_t flags) {
     if (!m || !cam || !tbound_dev) return HULK_E_ILLEGAL_INPUT;
     FwdParams P = fwd_params(m);
     P.camera = 1;
@@ -653,9 +687,11 @@ extern "C" int hulk_mlp_bound_camera(hulk_mlp* m, const hulk_camera* cam, uint32
     k.hf[0] = cam->horizontal_factor.x; k.hf[1] = cam->horizontal_factor.y; k.hf[2] = cam->horizontal_factor.z;
     k.ll[0] = cam->lower_left.x; k.ll[1] = cam->lower_left.y; k.ll[2] = cam->lower_left.z;
     P.cam = k;
+    if (stride == 0 || stride > 8) return HULK_E_ILLEGAL_INPUT;
     P.W = W;
     P.H = H;
-    P.n = W * H;
+    P.stride = stride;
+    P.n = ((W + stride - 1) / stride) * ((H + stride - 1) / stride);
     P.tbound = tbound_dev;
     P.margin = margin;
     P.max_log = max_log;

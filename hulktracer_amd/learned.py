@@ -29,6 +29,7 @@ class HitDistanceMLP:
         if rc:
             raise HulkError(rc, "hulk_mlp_create failed")
         self.margin = 0.0
+        self.stride = 1
 
     def _c(self, rc):
         if rc:
@@ -76,10 +77,11 @@ class HitDistanceMLP:
         return loss.value if want_loss else None
 
     def bound_camera(self, cam, W: int, H: int, tbound_dev, margin: float | None = None, max_log: float = 60.0,
-                     sync: bool = True):
+                     stride: int | None = None, sync: bool = True):
         m = self.margin if margin is None else margin
+        st = self.stride if stride is None else stride
         self._c(lib().hulk_mlp_bound_camera(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, m, max_log,
-                                            ptr(tbound_dev), 0 if sync else L.FLAG_NO_SYNC))
+                                            st, ptr(tbound_dev), 0 if sync else L.FLAG_NO_SYNC))
 
     # ------------------------------------------------------------ training --
 
@@ -117,6 +119,31 @@ class HitDistanceMLP:
         res = torch.cat(res)
         self.margin = float(torch.quantile(res.float()[:1 << 24], quantile).item())
         return self.margin
+
+
+def calibrate_frames(mlp: HitDistanceMLP, ctx: GPUTrace, cams, W: int, H: int, quantile: float = 0.95,
+                     stride: int = 1) -> float:
+    """margin from the deployed bound itself: on training frames, the
+    quantile of log(t) - log(bound at margin 0) over rays inside the box
+    (t = nearest hit, or the box exit for a miss)."""
+    import torch
+
+    res = []
+    tb = torch.empty(W * H, dtype=torch.float32, device="cuda")
+    hits = torch.empty(W * H * 32, dtype=torch.uint8, device="cuda")
+    for cam in cams:
+        _, texit = camera_rays(ctx, cam, W, H)
+        mlp.bound_camera(cam, W, H, tb, margin=0.0, max_log=80.0, stride=stride)
+        ctx.trace_camera_dev(cam, W, H, hits.data_ptr())
+        h = hits.view(torch.float32).view(-1, 8)
+        hit = h[:, 3].contiguous().view(torch.int32) != -1
+        lab = torch.where(hit, h[:, 0], texit)
+        keep = lab > 0
+        res.append(torch.log(lab[keep]) - torch.log(tb[keep]))
+    r = torch.cat(res)
+    mlp.stride = stride
+    mlp.margin = float(torch.quantile(r[: 1 << 24], quantile).item())
+    return mlp.margin
 
 
 def camera_rays(ctx: GPUTrace, cam, W: int, H: int):

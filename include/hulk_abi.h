@@ -292,9 +292,11 @@ int hulk_mlp_loss_grad(hulk_mlp* mlp, const hulk_ray* rays, const float* target,
 int hulk_mlp_train_step(hulk_mlp* mlp, const hulk_ray* rays, const float* target, uint32_t n, float lr,
                         float* loss);
 /* Per-pixel bound of a camera's primary rays: exp(y + margin), FLT_MAX
- * where y + margin >= max_log. */
+ * where y + margin >= max_log.  stride s (1..8): the network runs once per
+ * s x s pixel block, on the ray of its first pixel, and the block shares
+ * the bound. */
 int hulk_mlp_bound_camera(hulk_mlp* mlp, const hulk_camera* cam, uint32_t width, uint32_t height, float margin,
-                          float max_log, float* tbound, uint32_t flags);
+                          float max_log, uint32_t stride, float* tbound, uint32_t flags);
 /* Primary rays of a camera as ray records (device) and, optionally, each
  * ray's scene-box exit time (0 if it misses the box): training input of
  * the bound (castRay, GPU_common.cl:10-30; box test SharedFunctions.hpp:245-264). */

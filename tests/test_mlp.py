@@ -124,13 +124,14 @@ def test_learned_bound_keeps_hits_exact(variant):
     plain traversal's (the fallback re-runs rays the bound cut short)."""
     import torch
 
-    from hulktracer_amd.learned import training_pairs
+    from hulktracer_amd.learned import calibrate_frames, training_pairs
 
     s, ctx, mlp, _, _ = _gpu_setup()
     ctx.set_options(variant=variant)
-    rays, y = training_pairs(ctx, [s.camera(k, 1.0) for k in (0, 4, 8)], 256, 256)
+    cams = [s.camera(k, 1.0) for k in (0, 4, 8)]
+    rays, y = training_pairs(ctx, cams, 256, 256)
     mlp.fit(rays, y, steps=400, batch=16384, lr=2e-3)
-    mlp.calibrate(rays, y, 0.9)
+    calibrate_frames(mlp, ctx, cams, 256, 256, quantile=0.9, stride=2)
     for pose in (2, 6):
         cam = s.camera(pose, 1.0)
         plain = ctx.trace_camera(cam, 256, 256)
