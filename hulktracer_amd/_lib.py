@@ -124,6 +124,8 @@ EXPORTS = {
                                    C.c_uint32]),
     "hulk_kdtree_build": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                     C.POINTER(C.c_void_p)]),
+    "hulk_kdtree_build_threads": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                            C.c_uint32, C.POINTER(C.c_void_p)]),
     "hulk_kdtree_node_count": (C.c_uint32, [C.c_void_p]),
     "hulk_kdtree_leaf_poly_count": (C.c_uint32, [C.c_void_p]),
     "hulk_kdtree_nodes": (C.c_void_p, [C.c_void_p]),

@@ -56,7 +56,7 @@ def build(verbose: bool = False) -> Path:
         objs.append(obj)
     for src in CPP_SOURCES:
         obj = objdir / (src + ".o")
-        jobs.append(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", f"-I{ROOT / 'include'}",
+        jobs.append(["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", f"-I{ROOT / 'include'}",
                      "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
     procs = [subprocess.Popen(j, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for j in jobs]
@@ -71,7 +71,7 @@ def build(verbose: bool = False) -> Path:
         sys.stderr.write("\n".join(errs))
         raise RuntimeError("libhulk build failed")
     tmp = LIB.with_suffix(".so.tmp")
-    _run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

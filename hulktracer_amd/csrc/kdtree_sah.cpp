@@ -17,6 +17,8 @@
 #include <cfloat>
 #include <cstring>
 #include <new>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "hulk_abi.h"
@@ -56,19 +58,23 @@ inline void set_lane(hulk_float4& v, uint32_t a, float x) {
     if (a == 0) v.x = x; else if (a == 1) v.y = x; else v.z = x;
 }
 
+// Output of one subtree build: nodes in depth-first pre-order (the subtree
+// root at local index 0, empty child = 0), its leaf-polygon run and its leaf
+// count.  Subtrees built in parallel are concatenated in the sequential
+// order (above subtree, then below subtree) with their indices, leaf starts
+// and leaf ids shifted, which reproduces the sequential arrays exactly.
+struct Out {
+    std::vector<hulk_kd_node> nodes;
+    std::vector<uint32_t> leaf_polys;
+    uint32_t leaf_count = 0;
+};
+
 class Builder {
 public:
-    Builder(const hulk_float4* v, const hulk_tri* t, uint32_t nt)
-        : verts_(v), tris_(t), n_tris_(nt), rel_(nt) {}
+    Builder(const hulk_float4* v, const hulk_tri* t, uint32_t nt, uint32_t spawn_depth)
+        : verts_(v), tris_(t), n_tris_(nt), spawn_depth_(spawn_depth) {}
 
-    void build(const hulk_box& scene_box) {
-        nodes_.clear();
-        leaf_polys_.clear();
-        leaf_count_ = 0;
-        hulk_kd_node root;
-        std::memset(&root, 0, sizeof(root));
-        nodes_.push_back(root);  // KDTreeCPU.cpp:74-75
-
+    void build(const hulk_box& scene_box, Out& out) {
         Split s;
         s.polys.resize(n_tris_);
         for (uint32_t i = 0; i < n_tris_; i++) s.polys[i] = i;
@@ -90,15 +96,21 @@ public:
                 s.cand[j].push_back({mx, kEnd, p});
             }
         }
+        // the three candidate sorts are independent
+        std::vector<std::thread> sorts;
         for (uint32_t j = 0; j < 3; j++)
-            std::sort(s.cand[j].begin(), s.cand[j].end(),
-                      [](const Candidate& a, const Candidate& b) { return a.value < b.value; });
-        nodes_.reserve(5000);
-        build_node(s);
+            sorts.emplace_back([&s, j] {
+                std::sort(s.cand[j].begin(), s.cand[j].end(),
+                          [](const Candidate& a, const Candidate& b) { return a.value < b.value; });
+            });
+        for (auto& t : sorts) t.join();
+        hulk_kd_node root;
+        std::memset(&root, 0, sizeof(root));
+        out.nodes.reserve(5000);
+        out.nodes.push_back(root);  // KDTreeCPU.cpp:74-75
+        std::vector<Relation> rel(n_tris_);
+        build_node(s, out, rel);
     }
-
-    std::vector<hulk_kd_node> nodes_;
-    std::vector<uint32_t> leaf_polys_;
 
 private:
     // getBoxSurface, SharedFunctions.hpp:32-36.
@@ -118,18 +130,38 @@ private:
         return kStraddle;
     }
 
-    void make_leaf(const Split& s) {
-        hulk_kd_node& n = nodes_[s.node];
+    static void make_leaf(const Split& s, Out& out) {
+        hulk_kd_node& n = out.nodes[s.node];
         n.flags = HULK_NODE_LEAF;
-        n.u.leaf.leaf_id = leaf_count_++;
+        n.u.leaf.leaf_id = out.leaf_count++;
         n.u.leaf.count = static_cast<uint32_t>(s.polys.size());
-        n.u.leaf.start = static_cast<uint32_t>(leaf_polys_.size());
-        leaf_polys_.insert(leaf_polys_.end(), s.polys.begin(), s.polys.end());
+        n.u.leaf.start = static_cast<uint32_t>(out.leaf_polys.size());
+        out.leaf_polys.insert(out.leaf_polys.end(), s.polys.begin(), s.polys.end());
     }
 
-    void build_node(Split& s) {
+    // Append a subtree built on its own (root at local 0) after the nodes
+    // already in `out`; returns the subtree root's index in `out`.
+    static uint32_t append(Out& out, const Out& sub) {
+        const uint32_t base = static_cast<uint32_t>(out.nodes.size());
+        const uint32_t lbase = static_cast<uint32_t>(out.leaf_polys.size());
+        for (hulk_kd_node n : sub.nodes) {
+            if (n.flags & HULK_NODE_LEAF) {
+                n.u.leaf.start += lbase;
+                n.u.leaf.leaf_id += out.leaf_count;
+            } else if (n.flags & HULK_NODE_JUNCTION) {
+                if (n.u.junction.above) n.u.junction.above += base;
+                if (n.u.junction.below) n.u.junction.below += base;
+            }
+            out.nodes.push_back(n);
+        }
+        out.leaf_polys.insert(out.leaf_polys.end(), sub.leaf_polys.begin(), sub.leaf_polys.end());
+        out.leaf_count += sub.leaf_count;
+        return base;
+    }
+
+    void build_node(Split& s, Out& out, std::vector<Relation>& rel) {
         if (s.depth >= kMaxDepth || s.polys.size() < kMinLeafPolys) {
-            make_leaf(s);
+            make_leaf(s, out);
             return;
         }
         float best = FLT_MAX, best_upper = 0.0f, best_lower = 0.0f, best_value = 0.0f;
@@ -185,18 +217,18 @@ private:
         }
         const float leaf_cost = kIntersectionCost * static_cast<float>(s.eval);
         if (best >= leaf_cost) {
-            make_leaf(s);
+            make_leaf(s, out);
             return;
         }
         {
-            hulk_kd_node& n = nodes_[s.node];
+            hulk_kd_node& n = out.nodes[s.node];
             n.u.junction.plane = best_value;
             n.flags = best_dim | HULK_NODE_JUNCTION;
         }
         Split a, b;
         for (uint32_t p : s.polys) {
             Relation r = classify(best_dim, best_value, p);
-            rel_[p] = r;
+            rel[p] = r;
             if (r == kAbove) a.polys.push_back(p);
             else if (r == kBelow) b.polys.push_back(p);
             else { a.polys.push_back(p); b.polys.push_back(p); }
@@ -205,7 +237,7 @@ private:
             a.cand[j].reserve(s.cand[j].size());
             b.cand[j].reserve(s.cand[j].size());
             for (const Candidate& c : s.cand[j]) {
-                switch (rel_[c.poly]) {
+                switch (rel[c.poly]) {
                     case kAbove: a.cand[j].push_back(c); break;
                     case kBelow: b.cand[j].push_back(c); break;
                     case kStraddle:
@@ -228,45 +260,70 @@ private:
         }
         a.eval /= 6;
         b.eval /= 6;
-        // The parent's candidate lists are no longer needed: free them before
+        // The parent's lists are no longer needed: free them before
         // recursing so peak memory stays O(depth * level size).
         for (uint32_t j = 0; j < 3; j++) std::vector<Candidate>().swap(s.cand[j]);
         std::vector<uint32_t>().swap(s.polys);
 
         const uint32_t parent = s.node;
-        if (!a.polys.empty()) {
+        const bool has_a = !a.polys.empty(), has_b = !b.polys.empty();
+        if (has_a) {
             a.depth = s.depth + 1;
             a.box = s.box;
             a.surface = best_upper;
             set_lane(a.box.min, best_dim, best_value);
-            const uint32_t idx = static_cast<uint32_t>(nodes_.size());
-            a.node = idx;
-            hulk_kd_node empty;
-            std::memset(&empty, 0, sizeof(empty));
-            nodes_.push_back(empty);
-            nodes_[parent].u.junction.above = idx;
-            build_node(a);
         }
-        if (!b.polys.empty()) {
+        if (has_b) {
             b.depth = s.depth + 1;
             b.box = s.box;
             b.surface = best_lower;
             set_lane(b.box.max, best_dim, best_value);
-            const uint32_t idx = static_cast<uint32_t>(nodes_.size());
+        }
+        if (has_a && has_b && s.depth < spawn_depth_) {
+            // both subtrees in parallel, each into its own arrays
+            Out oa, ob;
+            hulk_kd_node empty;
+            std::memset(&empty, 0, sizeof(empty));
+            a.node = 0;
+            b.node = 0;
+            oa.nodes.push_back(empty);
+            ob.nodes.push_back(empty);
+            std::thread ta([this, &a, &oa] {
+                std::vector<Relation> rel_a(n_tris_);
+                build_node(a, oa, rel_a);
+            });
+            build_node(b, ob, rel);
+            ta.join();
+            const uint32_t ia = append(out, oa);
+            out.nodes[parent].u.junction.above = ia;
+            const uint32_t ib = append(out, ob);
+            out.nodes[parent].u.junction.below = ib;
+            return;
+        }
+        if (has_a) {
+            const uint32_t idx = static_cast<uint32_t>(out.nodes.size());
+            a.node = idx;
+            hulk_kd_node empty;
+            std::memset(&empty, 0, sizeof(empty));
+            out.nodes.push_back(empty);
+            out.nodes[parent].u.junction.above = idx;
+            build_node(a, out, rel);
+        }
+        if (has_b) {
+            const uint32_t idx = static_cast<uint32_t>(out.nodes.size());
             b.node = idx;
             hulk_kd_node empty;
             std::memset(&empty, 0, sizeof(empty));
-            nodes_.push_back(empty);
-            nodes_[parent].u.junction.below = idx;
-            build_node(b);
+            out.nodes.push_back(empty);
+            out.nodes[parent].u.junction.below = idx;
+            build_node(b, out, rel);
         }
     }
 
     const hulk_float4* verts_;
     const hulk_tri* tris_;
     uint32_t n_tris_;
-    uint32_t leaf_count_ = 0;
-    std::vector<Relation> rel_;
+    uint32_t spawn_depth_;
 };
 
 }  // namespace
@@ -276,8 +333,9 @@ struct hulk_kdtree {
     std::vector<uint32_t> leaf_polys;
 };
 
-extern "C" int hulk_kdtree_build(const hulk_float4* verts, uint32_t n_verts, const hulk_tri* tris,
-                                 uint32_t n_tris, const hulk_box* scene_box, hulk_kdtree** out) {
+extern "C" int hulk_kdtree_build_threads(const hulk_float4* verts, uint32_t n_verts, const hulk_tri* tris,
+                                         uint32_t n_tris, const hulk_box* scene_box, uint32_t max_threads,
+                                         hulk_kdtree** out) {
     if (!out || !scene_box || (n_tris && (!verts || !tris))) return HULK_E_ILLEGAL_INPUT;
     for (uint32_t i = 0; i < n_tris; i++)
         for (int k = 0; k < 3; k++)
@@ -285,16 +343,35 @@ extern "C" int hulk_kdtree_build(const hulk_float4* verts, uint32_t n_verts, con
     hulk_kdtree* t = new (std::nothrow) hulk_kdtree;
     if (!t) return HULK_E_ALLOC;
     try {
-        Builder b(verts, tris, n_tris);
-        b.build(*scene_box);
-        t->nodes.swap(b.nodes_);
-        t->leaf_polys.swap(b.leaf_polys_);
+        // Subtrees below the top `spawn` levels run on their own threads
+        // (up to 2^spawn concurrent builds).  0 = automatic: up to 16
+        // threads, one thread for scenes under 20k triangles.
+        uint32_t threads = max_threads;
+        if (threads == 0) {
+            threads = std::min(std::max(1u, std::thread::hardware_concurrency()), 16u);
+            if (n_tris < 20000) threads = 1;
+        }
+        uint32_t spawn = 0;
+        while (spawn < 10 && (1u << (spawn + 1)) <= threads) spawn++;
+        Builder b(verts, tris, n_tris, spawn);
+        Out res;
+        b.build(*scene_box, res);
+        t->nodes.swap(res.nodes);
+        t->leaf_polys.swap(res.leaf_polys);
     } catch (const std::bad_alloc&) {
+        delete t;
+        return HULK_E_ALLOC;
+    } catch (const std::system_error&) {
         delete t;
         return HULK_E_ALLOC;
     }
     *out = t;
     return HULK_OK;
+}
+
+extern "C" int hulk_kdtree_build(const hulk_float4* verts, uint32_t n_verts, const hulk_tri* tris,
+                                 uint32_t n_tris, const hulk_box* scene_box, hulk_kdtree** out) {
+    return hulk_kdtree_build_threads(verts, n_verts, tris, n_tris, scene_box, 0, out);
 }
 
 extern "C" uint32_t hulk_kdtree_node_count(const hulk_kdtree* t) {

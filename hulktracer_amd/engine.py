@@ -154,10 +154,13 @@ class KDTreeCPU:
         self.m_treeNodes = np.zeros((0, 4), np.uint32)
         self.m_leafPolygons = np.zeros(0, np.uint32)
 
-    def buildKDTree(self, scene: Scene) -> int:
+    def buildKDTree(self, scene: Scene, threads: int = 0) -> int:
+        """threads: 0 = automatic, 1 = single-threaded; the arrays are the
+        same for every value."""
         h = C.c_void_p()
-        _check(None, lib().hulk_kdtree_build(ptr(scene.vertices), len(scene.vertices), ptr(scene.triangles),
-                                             scene.n_tris, ptr(scene.box), C.byref(h)))
+        _check(None, lib().hulk_kdtree_build_threads(ptr(scene.vertices), len(scene.vertices),
+                                                     ptr(scene.triangles), scene.n_tris, ptr(scene.box),
+                                                     threads, C.byref(h)))
         try:
             nn = lib().hulk_kdtree_node_count(h)
             nl = lib().hulk_kdtree_leaf_poly_count(h)

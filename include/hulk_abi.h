@@ -313,6 +313,13 @@ int hulk_ctx_scene_box(const hulk_ctx* ctx, float out[6]);
 typedef struct hulk_kdtree hulk_kdtree;
 int hulk_kdtree_build(const hulk_float4* verts, uint32_t n_verts, const hulk_tri* tris,
                       uint32_t n_tris, const hulk_box* scene_box, hulk_kdtree** out);
+/* Same tree, built with at most `max_threads` threads (0 = automatic; 1 =
+ * the reference's single-threaded order).  Subtrees are built concurrently
+ * and concatenated in depth-first order, so the arrays are identical for
+ * every thread count. */
+int hulk_kdtree_build_threads(const hulk_float4* verts, uint32_t n_verts, const hulk_tri* tris,
+                              uint32_t n_tris, const hulk_box* scene_box, uint32_t max_threads,
+                              hulk_kdtree** out);
 uint32_t hulk_kdtree_node_count(const hulk_kdtree* t);
 uint32_t hulk_kdtree_leaf_poly_count(const hulk_kdtree* t);
 const hulk_kd_node* hulk_kdtree_nodes(const hulk_kdtree* t);

@@ -231,11 +231,28 @@ def test_builder_matches_python_restatement(case):
         s = H.Scene.procedural(1, seed=2, param=200)
         v, t = s.vertices, np.concatenate([s.triangles, s.triangles[:50]])
     box = H.scene_box(v)
-    kd = H.KDTreeCPU()
-    kd.buildKDTree(H.Scene(v, t, np.zeros((0, 8), F32), box))
     nodes, lp = kdtree_ref.build(v, t, box)
-    assert np.array_equal(kd.getNodeArray(), nodes)
-    assert np.array_equal(kd.getPolygonArray(), lp)
+    # single-threaded, and with subtrees built on up to 16 threads
+    for threads in (1, 16):
+        kd = H.KDTreeCPU()
+        kd.buildKDTree(H.Scene(v, t, np.zeros((0, 8), F32), box), threads=threads)
+        assert np.array_equal(kd.getNodeArray(), nodes), threads
+        assert np.array_equal(kd.getPolygonArray(), lp), threads
+
+
+@pytest.mark.parametrize("kind,seed", [(2, 1), (3, 7)])
+def test_parallel_builder_equals_single_threaded(kind, seed):
+    """Large scenes: concurrent subtree builds give the same arrays as the
+    single-threaded build (node indices, leaf starts and leaf ids shifted
+    on concatenation)."""
+    s = H.Scene.procedural(kind, seed=seed)
+    ref = H.KDTreeCPU()
+    ref.buildKDTree(s, threads=1)
+    for threads in (0, 3, 8):
+        kd = H.KDTreeCPU()
+        kd.buildKDTree(s, threads=threads)
+        assert np.array_equal(kd.getNodeArray(), ref.getNodeArray()), threads
+        assert np.array_equal(kd.getPolygonArray(), ref.getPolygonArray()), threads
 
 
 def _walk(nodes):
