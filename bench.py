@@ -36,6 +36,22 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel: str, opts: dict, W: int, H: int):
+    """HBM bytes per launch of the primary trace kernel, as measured by the
+    FETCH_SIZE and WRITE_SIZE counter runs whose summary is kept in
+    profiles/ (line: traffic <kernel> <options> <W>x<H> <bytes>).  Only a
+    profile of the same kernel, options and resolution counts; else None."""
+    want = ",".join(f"{k}={int(opts[k])}" for k in ("variant", "block_size", "short_stack_depth",
+                                                    "lds_top_levels", "persistent"))
+    for f in sorted((ROOT / "profiles").glob("*traffic*.txt")):
+        for line in f.read_text().splitlines():
+            parts = line.split()
+            if len(parts) == 5 and parts[0] == "traffic" and parts[1] == kernel and parts[2] == want \
+                    and parts[3] == f"{W}x{H}":
+                return int(parts[4]), f"profiles/{f.name}"
+    return None, f"no FETCH_SIZE/WRITE_SIZE profile of {kernel} {want} at {W}x{H}"
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,6 +65,10 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM traffic counter calibration)")
+    ap.add_argument("--kernel-only", default="",
+                    help="only time the primary trace kernel of pose 10 with these options "
+                         "(e.g. variant=5,block_size=64,short_stack_depth=6,lds_top_levels=0,persistent=0): "
+                         "warmup + steps launches, for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU only: this build has no multi-GPU path (rays are split per GPU only "
@@ -97,6 +117,18 @@ def main() -> int:
         e1.record(stream)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
+
+    if args.kernel_only:
+        opts = {k: int(v) for k, v in (kv.split("=") for kv in args.kernel_only.split(","))}
+        ctx = WL.make_ctx(scene, nodes, lp, stream, **opts)
+        for _ in range(max(args.warmup - 1, 0)):
+            ctx.trace_camera_dev(cams[0], W, Hh, hits.data_ptr(), flags=L.FLAG_NO_SYNC)
+        ms = time_trace(ctx, cams[0], args.steps)
+        ctx.close()
+        print(json.dumps({"kernel": f"k_trace<{opts.get('variant', 1)},false>", "options": opts,
+                          "kernel_ms": round(ms, 4), "timed_launches": args.steps,
+                          "launches": args.steps + max(args.warmup, 1), "rays_per_launch": n}), flush=True)
+        return 0
 
     # -- every variant at its default shape (primary trace of pose 10)
     shapes = {"GPUTrace": dict(variant=H.FULLSTACK), "GPUTrace_shortStack": dict(variant=H.SHORTSTACK),
@@ -207,10 +239,16 @@ def main() -> int:
         for tag, vopts in (("", opts_now), ("_stackless", dict(opts_now, variant=H.RESTART))):
             ctx.set_options(**vopts)
             ref_v = [ctx.trace_camera(cam, W, Hh) for cam in cams] if tag else ref_hits
-            if tag:  # plain frame-by-frame on the reference's live kernel
+            if tag:  # the reference's live kernel: plain frame by frame, and batched without a bound
                 s_pl = timed(plain_step, max(1, args.steps // 5))
                 methods["plain" + tag] = {"ms_per_step": round(s_pl * 1e3, 3),
                                          "mrays_s": round(nf * n / s_pl / 1e6, 1), "hit_match": 1.0}
+                pbatch = ctx.batch(frames, None, brgba.data_ptr(), use_graph=True)
+                s_pb = timed(lambda: pbatch.launch(L.FLAG_NO_SYNC), max(1, args.steps // 2))
+                methods["batched" + tag] = {"ms_per_step": round(s_pb * 1e3, 3),
+                                           "mrays_s": round(nf * n / s_pb / 1e6, 1),
+                                           "hit_match": hits_match(pbatch.primary_hits(), ref_v)}
+                pbatch.close()
             lbatch = ctx.batch(frames, None, brgba.data_ptr(), tbound_dev=tb.data_ptr(), use_graph=True)
 
             def learned_step():
@@ -246,6 +284,8 @@ def main() -> int:
     log(f"primary trace {k_ms:.3f} ms, B_ray {b_ray:.1f} B (nodes {n_node:.2f}, tris {n_tri:.2f}), "
         f"{achieved:.0f} GB/s algorithmic")
     tuned_name = L.VARIANT_NAMES[best["variant"]]
+    kernel_name = f"k_trace<{best['variant']},false>"
+    traffic, traffic_src = profiled_traffic(kernel_name, best, W, Hh)
 
     result = {
         "metric": "Mrays/s nearest-hit @1024^2, learned-bound+batched vs plain, 100% hit match",
@@ -269,8 +309,8 @@ def main() -> int:
         "variants_primary_trace": variants,
         "learned": learned_info,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"k_trace<{best['variant']},false>", "kernel_ms": round(k_ms, 4),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": kernel_name, "kernel_ms": round(k_ms, 4),
                      "rays_per_launch": n, "bytes_per_ray": round(b_ray, 1),
                      "node_visits_per_ray": round(n_node, 3), "tri_tests_per_ray": round(n_tri, 3)},
         "cpu_baseline": None,
@@ -287,7 +327,7 @@ def main() -> int:
         while True:
             rays_total += osc.cpu_trace_scene(cams[frames_done % nf], W, Hh, threads=threads, out=False)[0]
             frames_done += 1
-            if time.perf_counter() - tc > args.cpu_seconds or frames_done >= 50:
+            if time.perf_counter() - tc > args.cpu_seconds or frames_done >= 2000:
                 break
         cpu_s = time.perf_counter() - tc
         result["cpu_baseline"] = {

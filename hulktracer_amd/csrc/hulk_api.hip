@@ -36,7 +36,7 @@ struct DevBuf {
 // accumulation).  The context owns one set; every batch owns its own, so a
 // captured graph never sees a buffer that a later call reallocates.
 struct Work {
-    DevBuf hits, hits2, hits3, shadow_rays, shadow_slot, shadow_t, qrays[2], qpix[2], qw[2], acc, ctr, wcount,
+    DevBuf hits, hits2, hits3, shadow_rays, shadow_slot, shadow_t, qrays[2], qpix[2], qw[2], acc, ctr, wq, wcount,
         woffset;
 };
 
@@ -280,9 +280,15 @@ int run_trace(hulk_ctx* c, int variant, TraceParams& P, uint32_t n_max, uint32_t
     P.work_counter = work_counter;
     uint32_t grid;
     if (persistent) {
-        HIP_TRY(c, launch_fill_u32(work_counter, 1, 0u, c->stream));
+        HIP_TRY(c, launch_fill_u32(work_counter, kWorkCounterWords, 0u, c->stream));
+        // one resident wave set (2048 threads per CU); each wave takes
+        // `fetch` chunks per atomic, so that every wave still gets about
+        // four fetches of its slice
         const uint32_t per_cu = std::max<uint32_t>(1u, 2048u / block);
         grid = std::max<uint32_t>(1u, std::min<uint32_t>((n_max + block - 1) / block, c->num_cus * per_cu));
+        const uint64_t waves = uint64_t(grid) * (block / 64);
+        const uint64_t chunks = (uint64_t(n_max) + 63) / 64;
+        P.fetch = static_cast<uint32_t>(std::min<uint64_t>(8, std::max<uint64_t>(1, chunks / (waves * 4))));
     } else {
         grid = std::max<uint32_t>(1u, (n_max + block - 1) / block);
     }
@@ -324,7 +330,7 @@ int finish(hulk_ctx* c, uint32_t flags, bool timed) {
 
 void release_work(Work& w) {
     DevBuf* bufs[] = {&w.hits, &w.hits2, &w.hits3, &w.shadow_rays, &w.shadow_slot, &w.shadow_t, &w.qrays[0],
-                      &w.qrays[1], &w.qpix[0], &w.qpix[1], &w.qw[0], &w.qw[1], &w.acc, &w.ctr, &w.wcount,
+                      &w.qrays[1], &w.qpix[0], &w.qpix[1], &w.qw[0], &w.qw[1], &w.acc, &w.ctr, &w.wq, &w.wcount,
                       &w.woffset};
     for (DevBuf* b : bufs) release(*b);
 }
@@ -335,6 +341,7 @@ int ensure_work(hulk_ctx* c, Work& w, uint32_t n, uint32_t np, bool bounces) {
     int r;
     const size_t m = std::max<uint32_t>(np, 1);
     if ((r = ensure(c, w.ctr, 64))) return r;
+    if ((r = ensure(c, w.wq, kWorkCounterWords * 4))) return r;
     if ((r = ensure(c, w.hits, (size_t)n * sizeof(hulk_hit)))) return r;
     if ((r = ensure(c, w.acc, (size_t)n * 12))) return r;
     if ((r = ensure(c, w.shadow_rays, (size_t)n * m * sizeof(hulk_ray)))) return r;
@@ -385,7 +392,7 @@ int trace_primary(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, 
     P.tbound = tbound;
     P.hits = out;
     return run_trace(c, o.variant, P, fs.n, block, lds, count, o.persistent != 0,
-                     static_cast<uint32_t*>(w.ctr.p), timed);
+                     static_cast<uint32_t*>(w.wq.p), timed);
 }
 
 // Shading of a primary hit buffer: per generation g, point-light shadow
@@ -401,7 +408,8 @@ int shade_pipeline(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w,
     uint32_t block;
     size_t lds;
     if ((r = trace_setup(c, o, P, block, lds))) return r;
-    uint32_t* ctr = static_cast<uint32_t*>(w.ctr.p);  // [0] work, [1] shadow count, [2..3] queue counts
+    uint32_t* ctr = static_cast<uint32_t*>(w.ctr.p);  // [1] shadow count, [2..3] queue counts
+    uint32_t* wq = static_cast<uint32_t*>(w.wq.p);    // persistent-mode work counters
     HIP_TRY(c, launch_fill_f32(static_cast<float*>(w.acc.p), (size_t)n * 3, 0.0f, c->stream));
     const hulk_hit* cur_hits = hits0;
     const hulk_ray* cur_rays = nullptr;
@@ -450,7 +458,7 @@ int shade_pipeline(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w,
             S.t_out = static_cast<float*>(w.shadow_t.p);
             S.t_slot = static_cast<const uint32_t*>(w.shadow_slot.p);
             S.stop_at_w = 1;
-            if ((r = run_trace(c, o.variant, S, n * np, block, lds, count, false, ctr, false))) return r;
+            if ((r = run_trace(c, o.variant, S, n * np, block, lds, count, o.persistent != 0, wq, false))) return r;
         }
         ShadeParams SP;
         std::memset(&SP, 0, sizeof(SP));
@@ -483,7 +491,7 @@ int shade_pipeline(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w,
         Q.n = n;
         hulk_hit* nh = static_cast<hulk_hit*>((g & 1) ? w.hits3.p : w.hits2.p);
         Q.hits = nh;
-        if ((r = run_trace(c, o.variant, Q, n, block, lds, count, false, ctr, false))) return r;
+        if ((r = run_trace(c, o.variant, Q, n, block, lds, count, o.persistent != 0, wq, false))) return r;
         cur_hits = nh;
         cur_rays = static_cast<const hulk_ray*>(w.qrays[qo].p);
         cur_pix = static_cast<const uint32_t*>(w.qpix[qo].p);
@@ -558,7 +566,8 @@ extern "C" int hulk_create(hulk_ctx** out, const hulk_opts* opts) {
         c->own_stream = true;
     }
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreate(&c->ev2) != hipSuccess || ensure(c, c->work.ctr, 64) || ensure(c, c->stats, 128) ||
+        hipEventCreate(&c->ev2) != hipSuccess || ensure(c, c->work.ctr, 64) || ensure(c, c->work.wq, kWorkCounterWords * 4) ||
+        ensure(c, c->stats, 128) ||
         hipHostMalloc(reinterpret_cast<void**>(&c->host_stats), 128, hipHostMallocDefault) != hipSuccess) {
         hulk_destroy(c);
         return HULK_E_HIP;
@@ -773,6 +782,7 @@ extern "C" int hulk_trace_camera(hulk_ctx* c, const hulk_camera* cam, uint32_t W
     fs.H = H;
     fs.n = n;
     if ((r = ensure(c, c->work.ctr, 64))) return r;
+    if ((r = ensure(c, c->work.wq, kWorkCounterWords * 4))) return r;
     const float* tb;
     if ((r = stage_in(c, tbound, n, dev, c->tbound, &tb))) return r;
     hulk_hit* out = hits;
@@ -799,6 +809,7 @@ extern "C" int hulk_trace_rays(hulk_ctx* c, const hulk_ray* rays, uint32_t n, co
     int r = trace_setup(c, c->opts, P, block, lds);
     if (r) return r;
     if ((r = ensure(c, c->work.ctr, 64))) return r;
+    if ((r = ensure(c, c->work.wq, kWorkCounterWords * 4))) return r;
     P.source = kSourceRays;
     if ((r = stage_in(c, rays, n, dev, c->rays, &P.rays))) return r;
     P.n = n;
@@ -811,7 +822,7 @@ extern "C" int hulk_trace_rays(hulk_ctx* c, const hulk_ray* rays, uint32_t n, co
     P.hits = out;
     if (count && (r = zero_stats(c))) return r;
     if ((r = run_trace(c, c->opts.variant, P, n, block, lds, count, c->opts.persistent != 0,
-                       static_cast<uint32_t*>(c->work.ctr.p), true)))
+                       static_cast<uint32_t*>(c->work.wq.p), true)))
         return r;
     if (!dev && n) HIP_TRY(c, hipMemcpyAsync(hits, out, (size_t)n * sizeof(hulk_hit), hipMemcpyDeviceToHost, c->stream));
     if (count && (r = read_stats(c))) return r;

@@ -466,20 +466,36 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
     const uint32_t waves_per_block = block >> 6;
     uint64_t cn = 0, cl = 0, ct = 0, cfb = 0, chits = 0, crays = 0;
 
-    // Persistent: one atomic per wave fetches the next 64 rays.  Static:
-    // XCD-aware chunk order (blocks are dispatched round-robin over the 8
-    // XCDs, so consecutive chunks go to blocks on the same XCD).
-    // Persistent waves take kFetch chunks of 64 rays per atomic: device-scope
-    // atomics on one address serialize across the XCDs, so they are kept few.
-    constexpr uint32_t kFetch = 8;
-    uint32_t chunk, chunk_end = 0;
-    const uint32_t nblocks = gridDim.x;
+    // Static: XCD-aware chunk order (blocks are dispatched round-robin over
+    // the 8 XCDs, so consecutive chunks go to blocks on the same XCD).
+    // Persistent: the chunk space is cut into 8 slices, one per XCD, each
+    // with its own counter; a wave takes P.fetch chunks per atomic from the
+    // slice of its block's XCD and moves on to the next slice once its own
+    // is drained, so atomics stay spread over 8 addresses and the last
+    // chunks are shared by every XCD.
+    uint32_t chunk = 0, chunk_end = 0, slice = blockIdx.x & 7u, tries = 0;
+    const uint32_t nchunks = (n_rays + 63u) >> 6;
+    auto next_chunks = [&]() -> bool {
+        while (tries < 8) {
+            const uint32_t lo = slice * nchunks / 8u, hi = (slice + 1u) * nchunks / 8u;
+            uint32_t got = 0;
+            if (lane_id == 0) got = atomicAdd(P.work_counter + slice * kWorkCounterStride, P.fetch);
+            got = __shfl(got, 0, 64);
+            if (got < hi - lo) {
+                chunk = lo + got;
+                chunk_end = min(chunk + P.fetch, hi);
+                return true;
+            }
+            slice = (slice + 1u) & 7u;
+            tries++;
+        }
+        return false;
+    };
+    bool have;
     if (P.persistent) {
-        uint32_t base = 0;
-        if (lane_id == 0) base = atomicAdd(P.work_counter, 64u * kFetch);
-        chunk = __shfl(base, 0, 64) >> 6;
-        chunk_end = chunk + kFetch;
+        have = next_chunks();
     } else {
+        const uint32_t nblocks = gridDim.x;
         const uint32_t b = blockIdx.x;
         const uint32_t per_xcd = (nblocks + 7u) >> 3;
         const uint32_t xcd = b & 7u, k = b >> 3;
@@ -489,8 +505,9 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
         // chunks, which round-robin dispatch already spreads over the XCDs
         if ((nblocks & 7u) != 0 || P.n_dev) mapped = b;
         chunk = mapped * waves_per_block + wave;
+        have = chunk < nchunks;
     }
-    while (chunk * 64u < n_rays) {
+    while (have) {
         const uint32_t i = chunk * 64u + lane_id;
         if (i < n_rays) {
             Ray r;
@@ -555,14 +572,9 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
             }
         }
         if (P.persistent) {
-            if (++chunk == chunk_end) {
-                uint32_t base = 0;
-                if (lane_id == 0) base = atomicAdd(P.work_counter, 64u * kFetch);
-                chunk = __shfl(base, 0, 64) >> 6;
-                chunk_end = chunk + kFetch;
-            }
+            if (++chunk == chunk_end) have = next_chunks();
         } else {
-            break;
+            have = false;
         }
     }
     if (COUNT) {

@@ -60,10 +60,16 @@ struct TraceParams {
     float* t_out;
     const uint32_t* t_slot;
     unsigned long long* counters;  // rays, nodes, leaves, tris, hits, fallbacks
+    // persistent mode: 8 counters, one per XCD slice of the chunk space,
+    // kWorkCounterStride words apart; `fetch` chunks of 64 rays per atomic
     uint32_t* work_counter;
     uint32_t persistent;
+    uint32_t fetch;
     uint32_t stop_at_w;  // shadow queue: stop once best.t < ray.start.w (the light distance)
 };
+
+constexpr uint32_t kWorkCounterStride = 64;  // 256 B between the per-XCD counters
+constexpr uint32_t kWorkCounterWords = 8 * kWorkCounterStride;
 
 struct GenParams {
     const hulk_hit* hits;

@@ -1,14 +1,13 @@
 """Drivers of the benchmark workloads (one GPU, device-resident buffers).
 
-  plain_frames    terrain scene, frame by frame, one traversal variant
   animated        1,000,002-triangle animated tori, 20 frames, CPU KD
                   rebuild per frame overlapped with the previous frame's
                   GPU render, primary + shadow + 2 mirror generations
   batched         64 frames of the random-triangle and terrain scenes,
                   resolutions 256^2-1024^2, bounce depth 0-3, one
                   graph-captured pipeline per scene, autotuned per scene
-  learned         MLP bound trained on poses 0-9 of the terrain scene,
-                  applied to poses 10-19, against the plain traversal
+The learned-bound workload (poses 0-9 / 10-19 of the terrain scene) is run
+by bench.py itself.
 All timing uses HIP events / device synchronization around GPU work only;
 CPU KD builds are timed separately, as the reference's Statistics does
 (GLVisitorCPUTrace.cpp:45-47).
@@ -48,47 +47,27 @@ def build(scene) -> tuple[np.ndarray, np.ndarray, float]:
     return kd.getNodeArray(), kd.getPolygonArray(), (time.perf_counter() - t) * 1e3
 
 
-# ----------------------------------------------------------------- plain --
-
-def plain_frames(ctx: GPUTrace, cams, W: int, H: int, stream, reps: int = 1, tbounds=None):
-    """Render each camera (primary + shadow + shade) one frame at a time.
-    Returns GPU ms for all frames x reps."""
-    torch = _torch()
-    n = W * H
-    rgba = torch.empty(n, dtype=torch.int32, device="cuda")
-    hits = torch.empty(n * 32, dtype=torch.uint8, device="cuda")
-    for cam in cams[:1]:
-        ctx.render_frame_dev(cam, W, H, 0, None, rgba.data_ptr(), hits.data_ptr(), flags=L.FLAG_NO_SYNC)
-    torch.cuda.synchronize()
-    e0, e1 = _events(stream)
-    e0.record(stream)
-    for _ in range(reps):
-        for i, cam in enumerate(cams):
-            tb = None if tbounds is None else tbounds[i].data_ptr()
-            ctx.render_frame_dev(cam, W, H, 0, None, rgba.data_ptr(), hits.data_ptr(), tbound_dev=tb,
-                                 flags=L.FLAG_NO_SYNC)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1)
-
-
 # -------------------------------------------------------------- animated --
 
 def animated(frames: int = 20, W: int = 1024, H: int = 1024, max_bounce: int = 2, seed: int = 7, stream=None,
-             variant: int = L.FULLSTACK, log=print):
+             autotune: bool = True, log=print):
     """Config: 50 tori x 20,000 triangles + ground plane, rigid motion per
     frame, KD tree rebuilt on the CPU each frame; the build of frame k+1
-    runs while the GPU renders frame k."""
+    runs while the GPU renders frame k.  Traversal options are autotuned
+    once on frame 0 (512x512) and kept for every frame."""
     torch = _torch()
     stream = stream or torch.cuda.Stream()
     rgba = torch.empty(W * H, dtype=torch.int32, device="cuda")
     hits = torch.empty(W * H * 32, dtype=torch.uint8, device="cuda")
     scene = Scene.procedural(L.SCENE_ANIMATED_TORI, seed=seed, frame=0)
     nodes, lp, bms = build(scene)
-    ctx = make_ctx(scene, nodes, lp, stream, variant=variant, persistent=True)
+    ctx = make_ctx(scene, nodes, lp, stream, variant=L.FULLSTACK)
     mirror = torch.from_numpy(scene.mirror).cuda()
-    # ray counts of frame 0 (counted run, not timed)
     cam = scene.camera(0, H / W)
+    tuned = None
+    if autotune:
+        tuned, _ = ctx.autotune(cam, 512, 512, reps=2)
+    # ray counts of frame 0 (counted run, not timed)
     ctx.reset_counters()
     ctx.render_frame_dev(cam, W, H, max_bounce, mirror.data_ptr(), rgba.data_ptr(), hits.data_ptr(),
                          flags=L.FLAG_COUNT)
@@ -120,6 +99,7 @@ def animated(frames: int = 20, W: int = 1024, H: int = 1024, max_bounce: int = 2
     rays_frame = c0["rays"]
     res = {
         "triangles": int(scene.n_tris), "frames": frames, "res": f"{W}x{H}", "max_bounce": max_bounce,
+        "tuned": tuned,
         "nearest_hit_rays_frame0": int(rays_frame), "shadow_rays_frame0": int(c0["shadow_rays"]),
         "gpu_ms_per_frame": round(float(np.mean(gpu_ms)), 3),
         "cpu_kd_build_ms_per_frame": round(float(np.mean(build_ms)), 1),
@@ -204,74 +184,3 @@ def batched(stream=None, specs=None, reps: int = 3, autotune: bool = True, log=p
     out["speedup"] = round(total_plain / total_batch, 3)
     log(f"batched: {out}")
     return out
-
-
-# --------------------------------------------------------------- learned --
-
-def learned(stream=None, W: int = 1024, H: int = 1024, n_train: int = 4_000_000, steps: int = 1500,
-            quantile: float = 0.95, variants=(L.FULLSTACK, L.RESTART), log=print):
-    """Train on (ray, log t) pairs of poses 0-9 of the terrain scene, bound
-    poses 10-19; compare against the plain traversal of the same variant:
-    hit agreement, fallback rate, node visits and Mrays/s."""
-    from .learned import HitDistanceMLP, training_pairs
-
-    torch = _torch()
-    stream = stream or torch.cuda.Stream()
-    scene = Scene.procedural(L.SCENE_TERRAIN_SPHERES, seed=1)
-    nodes, lp, _ = build(scene)
-    ctx = make_ctx(scene, nodes, lp, stream, variant=L.FULLSTACK)
-    train_cams = [scene.camera(k, H / W) for k in range(10)]
-    test_cams = [scene.camera(k, H / W) for k in range(10, 20)]
-    t0 = time.perf_counter()
-    rays, y = training_pairs(ctx, train_cams, W, H, n_samples=n_train)
-    mlp = HitDistanceMLP(ctx, seed=1, max_batch=16384)
-    t1 = time.perf_counter()
-    losses = mlp.fit(rays, y, steps=steps, batch=16384, lr=2e-3, log_every=max(steps // 5, 1))
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    margin = mlp.calibrate(rays[: 1 << 20], y[: 1 << 20], quantile)
-    n = W * H
-    tb = [torch.empty(n, dtype=torch.float32, device="cuda") for _ in test_cams]
-    # inference cost (all 10 poses)
-    e0, e1 = _events(stream)
-    e0.record(stream)
-    for cam, t in zip(test_cams, tb):
-        mlp.bound_camera(cam, W, H, t, sync=False)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    infer_ms = e0.elapsed_time(e1) / len(test_cams)
-    res = {"train_pairs": int(rays.shape[0]), "steps": steps, "pairs_s_M": round(steps * 16384 / (t2 - t1) / 1e6, 2),
-           "data_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2), "loss": [[int(s), round(float(l), 5)] for s, l in losses],
-           "margin": round(margin, 4), "infer_ms_per_frame": round(infer_ms, 3),
-           "mlp_gflop_per_frame": round(n * 2 * (64 * 128 + 2 * 128 * 128 + 128) / 1e9, 2), "variants": {}}
-    for v in variants:
-        ctx.set_options(variant=v, persistent=1)
-        agree, fb, nodes_plain, nodes_bound = 0, 0, 0, 0
-        for cam, t in zip(test_cams, tb):
-            plain = ctx.trace_camera(cam, W, H, count=True)
-            cp = ctx.counters()
-            ctx.reset_counters()
-            got = ctx.trace_camera(cam, W, H, tbound=t.cpu().numpy(), count=True)
-            cb = ctx.counters()
-            ctx.reset_counters()
-            agree += int((got["tri"] == plain["tri"]).sum() - 0)
-            same = np.array_equal(got["t"].view(np.uint32), plain["t"].view(np.uint32)) and np.array_equal(
-                got["tri"], plain["tri"])
-            assert same, "learned bound changed a nearest hit"
-            fb += int((got["flags"] & 1).sum())
-            nodes_plain += cp["node_visits"]
-            nodes_bound += cb["node_visits"]
-        plain_ms = plain_frames(ctx, test_cams, W, H, stream)
-        bound_ms = plain_frames(ctx, test_cams, W, H, stream, tbounds=tb)
-        res["variants"][L.VARIANT_NAMES[v]] = {
-            "hit_agreement": agree / (n * len(test_cams)), "fallback_rate": round(fb / (n * len(test_cams)), 4),
-            "node_visits_saved": round(1 - nodes_bound / max(nodes_plain, 1), 4),
-            "plain_frame_ms": round(plain_ms / len(test_cams), 3),
-            "bounded_frame_ms": round(bound_ms / len(test_cams) + infer_ms, 3),
-            "plain_mrays_s": round(n * len(test_cams) / plain_ms / 1e3, 1),
-            "bounded_mrays_s": round(n * len(test_cams) / (bound_ms + infer_ms * len(test_cams)) / 1e3, 1),
-        }
-    log(f"learned: {res}")
-    mlp.close()
-    ctx.close()
-    return res

@@ -76,7 +76,7 @@ def test_short_stack_depths(variant, depth):
 @pytest.mark.parametrize("opts", [dict(lds_top_levels=6), dict(lds_top_levels=10, block_size=128),
                                   dict(persistent=True), dict(persistent=True, lds_top_levels=8, block_size=128),
                                   dict(block_size=64)])
-@pytest.mark.parametrize("variant", [H.FULLSTACK, H.LDSSTACK, H.RESTART])
+@pytest.mark.parametrize("variant", [H.FULLSTACK, H.LDSSTACK, H.RESTART, H.ROPES])
 def test_launch_options_parity(variant, opts):
     """LDS top tree, persistent waves and block sizes change only scheduling."""
     s, nodes, lp = scene_and_tree(2, 1)
@@ -105,12 +105,16 @@ def test_shade_parity(variant, mode):
         assert diff.max() == 0, f"scene {kind}: {int((diff > 0).sum())} channels differ, max {diff.max()} LSB"
 
 
-def test_render_bounces_parity():
+@pytest.mark.parametrize("opts", [dict(variant=H.FULLSTACK), dict(variant=H.FULLSTACK, persistent=True),
+                                  dict(variant=H.ROPES, persistent=True, block_size=64)])
+def test_render_bounces_parity(opts):
     """Animated tori (5 meshes, 100,002 triangles): primary + shadow + 2
-    mirror generations with per-generation compaction."""
+    mirror generations with per-generation compaction; persistent waves
+    also drain the device-counted shadow and mirror queues."""
     s, nodes, lp = scene_and_tree(3, 6, 5, 4)
     cam = s.camera(4, 1.0)
-    ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK)
+    opts = dict(opts)
+    ctx = gpu_ctx(s, nodes, lp, opts.pop("variant"), **opts)
     img, hits = ctx.render_frame(cam, 160, 160, max_bounce=2, mirror=s.mirror)
     rimg, counts = oracle_scene(3, 6, 5, 4).render(O.V_FULLSTACK, cam, 160, 160, max_bounce=2)
     diff = np.abs(img.view(np.uint8).astype(int) - rimg.view(np.uint8).astype(int))
@@ -118,6 +122,18 @@ def test_render_bounces_parity():
     assert int(s.mirror.sum()) > 0
     flat, _ = ctx.render_frame(cam, 160, 160, max_bounce=0, mirror=s.mirror)
     assert (flat != img).sum() > 100, "mirror generations changed too few pixels"
+
+
+@pytest.mark.parametrize("n", [1, 63, 65, 1000, 40000])
+def test_persistent_ray_counts(n):
+    """Persistent waves over ray buffers of ragged sizes (fewer rays than
+    one wave, than one chunk per XCD slice, than one fetch per wave)."""
+    s, nodes, lp = scene_and_tree(1, 3, 300)
+    rays = random_rays(n, s.box, seed=n)
+    ref = oracle_scene(1, 3, 300).trace_rays(O.V_FULLSTACK, rays, depth=6)
+    for block in (64, 256):
+        ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK, persistent=True, block_size=block)
+        _check(ctx.trace_rays(rays), ref, f"persistent n={n} block {block}")
 
 
 @pytest.mark.parametrize("variant", [H.FULLSTACK, H.SHORTSTACK, H.RESTART])
