@@ -29,6 +29,7 @@ FLAG_DEVICE_PTRS = 1
 FLAG_ANYHIT = 2
 FLAG_COUNT = 4
 FLAG_NO_SYNC = 8
+PIPELINE_AUTO, PIPELINE_QUEUED, PIPELINE_FUSED = range(3)
 SHADE_FUSED = 0
 SHADE_PICTURE = 1
 NO_HIT = 0xFFFFFFFF
@@ -58,7 +59,7 @@ class HulkError(RuntimeError):
 class Opts(C.Structure):
     _fields_ = [("device", C.c_int32), ("variant", C.c_int32), ("block_size", C.c_uint32),
                 ("short_stack_depth", C.c_uint32), ("lds_top_levels", C.c_uint32),
-                ("persistent", C.c_uint32), ("stream", C.c_void_p)]
+                ("persistent", C.c_uint32), ("pipeline", C.c_uint32), ("stream", C.c_void_p)]
 
 
 class Counters(C.Structure):

@@ -128,7 +128,16 @@ int check_opts(hulk_ctx* c, const hulk_opts& o) {
     if (o.short_stack_depth < 1 || o.short_stack_depth > kMaxShortStack)
         return fail(c, HULK_E_ILLEGAL_INPUT, "short_stack_depth must be in [1, 16]");
     if (o.lds_top_levels > kTopLevelsMax) return fail(c, HULK_E_ILLEGAL_INPUT, "lds_top_levels must be <= 12");
+    if (o.pipeline > HULK_PIPELINE_FUSED) return fail(c, HULK_E_ILLEGAL_INPUT, "unknown pipeline");
     return HULK_OK;
+}
+
+// Frame launch model of hulk_pipeline (AUTO: fused without mirror
+// generations).
+bool use_fused(const hulk_opts& o, bool bounces) {
+    if (o.pipeline == HULK_PIPELINE_FUSED) return true;
+    if (o.pipeline == HULK_PIPELINE_QUEUED) return false;
+    return !bounces;
 }
 
 Camera to_cam(const hulk_camera& c) {
@@ -274,7 +283,7 @@ int trace_setup(hulk_ctx* c, const hulk_opts& o, TraceParams& P, uint32_t& block
 // Launch one trace.  Persistent launches zero their work counter with a
 // fill kernel first (graph-capture safe: no memset nodes).
 int run_trace(hulk_ctx* c, int variant, TraceParams& P, uint32_t n_max, uint32_t block, size_t lds, bool count,
-              bool persistent, uint32_t* work_counter, bool timed) {
+              bool persistent, uint32_t* work_counter, bool timed, bool render = false) {
     if (n_max == 0) return HULK_OK;
     P.persistent = persistent ? 1u : 0u;
     P.work_counter = work_counter;
@@ -293,7 +302,7 @@ int run_trace(hulk_ctx* c, int variant, TraceParams& P, uint32_t n_max, uint32_t
         grid = std::max<uint32_t>(1u, (n_max + block - 1) / block);
     }
     if (timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-    HIP_TRY(c, launch_trace(variant, P, count, grid, block, lds, c->stream));
+    HIP_TRY(c, launch_trace(variant, P, count, render, grid, block, lds, c->stream));
     if (timed) HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return HULK_OK;
 }
@@ -337,12 +346,13 @@ void release_work(Work& w) {
 
 // Size a work set for n primary rays, np point lights, with or without
 // mirror generations.
-int ensure_work(hulk_ctx* c, Work& w, uint32_t n, uint32_t np, bool bounces) {
+int ensure_work(hulk_ctx* c, Work& w, uint32_t n, uint32_t np, bool bounces, bool queued = true) {
     int r;
     const size_t m = std::max<uint32_t>(np, 1);
     if ((r = ensure(c, w.ctr, 64))) return r;
     if ((r = ensure(c, w.wq, kWorkCounterWords * 4))) return r;
     if ((r = ensure(c, w.hits, (size_t)n * sizeof(hulk_hit)))) return r;
+    if (!queued) return HULK_OK;  // the fused pipeline keeps everything else in registers
     if ((r = ensure(c, w.acc, (size_t)n * 12))) return r;
     if ((r = ensure(c, w.shadow_rays, (size_t)n * m * sizeof(hulk_ray)))) return r;
     if ((r = ensure(c, w.shadow_slot, (size_t)n * m * 4))) return r;
@@ -393,6 +403,37 @@ int trace_primary(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, 
     P.hits = out;
     return run_trace(c, o.variant, P, fs.n, block, lds, count, o.persistent != 0,
                      static_cast<uint32_t*>(w.wq.p), timed);
+}
+
+// Whole frame in one launch (FUSED pipeline): primary hit records to `out`,
+// packed pixels to `rgba`.  No allocation, no synchronization.
+int render_fused(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, const float* tbound, hulk_hit* out,
+                 uint32_t max_bounce, const uint8_t* mirror_dev, uint32_t* rgba, bool count, bool timed) {
+    TraceParams P;
+    uint32_t block;
+    size_t lds;
+    int r = trace_setup(c, o, P, block, lds);
+    if (r) return r;
+    if (fs.frames) {
+        P.source = kSourceFrames;
+        P.frames = fs.frames;
+        P.n_frames = fs.n_frames;
+    } else {
+        P.source = kSourceCamera;
+        P.cam = fs.cam;
+        P.W = fs.W;
+        P.H = fs.H;
+    }
+    P.n = fs.n;
+    P.tbound = tbound;
+    P.hits = out;
+    P.lights = static_cast<const float4*>(c->d_lights.p);
+    P.n_lights = static_cast<uint32_t>(c->lights.size());
+    P.mirror = mirror_dev;
+    P.max_bounce = mirror_dev ? max_bounce : 0;
+    P.rgba = rgba;
+    return run_trace(c, o.variant, P, fs.n, block, lds, count, o.persistent != 0, static_cast<uint32_t*>(w.wq.p),
+                     timed, true);
 }
 
 // Shading of a primary hit buffer: per generation g, point-light shadow
@@ -446,8 +487,8 @@ int shade_pipeline(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w,
             G.weight_out = static_cast<float*>(w.qw[qo].p);
             G.next_count = ctr + 2 + qo;
         }
-        G.wave_counts = static_cast<uint32_t*>(w.wcount.p);
-        G.wave_offsets = static_cast<uint32_t*>(w.woffset.p);
+        G.block_counts = static_cast<uint32_t*>(w.wcount.p);
+        G.block_offsets = static_cast<uint32_t*>(w.woffset.p);
         HIP_TRY(c, launch_gen_secondary(G, n, c->stream));
         if (np) {
             TraceParams S = P;
@@ -894,7 +935,7 @@ extern "C" int hulk_render_frame(hulk_ctx* c, const hulk_camera* cam, uint32_t W
     fs.H = H;
     fs.n = n;
     const bool bounces = max_bounce > 0 && mirror_mask;
-    if ((r = ensure_work(c, c->work, n, c->n_point, bounces))) return r;
+    if ((r = ensure_work(c, c->work, n, c->n_point, bounces, !use_fused(c->opts, bounces)))) return r;
     const uint8_t* dm = nullptr;
     if (bounces && (r = stage_in(c, mirror_mask, c->n_tris, dev, c->mirror, &dm))) return r;
     const float* dt;
@@ -906,8 +947,13 @@ extern "C" int hulk_render_frame(hulk_ctx* c, const hulk_camera* cam, uint32_t W
         out = static_cast<uint32_t*>(c->rgba.p);
     }
     if (count && (r = zero_stats(c))) return r;
-    if ((r = trace_primary(c, c->opts, fs, c->work, dt, h0, count, true))) return r;
-    if ((r = shade_pipeline(c, c->opts, fs, c->work, h0, bounces ? max_bounce : 0, dm, out, count))) return r;
+    if (use_fused(c->opts, bounces)) {
+        if ((r = render_fused(c, c->opts, fs, c->work, dt, h0, bounces ? max_bounce : 0, dm, out, count, true)))
+            return r;
+    } else {
+        if ((r = trace_primary(c, c->opts, fs, c->work, dt, h0, count, true))) return r;
+        if ((r = shade_pipeline(c, c->opts, fs, c->work, h0, bounces ? max_bounce : 0, dm, out, count))) return r;
+    }
     if (!dev) {
         HIP_TRY(c, hipMemcpyAsync(rgba8, out, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
         if (primary_hits)
@@ -973,8 +1019,10 @@ int batch_record(hulk_batch* b, bool count) {
     fs.n = b->n;
     int r;
     hulk_hit* h0 = static_cast<hulk_hit*>(b->w.hits.p);
-    if ((r = trace_primary(c, b->opts, fs, b->w, b->tbound, h0, count, false))) return r;
     const uint8_t* dm = b->max_bounce ? static_cast<const uint8_t*>(b->d_mirror.p) : nullptr;
+    if (use_fused(b->opts, dm != nullptr))
+        return render_fused(c, b->opts, fs, b->w, b->tbound, h0, dm ? b->max_bounce : 0, dm, b->rgba, count, false);
+    if ((r = trace_primary(c, b->opts, fs, b->w, b->tbound, h0, count, false))) return r;
     return shade_pipeline(c, b->opts, fs, b->w, h0, dm ? b->max_bounce : 0, dm, b->rgba, count);
 }
 
@@ -1017,7 +1065,7 @@ extern "C" int hulk_batch_create(hulk_ctx* c, const hulk_frame_desc* frames, uin
     const bool bounces = b->max_bounce > 0 && mirror_mask;
     if (!bounces) b->max_bounce = 0;
     if ((r = ensure(c, b->d_frames, b->frames.size() * sizeof(FrameDev))) ||
-        (r = ensure_work(c, b->w, b->n, c->n_point, bounces))) {
+        (r = ensure_work(c, b->w, b->n, c->n_point, bounces, !use_fused(b->opts, bounces)))) {
         hulk_batch_destroy(b);
         return r;
     }

@@ -77,8 +77,9 @@ __device__ __forceinline__ void tile_pixel(uint32_t i, uint32_t W, uint32_t H, u
 
 // Batched primary ray i: pixel index (frame offset + y*W + x) returned,
 // 8x8 tiles inside a frame when its sides are multiples of 8.
-__device__ __forceinline__ uint32_t frame_ray_tiled(const FrameDev* F, uint32_t n, uint32_t i, Ray& r) {
-    const uint32_t f = find_frame(F, n, i);
+__device__ __forceinline__ uint32_t frame_ray_tiled(const FrameDev* F, uint32_t n, uint32_t i, Ray& r,
+                                                    uint32_t& f) {
+    f = find_frame(F, n, i);
     const FrameDev& fr = F[f];
     uint32_t x, y;
     tile_pixel(i - fr.offset, fr.W, fr.H, x, y);
@@ -439,6 +440,98 @@ __device__ __forceinline__ void trace_one(const TraceParams& P, const uint4* lds
 
 // ---------------------------------------------------------- trace kernel --
 
+// FrameBuffer::replacePixel (FrameBuffer.hpp:196-217): min(1, c) * 255
+// rounded to nearest even, packed R | G << 8 | B << 16.
+__device__ __forceinline__ uint32_t to_u8(float c) {
+    c = c > 1.0f ? 1.0f : c;
+    c = c > 0.0f ? c : 0.0f;
+    return (uint32_t)__builtin_rintf(c * 255.0f);
+}
+
+// Lambert term of one light at hit point Pp with normal nn (fused
+// semantics, GPUTrace.cl:193-228); for a point light, L and its distance
+// are returned for the shadow ray.
+__device__ __forceinline__ float light_intensity(const float4& col, const float4& dat, const V3& Pp, const V3& nn,
+                                                 V3& L, float& dist) {
+    float intensity = 0.0f;
+    if (col.w == 0.0f) intensity = dot(nn, v3(dat.x, dat.y, dat.z));
+    if (col.w == 1.0f) {
+        L = sub(v3(dat.x, dat.y, dat.z), Pp);
+        const float dd = dot(L, L);
+        dist = __builtin_sqrtf(dd);
+        L = scale(L, 1.0f / __builtin_sqrtf(dd));
+        intensity = dot(nn, L);
+    }
+    return intensity;
+}
+
+// Whole-frame path of one ray (render kernel): the same values, in the same
+// order, as the queued pipeline (k_gen_secondary, the shadow trace,
+// k_shade_accumulate and the mirror trace per generation, then k_pack),
+// with every intermediate kept in registers.  Generation g shades its hit:
+// c = 0.1 + sum_k color_k * max(I_k, 0), a point light's I halved when its
+// shadow ray (origin P + 0.001 L, nearest hit, stopped once t < distance)
+// is blocked; acc += c * keep, keep = 0.2 w for a mirror hit that spawns a
+// reflection (r = d - 2 (d.n) n, origin P + 0.001 r, weight 0.8 w), else w.
+template <int VARIANT, bool COUNT>
+__device__ void render_ray(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk, uint32_t stride, Ray r,
+                           Best best, uint32_t max_bounce, Cnt& c, uint64_t& crays, uint64_t& cshadow,
+                           uint64_t& chits, float& ar, float& ag, float& ab) {
+    float w = 1.0f;
+    for (uint32_t g = 0;; g++) {
+        if (best.tri == kNoHit) break;
+        const V3 Pp = add(r.o, scale(r.d, best.t));
+        const V3 nn = v3(best.nx, best.ny, best.nz);
+        float cr = 0.1f, cg = 0.1f, cb = 0.1f;
+        for (uint32_t k = 0; k < P.n_lights; k++) {
+            const float4 col = P.lights[2 * k];
+            const float4 dat = P.lights[2 * k + 1];
+            V3 L = v3(0.0f, 0.0f, 0.0f);
+            float dist = 0.0f;
+            float intensity = light_intensity(col, dat, Pp, nn, L, dist);
+            if (col.w == 1.0f) {
+                Ray sr;
+                sr.o = add(Pp, scale(L, 0.001f));
+                sr.d = L;
+                set_inv(sr);
+                Best sb;
+                best_init(sb);
+                uint32_t sf = 0;
+                trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, sr, kFltMax, dist, sb, c, sf);
+                if (COUNT) {
+                    cshadow++;
+                    chits += sb.tri != kNoHit ? 1 : 0;
+                }
+                if (dist > sb.t) intensity *= 0.5f;
+            }
+            const float m = intensity > 0.0f ? intensity : 0.0f;
+            cr += col.x * m;
+            cg += col.y * m;
+            cb += col.z * m;
+        }
+        const bool spawn = P.mirror && g < max_bounce && P.mirror[best.tri];
+        const float keep = spawn ? w * 0.2f : w;
+        ar += cr * keep;
+        ag += cg * keep;
+        ab += cb * keep;
+        if (!spawn) break;
+        const V3 dn = normalize(r.d);
+        const float kk = 2.0f * dot(dn, nn);
+        const V3 rr = sub(dn, scale(nn, kk));
+        r.o = add(Pp, scale(rr, 0.001f));
+        r.d = rr;
+        set_inv(r);
+        w = w * 0.8f;
+        best_init(best);
+        uint32_t mf = 0;
+        trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, kFltMax, -kFltMax, best, c, mf);
+        if (COUNT) {
+            crays++;
+            chits += best.tri != kNoHit ? 1 : 0;
+        }
+    }
+}
+
 __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -448,7 +541,9 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
 // Generic trace kernel.  Ray source: the camera (one ray per pixel) or a
 // ray buffer (optionally a compacted queue whose length is read on the
 // device).  Output: a full hit record, or only t scattered to t_slot[i].
-template <int VARIANT, bool COUNT>
+// RENDER: camera / frame rays only; after the primary hit record each ray
+// runs its whole shading path (render_ray) and writes its packed pixel.
+template <int VARIANT, bool COUNT, bool RENDER>
 __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
     extern __shared__ uint4 smem[];
     const uint32_t tid = threadIdx.x;
@@ -464,7 +559,7 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
     const uint32_t n_rays = P.n_dev ? *P.n_dev : P.n;
     const uint32_t wave = tid >> 6, lane_id = tid & 63u;
     const uint32_t waves_per_block = block >> 6;
-    uint64_t cn = 0, cl = 0, ct = 0, cfb = 0, chits = 0, crays = 0;
+    uint64_t cn = 0, cl = 0, ct = 0, cfb = 0, chits = 0, crays = 0, cshadow = 0;
 
     // Static: XCD-aware chunk order (blocks are dispatched round-robin over
     // the 8 XCDs, so consecutive chunks go to blocks on the same XCD).
@@ -512,6 +607,7 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
         if (i < n_rays) {
             Ray r;
             uint32_t o = i;  // output / bound index: the pixel index for camera rays
+            uint32_t max_bounce = P.max_bounce;
             float tstop = -kFltMax;
             if (P.source == kSourceCamera) {
                 uint32_t x, y;
@@ -519,7 +615,9 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
                 o = y * P.W + x;
                 cast_ray(P.cam, x, y, P.W, P.H, r);
             } else if (P.source == kSourceFrames) {
-                o = frame_ray_tiled(P.frames, P.n_frames, i, r);
+                uint32_t f;
+                o = frame_ray_tiled(P.frames, P.n_frames, i, r, f);
+                if (RENDER) max_bounce = P.frames[f].max_bounce;
             } else {
                 const hulk_ray ray = P.rays[i];
                 r.o = v3(ray.start.x, ray.start.y, ray.start.z);
@@ -564,11 +662,19 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
                 h[1] = make_float4(best.nx, best.ny, best.nz, __uint_as_float(flags));
             }
             if (COUNT) {
+                chits += best.tri != kNoHit ? 1 : 0;
+                crays += 1;
+            }
+            if (RENDER) {
+                float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+                render_ray<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, best, max_bounce, c, crays, cshadow, chits,
+                                           ar, ag, ab);
+                P.rgba[o] = to_u8(ar) | (to_u8(ag) << 8) | (to_u8(ab) << 16);
+            }
+            if (COUNT) {
                 cn += c.nodes;
                 cl += c.leaves;
                 ct += c.tris;
-                chits += best.tri != kNoHit ? 1 : 0;
-                crays += 1;
             }
         }
         if (P.persistent) {
@@ -584,9 +690,11 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
         cfb = wave_sum(cfb);
         chits = wave_sum(chits);
         crays = wave_sum(crays);
+        cshadow = wave_sum(cshadow);
         if (lane_id == 0) {
             // rays with a full hit record (primary / mirror) vs shadow rays
             atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[P.t_out ? 6 : 0]), (unsigned long long)crays);
+            if (RENDER) atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[6]), (unsigned long long)cshadow);
             atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[1]), (unsigned long long)cn);
             atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[2]), (unsigned long long)cl);
             atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[3]), (unsigned long long)ct);
@@ -634,9 +742,10 @@ __global__ void k_build_records(const float4* __restrict__ verts, const uint4* _
 // to the next generation with weight w*0.8.
 template <bool WRITE>
 __global__ void __launch_bounds__(256) k_gen_secondary(GenParams G) {
+    __shared__ uint32_t wsum[2][4];
     const uint32_t n = G.n_dev ? *G.n_dev : G.n;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t wave = i >> 6, lane_id = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6, lane_id = threadIdx.x & 63u;
     const bool active = i < n;
     hulk_hit h;
     Ray r;
@@ -666,12 +775,23 @@ __global__ void __launch_bounds__(256) k_gen_secondary(GenParams G) {
     const bool mirror = hit && G.next_rays && may_spawn && G.mirror && G.mirror[h.tri];
     const uint64_t hit_mask = __ballot(hit);
     const uint64_t mir_mask = __ballot(mirror);
+    if (lane_id == 0) {
+        wsum[0][wave] = (uint32_t)__builtin_popcountll(hit_mask) * G.n_point;
+        wsum[1][wave] = (uint32_t)__builtin_popcountll(mir_mask);
+    }
+    __syncthreads();
     if (!WRITE) {
-        if (lane_id == 0) {
-            G.wave_counts[2 * wave] = (uint32_t)__builtin_popcountll(hit_mask) * G.n_point;
-            G.wave_counts[2 * wave + 1] = (uint32_t)__builtin_popcountll(mir_mask);
+        if (threadIdx.x == 0) {
+            G.block_counts[2 * blockIdx.x] = wsum[0][0] + wsum[0][1] + wsum[0][2] + wsum[0][3];
+            G.block_counts[2 * blockIdx.x + 1] = wsum[1][0] + wsum[1][1] + wsum[1][2] + wsum[1][3];
         }
         return;
+    }
+    // this wave's offsets: the block's scanned offset plus the earlier waves
+    uint32_t shadow_base = G.block_offsets[2 * blockIdx.x], mirror_base = G.block_offsets[2 * blockIdx.x + 1];
+    for (uint32_t k = 0; k < wave; k++) {
+        shadow_base += wsum[0][k];
+        mirror_base += wsum[1][k];
     }
     const uint32_t hit_rank =
         __builtin_amdgcn_mbcnt_hi((uint32_t)(hit_mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hit_mask, 0u));
@@ -680,7 +800,7 @@ __global__ void __launch_bounds__(256) k_gen_secondary(GenParams G) {
     const uint32_t n_hit = (uint32_t)__builtin_popcountll(hit_mask);
     if (!hit) return;
     const V3 P = add(r.o, scale(r.d, h.t));
-    const uint32_t sbase = G.wave_offsets[2 * wave];
+    const uint32_t sbase = shadow_base;
     for (uint32_t k = 0; k < G.n_point; k++) {
         const float4 lp = G.point_pos[k];
         const uint32_t j = sbase + k * n_hit + hit_rank;
@@ -696,7 +816,7 @@ __global__ void __launch_bounds__(256) k_gen_secondary(GenParams G) {
         G.shadow_slot[j] = i * G.n_point + k;
     }
     if (mirror) {
-        const uint32_t j = G.wave_offsets[2 * wave + 1] + mir_rank;
+        const uint32_t j = mirror_base + mir_rank;
         const V3 nn = v3(h.nx, h.ny, h.nz);
         const V3 dn = normalize(r.d);
         const float kk = 2.0f * dot(dn, nn);
@@ -711,19 +831,22 @@ __global__ void __launch_bounds__(256) k_gen_secondary(GenParams G) {
     }
 }
 
-// Exclusive scan of n pairs (shadow, mirror) of wave counts by one block:
-// each thread sums a contiguous run, the block scans the run sums, then
-// each thread writes its run's offsets.  Totals go to the queue counters.
-__global__ void __launch_bounds__(1024) k_scan_pairs(const uint32_t* counts, uint32_t n, uint32_t* offsets,
+// Exclusive scan of n pairs (shadow, mirror) of block counts by one block:
+// each thread sums a contiguous run (unrolled, so its loads are in flight
+// together), the block scans the run sums in LDS, then each thread writes
+// its run's offsets.  Totals go to the queue counters.
+__global__ void __launch_bounds__(1024) k_scan_pairs(const uint2* counts, uint32_t n, uint2* offsets,
                                                      uint32_t* total0, uint32_t* total1) {
     __shared__ uint32_t s0[1024], s1[1024];
     const uint32_t t = threadIdx.x, T = blockDim.x;
     const uint32_t per = (n + T - 1) / T;
-    const uint32_t a = t * per, b = min(n, a + per);
+    const uint32_t a = min(n, t * per), b = min(n, a + per);
     uint32_t c0 = 0, c1 = 0;
+#pragma unroll 8
     for (uint32_t k = a; k < b; k++) {
-        c0 += counts[2 * k];
-        c1 += counts[2 * k + 1];
+        const uint2 v = counts[k];
+        c0 += v.x;
+        c1 += v.y;
     }
     s0[t] = c0;
     s1[t] = c1;
@@ -736,11 +859,12 @@ __global__ void __launch_bounds__(1024) k_scan_pairs(const uint32_t* counts, uin
         __syncthreads();
     }
     uint32_t o0 = s0[t] - c0, o1 = s1[t] - c1;
+#pragma unroll 8
     for (uint32_t k = a; k < b; k++) {
-        offsets[2 * k] = o0;
-        offsets[2 * k + 1] = o1;
-        o0 += counts[2 * k];
-        o1 += counts[2 * k + 1];
+        const uint2 v = counts[k];
+        offsets[k] = make_uint2(o0, o1);
+        o0 += v.x;
+        o1 += v.y;
     }
     if (t == T - 1) {
         if (total0) *total0 = s0[t];
@@ -786,14 +910,10 @@ __global__ void k_shade_accumulate(ShadeParams S) {
     for (uint32_t k = 0; k < S.n_lights; k++) {
         const float4 col = S.lights[2 * k];
         const float4 dat = S.lights[2 * k + 1];
-        float intensity = 0.0f;
-        if (col.w == 0.0f) intensity = dot(nn, v3(dat.x, dat.y, dat.z));
+        V3 L = v3(0.0f, 0.0f, 0.0f);
+        float dist = 0.0f;
+        float intensity = light_intensity(col, dat, P, nn, L, dist);
         if (col.w == 1.0f) {
-            V3 L = sub(v3(dat.x, dat.y, dat.z), P);
-            const float dd = dot(L, L);
-            const float dist = __builtin_sqrtf(dd);
-            L = scale(L, 1.0f / __builtin_sqrtf(dd));
-            intensity = dot(nn, L);
             const float tsh = S.shadow_t[i * S.n_point + kp];
             if (dist > tsh) intensity *= 0.5f;
             kp++;
@@ -811,14 +931,6 @@ __global__ void k_shade_accumulate(ShadeParams S) {
     acc[0] += cr * keep;
     acc[1] += cg * keep;
     acc[2] += cb * keep;
-}
-
-// FrameBuffer::replacePixel (FrameBuffer.hpp:196-217): min(1, c) * 255
-// rounded to nearest even, packed R | G << 8 | B << 16.
-__device__ __forceinline__ uint32_t to_u8(float c) {
-    c = c > 1.0f ? 1.0f : c;
-    c = c > 0.0f ? c : 0.0f;
-    return (uint32_t)__builtin_rintf(c * 255.0f);
 }
 
 __global__ void k_pack(const float* __restrict__ acc, uint32_t n, uint32_t* __restrict__ out) {
@@ -891,26 +1003,34 @@ __global__ void k_camera_rays(Camera cam, uint32_t W, uint32_t H, SceneBox box, 
 // ---------------------------------------------------------- launchers --
 
 template <int V, bool C>
-static hipError_t launch_trace_t(const TraceParams& P, uint32_t grid, uint32_t block, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((k_trace<V, C>), dim3(grid), dim3(block), lds, s, P);
+static hipError_t launch_trace_t(const TraceParams& P, bool render, uint32_t grid, uint32_t block, size_t lds,
+                                 hipStream_t s) {
+    if (render) hipLaunchKernelGGL((k_trace<V, C, true>), dim3(grid), dim3(block), lds, s, P);
+    else hipLaunchKernelGGL((k_trace<V, C, false>), dim3(grid), dim3(block), lds, s, P);
     return hipGetLastError();
 }
 
 template <int V>
-static hipError_t launch_trace_v(const TraceParams& P, bool count, uint32_t grid, uint32_t block, size_t lds,
-                                 hipStream_t s) {
-    return count ? launch_trace_t<V, true>(P, grid, block, lds, s) : launch_trace_t<V, false>(P, grid, block, lds, s);
+static hipError_t launch_trace_v(const TraceParams& P, bool count, bool render, uint32_t grid, uint32_t block,
+                                 size_t lds, hipStream_t s) {
+    return count ? launch_trace_t<V, true>(P, render, grid, block, lds, s)
+                 : launch_trace_t<V, false>(P, render, grid, block, lds, s);
 }
 
-hipError_t launch_trace(int variant, const TraceParams& P, bool count, uint32_t grid, uint32_t block, size_t lds,
-                        hipStream_t s) {
+hipError_t launch_trace(int variant, const TraceParams& P, bool count, bool render, uint32_t grid, uint32_t block,
+                        size_t lds, hipStream_t s) {
     switch (variant) {
-        case HULK_VARIANT_BRUTEFORCE: return launch_trace_v<HULK_VARIANT_BRUTEFORCE>(P, count, grid, block, lds, s);
-        case HULK_VARIANT_FULLSTACK: return launch_trace_v<HULK_VARIANT_FULLSTACK>(P, count, grid, block, lds, s);
-        case HULK_VARIANT_SHORTSTACK: return launch_trace_v<HULK_VARIANT_SHORTSTACK>(P, count, grid, block, lds, s);
-        case HULK_VARIANT_LDSSTACK: return launch_trace_v<HULK_VARIANT_LDSSTACK>(P, count, grid, block, lds, s);
-        case HULK_VARIANT_RESTART: return launch_trace_v<HULK_VARIANT_RESTART>(P, count, grid, block, lds, s);
-        case HULK_VARIANT_ROPES: return launch_trace_v<HULK_VARIANT_ROPES>(P, count, grid, block, lds, s);
+        case HULK_VARIANT_BRUTEFORCE:
+            return launch_trace_v<HULK_VARIANT_BRUTEFORCE>(P, count, render, grid, block, lds, s);
+        case HULK_VARIANT_FULLSTACK:
+            return launch_trace_v<HULK_VARIANT_FULLSTACK>(P, count, render, grid, block, lds, s);
+        case HULK_VARIANT_SHORTSTACK:
+            return launch_trace_v<HULK_VARIANT_SHORTSTACK>(P, count, render, grid, block, lds, s);
+        case HULK_VARIANT_LDSSTACK:
+            return launch_trace_v<HULK_VARIANT_LDSSTACK>(P, count, render, grid, block, lds, s);
+        case HULK_VARIANT_RESTART:
+            return launch_trace_v<HULK_VARIANT_RESTART>(P, count, render, grid, block, lds, s);
+        case HULK_VARIANT_ROPES: return launch_trace_v<HULK_VARIANT_ROPES>(P, count, render, grid, block, lds, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -924,10 +1044,10 @@ hipError_t launch_build_records(const float4* verts, const uint4* tris, const ui
 
 hipError_t launch_gen_secondary(const GenParams& G, uint32_t n_max, hipStream_t s) {
     if (n_max == 0) return hipSuccess;
-    const uint32_t blocks = (n_max + 255) / 256, waves = blocks * 4;
+    const uint32_t blocks = (n_max + 255) / 256;
     hipLaunchKernelGGL(k_gen_secondary<false>, dim3(blocks), dim3(256), 0, s, G);
-    hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, s, G.wave_counts, waves, G.wave_offsets,
-                       G.shadow_count, G.next_count);
+    hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, s, reinterpret_cast<const uint2*>(G.block_counts),
+                       blocks, reinterpret_cast<uint2*>(G.block_offsets), G.shadow_count, G.next_count);
     hipLaunchKernelGGL(k_gen_secondary<true>, dim3(blocks), dim3(256), 0, s, G);
     return hipGetLastError();
 }

@@ -65,6 +65,13 @@ struct TraceParams {
     uint32_t* work_counter;
     uint32_t persistent;
     uint32_t fetch;
+    // whole-frame launch (render kernel): shading with point-light shadow
+    // rays and mirror generations per ray, packed pixels to rgba[o]
+    const float4* lights;  // 2 float4 per light
+    uint32_t n_lights;
+    const uint8_t* mirror;  // mirror flag per triangle, or null (no bounces)
+    uint32_t max_bounce;    // camera source; frames carry their own
+    uint32_t* rgba;
     uint32_t stop_at_w;  // shadow queue: stop once best.t < ray.start.w (the light distance)
 };
 
@@ -93,8 +100,8 @@ struct GenParams {
     uint32_t* pixel_out;
     float* weight_out;
     uint32_t* next_count;
-    uint32_t* wave_counts;   // 2 per wave of the count pass
-    uint32_t* wave_offsets;  // 2 per wave, exclusive scan of wave_counts
+    uint32_t* block_counts;   // 2 per 256-ray block of the count pass
+    uint32_t* block_offsets;  // 2 per block, exclusive scan of block_counts
 };
 
 struct ShadeParams {
@@ -118,8 +125,8 @@ struct ShadeParams {
     float* acc;
 };
 
-hipError_t launch_trace(int variant, const TraceParams& P, bool count, uint32_t grid, uint32_t block, size_t lds,
-                        hipStream_t s);
+hipError_t launch_trace(int variant, const TraceParams& P, bool count, bool render, uint32_t grid, uint32_t block,
+                        size_t lds, hipStream_t s);
 hipError_t launch_build_records(const float4* verts, const uint4* tris, const uint32_t* lp, uint32_t n,
                                 TriRec* recs, hipStream_t s);
 hipError_t launch_gen_secondary(const GenParams& G, uint32_t n_max, hipStream_t s);

@@ -186,11 +186,15 @@ class GPUTrace:
     """One libhulk context (one HIP stream, device buffers owned here)."""
 
     def __init__(self, variant: int | str = L.FULLSTACK, block_size: int = 256, short_stack_depth: int = 6,
-                 lds_top_levels: int = 0, persistent: bool = False, device: int = 0, stream: int | None = None):
+                 lds_top_levels: int = 0, persistent: bool = False, device: int = 0, stream: int | None = None,
+                 pipeline: int = 0):
+        """pipeline: 0 = automatic, 1 = queued shadow / mirror launches,
+        2 = one fused launch per frame (hulk_pipeline)."""
         if isinstance(variant, str):
             variant = L.VARIANTS[variant]
-        self.opts = L.Opts(device, variant, block_size, short_stack_depth, lds_top_levels, int(persistent),
-                           stream)
+        self.opts = L.Opts(device=device, variant=variant, block_size=block_size,
+                           short_stack_depth=short_stack_depth, lds_top_levels=lds_top_levels,
+                           persistent=int(persistent), pipeline=int(pipeline), stream=stream)
         self.h = C.c_void_p()
         _check(None, lib().hulk_create(C.byref(self.h), C.byref(self.opts)))
 

@@ -116,8 +116,21 @@ typedef struct hulk_opts {
     uint32_t short_stack_depth; /* 1..16, short-stack / LDS-stack variants   */
     uint32_t lds_top_levels;    /* 0..12: top tree levels staged in LDS      */
     uint32_t persistent;        /* 1 = persistent waves with atomic fetch    */
+    uint32_t pipeline;          /* hulk_pipeline: frame launch model         */
     void*    stream;            /* hipStream_t to use; NULL = own stream     */
 } hulk_opts;
+
+/* Launch model of a frame (render_frame, batches).  QUEUED: primary trace,
+ * then per generation compacted shadow and mirror queues, each traced by
+ * its own launch.  FUSED: one launch in which every ray runs its whole
+ * path (primary hit, shadow rays, mirror generations) in registers.  Both
+ * give identical pixels and hit records.  AUTO: FUSED for frames without
+ * mirror generations, QUEUED otherwise. */
+typedef enum hulk_pipeline {
+    HULK_PIPELINE_AUTO = 0,
+    HULK_PIPELINE_QUEUED = 1,
+    HULK_PIPELINE_FUSED = 2
+} hulk_pipeline;
 
 /* Call flags. */
 #define HULK_FLAG_DEVICE_PTRS  1u   /* all pointer args are device pointers */

@@ -17,12 +17,14 @@ def _frames(scene, specs):
     return [(scene.camera(pose, h / w), w, h, mb) for pose, w, h, mb in specs]
 
 
+@pytest.mark.parametrize("pipeline", [1, 2])
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_equals_single_frames(use_graph):
+def test_batch_equals_single_frames(use_graph, pipeline):
+    """Queued (1) and fused (2) frame pipelines, eager and graph-captured."""
     import torch
 
     s, nodes, lp = scene_and_tree(3, 6, 5, 4)
-    ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK, persistent=True)
+    ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK, persistent=True, pipeline=pipeline)
     specs = [(0, 64, 64, 0), (3, 128, 96, 2), (5, 96, 128, 1), (7, 192, 192, 3), (9, 64, 128, 2)]
     frames = _frames(s, specs)
     total = sum(w * h for _, w, h, _ in frames)
@@ -44,14 +46,15 @@ def test_batch_equals_single_frames(use_graph):
     b.close()
 
 
-def test_batch_matches_oracle_two_scenes():
+@pytest.mark.parametrize("pipeline", [1, 2])
+def test_batch_matches_oracle_two_scenes(pipeline):
     """Config-4 style: frames of the random-triangle and terrain scenes, one
     batch per scene, uneven sizes, bounce depths 0-3, against orc_render."""
     import torch
 
     for kind in (1, 2):
         s, nodes, lp = scene_and_tree(kind, 1)
-        ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK)
+        ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK, pipeline=pipeline)
         specs = [(1, 64, 64, 0), (4, 128, 64, 3), (12, 96, 96, 1)]
         frames = _frames(s, specs)
         total = sum(w * h for _, w, h, _ in frames)

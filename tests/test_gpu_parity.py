@@ -106,7 +106,9 @@ def test_shade_parity(variant, mode):
 
 
 @pytest.mark.parametrize("opts", [dict(variant=H.FULLSTACK), dict(variant=H.FULLSTACK, persistent=True),
-                                  dict(variant=H.ROPES, persistent=True, block_size=64)])
+                                  dict(variant=H.ROPES, persistent=True, block_size=64),
+                                  dict(variant=H.FULLSTACK, pipeline=2),
+                                  dict(variant=H.ROPES, pipeline=2, persistent=True, block_size=128)])
 def test_render_bounces_parity(opts):
     """Animated tori (5 meshes, 100,002 triangles): primary + shadow + 2
     mirror generations with per-generation compaction; persistent waves
@@ -122,6 +124,29 @@ def test_render_bounces_parity(opts):
     assert int(s.mirror.sum()) > 0
     flat, _ = ctx.render_frame(cam, 160, 160, max_bounce=0, mirror=s.mirror)
     assert (flat != img).sum() > 100, "mirror generations changed too few pixels"
+
+
+@pytest.mark.parametrize("variant", [H.FULLSTACK, H.LDSSTACK, H.RESTART, H.ROPES])
+def test_fused_pipeline_equals_queued(variant):
+    """One fused launch per frame against the queued launches: pixels,
+    primary hit records and every counter identical, with mirror
+    generations and a learned-style bound on the primary rays."""
+    s, nodes, lp = scene_and_tree(3, 6, 5, 4)
+    cam = s.camera(4, 1.0)
+    W = H_ = 128
+    plain = gpu_ctx(s, nodes, lp, variant).trace_camera(cam, W, H_)
+    tb = np.where(plain["tri"] != H.NO_HIT, plain["t"] * np.float32(1.3), np.float32(0.5)).astype(np.float32)
+    out = {}
+    for pipeline in (1, 2):
+        ctx = gpu_ctx(s, nodes, lp, variant, pipeline=pipeline, lds_top_levels=6)
+        img, hits = ctx.render_frame(cam, W, H_, max_bounce=2, mirror=s.mirror, tbound=tb, count=True)
+        out[pipeline] = (img, hits, ctx.counters())
+    (i1, h1, c1), (i2, h2, c2) = out[1], out[2]
+    assert np.array_equal(i1, i2)
+    assert np.array_equal(h1.view(np.uint32), h2.view(np.uint32))
+    for k in ("rays", "node_visits", "leaf_visits", "tri_tests", "hits", "fallbacks", "shadow_rays"):
+        assert c1[k] == c2[k], (k, c1[k], c2[k])
+    assert c1["shadow_rays"] > 0 and c1["rays"] > W * H_
 
 
 @pytest.mark.parametrize("n", [1, 63, 65, 1000, 40000])
