@@ -6,16 +6,19 @@ pinhole primary rays (nearest hit), one nearest-hit shadow ray per hit and
 point light, Lambert shading into RGBA8.  Inputs are resident in HBM before
 the timed region; outputs stay in HBM.
 
-Methods on the same 10 frames:
-  plain    - frame after frame (hulk_render_frame), autotuned variant
-  batched  - the 10 frames in one graph-captured pipeline (hulk_batch)
+Options (traversal variant, block size, stack / LDS depths, persistent
+waves, fused or queued frame pipeline) come from the autotuner, timing
+whole frames of pose 10 at the workload resolution.  Methods on the same
+10 frames:
+  plain    - frame after frame (hulk_render_frame)
+  batched  - the 10 frames in one graph-captured launch (hulk_batch)
   learned  - batched, with per-pixel bounds from the MLP trained on poses
              0-9 (inference inside the timed region; training is not)
-Every method's primary hits are compared with plain's on every ray.
-value = nearest-hit (primary) rays per second of the fastest method whose
-hits match 100%.  Extras: every variant's rate, the roofline of the
-primary trace kernel, the reference CPU path (port) on the host, and the
-animated 1M-triangle and 64-frame batched workloads.
+Every method's primary hits are compared with the plain trace's on every
+ray.  value = nearest-hit (primary) rays per second of the fastest method
+whose hits match 100%.  Extras: every variant's primary-trace rate, the
+roofline of the batched frame kernel, the reference CPU path (port) on the
+host, and the animated 1M-triangle and 64-frame batched workloads.
 """
 from __future__ import annotations
 
@@ -36,13 +39,23 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+OPT_KEYS = ("variant", "block_size", "short_stack_depth", "lds_top_levels", "persistent", "pipeline")
+
+
+def kernel_name(opts: dict) -> str:
+    """The batched launch's dominant kernel: the fused frame kernel, or the
+    queued pipeline's graph of trace, compaction and shading kernels."""
+    if int(opts.get("pipeline", 0)) == 1:
+        return "queued pipeline graph"
+    return f"k_trace<{int(opts['variant'])},false,true>"
+
+
 def profiled_traffic(kernel: str, opts: dict, W: int, H: int):
     """HBM bytes per launch of the primary trace kernel, as measured by the
     FETCH_SIZE and WRITE_SIZE counter runs whose summary is kept in
     profiles/ (line: traffic <kernel> <options> <W>x<H> <bytes>).  Only a
     profile of the same kernel, options and resolution counts; else None."""
-    want = ",".join(f"{k}={int(opts[k])}" for k in ("variant", "block_size", "short_stack_depth",
-                                                    "lds_top_levels", "persistent"))
+    want = ",".join(f"{k}={int(opts[k])}" for k in OPT_KEYS)
     for f in sorted((ROOT / "profiles").glob("*traffic*.txt")):
         for line in f.read_text().splitlines():
             parts = line.split()
@@ -118,16 +131,33 @@ def main() -> int:
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
+    frames = [(cam, W, Hh, 0) for cam in cams]
+    brgba = torch.empty(nf * n, dtype=torch.int32, device=dev)
+
+    def time_batch(batch, reps):
+        batch.launch(L.FLAG_NO_SYNC)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            batch.launch(L.FLAG_NO_SYNC)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
     if args.kernel_only:
+        # the batched launch of the 10 frames: one kernel when the pipeline is fused
         opts = {k: int(v) for k, v in (kv.split("=") for kv in args.kernel_only.split(","))}
         ctx = WL.make_ctx(scene, nodes, lp, stream, **opts)
+        batch = ctx.batch(frames, None, brgba.data_ptr(), use_graph=True)
         for _ in range(max(args.warmup - 1, 0)):
-            ctx.trace_camera_dev(cams[0], W, Hh, hits.data_ptr(), flags=L.FLAG_NO_SYNC)
-        ms = time_trace(ctx, cams[0], args.steps)
+            batch.launch(L.FLAG_NO_SYNC)
+        ms = time_batch(batch, args.steps)
+        batch.close()
         ctx.close()
-        print(json.dumps({"kernel": f"k_trace<{opts.get('variant', 1)},false>", "options": opts,
-                          "kernel_ms": round(ms, 4), "timed_launches": args.steps,
-                          "launches": args.steps + max(args.warmup, 1), "rays_per_launch": n}), flush=True)
+        print(json.dumps({"kernel": kernel_name(opts), "options": opts, "kernel_ms": round(ms, 4),
+                          "timed_launches": args.steps, "launches": args.steps + max(args.warmup, 1),
+                          "primary_rays_per_launch": nf * n}), flush=True)
         return 0
 
     # -- every variant at its default shape (primary trace of pose 10)
@@ -142,19 +172,24 @@ def main() -> int:
         log(f"  {name:24s} primary trace {ms:8.3f} ms  {n / ms / 1e3:9.1f} Mrays/s")
         ctx.close()
 
-    # -- autotuned options for this scene
+    # -- autotuned options for this scene: whole frames at the workload resolution
     ctx = WL.make_ctx(scene, nodes, lp, stream, variant=H.FULLSTACK)
     tt = time.time()
-    best, table = ctx.autotune(cams[0], 512, 512, reps=2)
+    best, table = ctx.autotune(cams[0], W, Hh, reps=2, frame=True)
     log(f"autotune {time.time() - tt:.1f}s over {len(table)} shapes -> {best}")
 
-    # -- algorithmic bytes per ray: full-stack node / triangle counts on pose 10
+    # -- algorithmic bytes of one batched launch (10 frames): full-stack
+    # counts of every traced ray (primary and shadow), 32 B ray + 16 B
+    # result each, 16 B per node visit, 64 B per triangle test
     cnt = WL.make_ctx(scene, nodes, lp, stream, variant=H.FULLSTACK)
-    cnt.trace_camera(cams[0], W, Hh, count=True)
+    cbatch = cnt.batch(frames, None, brgba.data_ptr(), use_graph=False)
+    cbatch.launch(L.FLAG_COUNT)
     c = cnt.counters()
+    cbatch.close()
     cnt.close()
-    n_node, n_tri = c["node_visits"] / c["rays"], c["tri_tests"] / c["rays"]
-    b_ray = 32 + 16 + 16 * n_node + 64 * n_tri
+    traced = c["rays"] + c["shadow_rays"]
+    bytes_launch = 48 * traced + 16 * c["node_visits"] + 64 * c["tri_tests"]
+    n_node, n_tri = c["node_visits"] / traced, c["tri_tests"] / traced
 
     # reference hits of every frame (plain, tuned options)
     ref_hits = [ctx.trace_camera(cam, W, Hh) for cam in cams]
@@ -190,12 +225,13 @@ def main() -> int:
                         "hit_match": 1.0}
 
     # batched: one graph for the 10 frames
-    frames = [(cam, W, Hh, 0) for cam in cams]
-    brgba = torch.empty(nf * n, dtype=torch.int32, device=dev)
     batch = ctx.batch(frames, None, brgba.data_ptr(), use_graph=True)
     s_batch = timed(lambda: batch.launch(L.FLAG_NO_SYNC), args.steps)
     methods["batched"] = {"ms_per_step": round(s_batch * 1e3, 3), "mrays_s": round(nf * n / s_batch / 1e6, 1),
                           "hit_match": hits_match(batch.primary_hits())}
+    # -- the batched launch is the dominant kernel (the whole step when the
+    # pipeline is fused): live HIP events on its stream
+    k_ms = time_batch(batch, max(args.steps, 10))
     batch.close()
 
     # learned + batched, on the tuned variant and on the reference's live
@@ -278,14 +314,12 @@ def main() -> int:
     best_m = max(ok, key=lambda k: ok[k]["mrays_s"])
     s_best = methods[best_m]["ms_per_step"] / 1e3
 
-    # -- dominant kernel: primary trace with the tuned options, live events
-    k_ms = time_trace(ctx, cams[0], max(args.steps, 10))
-    achieved = n * b_ray / (k_ms * 1e-3) / 1e9
-    log(f"primary trace {k_ms:.3f} ms, B_ray {b_ray:.1f} B (nodes {n_node:.2f}, tris {n_tri:.2f}), "
-        f"{achieved:.0f} GB/s algorithmic")
+    achieved = bytes_launch / (k_ms * 1e-3) / 1e9
+    log(f"batched launch {k_ms:.3f} ms, {bytes_launch / 1e9:.2f} GB algorithmic ({traced} rays, "
+        f"nodes {n_node:.2f} / tris {n_tri:.2f} per ray), {achieved:.0f} GB/s")
     tuned_name = L.VARIANT_NAMES[best["variant"]]
-    kernel_name = f"k_trace<{best['variant']},false>"
-    traffic, traffic_src = profiled_traffic(kernel_name, best, W, Hh)
+    kname = kernel_name(best)
+    traffic, traffic_src = profiled_traffic(kname, best, W, Hh)
 
     result = {
         "metric": "Mrays/s nearest-hit @1024^2, learned-bound+batched vs plain, 100% hit match",
@@ -310,8 +344,9 @@ def main() -> int:
         "learned": learned_info,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": kernel_name, "kernel_ms": round(k_ms, 4),
-                     "rays_per_launch": n, "bytes_per_ray": round(b_ray, 1),
+                     "kernel": kname, "kernel_ms": round(k_ms, 4), "launch": "10-frame batch (graph)",
+                     "primary_rays_per_launch": nf * n, "traced_rays_per_launch": int(traced),
+                     "algorithmic_bytes_per_launch": int(bytes_launch),
                      "node_visits_per_ray": round(n_node, 3), "tri_tests_per_ray": round(n_tri, 3)},
         "cpu_baseline": None,
     }

@@ -30,6 +30,7 @@ FLAG_ANYHIT = 2
 FLAG_COUNT = 4
 FLAG_NO_SYNC = 8
 PIPELINE_AUTO, PIPELINE_QUEUED, PIPELINE_FUSED = range(3)
+TUNE_TRACE, TUNE_FRAME = 0, 1
 SHADE_FUSED = 0
 SHADE_PICTURE = 1
 NO_HIT = 0xFFFFFFFF
@@ -75,7 +76,8 @@ class FrameDesc(C.Structure):
 
 class TuneEntry(C.Structure):
     _fields_ = [("variant", C.c_int32), ("block_size", C.c_uint32), ("short_stack_depth", C.c_uint32),
-                ("lds_top_levels", C.c_uint32), ("persistent", C.c_uint32), ("ms", C.c_float)]
+                ("lds_top_levels", C.c_uint32), ("persistent", C.c_uint32), ("pipeline", C.c_uint32),
+                ("ms", C.c_float)]
 
 
 EXPORTS = {
@@ -106,7 +108,8 @@ EXPORTS = {
     "hulk_batch_destroy": (None, [C.c_void_p]),
     "hulk_batch_primary_hits": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "hulk_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                C.POINTER(TuneEntry), C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(Opts)]),
+                                C.c_uint32, C.POINTER(TuneEntry), C.c_uint32, C.POINTER(C.c_uint32),
+                                C.POINTER(Opts)]),
     "hulk_mlp_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
     "hulk_mlp_destroy": (None, [C.c_void_p]),
     "hulk_mlp_last_error": (C.c_char_p, [C.c_void_p]),

@@ -59,9 +59,8 @@ struct hulk_ctx {
     uint32_t n_nodes = 0, n_lp = 0, max_jdepth = 0;
     uint32_t level_prefix[kTopLevelsMax + 1] = {0};
     bool have_tree = false;
-    std::vector<hulk_kd_node> host_nodes;  // original order, for ropes
     bool ropes_ready = false;
-    DevBuf rope_nodes, rope_slots;
+    DevBuf rope_nodes, rope_slots, rope_links;
     uint32_t n_slots = 0;
 
     // lights
@@ -132,13 +131,8 @@ int check_opts(hulk_ctx* c, const hulk_opts& o) {
     return HULK_OK;
 }
 
-// Frame launch model of hulk_pipeline (AUTO: fused without mirror
-// generations).
-bool use_fused(const hulk_opts& o, bool bounces) {
-    if (o.pipeline == HULK_PIPELINE_FUSED) return true;
-    if (o.pipeline == HULK_PIPELINE_QUEUED) return false;
-    return !bounces;
-}
+// Frame launch model of hulk_pipeline (AUTO: fused).
+bool use_fused(const hulk_opts& o, bool /*bounces*/) { return o.pipeline != HULK_PIPELINE_QUEUED; }
 
 Camera to_cam(const hulk_camera& c) {
     Camera k;
@@ -149,88 +143,19 @@ Camera to_cam(const hulk_camera& c) {
     return k;
 }
 
-// Rope slots for the rope variant: slot s < n_nodes = tree node s, empty
-// children become virtual empty leaves numbered from n_nodes.  Junction
-// children are rewritten to slots; ropes: the above child's min face on the
-// split axis points at the below child and vice versa, everything else is
-// inherited from the parent; the root's ropes are kRopeNone.
-void build_ropes(const std::vector<hulk_kd_node>& nodes, const SceneBox& box, std::vector<uint4>& rn,
-                 std::vector<RopeSlot>& rs) {
-    const uint32_t n = static_cast<uint32_t>(nodes.size());
-    rn.assign(n, make_uint4(0, 0, 0, 0));
-    rs.assign(n, RopeSlot{});
-    struct Item {
-        uint32_t slot;
-        float lo[3], hi[3];
-        uint32_t r[6];
-    };
-    std::vector<Item> st;
-    Item root;
-    root.slot = 0;
-    for (int a = 0; a < 3; a++) {
-        root.lo[a] = box.mn[a];
-        root.hi[a] = box.mx[a];
-    }
-    for (int k = 0; k < 6; k++) root.r[k] = kRopeNone;
-    st.push_back(root);
-    while (!st.empty()) {
-        Item it = st.back();
-        st.pop_back();
-        RopeSlot s{};
-        s.lo = make_float4(it.lo[0], it.lo[1], it.lo[2], 0.0f);
-        s.hi = make_float4(it.hi[0], it.hi[1], it.hi[2], 0.0f);
-        for (int k = 0; k < 6; k++) s.rope[k] = it.r[k];
-        if (it.slot >= rs.size()) {
-            rs.resize(it.slot + 1);
-            rn.resize(it.slot + 1, make_uint4(0, 0, 0, 0));
-        }
-        rs[it.slot] = s;
-        if (it.slot >= n) {
-            rn[it.slot] = make_uint4(kNodeLeaf, 0, 0, 0);
-            continue;
-        }
-        const hulk_kd_node& nd = nodes[it.slot];
-        if (!(nd.flags & kNodeJunction)) {
-            rn[it.slot] = make_uint4(nd.flags, nd.u.leaf.start, nd.u.leaf.count, nd.u.leaf.leaf_id);
-            continue;
-        }
-        const uint32_t a = nd.flags & 3u;
-        uint32_t above = nd.u.junction.above, below = nd.u.junction.below;
-        uint32_t next = static_cast<uint32_t>(std::max<size_t>(rs.size(), n));
-        if (above == 0) above = next++;
-        if (below == 0) below = next++;
-        if (next > rs.size()) {
-            rs.resize(next);
-            rn.resize(next, make_uint4(0, 0, 0, 0));
-        }
-        uint32_t plane_bits;
-        std::memcpy(&plane_bits, &nd.u.junction.plane, 4);
-        rn[it.slot] = make_uint4(nd.flags, above, below, plane_bits);
-        Item ia = it, ib = it;
-        ia.slot = above;
-        ib.slot = below;
-        ia.lo[a] = nd.u.junction.plane;
-        ib.hi[a] = nd.u.junction.plane;
-        ia.r[2 * a] = below;
-        ib.r[2 * a + 1] = above;
-        st.push_back(ib);
-        st.push_back(ia);
-    }
-}
-
+// Rope slots for the rope variant, built on the device (launch_build_ropes)
+// the first time a trace needs them after a tree upload.
 int ensure_ropes(hulk_ctx* c) {
     if (c->ropes_ready) return HULK_OK;
-    std::vector<uint4> rn;
-    std::vector<RopeSlot> rs;
-    build_ropes(c->host_nodes, c->box, rn, rs);
-    c->n_slots = static_cast<uint32_t>(rn.size());
+    const uint32_t n_slots = 3 * c->n_nodes;
     int r;
-    if ((r = ensure(c, c->rope_nodes, rn.size() * sizeof(uint4)))) return r;
-    if ((r = ensure(c, c->rope_slots, rs.size() * sizeof(RopeSlot)))) return r;
-    HIP_TRY(c, hipMemcpyAsync(c->rope_nodes.p, rn.data(), rn.size() * sizeof(uint4), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->rope_slots.p, rs.data(), rs.size() * sizeof(RopeSlot), hipMemcpyHostToDevice,
-                              c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if ((r = ensure(c, c->rope_nodes, (size_t)n_slots * sizeof(uint4)))) return r;
+    if ((r = ensure(c, c->rope_slots, (size_t)n_slots * sizeof(RopeSlot)))) return r;
+    if ((r = ensure(c, c->rope_links, (size_t)n_slots * 4))) return r;
+    HIP_TRY(c, launch_build_ropes(static_cast<const uint4*>(c->nodes.p), c->n_nodes,
+                                  static_cast<uint32_t*>(c->rope_links.p), static_cast<uint4*>(c->rope_nodes.p),
+                                  static_cast<RopeSlot*>(c->rope_slots.p), c->box, c->stream));
+    c->n_slots = n_slots;
     c->ropes_ready = true;
     return HULK_OK;
 }
@@ -620,7 +545,7 @@ extern "C" int hulk_create(hulk_ctx** out, const hulk_opts* opts) {
 extern "C" void hulk_destroy(hulk_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DevBuf* bufs[] = {&c->verts, &c->tris, &c->nodes, &c->lp, &c->recs, &c->rope_nodes, &c->rope_slots,
+    DevBuf* bufs[] = {&c->verts, &c->tris, &c->nodes, &c->lp, &c->recs, &c->rope_nodes, &c->rope_slots, &c->rope_links,
                       &c->d_lights, &c->d_point, &c->rays, &c->tbound, &c->rgba, &c->mirror, &c->stats, &c->hits_io};
     for (DevBuf* b : bufs) release(*b);
     release_work(c->work);
@@ -687,6 +612,7 @@ extern "C" int hulk_upload_kdtree(hulk_ctx* c, const hulk_kd_node* nodes, uint32
     if (!c) return HULK_E_ILLEGAL_INPUT;
     if (!c->have_scene) return fail(c, HULK_E_NO_SCENE, "upload the scene before the tree");
     if (!nodes || n_nodes == 0 || (n_lp && !leaf_polys)) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    if (n_nodes >= (1u << 30)) return fail(c, HULK_E_ILLEGAL_INPUT, "more than 2^30 nodes");
     // Validate: children in range, every node reached once from the root
     // (no cycles / sharing), leaf ranges inside leaf_polys, references < n_tris.
     std::vector<uint8_t> seen(n_nodes, 0);
@@ -738,10 +664,11 @@ extern "C" int hulk_upload_kdtree(hulk_ctx* c, const hulk_kd_node* nodes, uint32
                 dfs.pop_back();
                 perm[k] = static_cast<uint32_t>(order.size());
                 order.push_back(k);
-                std::vector<uint32_t> kids;
-                int r = visit_children(k, [&](uint32_t ch) { kids.push_back(ch); });
+                uint32_t kids[2];
+                int nk = 0;
+                int r = visit_children(k, [&](uint32_t ch) { kids[nk++] = ch; });
                 if (r) return r;
-                for (auto it = kids.rbegin(); it != kids.rend(); ++it) dfs.push_back(*it);
+                while (nk > 0) dfs.push_back(kids[--nk]);
             }
         }
     }
@@ -781,7 +708,6 @@ extern "C" int hulk_upload_kdtree(hulk_ctx* c, const hulk_kd_node* nodes, uint32
     c->n_lp = n_lp;
     c->max_jdepth = max_jdepth;
     std::memcpy(c->level_prefix, prefix, sizeof(prefix));
-    c->host_nodes.assign(nodes, nodes + n_nodes);
     c->ropes_ready = false;
     c->have_tree = true;
     return HULK_OK;
@@ -1163,7 +1089,7 @@ namespace {
 
 struct Cand {
     int32_t variant;
-    uint32_t block, depth, lds, persistent;
+    uint32_t block, depth, lds, persistent, pipeline;
 };
 
 }  // namespace
@@ -1174,9 +1100,12 @@ struct Cand {
 // default shape, stage 2 sweeps the shape of the two fastest variants.
 // The fastest candidate becomes the context's options.
 extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H, uint32_t reps,
-                             uint32_t variant_mask, hulk_tune_entry* table, uint32_t cap, uint32_t* n_out,
-                             hulk_opts* best_out) {
+                             uint32_t variant_mask, uint32_t objective, hulk_tune_entry* table, uint32_t cap,
+                             uint32_t* n_out, hulk_opts* best_out) {
     if (!c || !cam || !W || !H) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    if (objective > HULK_TUNE_FRAME) return fail(c, HULK_E_ILLEGAL_INPUT, "unknown autotune objective");
+    if ((uint64_t)W * H > 0xFFFFFFC0ull) return fail(c, HULK_E_ILLEGAL_INPUT, "image too large");
+    const bool frame = objective == HULK_TUNE_FRAME;
     if (!variant_mask) variant_mask = (1u << HULK_VARIANT_FULLSTACK) | (1u << HULK_VARIANT_SHORTSTACK) |
                                       (1u << HULK_VARIANT_LDSSTACK) | (1u << HULK_VARIANT_RESTART) |
                                       (1u << HULK_VARIANT_ROPES);
@@ -1188,8 +1117,11 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
     fs.H = H;
     fs.n = n;
     int r;
-    if ((r = ensure_work(c, c->work, n, c->n_point, false))) return r;
+    if ((r = ensure_work(c, c->work, n, c->n_point, false, frame))) return r;
+    if (frame && (r = ensure(c, c->rgba, (size_t)n * 4))) return r;
     hulk_hit* out = static_cast<hulk_hit*>(c->work.hits.p);
+    uint32_t* px = static_cast<uint32_t*>(c->rgba.p);
+    const uint32_t pipe0 = c->opts.pipeline == HULK_PIPELINE_QUEUED ? HULK_PIPELINE_QUEUED : HULK_PIPELINE_FUSED;
     std::vector<std::pair<Cand, float>> done;
     auto measure = [&](const Cand& k, float& ms) -> int {
         hulk_opts o = c->opts;
@@ -1198,15 +1130,22 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
         o.short_stack_depth = k.depth;
         o.lds_top_levels = k.lds;
         o.persistent = k.persistent;
+        o.pipeline = k.pipeline;
         TraceParams P;
         uint32_t block;
         size_t lds;
         if (check_opts(c, o) || trace_setup(c, o, P, block, lds)) return 1;  // shape not allowed: skip
+        auto once = [&]() -> int {
+            if (!frame) return trace_primary(c, o, fs, c->work, nullptr, out, false, false);
+            if (use_fused(o, false)) return render_fused(c, o, fs, c->work, nullptr, out, 0, nullptr, px, false, false);
+            int q = trace_primary(c, o, fs, c->work, nullptr, out, false, false);
+            return q ? q : shade_pipeline(c, o, fs, c->work, out, 0, nullptr, px, false);
+        };
         int rr;
-        if ((rr = trace_primary(c, o, fs, c->work, nullptr, out, false, false))) return rr;
+        if ((rr = once())) return rr;
         HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
         for (uint32_t i = 0; i < reps; i++)
-            if ((rr = trace_primary(c, o, fs, c->work, nullptr, out, false, false))) return rr;
+            if ((rr = once())) return rr;
         HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -1218,7 +1157,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
     std::vector<std::pair<float, Cand>> firsts;
     for (int v = HULK_VARIANT_FULLSTACK; v < HULK_VARIANT_COUNT; v++) {
         if (!(variant_mask & (1u << v))) continue;
-        Cand k{v, 256, 6, 0, 0};
+        Cand k{v, 256, 6, 0, 0, pipe0};
         float ms;
         r = measure(k, ms);
         if (r > 1) return r;
@@ -1240,7 +1179,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
                 for (uint32_t ld : ldss) {
                     if (!ldsv && ld) continue;
                     for (uint32_t ps = 0; ps < 2; ps++) {
-                        Cand k{v, bs, dp, ld, ps};
+                        Cand k{v, bs, dp, ld, ps, pipe0};
                         if (bs == 256 && dp == 6 && ld == 0 && ps == 0) continue;  // stage 1
                         // persistent + LDS top tree pays off; skip the
                         // non-persistent LDS shapes beyond depth 8
@@ -1255,12 +1194,21 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
     size_t bi = 0;
     for (size_t i = 1; i < done.size(); i++)
         if (done[i].second < done[bi].second) bi = i;
+    if (frame) {  // stage 3: the other frame pipeline on the best shape
+        Cand k = done[bi].first;
+        k.pipeline = k.pipeline == HULK_PIPELINE_FUSED ? HULK_PIPELINE_QUEUED : HULK_PIPELINE_FUSED;
+        float ms;
+        r = measure(k, ms);
+        if (r > 1) return r;
+        if (r == 0 && ms < done[bi].second) bi = done.size() - 1;
+    }
     const Cand& b = done[bi].first;
     c->opts.variant = b.variant;
     c->opts.block_size = b.block;
     c->opts.short_stack_depth = b.depth;
     c->opts.lds_top_levels = b.lds;
     c->opts.persistent = b.persistent;
+    if (frame) c->opts.pipeline = b.pipeline;
     if (best_out) *best_out = c->opts;
     if (table) {
         for (size_t i = 0; i < done.size() && i < cap; i++) {
@@ -1269,6 +1217,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
             table[i].short_stack_depth = done[i].first.depth;
             table[i].lds_top_levels = done[i].first.lds;
             table[i].persistent = done[i].first.persistent;
+            table[i].pipeline = done[i].first.pipeline;
             table[i].ms = done[i].second;
         }
     }

@@ -1000,6 +1000,82 @@ __global__ void k_camera_rays(Camera cam, uint32_t W, uint32_t H, SceneBox box, 
     }
 }
 
+// ---------------------------------------------------------------- ropes --
+
+// Rope slots of the rope variant, built on the device from the uploaded
+// node array.  Slot s < n is node s; the empty above / below child of
+// junction i becomes the empty leaf slot n + 2i / n + 2i + 1.  A child's
+// cell is its parent's cell with the split plane as its min (above) or max
+// (below) on the split axis, and its rope on that face points at its
+// sibling; every other face is inherited.  So for a slot the deepest
+// ancestor split on each (axis, side) fixes that bound and rope, found by
+// walking up the parent links; faces no ancestor fixes keep the scene box
+// and kRopeNone.
+constexpr uint32_t kLinkRoot = 0xFFFFFFFFu, kLinkUnused = 0xFFFFFFFEu;
+
+__global__ void k_rope_links(const uint4* __restrict__ nodes, uint32_t n, uint32_t* __restrict__ up,
+                             uint4* __restrict__ rn) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4 nd = nodes[i];
+    if (i == 0) up[0] = kLinkRoot;
+    if (nd.x & kNodeJunction) {
+        const uint32_t a = nd.y ? nd.y : n + 2 * i, b = nd.z ? nd.z : n + 2 * i + 1;
+        rn[i] = make_uint4(nd.x, a, b, nd.w);
+        up[a] = i;
+        up[b] = i | 0x80000000u;
+        if (!nd.y) rn[a] = make_uint4(kNodeLeaf, 0, 0, 0);
+        if (!nd.z) rn[b] = make_uint4(kNodeLeaf, 0, 0, 0);
+    } else {
+        rn[i] = nd;
+    }
+}
+
+__global__ void k_rope_slots(const uint4* __restrict__ rn, const uint32_t* __restrict__ up, uint32_t n_slots,
+                             SceneBox box, RopeSlot* __restrict__ rs) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots) return;
+    uint32_t link = up[s];
+    if (link == kLinkUnused) return;
+    float lo[3] = {box.mn[0], box.mn[1], box.mn[2]}, hi[3] = {box.mx[0], box.mx[1], box.mx[2]};
+    uint32_t rope[6] = {kRopeNone, kRopeNone, kRopeNone, kRopeNone, kRopeNone, kRopeNone};
+    uint32_t fixed = 0;  // bit f: face f (2a = min, 2a + 1 = max on axis a) already set by a deeper split
+    for (uint32_t step = 0; link != kLinkRoot && step < 2 * kMaxFullStack; step++) {
+        const uint32_t p = link & 0x7FFFFFFFu;
+        const uint4 pn = rn[p];
+        const uint32_t a = pn.x & 3u;
+        const uint32_t f = (link >> 31) ? 2 * a + 1 : 2 * a;  // below child: max face; above child: min face
+        if (!(fixed & (1u << f))) {
+            fixed |= 1u << f;
+            const float v = __uint_as_float(pn.w);
+            if (f & 1u) {
+                hi[a] = v;
+                rope[f] = pn.y;  // the above sibling
+            } else {
+                lo[a] = v;
+                rope[f] = pn.z;  // the below sibling
+            }
+        }
+        link = up[p];
+    }
+    RopeSlot r;
+    r.lo = make_float4(lo[0], lo[1], lo[2], 0.0f);
+    r.hi = make_float4(hi[0], hi[1], hi[2], 0.0f);
+    for (int k = 0; k < 6; k++) r.rope[k] = rope[k];
+    r.pad[0] = r.pad[1] = 0;
+    rs[s] = r;
+}
+
+hipError_t launch_build_ropes(const uint4* nodes, uint32_t n, uint32_t* up, uint4* rn, RopeSlot* rs,
+                              const SceneBox& box, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint32_t n_slots = 3 * n;
+    hipLaunchKernelGGL(k_fill_u32, dim3((n_slots + 255) / 256), dim3(256), 0, s, up, (size_t)n_slots, kLinkUnused);
+    hipLaunchKernelGGL(k_rope_links, dim3((n + 255) / 256), dim3(256), 0, s, nodes, n, up, rn);
+    hipLaunchKernelGGL(k_rope_slots, dim3((n_slots + 255) / 256), dim3(256), 0, s, rn, up, n_slots, box, rs);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------- launchers --
 
 template <int V, bool C>

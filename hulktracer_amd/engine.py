@@ -297,8 +297,10 @@ class GPUTrace:
                                         int(use_graph), C.byref(hb)))
         return Batch(self, hb)
 
-    def autotune(self, cam: np.ndarray, W: int, H: int, reps: int = 3, variants=None):
-        """Per-scene search (hulk_autotune); returns (best options dict, table)."""
+    def autotune(self, cam: np.ndarray, W: int, H: int, reps: int = 3, variants=None, frame: bool = False):
+        """Per-scene search (hulk_autotune); returns (best options dict, table).
+        frame=True times whole frames (primary + shadow + shading, fused and
+        queued pipelines) instead of the primary trace."""
         mask = 0
         for v in variants or []:
             mask |= 1 << (L.VARIANTS[v] if isinstance(v, str) else int(v))
@@ -306,8 +308,9 @@ class GPUTrace:
         table = (L.TuneEntry * cap)()
         n = C.c_uint32()
         best = L.Opts()
-        self._c(lib().hulk_autotune(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, reps, mask, table,
-                                    cap, C.byref(n), C.byref(best)))
+        self._c(lib().hulk_autotune(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, reps, mask,
+                                    L.TUNE_FRAME if frame else L.TUNE_TRACE, table, cap, C.byref(n),
+                                    C.byref(best)))
         self.opts = best
         rows = [{f: getattr(table[i], f) for f, _ in L.TuneEntry._fields_} for i in range(n.value)]
         return {f: getattr(best, f) for f, _ in L.Opts._fields_ if f != "stream"}, rows

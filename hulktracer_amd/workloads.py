@@ -54,7 +54,8 @@ def animated(frames: int = 20, W: int = 1024, H: int = 1024, max_bounce: int = 2
     """Config: 50 tori x 20,000 triangles + ground plane, rigid motion per
     frame, KD tree rebuilt on the CPU each frame; the build of frame k+1
     runs while the GPU renders frame k.  Traversal options are autotuned
-    once on frame 0 (512x512) and kept for every frame."""
+    once on frame 0 (whole frames at the workload resolution) and kept for
+    every frame."""
     torch = _torch()
     stream = stream or torch.cuda.Stream()
     rgba = torch.empty(W * H, dtype=torch.int32, device="cuda")
@@ -66,7 +67,7 @@ def animated(frames: int = 20, W: int = 1024, H: int = 1024, max_bounce: int = 2
     cam = scene.camera(0, H / W)
     tuned = None
     if autotune:
-        tuned, _ = ctx.autotune(cam, 512, 512, reps=2)
+        tuned, _ = ctx.autotune(cam, W, H, reps=2, frame=True)
     # ray counts of frame 0 (counted run, not timed)
     ctx.reset_counters()
     ctx.render_frame_dev(cam, W, H, max_bounce, mirror.data_ptr(), rgba.data_ptr(), hits.data_ptr(),
@@ -138,7 +139,7 @@ def batched(stream=None, specs=None, reps: int = 3, autotune: bool = True, log=p
         ctx = make_ctx(scene, nodes, lp, stream, variant=L.FULLSTACK)
         tuned = None
         if autotune:
-            best, table = ctx.autotune(scene.camera(mine[0][1], 1.0), 512, 512, reps=2)
+            best, table = ctx.autotune(scene.camera(mine[0][1], 1.0), 512, 512, reps=2, frame=True)
             tuned = best
         mirror = scene.mirror
         frames = [(scene.camera(p, 1.0), r, r, mb) for _, p, r, mb in mine]

@@ -124,8 +124,7 @@ typedef struct hulk_opts {
  * then per generation compacted shadow and mirror queues, each traced by
  * its own launch.  FUSED: one launch in which every ray runs its whole
  * path (primary hit, shadow rays, mirror generations) in registers.  Both
- * give identical pixels and hit records.  AUTO: FUSED for frames without
- * mirror generations, QUEUED otherwise. */
+ * give identical pixels and hit records.  AUTO: FUSED. */
 typedef enum hulk_pipeline {
     HULK_PIPELINE_AUTO = 0,
     HULK_PIPELINE_QUEUED = 1,
@@ -267,18 +266,25 @@ void hulk_batch_destroy(hulk_batch* batch);
 
 typedef struct hulk_tune_entry {
     int32_t variant;
-    uint32_t block_size, short_stack_depth, lds_top_levels, persistent;
-    float ms;                 /* primary-trace time per launch */
+    uint32_t block_size, short_stack_depth, lds_top_levels, persistent, pipeline;
+    float ms;                 /* time per launch of the tuned objective */
 } hulk_tune_entry;
 
-/* Per-scene autotuner: times the primary trace of `cam` over traversal
- * variant (variant_mask bit per hulk_variant, 0 = all), block size 64-256,
+/* Autotuner objectives. */
+#define HULK_TUNE_TRACE 0u   /* the primary trace of the camera            */
+#define HULK_TUNE_FRAME 1u   /* the whole frame (primary, shadow rays,
+                                shading) as hulk_render_frame runs it
+                                without mirror generations; the fused and
+                                queued pipelines are both tried            */
+
+/* Per-scene autotuner: times `objective` for `cam` over traversal variant
+ * (variant_mask bit per hulk_variant, 0 = all), block size 64-256,
  * short-stack depth 4-16 and LDS top-tree depth 0-12, persistent or not,
  * and makes the fastest the context's options.  No reference counterpart
  * (the reference picks the kernel by --kernelType). */
 int hulk_autotune(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height, uint32_t reps,
-                  uint32_t variant_mask, hulk_tune_entry* table, uint32_t table_cap, uint32_t* n_entries,
-                  hulk_opts* best);
+                  uint32_t variant_mask, uint32_t objective, hulk_tune_entry* table, uint32_t table_cap,
+                  uint32_t* n_entries, hulk_opts* best);
 
 /* ----------------------------------------------- learned distance bound -- */
 

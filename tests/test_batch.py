@@ -85,3 +85,18 @@ def test_autotune_picks_a_parity_clean_config():
     var = {1: O.V_FULLSTACK, 2: O.V_SHORTSTACK, 3: O.V_LDSSTACK, 4: O.V_RESTART, 5: O.V_ROPES}[best["variant"]]
     ref = oracle_scene(2, 1).trace_camera(var, cam, 256, 256, depth=best["short_stack_depth"])
     assert len(hit_mismatch(got, ref)) == 0
+
+
+def test_autotune_whole_frames():
+    """Frame objective: both pipelines are tried on the best shape, and the
+    chosen options render the oracle's pixels."""
+    s, nodes, lp = scene_and_tree(2, 1)
+    ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK)
+    cam = s.camera(6, 1.0)
+    best, table = ctx.autotune(cam, 128, 128, reps=1, frame=True, variants=[H.ROPES, H.LDSSTACK])
+    assert {r["pipeline"] for r in table} == {1, 2}
+    assert best["pipeline"] in (1, 2)
+    img, _ = ctx.render_frame(cam, 128, 128)
+    var = {3: O.V_LDSSTACK, 5: O.V_ROPES}[best["variant"]]
+    ref, _ = oracle_scene(2, 1).render(var, cam, 128, 128, max_bounce=0, depth=best["short_stack_depth"])
+    assert np.array_equal(img, ref)
