@@ -124,6 +124,7 @@ EXPORTS = {
     "hulk_mlp_bound_camera": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
                                         C.c_uint32, C.c_void_p, C.c_uint32]),
     "hulk_ctx_scene_box": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hulk_selftest_reciprocal": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hulk_camera_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                    C.c_uint32]),
     "hulk_kdtree_build": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,

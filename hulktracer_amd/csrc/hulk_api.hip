@@ -61,6 +61,7 @@ struct hulk_ctx {
     bool have_tree = false;
     bool ropes_ready = false;
     DevBuf rope_nodes, rope_slots, rope_links;
+    DevBuf selftest;
     uint32_t n_slots = 0;
 
     // lights
@@ -545,7 +546,7 @@ extern "C" int hulk_create(hulk_ctx** out, const hulk_opts* opts) {
 extern "C" void hulk_destroy(hulk_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DevBuf* bufs[] = {&c->verts, &c->tris, &c->nodes, &c->lp, &c->recs, &c->rope_nodes, &c->rope_slots, &c->rope_links,
+    DevBuf* bufs[] = {&c->verts, &c->tris, &c->nodes, &c->lp, &c->recs, &c->rope_nodes, &c->rope_slots, &c->rope_links, &c->selftest,
                       &c->d_lights, &c->d_point, &c->rays, &c->tbound, &c->rgba, &c->mirror, &c->stats, &c->hits_io};
     for (DevBuf* b : bufs) release(*b);
     release_work(c->work);
@@ -895,6 +896,16 @@ extern "C" int hulk_camera_rays(hulk_ctx* c, const hulk_camera* cam, uint32_t W,
     if (!c->have_scene) return fail(c, HULK_E_NO_SCENE, "no scene uploaded");
     HIP_TRY(c, launch_camera_rays(to_cam(*cam), W, H, c->box, rays_dev, texit_dev, c->stream));
     return finish(c, flags, false);
+}
+
+extern "C" int hulk_selftest_reciprocal(hulk_ctx* c, unsigned long long out[4]) {
+    if (!c || !out) return HULK_E_ILLEGAL_INPUT;
+    int r;
+    if ((r = ensure(c, c->selftest, 8 * sizeof(unsigned long long)))) return r;
+    HIP_TRY(c, launch_selftest_rcp(static_cast<unsigned long long*>(c->selftest.p), c->stream));
+    HIP_TRY(c, hipMemcpyAsync(out, c->selftest.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HULK_OK;
 }
 
 extern "C" int hulk_ctx_scene_box(const hulk_ctx* c, float out[6]) {

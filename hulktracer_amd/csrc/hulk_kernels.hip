@@ -459,7 +459,7 @@ __device__ __forceinline__ float light_intensity(const float4& col, const float4
         L = sub(v3(dat.x, dat.y, dat.z), Pp);
         const float dd = dot(L, L);
         dist = __builtin_sqrtf(dd);
-        L = scale(L, 1.0f / __builtin_sqrtf(dd));
+        L = scale(L, rcp_exact(__builtin_sqrtf(dd)));
         intensity = dot(nn, L);
     }
     return intensity;
@@ -807,7 +807,7 @@ __global__ void __launch_bounds__(256) k_gen_secondary(GenParams G) {
         V3 L = sub(v3(lp.x, lp.y, lp.z), P);
         const float dd = dot(L, L);
         const float dist = __builtin_sqrtf(dd);
-        L = scale(L, 1.0f / __builtin_sqrtf(dd));
+        L = scale(L, rcp_exact(__builtin_sqrtf(dd)));
         hulk_ray sr;
         const V3 so = add(P, scale(L, 0.001f));
         sr.start = hulk_float4{so.x, so.y, so.z, dist};
@@ -968,7 +968,7 @@ __global__ void k_shade_picture(ShadeParams S, uint32_t* out) {
             float intensity = 0.0f;
             if (col.w == 1.0f) {
                 V3 L = sub(v3(dat.x, dat.y, dat.z), P);
-                L = scale(L, 1.0f / __builtin_sqrtf(dot(L, L)));
+                L = scale(L, rcp_exact(__builtin_sqrtf(dot(L, L))));
                 intensity = dot(nn, L) / 2.0f;
             }
             if (col.w == 0.0f) intensity = dot(nn, v3(dat.x, dat.y, dat.z));
@@ -998,6 +998,44 @@ __global__ void k_camera_rays(Camera cam, uint32_t W, uint32_t H, SceneBox box, 
         if (!ray_box(box, r, ts, te)) te = 0.0f;
         texit[i] = te;
     }
+}
+
+// -------------------------------------------------------------- selftest --
+
+// Every finite nonzero float x: out[0] counts those with
+// 2^-126 <= |x| < 2^126, out[1] those of them where v_rcp_f32 + one Newton
+// step differs from the IEEE 1.0f / x, out[2] the differences of that
+// formula outside the range, out[3] the differences of rcp_exact (range
+// test + fallback) over every x.
+__global__ void k_selftest_rcp(unsigned long long* out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long in_range = 0, d_in = 0, d_out = 0, d_exact = 0;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < (1ull << 32); b += stride) {
+        const uint32_t bits = (uint32_t)b;
+        if ((bits & 0x7F800000u) == 0x7F800000u || (bits & 0x7FFFFFFFu) == 0) continue;
+        const float x = __uint_as_float(bits);
+        const float y0 = __builtin_amdgcn_rcpf(x);
+        const uint32_t nr = __float_as_uint(__builtin_fmaf(__builtin_fmaf(-x, y0, 1.0f), y0, y0));
+        const uint32_t ieee = __float_as_uint(1.0f / x);
+        const float ax = __builtin_fabsf(x);
+        if (ax >= 0x1p-126f && ax < 0x1p126f) {
+            in_range++;
+            d_in += nr != ieee;
+        } else {
+            d_out += nr != ieee;
+        }
+        d_exact += __float_as_uint(rcp_exact(x)) != ieee;
+    }
+    atomicAdd(&out[0], in_range);
+    atomicAdd(&out[1], d_in);
+    atomicAdd(&out[2], d_out);
+    atomicAdd(&out[3], d_exact);
+}
+
+hipError_t launch_selftest_rcp(unsigned long long* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(out), (size_t)8, 0u);
+    hipLaunchKernelGGL(k_selftest_rcp, dim3(4096), dim3(256), 0, s, out);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- ropes --

@@ -137,6 +137,7 @@ hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 hipError_t launch_camera_rays(const Camera& cam, uint32_t W, uint32_t H, const SceneBox& box, hulk_ray* rays,
                               float* texit, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+hipError_t launch_selftest_rcp(unsigned long long* out, hipStream_t s);  // out: 4 counters
 // rope slots (3 n slots: nodes, then two empty-leaf slots per junction)
 hipError_t launch_build_ropes(const uint4* nodes, uint32_t n, uint32_t* up, uint4* rn, RopeSlot* rs,
                               const SceneBox& box, hipStream_t s);

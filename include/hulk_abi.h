@@ -321,6 +321,16 @@ int hulk_mlp_bound_camera(hulk_mlp* mlp, const hulk_camera* cam, uint32_t width,
  * the bound (castRay, GPU_common.cl:10-30; box test SharedFunctions.hpp:245-264). */
 int hulk_camera_rays(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height, hulk_ray* rays,
                      float* texit, uint32_t flags);
+/* Device self-test of the reciprocal the kernels use in place of 1.0f / x
+ * (no reference counterpart): over every finite nonzero float, out[0] =
+ * count with 2^-126 <= |x| < 2^126, out[1] = how many of them differ
+ * between v_rcp_f32 + one Newton-Raphson step and the IEEE division,
+ * out[2] = the same formula's differences outside that range (taken by
+ * the IEEE division in the kernels), out[3] = differences of the kernels'
+ * reciprocal from 1.0f / x over every x.  Exact parity needs out[1] ==
+ * out[3] == 0.  Runs about 4.3e9 evaluations. */
+int hulk_selftest_reciprocal(hulk_ctx* ctx, unsigned long long out[4]);
+
 /* Scene box of a context (min xyz, max xyz). */
 int hulk_ctx_scene_box(const hulk_ctx* ctx, float out[6]);
 

@@ -211,3 +211,18 @@ def test_full_size_terrain_1024():
     ctx.set_options(variant=H.SHORTSTACK)
     ss = ctx.trace_camera(cam, 1024, 1024)
     assert (ss["tri"] != got["tri"]).sum() <= 16, "short stack and full stack disagree beyond ties"
+
+
+def test_reciprocal_selftest():
+    """The kernels' 1 / x (v_rcp_f32 + one Newton step, IEEE division
+    outside 2^-126 <= |x| < 2^126) equals 1.0f / x for every float."""
+    import ctypes as C
+
+    s, nodes, lp = scene_and_tree(1, 1)
+    ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK)
+    out = (C.c_ulonglong * 4)()
+    assert H.lib().hulk_selftest_reciprocal(ctx.h, out) == 0
+    in_range, diff_in, diff_out, diff_exact = list(out)
+    assert in_range == 2 * 252 * (1 << 23)
+    assert diff_in == 0 and diff_exact == 0, (diff_in, diff_exact)
+    assert diff_out > 0  # the refinement alone is not exact outside the range: the fallback is needed
