@@ -65,6 +65,44 @@ def profiled_traffic(kernel: str, opts: dict, W: int, H: int):
     return None, f"no FETCH_SIZE/WRITE_SIZE profile of {kernel} {want} at {W}x{H}"
 
 
+def animated_kernel(args, W: int, H: int) -> int:
+    """--kernel-only on frame 0 of the animated scene (50 tori + ground, 1,000,002
+    triangles, 2 mirror generations): warmup + steps renders, HIP events."""
+    import torch
+
+    import hulktracer_amd as HT
+    from hulktracer_amd import _lib as L
+    from hulktracer_amd import workloads as WL
+
+    opts = {k: int(v) for k, v in (kv.split("=") for kv in args.kernel_only.split(","))}
+    scene = HT.Scene.procedural(L.SCENE_ANIMATED_TORI, seed=7, frame=0)
+    nodes, lp, _ = WL.build(scene)
+    stream = torch.cuda.Stream()
+    ctx = WL.make_ctx(scene, nodes, lp, stream, **opts)
+    mirror = torch.from_numpy(scene.mirror).cuda()
+    rgba = torch.empty(W * H, dtype=torch.int32, device="cuda")
+    hits = torch.empty(W * H * 32, dtype=torch.uint8, device="cuda")
+    cam = scene.camera(0, H / W)
+
+    def frame():
+        ctx.render_frame_dev(cam, W, H, 2, mirror.data_ptr(), rgba.data_ptr(), hits.data_ptr(), flags=L.FLAG_NO_SYNC)
+
+    for _ in range(max(args.warmup, 1)):
+        frame()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        frame()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel": kernel_name(opts), "scene": "animated_1M_frame0", "options": opts,
+                      "kernel_ms": round(e0.elapsed_time(e1) / args.steps, 4), "timed_launches": args.steps,
+                      "launches": args.steps + max(args.warmup, 1), "primary_rays_per_launch": W * H}), flush=True)
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +120,9 @@ def main() -> int:
                     help="only time the primary trace kernel of pose 10 with these options "
                          "(e.g. variant=5,block_size=64,short_stack_depth=6,lds_top_levels=0,persistent=0): "
                          "warmup + steps launches, for profiler runs")
+    ap.add_argument("--kernel-scene", default="terrain", choices=("terrain", "animated"),
+                    help="--kernel-only workload: the 10-frame terrain batch, or frame 0 of the animated "
+                         "1M-triangle scene (1024x1024, 2 mirror generations, one render per launch)")
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU only: this build has no multi-GPU path (rays are split per GPU only "
@@ -107,6 +148,8 @@ def main() -> int:
 
     W = Hh = args.res
     n = W * Hh
+    if args.kernel_only and args.kernel_scene == "animated":
+        return animated_kernel(args, W, Hh)
     t0 = time.time()
     scene = H.Scene.procedural(L.SCENE_TERRAIN_SPHERES, seed=1)
     nodes, lp, build_ms = WL.build(scene)
