@@ -39,7 +39,7 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-OPT_KEYS = ("variant", "block_size", "short_stack_depth", "lds_top_levels", "persistent", "pipeline")
+OPT_KEYS = ("variant", "block_size", "short_stack_depth", "lds_top_levels", "persistent", "pipeline", "leaf_batch")
 
 
 def kernel_name(opts: dict) -> str:
@@ -47,7 +47,7 @@ def kernel_name(opts: dict) -> str:
     queued pipeline's graph of trace, compaction and shading kernels."""
     if int(opts.get("pipeline", 0)) == 1:
         return "queued pipeline graph"
-    return f"k_trace<{int(opts['variant'])},false,true>"
+    return f"k_trace<{int(opts['variant'])},false,true,{2 if int(opts.get('leaf_batch', 1)) == 2 else 1}>"
 
 
 def profiled_traffic(kernel: str, opts: dict, W: int, H: int):

@@ -129,6 +129,7 @@ int check_opts(hulk_ctx* c, const hulk_opts& o) {
         return fail(c, HULK_E_ILLEGAL_INPUT, "short_stack_depth must be in [1, 16]");
     if (o.lds_top_levels > kTopLevelsMax) return fail(c, HULK_E_ILLEGAL_INPUT, "lds_top_levels must be <= 12");
     if (o.pipeline > HULK_PIPELINE_FUSED) return fail(c, HULK_E_ILLEGAL_INPUT, "unknown pipeline");
+    if (o.leaf_batch > 2) return fail(c, HULK_E_ILLEGAL_INPUT, "leaf_batch must be 0, 1 or 2");
     return HULK_OK;
 }
 
@@ -172,6 +173,7 @@ int trace_setup(hulk_ctx* c, const hulk_opts& o, TraceParams& P, uint32_t& block
         if (r) return r;
     }
     std::memset(&P, 0, sizeof(P));
+    P.leaf_batch = o.leaf_batch == 2 ? 2u : 1u;
     P.nodes = static_cast<const uint4*>(c->nodes.p);
     P.recs = static_cast<const TriRec*>(c->recs.p);
     P.verts = static_cast<const float4*>(c->verts.p);
@@ -228,7 +230,7 @@ int run_trace(hulk_ctx* c, int variant, TraceParams& P, uint32_t n_max, uint32_t
         grid = std::max<uint32_t>(1u, (n_max + block - 1) / block);
     }
     if (timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-    HIP_TRY(c, launch_trace(variant, P, count, render, grid, block, lds, c->stream));
+    HIP_TRY(c, launch_trace(variant, P, count, render, P.leaf_batch, grid, block, lds, c->stream));
     if (timed) HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return HULK_OK;
 }
@@ -1100,7 +1102,7 @@ namespace {
 
 struct Cand {
     int32_t variant;
-    uint32_t block, depth, lds, persistent, pipeline;
+    uint32_t block, depth, lds, persistent, pipeline, leaf_batch;
 };
 
 }  // namespace
@@ -1142,6 +1144,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
         o.lds_top_levels = k.lds;
         o.persistent = k.persistent;
         o.pipeline = k.pipeline;
+        o.leaf_batch = k.leaf_batch;
         TraceParams P;
         uint32_t block;
         size_t lds;
@@ -1168,7 +1171,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
     std::vector<std::pair<float, Cand>> firsts;
     for (int v = HULK_VARIANT_FULLSTACK; v < HULK_VARIANT_COUNT; v++) {
         if (!(variant_mask & (1u << v))) continue;
-        Cand k{v, 256, 6, 0, 0, pipe0};
+        Cand k{v, 256, 6, 0, 0, pipe0, 1};
         float ms;
         r = measure(k, ms);
         if (r > 1) return r;
@@ -1189,16 +1192,19 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
                 if (!shortv && dp != 6) continue;
                 for (uint32_t ld : ldss) {
                     if (!ldsv && ld) continue;
-                    for (uint32_t ps = 0; ps < 2; ps++) {
-                        Cand k{v, bs, dp, ld, ps, pipe0};
-                        if (bs == 256 && dp == 6 && ld == 0 && ps == 0) continue;  // stage 1
-                        // persistent + LDS top tree pays off; skip the
-                        // non-persistent LDS shapes beyond depth 8
-                        if (!ps && ld > 8) continue;
-                        float ms;
-                        r = measure(k, ms);
-                        if (r > 1) return r;
-                    }
+                    for (uint32_t ps = 0; ps < 2; ps++)
+                        for (uint32_t lb = 1; lb <= 2; lb++) {
+                            Cand k{v, bs, dp, ld, ps, pipe0, lb};
+                            if (bs == 256 && dp == 6 && ld == 0 && ps == 0 && lb == 1) continue;  // stage 1
+                            // persistent + LDS top tree pays off; skip the
+                            // non-persistent LDS shapes beyond depth 8
+                            if (!ps && ld > 8) continue;
+                            // two records in flight: default stack / LDS depths only
+                            if (lb == 2 && (dp != 6 || (ld != 0 && ld != 8))) continue;
+                            float ms;
+                            r = measure(k, ms);
+                            if (r > 1) return r;
+                        }
                 }
             }
     }
@@ -1219,6 +1225,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
     c->opts.short_stack_depth = b.depth;
     c->opts.lds_top_levels = b.lds;
     c->opts.persistent = b.persistent;
+    c->opts.leaf_batch = b.leaf_batch;
     if (frame) c->opts.pipeline = b.pipeline;
     if (best_out) *best_out = c->opts;
     if (table) {
@@ -1229,6 +1236,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
             table[i].lds_top_levels = done[i].first.lds;
             table[i].persistent = done[i].first.persistent;
             table[i].pipeline = done[i].first.pipeline;
+            table[i].leaf_batch = done[i].first.leaf_batch;
             table[i].ms = done[i].second;
         }
     }

@@ -100,13 +100,28 @@ __device__ __forceinline__ uint32_t frame_ray(const FrameDev* F, uint32_t n, uin
 // hit lies below the light distance; once best.t < tstop that answer is
 // fixed (best.t only decreases), so the ray stops.  Primary and mirror rays
 // pass tstop = -FLT_MAX, which never stops them.
-template <bool COUNT>
+template <bool COUNT, int LB>
 __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, const uint4& n, const Ray& r, Best& best,
                                           Cnt& c, float tstop) {
     const uint32_t start = n.y, end = n.y + n.z;
     if (COUNT) {
         c.leaves++;
         c.tris += n.z;
+    }
+    if (LB == 2) {
+        // two records in flight per lane: the loads of a pair are issued
+        // together (pays off where records come from L2 / MALL, not L1)
+        uint32_t i = start;
+        for (; i + 1 < end; i += 2) {
+            const TriRec r0 = recs[i], r1 = recs[i + 1];
+            if (test_record(r, r0, best) && best.t < tstop) return;
+            if (test_record(r, r1, best) && best.t < tstop) return;
+        }
+        if (i < end) {
+            const TriRec r0 = recs[i];
+            test_record(r, r0, best);
+        }
+        return;
     }
     for (uint32_t i = start; i < end; i++) {
         TriRec rec;
@@ -125,7 +140,7 @@ __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, const
 // column per lane.  Entries keep (node, tStart) only: the tEnd of entry k
 // equals the tStart of entry k-1 (tEnd0 for the bottom entry), because tEnd
 // changes only by a push (to the pushed split) or a pop.
-template <bool COUNT>
+template <bool COUNT, int LB>
 __device__ void trav_fullstack(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk, uint32_t stride,
                                const Ray& r, float tclip, float tstop, Best& best, Cnt& c, uint32_t& flags) {
     float ts = 0.0f, te = kFltMax;
@@ -138,7 +153,7 @@ __device__ void trav_fullstack(const TraceParams& P, const uint4* lds_nodes, uin
         const uint4 n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
         if (COUNT) c.nodes++;
         if (n.x & kNodeLeaf) {
-            test_leaf<COUNT>(P.recs, n, r, best, c, tstop);
+            test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
             if (best.t < tstop) break;
             node = 0;
         } else {
@@ -181,7 +196,7 @@ __device__ void trav_fullstack(const TraceParams& P, const uint4* lds_nodes, uin
 // `depth` entries that drops the oldest on overflow; an empty stack
 // restarts from the root with tmin = tmax.  Frame i of lane L lives at
 // stk[(3 i + k) * stride] (k = tStart, tEnd, node).
-template <bool COUNT>
+template <bool COUNT, int LB>
 __device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk, uint32_t stride,
                                 const Ray& r, float tclip, float tstop, Best& best, Cnt& c, uint32_t& flags) {
     float ts = 0.0f, te = kFltMax;
@@ -228,7 +243,7 @@ __device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, ui
             if (node == 0) break;
         }
         if (n.x & kNodeLeaf) {
-            test_leaf<COUNT>(P.recs, n, r, best, c, tstop);
+            test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
             if (best.t < tmax || best.t < tstop) break;
         }
         if (++guard > P.guard) {
@@ -239,7 +254,7 @@ __device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, ui
 }
 
 // The same short stack kept in a private array (registers / scratch).
-template <bool COUNT>
+template <bool COUNT, int LB>
 __device__ void trav_shortstack_private(const TraceParams& P, const uint4* lds_nodes, const Ray& r, float tclip,
                                         float tstop, Best& best, Cnt& c, uint32_t& flags) {
     float ts = 0.0f, te = kFltMax;
@@ -288,7 +303,7 @@ __device__ void trav_shortstack_private(const TraceParams& P, const uint4* lds_n
             if (node == 0) break;
         }
         if (n.x & kNodeLeaf) {
-            test_leaf<COUNT>(P.recs, n, r, best, c, tstop);
+            test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
             if (best.t < tmax || best.t < tstop) break;
         }
         if (++guard > P.guard) {
@@ -302,7 +317,7 @@ __device__ void trav_shortstack_private(const TraceParams& P, const uint4* lds_n
 // resident.  ThreadContext starts zeroed, so tmin = tmax = 0 (the box entry
 // is not used); each leaf is followed by a restart from the root; there is
 // no early exit after a hit.
-template <bool COUNT>
+template <bool COUNT, int LB>
 __device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const Ray& r, float tclip, float tstop,
                              Best& best, Cnt& c, uint32_t& flags) {
     float ts = 0.0f, te = kFltMax;
@@ -334,7 +349,7 @@ __device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const
             if (node == 0) break;
         }
         if (n.x & kNodeLeaf) {
-            test_leaf<COUNT>(P.recs, n, r, best, c, tstop);
+            test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
             if (best.t < tstop) break;
         }
         if (++guard > P.guard) {
@@ -352,7 +367,7 @@ __device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const
 // split > t, else second; test the leaf; leave through the face of minimum
 // slab time (lowest axis on ties); stop when best.t <= t_out, t_out >= te,
 // or there is no rope; else t = max(t, t_out) and follow the rope.
-template <bool COUNT>
+template <bool COUNT, int LB>
 __device__ void trav_ropes(const TraceParams& P, const Ray& r, float tclip, float tstop, Best& best, Cnt& c,
                            uint32_t& flags) {
     float ts = 0.0f, te = kFltMax;
@@ -371,7 +386,7 @@ __device__ void trav_ropes(const TraceParams& P, const Ray& r, float tclip, floa
             n = P.rope_nodes[slot];
             if (COUNT) c.nodes++;
         }
-        test_leaf<COUNT>(P.recs, n, r, best, c, tstop);
+        test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
         if (best.t < tstop) break;
         const RopeSlot& rs = P.rope_slots[slot];
         const float4 lo = rs.lo, hi = rs.hi;
@@ -412,7 +427,7 @@ __device__ void trav_ropes(const TraceParams& P, const Ray& r, float tclip, floa
     }
 }
 
-template <bool COUNT>
+template <bool COUNT, int LB>
 __device__ void trav_brute(const TraceParams& P, const Ray& r, float tstop, Best& best, Cnt& c) {
     for (uint32_t i = 0; i < P.n_tris; i++) {
         const uint4 t = P.tris[i];
@@ -423,19 +438,19 @@ __device__ void trav_brute(const TraceParams& P, const Ray& r, float tstop, Best
     if (COUNT) c.tris += P.n_tris;
 }
 
-template <int VARIANT, bool COUNT>
+template <int VARIANT, bool COUNT, int LB>
 __device__ __forceinline__ void trace_one(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk,
                                           uint32_t stride, const Ray& r, float tclip, float tstop, Best& best,
                                           Cnt& c, uint32_t& flags) {
     if (VARIANT == HULK_VARIANT_FULLSTACK)
-        trav_fullstack<COUNT>(P, lds_nodes, stk, stride, r, tclip, tstop, best, c, flags);
+        trav_fullstack<COUNT, LB>(P, lds_nodes, stk, stride, r, tclip, tstop, best, c, flags);
     else if (VARIANT == HULK_VARIANT_SHORTSTACK)
-        trav_shortstack_private<COUNT>(P, lds_nodes, r, tclip, tstop, best, c, flags);
+        trav_shortstack_private<COUNT, LB>(P, lds_nodes, r, tclip, tstop, best, c, flags);
     else if (VARIANT == HULK_VARIANT_LDSSTACK)
-        trav_shortstack<COUNT>(P, lds_nodes, stk, stride, r, tclip, tstop, best, c, flags);
-    else if (VARIANT == HULK_VARIANT_RESTART) trav_restart<COUNT>(P, lds_nodes, r, tclip, tstop, best, c, flags);
-    else if (VARIANT == HULK_VARIANT_ROPES) trav_ropes<COUNT>(P, r, tclip, tstop, best, c, flags);
-    else trav_brute<COUNT>(P, r, tstop, best, c);
+        trav_shortstack<COUNT, LB>(P, lds_nodes, stk, stride, r, tclip, tstop, best, c, flags);
+    else if (VARIANT == HULK_VARIANT_RESTART) trav_restart<COUNT, LB>(P, lds_nodes, r, tclip, tstop, best, c, flags);
+    else if (VARIANT == HULK_VARIANT_ROPES) trav_ropes<COUNT, LB>(P, r, tclip, tstop, best, c, flags);
+    else trav_brute<COUNT, LB>(P, r, tstop, best, c);
 }
 
 // ---------------------------------------------------------- trace kernel --
@@ -473,7 +488,7 @@ __device__ __forceinline__ float light_intensity(const float4& col, const float4
 // shadow ray (origin P + 0.001 L, nearest hit, stopped once t < distance)
 // is blocked; acc += c * keep, keep = 0.2 w for a mirror hit that spawns a
 // reflection (r = d - 2 (d.n) n, origin P + 0.001 r, weight 0.8 w), else w.
-template <int VARIANT, bool COUNT>
+template <int VARIANT, bool COUNT, int LB>
 __device__ void render_ray(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk, uint32_t stride, Ray r,
                            Best best, uint32_t max_bounce, Cnt& c, uint64_t& crays, uint64_t& cshadow,
                            uint64_t& chits, float& ar, float& ag, float& ab) {
@@ -497,7 +512,7 @@ __device__ void render_ray(const TraceParams& P, const uint4* lds_nodes, uint32_
                 Best sb;
                 best_init(sb);
                 uint32_t sf = 0;
-                trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, sr, kFltMax, dist, sb, c, sf);
+                trace_one<VARIANT, COUNT, LB>(P, lds_nodes, stk, stride, sr, kFltMax, dist, sb, c, sf);
                 if (COUNT) {
                     cshadow++;
                     chits += sb.tri != kNoHit ? 1 : 0;
@@ -524,7 +539,7 @@ __device__ void render_ray(const TraceParams& P, const uint4* lds_nodes, uint32_
         w = w * 0.8f;
         best_init(best);
         uint32_t mf = 0;
-        trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, kFltMax, -kFltMax, best, c, mf);
+        trace_one<VARIANT, COUNT, LB>(P, lds_nodes, stk, stride, r, kFltMax, -kFltMax, best, c, mf);
         if (COUNT) {
             crays++;
             chits += best.tri != kNoHit ? 1 : 0;
@@ -543,8 +558,9 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
 // device).  Output: a full hit record, or only t scattered to t_slot[i].
 // RENDER: camera / frame rays only; after the primary hit record each ray
 // runs its whole shading path (render_ray) and writes its packed pixel.
-template <int VARIANT, bool COUNT, bool RENDER>
-__global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
+template <int VARIANT, bool COUNT, bool RENDER, int LB>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(LB == 2 ? 6 : 1)))
+k_trace(TraceParams P) {
     extern __shared__ uint4 smem[];
     const uint32_t tid = threadIdx.x;
     const uint32_t block = blockDim.x;
@@ -639,15 +655,15 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
             const float tclip = tb * (1.0f + 1.0f / 1024.0f);
             const bool clipped = tb < kFltMax && ray_box(P.box, r, bts, bte) && tclip < bte;
             if (clipped) {
-                trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, tclip, tstop, best, c, flags);
+                trace_one<VARIANT, COUNT, LB>(P, lds_nodes, stk, stride, r, tclip, tstop, best, c, flags);
                 if (!(best.tri != kNoHit && best.t < tb)) {
                     best_init(best);
                     flags |= HULK_HITF_BOUNDED_FALLBACK;
-                    trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, kFltMax, tstop, best, c, flags);
+                    trace_one<VARIANT, COUNT, LB>(P, lds_nodes, stk, stride, r, kFltMax, tstop, best, c, flags);
                     if (COUNT) cfb++;
                 }
             } else {
-                trace_one<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, kFltMax, tstop, best, c, flags);
+                trace_one<VARIANT, COUNT, LB>(P, lds_nodes, stk, stride, r, kFltMax, tstop, best, c, flags);
             }
             if (P.t_out) {
                 P.t_out[P.t_slot ? P.t_slot[i] : o] = best.t;
@@ -667,7 +683,7 @@ __global__ void __launch_bounds__(1024) k_trace(TraceParams P) {
             }
             if (RENDER) {
                 float ar = 0.0f, ag = 0.0f, ab = 0.0f;
-                render_ray<VARIANT, COUNT>(P, lds_nodes, stk, stride, r, best, max_bounce, c, crays, cshadow, chits,
+                render_ray<VARIANT, COUNT, LB>(P, lds_nodes, stk, stride, r, best, max_bounce, c, crays, cshadow, chits,
                                            ar, ag, ab);
                 P.rgba[o] = to_u8(ar) | (to_u8(ag) << 8) | (to_u8(ab) << 16);
             }
@@ -1117,34 +1133,40 @@ hipError_t launch_build_ropes(const uint4* nodes, uint32_t n, uint32_t* up, uint
 // ---------------------------------------------------------- launchers --
 
 template <int V, bool C>
-static hipError_t launch_trace_t(const TraceParams& P, bool render, uint32_t grid, uint32_t block, size_t lds,
-                                 hipStream_t s) {
-    if (render) hipLaunchKernelGGL((k_trace<V, C, true>), dim3(grid), dim3(block), lds, s, P);
-    else hipLaunchKernelGGL((k_trace<V, C, false>), dim3(grid), dim3(block), lds, s, P);
+static hipError_t launch_trace_t(const TraceParams& P, bool render, uint32_t leaf_batch, uint32_t grid,
+                                 uint32_t block, size_t lds, hipStream_t s) {
+    if (leaf_batch == 2) {
+        if (render) hipLaunchKernelGGL((k_trace<V, C, true, 2>), dim3(grid), dim3(block), lds, s, P);
+        else hipLaunchKernelGGL((k_trace<V, C, false, 2>), dim3(grid), dim3(block), lds, s, P);
+    } else {
+        if (render) hipLaunchKernelGGL((k_trace<V, C, true, 1>), dim3(grid), dim3(block), lds, s, P);
+        else hipLaunchKernelGGL((k_trace<V, C, false, 1>), dim3(grid), dim3(block), lds, s, P);
+    }
     return hipGetLastError();
 }
 
 template <int V>
-static hipError_t launch_trace_v(const TraceParams& P, bool count, bool render, uint32_t grid, uint32_t block,
-                                 size_t lds, hipStream_t s) {
-    return count ? launch_trace_t<V, true>(P, render, grid, block, lds, s)
-                 : launch_trace_t<V, false>(P, render, grid, block, lds, s);
+static hipError_t launch_trace_v(const TraceParams& P, bool count, bool render, uint32_t leaf_batch, uint32_t grid,
+                                 uint32_t block, size_t lds, hipStream_t s) {
+    return count ? launch_trace_t<V, true>(P, render, leaf_batch, grid, block, lds, s)
+                 : launch_trace_t<V, false>(P, render, leaf_batch, grid, block, lds, s);
 }
 
-hipError_t launch_trace(int variant, const TraceParams& P, bool count, bool render, uint32_t grid, uint32_t block,
-                        size_t lds, hipStream_t s) {
+hipError_t launch_trace(int variant, const TraceParams& P, bool count, bool render, uint32_t leaf_batch,
+                        uint32_t grid, uint32_t block, size_t lds, hipStream_t s) {
     switch (variant) {
         case HULK_VARIANT_BRUTEFORCE:
-            return launch_trace_v<HULK_VARIANT_BRUTEFORCE>(P, count, render, grid, block, lds, s);
+            return launch_trace_v<HULK_VARIANT_BRUTEFORCE>(P, count, render, leaf_batch, grid, block, lds, s);
         case HULK_VARIANT_FULLSTACK:
-            return launch_trace_v<HULK_VARIANT_FULLSTACK>(P, count, render, grid, block, lds, s);
+            return launch_trace_v<HULK_VARIANT_FULLSTACK>(P, count, render, leaf_batch, grid, block, lds, s);
         case HULK_VARIANT_SHORTSTACK:
-            return launch_trace_v<HULK_VARIANT_SHORTSTACK>(P, count, render, grid, block, lds, s);
+            return launch_trace_v<HULK_VARIANT_SHORTSTACK>(P, count, render, leaf_batch, grid, block, lds, s);
         case HULK_VARIANT_LDSSTACK:
-            return launch_trace_v<HULK_VARIANT_LDSSTACK>(P, count, render, grid, block, lds, s);
+            return launch_trace_v<HULK_VARIANT_LDSSTACK>(P, count, render, leaf_batch, grid, block, lds, s);
         case HULK_VARIANT_RESTART:
-            return launch_trace_v<HULK_VARIANT_RESTART>(P, count, render, grid, block, lds, s);
-        case HULK_VARIANT_ROPES: return launch_trace_v<HULK_VARIANT_ROPES>(P, count, render, grid, block, lds, s);
+            return launch_trace_v<HULK_VARIANT_RESTART>(P, count, render, leaf_batch, grid, block, lds, s);
+        case HULK_VARIANT_ROPES:
+            return launch_trace_v<HULK_VARIANT_ROPES>(P, count, render, leaf_batch, grid, block, lds, s);
         default: return hipErrorInvalidValue;
     }
 }

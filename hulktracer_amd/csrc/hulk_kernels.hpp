@@ -72,6 +72,7 @@ struct TraceParams {
     const uint8_t* mirror;  // mirror flag per triangle, or null (no bounces)
     uint32_t max_bounce;    // camera source; frames carry their own
     uint32_t* rgba;
+    uint32_t leaf_batch;    // host side: picks the kernel instance (1 or 2 records in flight)
     uint32_t stop_at_w;  // shadow queue: stop once best.t < ray.start.w (the light distance)
 };
 
@@ -125,8 +126,9 @@ struct ShadeParams {
     float* acc;
 };
 
-hipError_t launch_trace(int variant, const TraceParams& P, bool count, bool render, uint32_t grid, uint32_t block,
-                        size_t lds, hipStream_t s);
+// leaf_batch: triangle records in flight per lane in the leaf loop (1 or 2)
+hipError_t launch_trace(int variant, const TraceParams& P, bool count, bool render, uint32_t leaf_batch,
+                        uint32_t grid, uint32_t block, size_t lds, hipStream_t s);
 hipError_t launch_build_records(const float4* verts, const uint4* tris, const uint32_t* lp, uint32_t n,
                                 TriRec* recs, hipStream_t s);
 hipError_t launch_gen_secondary(const GenParams& G, uint32_t n_max, hipStream_t s);

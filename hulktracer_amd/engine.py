@@ -187,14 +187,16 @@ class GPUTrace:
 
     def __init__(self, variant: int | str = L.FULLSTACK, block_size: int = 256, short_stack_depth: int = 6,
                  lds_top_levels: int = 0, persistent: bool = False, device: int = 0, stream: int | None = None,
-                 pipeline: int = 0):
+                 pipeline: int = 0, leaf_batch: int = 1):
         """pipeline: 0 = automatic, 1 = queued shadow / mirror launches,
-        2 = one fused launch per frame (hulk_pipeline)."""
+        2 = one fused launch per frame (hulk_pipeline).  leaf_batch: triangle
+        records in flight per lane (1 or 2)."""
         if isinstance(variant, str):
             variant = L.VARIANTS[variant]
         self.opts = L.Opts(device=device, variant=variant, block_size=block_size,
                            short_stack_depth=short_stack_depth, lds_top_levels=lds_top_levels,
-                           persistent=int(persistent), pipeline=int(pipeline), stream=stream)
+                           persistent=int(persistent), pipeline=int(pipeline), leaf_batch=int(leaf_batch),
+                           stream=stream)
         self.h = C.c_void_p()
         _check(None, lib().hulk_create(C.byref(self.h), C.byref(self.opts)))
 

@@ -117,6 +117,9 @@ typedef struct hulk_opts {
     uint32_t lds_top_levels;    /* 0..12: top tree levels staged in LDS      */
     uint32_t persistent;        /* 1 = persistent waves with atomic fetch    */
     uint32_t pipeline;          /* hulk_pipeline: frame launch model         */
+    uint32_t leaf_batch;        /* triangle records in flight per lane in a
+                                   leaf: 0 or 1 = one, 2 = two (for scenes
+                                   whose records do not stay in L1)          */
     void*    stream;            /* hipStream_t to use; NULL = own stream     */
 } hulk_opts;
 
@@ -266,7 +269,7 @@ void hulk_batch_destroy(hulk_batch* batch);
 
 typedef struct hulk_tune_entry {
     int32_t variant;
-    uint32_t block_size, short_stack_depth, lds_top_levels, persistent, pipeline;
+    uint32_t block_size, short_stack_depth, lds_top_levels, persistent, pipeline, leaf_batch;
     float ms;                 /* time per launch of the tuned objective */
 } hulk_tune_entry;
 

@@ -48,7 +48,7 @@ def test_struct_sizes_match_reference_layout():
     # CameraPos 64 B, Light 32 B, Ray 32 B
     assert H.HIT_DTYPE.itemsize == 32
     assert H.RAY_DTYPE.itemsize == 32
-    assert C.sizeof(_lib.Opts) == 40
+    assert C.sizeof(_lib.Opts) == 40  # 8 x 4 B + stream pointer
 
 
 def test_procedural_scenes_and_counts():

@@ -75,10 +75,11 @@ def test_short_stack_depths(variant, depth):
 
 @pytest.mark.parametrize("opts", [dict(lds_top_levels=6), dict(lds_top_levels=10, block_size=128),
                                   dict(persistent=True), dict(persistent=True, lds_top_levels=8, block_size=128),
-                                  dict(block_size=64)])
+                                  dict(block_size=64), dict(leaf_batch=2), dict(leaf_batch=2, persistent=True)])
 @pytest.mark.parametrize("variant", [H.FULLSTACK, H.LDSSTACK, H.RESTART, H.ROPES])
 def test_launch_options_parity(variant, opts):
-    """LDS top tree, persistent waves and block sizes change only scheduling."""
+    """LDS top tree, persistent waves, block sizes and records in flight
+    change only scheduling."""
     s, nodes, lp = scene_and_tree(2, 1)
     cam = s.camera(9, 1.0)
     ctx = gpu_ctx(s, nodes, lp, variant, **opts)
@@ -108,7 +109,8 @@ def test_shade_parity(variant, mode):
 @pytest.mark.parametrize("opts", [dict(variant=H.FULLSTACK), dict(variant=H.FULLSTACK, persistent=True),
                                   dict(variant=H.ROPES, persistent=True, block_size=64),
                                   dict(variant=H.FULLSTACK, pipeline=2),
-                                  dict(variant=H.ROPES, pipeline=2, persistent=True, block_size=128)])
+                                  dict(variant=H.ROPES, pipeline=2, persistent=True, block_size=128),
+                                  dict(variant=H.LDSSTACK, pipeline=2, leaf_batch=2, lds_top_levels=8)])
 def test_render_bounces_parity(opts):
     """Animated tori (5 meshes, 100,002 triangles): primary + shadow + 2
     mirror generations with per-generation compaction; persistent waves
