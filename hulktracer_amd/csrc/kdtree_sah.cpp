@@ -14,7 +14,6 @@
 // keep their 32-bit wrap, and min / max keep the SSE operand rule
 // (minps(a, b) = a < b ? a : b).  Built with -ffp-contract=off.
 #include <algorithm>
-#include <atomic>
 #include <cfloat>
 #include <cstring>
 #include <new>
@@ -70,32 +69,10 @@ struct Out {
     uint32_t leaf_count = 0;
 };
 
-// Run f(0) .. f(n - 1), f(0) on the calling thread, the rest on their own.
-template <class F>
-void run_parallel(uint32_t n, F&& f) {
-    std::vector<std::thread> ts;
-    for (uint32_t k = 1; k < n; k++) ts.emplace_back([&f, k] { f(k); });
-    f(0);
-    for (auto& t : ts) t.join();
-}
-
-// Nodes with at least this many polygons split their own work (the three
-// axis sweeps, the classification, the three candidate splits) over
-// threads when the build has more than one.
-constexpr size_t kParallelNodePolys = 32768;
-
-struct AxisBest {
-    float best = FLT_MAX, value = 0.0f, upper = 0.0f, lower = 0.0f;
-};
-
-// A junction whose polygon list has at least this many entries may hand
-// its above subtree to a new thread, when one of the build's threads is free.
-constexpr size_t kSpawnMinPolys = 8192;
-
 class Builder {
 public:
-    Builder(const hulk_float4* v, const hulk_tri* t, uint32_t nt, uint32_t threads)
-        : verts_(v), tris_(t), n_tris_(nt), threads_(threads), free_threads_(static_cast<int>(threads) - 1) {}
+    Builder(const hulk_float4* v, const hulk_tri* t, uint32_t nt, uint32_t spawn_depth)
+        : verts_(v), tris_(t), n_tris_(nt), spawn_depth_(spawn_depth) {}
 
     void build(const hulk_box& scene_box, Out& out) {
         Split s;
@@ -119,30 +96,14 @@ public:
                 s.cand[j].push_back({mx, kEnd, p});
             }
         }
-        // The three candidate sorts are independent; with threads to spare
-        // each is sorted in chunks that are then merged.  Only the order of
-        // equal values can differ between sort orders, and the sweep counts
-        // a run of equal values as one group, so the tree does not depend on it.
-        const auto by_value = [](const Candidate& a, const Candidate& b) { return a.value < b.value; };
-        const uint32_t chunks = std::max<uint32_t>(1, std::min<uint32_t>(threads_ / 3, 8));
-        if (threads_ <= 1) {
-            for (uint32_t j = 0; j < 3; j++) std::sort(s.cand[j].begin(), s.cand[j].end(), by_value);
-        } else run_parallel(3 * chunks, [&](uint32_t k) {
-            std::vector<Candidate>& c = s.cand[k / chunks];
-            const size_t part = k % chunks, m = c.size();
-            std::sort(c.begin() + m * part / chunks, c.begin() + m * (part + 1) / chunks, by_value);
-        });
-        for (uint32_t width = 1; width < chunks; width *= 2) {
-            run_parallel(3 * chunks, [&](uint32_t k) {
-                const uint32_t part = k % chunks;
-                if (part % (2 * width) != 0 || part + width >= chunks) return;
-                std::vector<Candidate>& c = s.cand[k / chunks];
-                const size_t m = c.size();
-                const uint32_t hi = std::min(chunks, part + 2 * width);
-                std::inplace_merge(c.begin() + m * part / chunks, c.begin() + m * (part + width) / chunks,
-                                   c.begin() + m * hi / chunks, by_value);
+        // the three candidate sorts are independent
+        std::vector<std::thread> sorts;
+        for (uint32_t j = 0; j < 3; j++)
+            sorts.emplace_back([&s, j] {
+                std::sort(s.cand[j].begin(), s.cand[j].end(),
+                          [](const Candidate& a, const Candidate& b) { return a.value < b.value; });
             });
-        }
+        for (auto& t : sorts) t.join();
         hulk_kd_node root;
         std::memset(&root, 0, sizeof(root));
         out.nodes.reserve(5000);
@@ -156,58 +117,6 @@ private:
     static float box_surface(const hulk_box& b) {
         float sx = b.max.x - b.min.x, sy = b.max.y - b.min.y, sz = b.max.z - b.min.z;
         return 2.0f * (sx * sy + sz * sy + sx * sz);
-    }
-
-    // Event sweep of one axis (buildNodeSAHNlogN, KDTreeSAH.cpp:343-420):
-    // the first candidate value with the smallest SAH cost on this axis.
-    static AxisBest sweep(const Split& s, uint32_t j) {
-        AxisBest ab;
-        const float dim[3] = {s.box.max.x - s.box.min.x, s.box.max.y - s.box.min.y,
-                              s.box.max.z - s.box.min.z};
-        const float inv_surface = 1.0f / s.surface;
-        uint32_t below = 0, cop = 0, above = 0;
-        uint32_t counters[3] = {0, 0, 0};
-        uint32_t plus_prev = 0, cop_prev = 0;
-        const float x = dim[(j + 1) % 3];
-        const float y = dim[(j + 2) % 3];
-        const float xpy = x + y;
-        const float area = x * y;
-        bool first = true;
-        const std::vector<Candidate>& c = s.cand[j];
-        size_t i = 0;
-        while (i < c.size()) {
-            const float value = c[i].value;
-            while (i < c.size() && c[i].value == value) {
-                counters[c[i].marker]++;
-                i++;
-            }
-            below = below + cop_prev + plus_prev;
-            cop = cop_prev;
-            above = above - counters[kCoplanarMark] - counters[kEnd];
-            if (first) {
-                below = 0;
-                cop = counters[kCoplanarMark];
-                above = static_cast<uint32_t>(s.eval) - counters[kCoplanarMark];
-                first = false;
-            }
-            plus_prev = counters[kBegin];
-            cop_prev = counters[kCoplanarMark];
-            counters[0] = counters[1] = counters[2] = 0;
-
-            const float upper = area + (lane(s.box.max, j) - value) * xpy;
-            const float lower = area + (value - lane(s.box.min, j)) * xpy;
-            const float cost_below = lower * static_cast<float>(below);
-            const float cost_above = upper * static_cast<float>(above);
-            const float cost_cop = area * static_cast<float>(cop);
-            const float sah = kTraverseCost + kIntersectionCost * (((cost_cop + cost_below) + cost_above) * inv_surface);
-            if (sah < ab.best) {
-                ab.best = sah;
-                ab.value = value;
-                ab.upper = upper;
-                ab.lower = lower;
-            }
-        }
-        return ab;
     }
 
     Relation classify(uint32_t axis, float v, uint32_t poly) const {
@@ -255,20 +164,55 @@ private:
             make_leaf(s, out);
             return;
         }
-        const bool par = threads_ > 1 && s.polys.size() >= kParallelNodePolys;
-        AxisBest ab[3];
-        if (par) run_parallel(3, [&](uint32_t j) { ab[j] = sweep(s, j); });
-        else for (uint32_t j = 0; j < 3; j++) ab[j] = sweep(s, j);
-        // first minimum in (axis, value) order, as one sweep over the axes in turn
         float best = FLT_MAX, best_upper = 0.0f, best_lower = 0.0f, best_value = 0.0f;
         uint32_t best_dim = 0;
+        const float dim[3] = {s.box.max.x - s.box.min.x, s.box.max.y - s.box.min.y,
+                              s.box.max.z - s.box.min.z};
+        const float inv_surface = 1.0f / s.surface;
         for (uint32_t j = 0; j < 3; j++) {
-            if (ab[j].best < best) {
-                best = ab[j].best;
-                best_value = ab[j].value;
-                best_dim = j;
-                best_upper = ab[j].upper;
-                best_lower = ab[j].lower;
+            uint32_t below = 0, cop = 0, above = 0;
+            uint32_t counters[3] = {0, 0, 0};
+            uint32_t plus_prev = 0, cop_prev = 0;
+            const float x = dim[(j + 1) % 3];
+            const float y = dim[(j + 2) % 3];
+            const float xpy = x + y;
+            const float area = x * y;
+            bool first = true;
+            const std::vector<Candidate>& c = s.cand[j];
+            size_t i = 0;
+            while (i < c.size()) {
+                const float value = c[i].value;
+                while (i < c.size() && c[i].value == value) {
+                    counters[c[i].marker]++;
+                    i++;
+                }
+                below = below + cop_prev + plus_prev;
+                cop = cop_prev;
+                above = above - counters[kCoplanarMark] - counters[kEnd];
+                if (first) {
+                    below = 0;
+                    cop = counters[kCoplanarMark];
+                    above = static_cast<uint32_t>(s.eval) - counters[kCoplanarMark];
+                    first = false;
+                }
+                plus_prev = counters[kBegin];
+                cop_prev = counters[kCoplanarMark];
+                counters[0] = counters[1] = counters[2] = 0;
+
+                const float upper = area + (lane(s.box.max, j) - value) * xpy;
+                const float lower = area + (value - lane(s.box.min, j)) * xpy;
+                const float cost_below = lower * static_cast<float>(below);
+                const float cost_above = upper * static_cast<float>(above);
+                const float cost_cop = area * static_cast<float>(cop);
+                const float sah =
+                    kTraverseCost + kIntersectionCost * (((cost_cop + cost_below) + cost_above) * inv_surface);
+                if (sah < best) {
+                    best = sah;
+                    best_value = value;
+                    best_dim = j;
+                    best_upper = upper;
+                    best_lower = lower;
+                }
             }
         }
         const float leaf_cost = kIntersectionCost * static_cast<float>(s.eval);
@@ -282,35 +226,14 @@ private:
             n.flags = best_dim | HULK_NODE_JUNCTION;
         }
         Split a, b;
-        if (par) {
-            // classification in chunks; the children's lists are the chunks'
-            // lists concatenated in order
-            const uint32_t nc = std::min<uint32_t>(threads_, 16);
-            std::vector<std::vector<uint32_t>> ca(nc), cb(nc);
-            const size_t m = s.polys.size();
-            run_parallel(nc, [&](uint32_t k) {
-                for (size_t i = m * k / nc; i < m * (k + 1) / nc; i++) {
-                    const uint32_t p = s.polys[i];
-                    const Relation r = classify(best_dim, best_value, p);
-                    rel[p] = r;
-                    if (r != kBelow) ca[k].push_back(p);
-                    if (r != kAbove) cb[k].push_back(p);
-                }
-            });
-            for (uint32_t k = 0; k < nc; k++) {
-                a.polys.insert(a.polys.end(), ca[k].begin(), ca[k].end());
-                b.polys.insert(b.polys.end(), cb[k].begin(), cb[k].end());
-            }
-        } else {
-            for (uint32_t p : s.polys) {
-                Relation r = classify(best_dim, best_value, p);
-                rel[p] = r;
-                if (r == kAbove) a.polys.push_back(p);
-                else if (r == kBelow) b.polys.push_back(p);
-                else { a.polys.push_back(p); b.polys.push_back(p); }
-            }
+        for (uint32_t p : s.polys) {
+            Relation r = classify(best_dim, best_value, p);
+            rel[p] = r;
+            if (r == kAbove) a.polys.push_back(p);
+            else if (r == kBelow) b.polys.push_back(p);
+            else { a.polys.push_back(p); b.polys.push_back(p); }
         }
-        auto split_axis = [&](uint32_t j) {
+        for (uint32_t j = 0; j < 3; j++) {
             a.cand[j].reserve(s.cand[j].size());
             b.cand[j].reserve(s.cand[j].size());
             for (const Candidate& c : s.cand[j]) {
@@ -332,10 +255,6 @@ private:
                     case kCoplanar: break;  // candidates of coplanar polygons are dropped
                 }
             }
-        };
-        if (par) run_parallel(3, split_axis);
-        else for (uint32_t j = 0; j < 3; j++) split_axis(j);
-        for (uint32_t j = 0; j < 3; j++) {
             a.eval += a.cand[j].size();
             b.eval += b.cand[j].size();
         }
@@ -360,8 +279,7 @@ private:
             b.surface = best_lower;
             set_lane(b.box.max, best_dim, best_value);
         }
-        if (has_a && has_b && a.polys.size() >= kSpawnMinPolys && b.polys.size() >= kSpawnMinPolys &&
-            acquire_thread()) {
+        if (has_a && has_b && s.depth < spawn_depth_) {
             // both subtrees in parallel, each into its own arrays
             Out oa, ob;
             hulk_kd_node empty;
@@ -376,7 +294,6 @@ private:
             });
             build_node(b, ob, rel);
             ta.join();
-            free_threads_.fetch_add(1);
             const uint32_t ia = append(out, oa);
             out.nodes[parent].u.junction.above = ia;
             const uint32_t ib = append(out, ob);
@@ -403,18 +320,10 @@ private:
         }
     }
 
-    bool acquire_thread() {
-        int f = free_threads_.load();
-        while (f > 0)
-            if (free_threads_.compare_exchange_weak(f, f - 1)) return true;
-        return false;
-    }
-
     const hulk_float4* verts_;
     const hulk_tri* tris_;
     uint32_t n_tris_;
-    uint32_t threads_;
-    std::atomic<int> free_threads_;
+    uint32_t spawn_depth_;
 };
 
 }  // namespace
@@ -434,15 +343,17 @@ extern "C" int hulk_kdtree_build_threads(const hulk_float4* verts, uint32_t n_ve
     hulk_kdtree* t = new (std::nothrow) hulk_kdtree;
     if (!t) return HULK_E_ALLOC;
     try {
-        // Up to `threads` threads: large nodes split their own work, and a
-        // junction hands its above subtree to a free thread.  0 = automatic: up to 16
+        // Subtrees below the top `spawn` levels run on their own threads
+        // (up to 2^spawn concurrent builds).  0 = automatic: up to 16
         // threads, one thread for scenes under 20k triangles.
         uint32_t threads = max_threads;
         if (threads == 0) {
             threads = std::min(std::max(1u, std::thread::hardware_concurrency()), 16u);
             if (n_tris < 20000) threads = 1;
         }
-        Builder b(verts, tris, n_tris, threads);
+        uint32_t spawn = 0;
+        while (spawn < 10 && (1u << (spawn + 1)) <= threads) spawn++;
+        Builder b(verts, tris, n_tris, spawn);
         Out res;
         b.build(*scene_box, res);
         t->nodes.swap(res.nodes);
