@@ -81,6 +81,12 @@ class TuneEntry(C.Structure):
                 ("leaf_batch", C.c_uint32), ("ms", C.c_float)]
 
 
+class TuneRequest(C.Structure):
+    _fields_ = [("cam", C.c_float * 16), ("width", C.c_uint32), ("height", C.c_uint32), ("reps", C.c_uint32),
+                ("variant_mask", C.c_uint32), ("objective", C.c_uint32), ("max_bounce", C.c_uint32),
+                ("mirror_mask", C.c_void_p)]
+
+
 EXPORTS = {
     # name: (restype, argtypes)
     "hulk_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(Opts)]),
@@ -108,8 +114,7 @@ EXPORTS = {
     "hulk_batch_ray_count": (C.c_uint32, [C.c_void_p]),
     "hulk_batch_destroy": (None, [C.c_void_p]),
     "hulk_batch_primary_hits": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
-    "hulk_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                C.c_uint32, C.POINTER(TuneEntry), C.c_uint32, C.POINTER(C.c_uint32),
+    "hulk_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(TuneEntry), C.c_uint32, C.POINTER(C.c_uint32),
                                 C.POINTER(Opts)]),
     "hulk_mlp_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
     "hulk_mlp_destroy": (None, [C.c_void_p]),

@@ -21,6 +21,12 @@
 
 using namespace hulk;
 
+// ABI layouts the bindings rely on (SharedStructs.hpp sizes for the first five).
+static_assert(sizeof(hulk_float4) == 16 && sizeof(hulk_tri) == 16 && sizeof(hulk_kd_node) == 16, "layout");
+static_assert(sizeof(hulk_box) == 32 && sizeof(hulk_camera) == 64 && sizeof(hulk_light) == 32, "layout");
+static_assert(sizeof(hulk_ray) == 32 && sizeof(hulk_hit) == 32, "layout");
+static_assert(sizeof(hulk_opts) == 40 && sizeof(hulk_tune_entry) == 32 && sizeof(hulk_tune_request) == 96, "layout");
+
 namespace {
 
 constexpr uint32_t kTopLevelsMax = 12;
@@ -1112,13 +1118,19 @@ struct Cand {
 // given camera with HIP events: stage 1 times every allowed variant at its
 // default shape, stage 2 sweeps the shape of the two fastest variants.
 // The fastest candidate becomes the context's options.
-extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H, uint32_t reps,
-                             uint32_t variant_mask, uint32_t objective, hulk_tune_entry* table, uint32_t cap,
+extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tune_entry* table, uint32_t cap,
                              uint32_t* n_out, hulk_opts* best_out) {
-    if (!c || !cam || !W || !H) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
-    if (objective > HULK_TUNE_FRAME) return fail(c, HULK_E_ILLEGAL_INPUT, "unknown autotune objective");
+    if (!c || !req) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    const hulk_camera* cam = &req->cam;
+    const uint32_t W = req->width, H = req->height;
+    uint32_t reps = req->reps, variant_mask = req->variant_mask;
+    if (!W || !H) return fail(c, HULK_E_ILLEGAL_INPUT, "width and height must be > 0");
+    if (req->objective > HULK_TUNE_FRAME) return fail(c, HULK_E_ILLEGAL_INPUT, "unknown autotune objective");
     if ((uint64_t)W * H > 0xFFFFFFC0ull) return fail(c, HULK_E_ILLEGAL_INPUT, "image too large");
-    const bool frame = objective == HULK_TUNE_FRAME;
+    if (req->max_bounce > 8) return fail(c, HULK_E_ILLEGAL_INPUT, "max_bounce must be <= 8");
+    const bool frame = req->objective == HULK_TUNE_FRAME;
+    const bool bounces = frame && req->max_bounce > 0 && req->mirror_mask;
+    const uint32_t max_bounce = bounces ? req->max_bounce : 0;
     if (!variant_mask) variant_mask = (1u << HULK_VARIANT_FULLSTACK) | (1u << HULK_VARIANT_SHORTSTACK) |
                                       (1u << HULK_VARIANT_LDSSTACK) | (1u << HULK_VARIANT_RESTART) |
                                       (1u << HULK_VARIANT_ROPES);
@@ -1130,8 +1142,10 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
     fs.H = H;
     fs.n = n;
     int r;
-    if ((r = ensure_work(c, c->work, n, c->n_point, false, frame))) return r;
+    if ((r = ensure_work(c, c->work, n, c->n_point, bounces, frame))) return r;
     if (frame && (r = ensure(c, c->rgba, (size_t)n * 4))) return r;
+    const uint8_t* dm = nullptr;
+    if (bounces && (r = stage_in(c, req->mirror_mask, c->n_tris, false, c->mirror, &dm))) return r;
     hulk_hit* out = static_cast<hulk_hit*>(c->work.hits.p);
     uint32_t* px = static_cast<uint32_t*>(c->rgba.p);
     const uint32_t pipe0 = c->opts.pipeline == HULK_PIPELINE_QUEUED ? HULK_PIPELINE_QUEUED : HULK_PIPELINE_FUSED;
@@ -1151,9 +1165,10 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_camera* cam, uint32_t W, ui
         if (check_opts(c, o) || trace_setup(c, o, P, block, lds)) return 1;  // shape not allowed: skip
         auto once = [&]() -> int {
             if (!frame) return trace_primary(c, o, fs, c->work, nullptr, out, false, false);
-            if (use_fused(o, false)) return render_fused(c, o, fs, c->work, nullptr, out, 0, nullptr, px, false, false);
+            if (use_fused(o, bounces))
+                return render_fused(c, o, fs, c->work, nullptr, out, max_bounce, dm, px, false, false);
             int q = trace_primary(c, o, fs, c->work, nullptr, out, false, false);
-            return q ? q : shade_pipeline(c, o, fs, c->work, out, 0, nullptr, px, false);
+            return q ? q : shade_pipeline(c, o, fs, c->work, out, max_bounce, dm, px, false);
         };
         int rr;
         if ((rr = once())) return rr;

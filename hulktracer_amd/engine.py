@@ -299,10 +299,12 @@ class GPUTrace:
                                         int(use_graph), C.byref(hb)))
         return Batch(self, hb)
 
-    def autotune(self, cam: np.ndarray, W: int, H: int, reps: int = 3, variants=None, frame: bool = False):
+    def autotune(self, cam: np.ndarray, W: int, H: int, reps: int = 3, variants=None, frame: bool = False,
+                 max_bounce: int = 0, mirror: np.ndarray | None = None):
         """Per-scene search (hulk_autotune); returns (best options dict, table).
-        frame=True times whole frames (primary + shadow + shading, fused and
-        queued pipelines) instead of the primary trace."""
+        frame=True times whole frames (primary + shadow + shading, with
+        max_bounce mirror generations when `mirror` is given; fused and queued
+        pipelines) instead of the primary trace."""
         mask = 0
         for v in variants or []:
             mask |= 1 << (L.VARIANTS[v] if isinstance(v, str) else int(v))
@@ -310,9 +312,14 @@ class GPUTrace:
         table = (L.TuneEntry * cap)()
         n = C.c_uint32()
         best = L.Opts()
-        self._c(lib().hulk_autotune(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, reps, mask,
-                                    L.TUNE_FRAME if frame else L.TUNE_TRACE, table, cap, C.byref(n),
-                                    C.byref(best)))
+        m = None if mirror is None else np.ascontiguousarray(mirror, np.uint8)
+        req = L.TuneRequest()
+        req.cam[:] = [float(x) for x in np.ascontiguousarray(cam, np.float32).ravel()[:16]]
+        req.width, req.height, req.reps, req.variant_mask = W, H, reps, mask
+        req.objective = L.TUNE_FRAME if frame else L.TUNE_TRACE
+        req.max_bounce = max_bounce
+        req.mirror_mask = None if m is None else m.ctypes.data
+        self._c(lib().hulk_autotune(self.h, C.byref(req), table, cap, C.byref(n), C.byref(best)))
         self.opts = best
         rows = [{f: getattr(table[i], f) for f, _ in L.TuneEntry._fields_} for i in range(n.value)]
         return {f: getattr(best, f) for f, _ in L.Opts._fields_ if f != "stream"}, rows

@@ -67,7 +67,7 @@ def animated(frames: int = 20, W: int = 1024, H: int = 1024, max_bounce: int = 2
     cam = scene.camera(0, H / W)
     tuned = None
     if autotune:
-        tuned, _ = ctx.autotune(cam, W, H, reps=2, frame=True)
+        tuned, _ = ctx.autotune(cam, W, H, reps=2, frame=True, max_bounce=max_bounce, mirror=scene.mirror)
     # ray counts of frame 0 (counted run, not timed)
     ctx.reset_counters()
     ctx.render_frame_dev(cam, W, H, max_bounce, mirror.data_ptr(), rgba.data_ptr(), hits.data_ptr(),

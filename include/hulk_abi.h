@@ -280,13 +280,22 @@ typedef struct hulk_tune_entry {
                                 without mirror generations; the fused and
                                 queued pipelines are both tried            */
 
-/* Per-scene autotuner: times `objective` for `cam` over traversal variant
- * (variant_mask bit per hulk_variant, 0 = all), block size 64-256,
- * short-stack depth 4-16 and LDS top-tree depth 0-12, persistent or not,
- * and makes the fastest the context's options.  No reference counterpart
- * (the reference picks the kernel by --kernelType). */
-int hulk_autotune(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height, uint32_t reps,
-                  uint32_t variant_mask, uint32_t objective, hulk_tune_entry* table, uint32_t table_cap,
+typedef struct hulk_tune_request {
+    hulk_camera cam;            /* the frame to time                         */
+    uint32_t width, height;
+    uint32_t reps;              /* timed launches per candidate (>= 1)        */
+    uint32_t variant_mask;      /* bit per hulk_variant; 0 = every tree variant */
+    uint32_t objective;         /* HULK_TUNE_TRACE or HULK_TUNE_FRAME         */
+    uint32_t max_bounce;        /* frame objective: mirror generations (<= 8) */
+    const uint8_t* mirror_mask; /* frame objective: host flag per triangle, or NULL */
+} hulk_tune_request;
+
+/* Per-scene autotuner: times the request's objective over traversal variant,
+ * block size 64-256, short-stack depth 4-16, LDS top-tree depth 0-12,
+ * persistent or not, one or two records in flight (and, for frames, both
+ * pipelines), and makes the fastest the context's options.  No reference
+ * counterpart (the reference picks the kernel by --kernelType). */
+int hulk_autotune(hulk_ctx* ctx, const hulk_tune_request* request, hulk_tune_entry* table, uint32_t table_cap,
                   uint32_t* n_entries, hulk_opts* best);
 
 /* ----------------------------------------------- learned distance bound -- */
