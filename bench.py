@@ -51,56 +51,162 @@ def kernel_name(opts: dict) -> str:
 
 
 def profiled_traffic(kernel: str, opts: dict, W: int, H: int):
-    """HBM bytes per launch of the primary trace kernel, as measured by the
+    """HBM bytes per launch of the batched frame kernel, as measured by the
     FETCH_SIZE and WRITE_SIZE counter runs whose summary is kept in
-    profiles/ (line: traffic <kernel> <options> <W>x<H> <bytes>).  Only a
-    profile of the same kernel, options and resolution counts; else None."""
-    want = ",".join(f"{k}={int(opts[k])}" for k in OPT_KEYS)
+    profiles/ (line: traffic <kernel> <options> <W>x<H> <bytes>).  A profile
+    of the same kernel instance (variant, frame kernel, records in flight) at
+    the same resolution counts: its bytes are the hit records and pixels
+    written plus the scene reads that miss L2 / MALL, which the launch shape
+    (block size, LDS depth) hardly changes; the profiled options are named
+    in the source field.  Else None."""
+    want = ",".join(f"{k}={int(opts.get(k, 0))}" for k in OPT_KEYS)
+    found = None
     for f in sorted((ROOT / "profiles").glob("*traffic*.txt")):
         for line in f.read_text().splitlines():
             parts = line.split()
-            if len(parts) == 5 and parts[0] == "traffic" and parts[1] == kernel and parts[2] == want \
-                    and parts[3] == f"{W}x{H}":
-                return int(parts[4]), f"profiles/{f.name}"
-    return None, f"no FETCH_SIZE/WRITE_SIZE profile of {kernel} {want} at {W}x{H}"
+            if len(parts) == 5 and parts[0] == "traffic" and parts[1] == kernel and parts[3] == f"{W}x{H}":
+                if found is None or parts[2] == want:
+                    found = (int(parts[4]), f"profiles/{f.name} (profiled options {parts[2]})")
+    if found:
+        return found
+    return None, f"no FETCH_SIZE/WRITE_SIZE profile of {kernel} at {W}x{H}"
 
 
-def animated_kernel(args, W: int, H: int) -> int:
-    """--kernel-only on frame 0 of the animated scene (50 tori + ground, 1,000,002
-    triangles, 2 mirror generations): warmup + steps renders, HIP events."""
+def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, best: dict, log):
+    """plain / batched / learned-bound + batched on `cams` (bounds from an MLP
+    trained on `train_cams`), for the tuned options and for kd-restart.
+    Returns (methods, learned info, batched-launch ms of the tuned options)."""
+    import numpy as np
     import torch
 
     import hulktracer_amd as HT
     from hulktracer_amd import _lib as L
-    from hulktracer_amd import workloads as WL
 
-    opts = {k: int(v) for k, v in (kv.split("=") for kv in args.kernel_only.split(","))}
-    scene = HT.Scene.procedural(L.SCENE_ANIMATED_TORI, seed=7, frame=0)
-    nodes, lp, _ = WL.build(scene)
-    stream = torch.cuda.Stream()
-    ctx = WL.make_ctx(scene, nodes, lp, stream, **opts)
-    mirror = torch.from_numpy(scene.mirror).cuda()
-    rgba = torch.empty(W * H, dtype=torch.int32, device="cuda")
-    hits = torch.empty(W * H * 32, dtype=torch.uint8, device="cuda")
-    cam = scene.camera(0, H / W)
+    n = W * H
+    nf = len(cams)
+    dev = torch.device("cuda:0")
+    hits = torch.empty(n * 32, dtype=torch.uint8, device=dev)
+    rgba = torch.empty(n, dtype=torch.int32, device=dev)
+    frames = [(cam, W, H, 0) for cam in cams]
+    brgba = torch.empty(nf * n, dtype=torch.int32, device=dev)
+    ref_hits = [ctx.trace_camera(cam, W, H) for cam in cams]
+    methods = {}
 
-    def frame():
-        ctx.render_frame_dev(cam, W, H, 2, mirror.data_ptr(), rgba.data_ptr(), hits.data_ptr(), flags=L.FLAG_NO_SYNC)
+    def timed(fn, reps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
 
-    for _ in range(max(args.warmup, 1)):
-        frame()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        frame()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    print(json.dumps({"kernel": kernel_name(opts), "scene": "animated_1M_frame0", "options": opts,
-                      "kernel_ms": round(e0.elapsed_time(e1) / args.steps, 4), "timed_launches": args.steps,
-                      "launches": args.steps + max(args.warmup, 1), "primary_rays_per_launch": W * H}), flush=True)
-    ctx.close()
-    return 0
+    def time_batch(batch, reps):
+        batch.launch(L.FLAG_NO_SYNC)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            batch.launch(L.FLAG_NO_SYNC)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def hits_match(h, ref=None):
+        ref = ref_hits if ref is None else ref
+        return float(np.mean([np.array_equal(h[k * n:(k + 1) * n]["tri"], ref[k]["tri"]) and
+                              np.array_equal(h[k * n:(k + 1) * n]["t"].view(np.uint32),
+                                             ref[k]["t"].view(np.uint32)) for k in range(nf)]))
+
+    def entry(sec, match=1.0):
+        return {"ms_per_step": round(sec * 1e3, 3), "mrays_s": round(nf * n / sec / 1e6, 1), "hit_match": match}
+
+    # plain: frame after frame
+    def plain_step():
+        for cam in cams:
+            ctx.render_frame_dev(cam, W, H, 0, None, rgba.data_ptr(), hits.data_ptr(), flags=L.FLAG_NO_SYNC)
+
+    methods["plain"] = entry(timed(plain_step, args.steps))
+    # batched: one graph for the frames
+    batch = ctx.batch(frames, None, brgba.data_ptr(), use_graph=True)
+    methods["batched"] = entry(timed(lambda: batch.launch(L.FLAG_NO_SYNC), args.steps),
+                               hits_match(batch.primary_hits()))
+    # the batched launch is the dominant kernel (the whole step when the
+    # pipeline is fused): live HIP events on its stream
+    k_ms = time_batch(batch, max(args.steps, 10))
+    batch.close()
+
+    # learned + batched, on the tuned variant and on the reference's live
+    # kernel (kd-restart, GPUTrace_stackless: no exit before the box exit,
+    # so a bound is what lets it stop early)
+    learned_info = None
+    if not args.no_learned:
+        from hulktracer_amd.learned import HitDistanceMLP, calibrate_frames, training_pairs
+
+        tl = time.time()
+        rays, y = training_pairs(ctx, train_cams, W, H, n_samples=4_000_000)
+        n_pairs = int(rays.shape[0])
+        mlp = HitDistanceMLP(ctx, seed=1, max_batch=16384)
+        tt = time.time()
+        losses = mlp.fit(rays, y, steps=args.train_steps, batch=16384, lr=2e-3,
+                         log_every=max(args.train_steps // 4, 1))
+        torch.cuda.synchronize()
+        train_s = time.time() - tt
+        del rays, y
+        margin = calibrate_frames(mlp, ctx, train_cams[::3], W, H, quantile=0.95, stride=2)
+        tb = torch.empty(nf * n, dtype=torch.float32, device=dev)
+
+        def bounds():
+            for k, cam in enumerate(cams):
+                mlp.bound_camera(cam, W, H, tb[k * n:(k + 1) * n], sync=False)
+
+        bounds()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        bounds()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        infer_ms = e0.elapsed_time(e1) / nf
+        learned_info = {"train_pairs": n_pairs, "train_steps": args.train_steps, "train_s": round(train_s, 2),
+                        "data_s": round(tt - tl, 2), "margin": round(margin, 4), "bound_stride": 2,
+                        "inference_ms_per_frame": round(infer_ms, 4),
+                        "inference_tflops": round(n / 4 * 82176 / (infer_ms * 1e-3) / 1e12, 2),
+                        "loss": [[int(a), round(float(b), 4)] for a, b in losses]}
+        opts_now = {k: int(v) for k, v in best.items() if k != "device"}
+        for tag, vopts in (("", opts_now), ("_stackless", dict(opts_now, variant=HT.RESTART))):
+            ctx.set_options(**vopts)
+            ref_v = [ctx.trace_camera(cam, W, H) for cam in cams] if tag else ref_hits
+            if tag:  # the reference's live kernel: plain frame by frame, and batched without a bound
+                methods["plain" + tag] = entry(timed(plain_step, max(1, args.steps // 5)))
+                pbatch = ctx.batch(frames, None, brgba.data_ptr(), use_graph=True)
+                methods["batched" + tag] = entry(timed(lambda: pbatch.launch(L.FLAG_NO_SYNC),
+                                                       max(1, args.steps // 2)),
+                                                 hits_match(pbatch.primary_hits(), ref_v))
+                pbatch.close()
+            lbatch = ctx.batch(frames, None, brgba.data_ptr(), tbound_dev=tb.data_ptr(), use_graph=True)
+
+            def learned_step():
+                bounds()
+                lbatch.launch(L.FLAG_NO_SYNC)
+
+            s_learn = timed(learned_step, args.steps)
+            lh = lbatch.primary_hits()
+            fb = float(np.mean((lh["flags"] & 1) != 0))
+            ctx.reset_counters()
+            ctx.trace_camera(cams[0], W, H, count=True)
+            cp = ctx.counters()["node_visits"]
+            ctx.reset_counters()
+            ctx.trace_camera(cams[0], W, H, tbound=tb[:n].cpu().numpy(), count=True)
+            cb = ctx.counters()["node_visits"]
+            methods["learned+batched" + tag] = dict(entry(s_learn, hits_match(lh, ref_v)),
+                                                    fallback_rate=round(fb, 4),
+                                                    node_visits_saved=round(1 - cb / cp, 4))
+            lbatch.close()
+        ctx.set_options(**opts_now)
+        mlp.close()
+    return methods, learned_info, k_ms
 
 
 def main() -> int:
@@ -234,122 +340,11 @@ def main() -> int:
     bytes_launch = 48 * traced + 16 * c["node_visits"] + 64 * c["tri_tests"]
     n_node, n_tri = c["node_visits"] / traced, c["tri_tests"] / traced
 
-    # reference hits of every frame (plain, tuned options)
-    ref_hits = [ctx.trace_camera(cam, W, Hh) for cam in cams]
     ctx.reset_counters()
     ctx.render_frame(cams[0], W, Hh, count=True)
     shadow_per_frame = ctx.counters()["shadow_rays"]
-
-    methods = {}
-
-    def timed(fn, reps):
-        for _ in range(args.warmup):
-            fn()
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(reps):
-            fn()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t) / reps
-
-    def hits_match(h, ref=None):
-        ref = ref_hits if ref is None else ref
-        return float(np.mean([np.array_equal(h[k * n:(k + 1) * n]["tri"], ref[k]["tri"]) and
-                              np.array_equal(h[k * n:(k + 1) * n]["t"].view(np.uint32),
-                                             ref[k]["t"].view(np.uint32)) for k in range(nf)]))
-
-    # plain: frame after frame
-    def plain_step():
-        for cam in cams:
-            ctx.render_frame_dev(cam, W, Hh, 0, None, rgba.data_ptr(), hits.data_ptr(), flags=L.FLAG_NO_SYNC)
-
-    s_plain = timed(plain_step, args.steps)
-    methods["plain"] = {"ms_per_step": round(s_plain * 1e3, 3), "mrays_s": round(nf * n / s_plain / 1e6, 1),
-                        "hit_match": 1.0}
-
-    # batched: one graph for the 10 frames
-    batch = ctx.batch(frames, None, brgba.data_ptr(), use_graph=True)
-    s_batch = timed(lambda: batch.launch(L.FLAG_NO_SYNC), args.steps)
-    methods["batched"] = {"ms_per_step": round(s_batch * 1e3, 3), "mrays_s": round(nf * n / s_batch / 1e6, 1),
-                          "hit_match": hits_match(batch.primary_hits())}
-    # -- the batched launch is the dominant kernel (the whole step when the
-    # pipeline is fused): live HIP events on its stream
-    k_ms = time_batch(batch, max(args.steps, 10))
-    batch.close()
-
-    # learned + batched, on the tuned variant and on the reference's live
-    # kernel (kd-restart, GPUTrace_stackless: no exit before the box exit,
-    # so a bound is what lets it stop early)
-    learned_info = None
-    if not args.no_learned:
-        from hulktracer_amd.learned import HitDistanceMLP, calibrate_frames, training_pairs
-
-        tl = time.time()
-        train_cams = [scene.camera(k, Hh / W) for k in range(10)]
-        rays, y = training_pairs(ctx, train_cams, W, Hh, n_samples=4_000_000)
-        mlp = HitDistanceMLP(ctx, seed=1, max_batch=16384)
-        tt = time.time()
-        losses = mlp.fit(rays, y, steps=args.train_steps, batch=16384, lr=2e-3,
-                         log_every=max(args.train_steps // 4, 1))
-        torch.cuda.synchronize()
-        train_s = time.time() - tt
-        del rays, y
-        margin = calibrate_frames(mlp, ctx, train_cams[::3], W, Hh, quantile=0.95, stride=2)
-        tb = torch.empty(nf * n, dtype=torch.float32, device=dev)
-
-        def bounds():
-            for k, cam in enumerate(cams):
-                mlp.bound_camera(cam, W, Hh, tb[k * n:(k + 1) * n], sync=False)
-
-        bounds()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        bounds()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        infer_ms = e0.elapsed_time(e1) / nf
-        learned_info = {"train_pairs": 4_000_000, "train_steps": args.train_steps, "train_s": round(train_s, 2),
-                        "data_s": round(tt - tl, 2), "margin": round(margin, 4), "bound_stride": 2,
-                        "inference_ms_per_frame": round(infer_ms, 4),
-                        "inference_tflops": round(n / 4 * 82176 / (infer_ms * 1e-3) / 1e12, 2),
-                        "loss": [[int(a), round(float(b), 4)] for a, b in losses]}
-        opts_now = {k: int(v) for k, v in best.items() if k != "device"}
-        for tag, vopts in (("", opts_now), ("_stackless", dict(opts_now, variant=H.RESTART))):
-            ctx.set_options(**vopts)
-            ref_v = [ctx.trace_camera(cam, W, Hh) for cam in cams] if tag else ref_hits
-            if tag:  # the reference's live kernel: plain frame by frame, and batched without a bound
-                s_pl = timed(plain_step, max(1, args.steps // 5))
-                methods["plain" + tag] = {"ms_per_step": round(s_pl * 1e3, 3),
-                                         "mrays_s": round(nf * n / s_pl / 1e6, 1), "hit_match": 1.0}
-                pbatch = ctx.batch(frames, None, brgba.data_ptr(), use_graph=True)
-                s_pb = timed(lambda: pbatch.launch(L.FLAG_NO_SYNC), max(1, args.steps // 2))
-                methods["batched" + tag] = {"ms_per_step": round(s_pb * 1e3, 3),
-                                           "mrays_s": round(nf * n / s_pb / 1e6, 1),
-                                           "hit_match": hits_match(pbatch.primary_hits(), ref_v)}
-                pbatch.close()
-            lbatch = ctx.batch(frames, None, brgba.data_ptr(), tbound_dev=tb.data_ptr(), use_graph=True)
-
-            def learned_step():
-                bounds()
-                lbatch.launch(L.FLAG_NO_SYNC)
-
-            s_learn = timed(learned_step, args.steps)
-            lh = lbatch.primary_hits()
-            fb = float(np.mean((lh["flags"] & 1) != 0))
-            ctx.reset_counters()
-            ctx.trace_camera(cams[0], W, Hh, count=True)
-            cp = ctx.counters()["node_visits"]
-            ctx.reset_counters()
-            ctx.trace_camera(cams[0], W, Hh, tbound=tb[:n].cpu().numpy(), count=True)
-            cb = ctx.counters()["node_visits"]
-            methods["learned+batched" + tag] = {
-                "ms_per_step": round(s_learn * 1e3, 3), "mrays_s": round(nf * n / s_learn / 1e6, 1),
-                "hit_match": hits_match(lh, ref_v), "fallback_rate": round(fb, 4),
-                "node_visits_saved": round(1 - cb / cp, 4)}
-            lbatch.close()
-        ctx.set_options(**opts_now)
-        mlp.close()
+    train_cams = [scene.camera(k, Hh / W) for k in range(10)]
+    methods, learned_info, k_ms = compare_methods(ctx, scene, cams, train_cams, W, Hh, stream, args, best, log)
     for k, v in methods.items():
         log(f"method {k:16s} {v}")
 
@@ -418,7 +413,24 @@ def main() -> int:
 
     if not args.no_extras:
         extras = {}
-        for name, fn in (("animated_1M", lambda: WL.animated(frames=20, stream=stream, log=log)),
+        def methods_1m():
+            # the same comparison at 1M triangles: geometry of frame 0 of the
+            # animated scene, its camera path poses 0-9 train, 10-19 measured
+            sc = H.Scene.procedural(L.SCENE_ANIMATED_TORI, seed=7, frame=0)
+            nd, lpp, bms = WL.build(sc)
+            cx = WL.make_ctx(sc, nd, lpp, stream, variant=H.FULLSTACK)
+            cams1 = [sc.camera(k, Hh / W) for k in range(10, 20)]
+            b1, _ = cx.autotune(cams1[0], W, Hh, reps=2, frame=True)
+            m1, li1, _ = compare_methods(cx, sc, cams1, [sc.camera(k, Hh / W) for k in range(10)], W, Hh, stream,
+                                         args, b1, log)
+            cx.close()
+            for k, v in m1.items():
+                log(f"1M method {k:16s} {v}")
+            return {"workload": f"animated_tori_{sc.n_tris}tris_frame0_poses10-19_{W}x{Hh}_primary+shadow+shade",
+                    "kd_build_ms": round(bms, 1), "tuned": b1, "methods": m1, "learned": li1}
+
+        for name, fn in (("methods_1M", methods_1m),
+                         ("animated_1M", lambda: WL.animated(frames=20, stream=stream, log=log)),
                          ("batched_64_frames", lambda: WL.batched(stream=stream, log=log))):
             try:
                 extras[name] = fn()
