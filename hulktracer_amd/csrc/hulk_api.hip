@@ -1150,7 +1150,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
     uint32_t* px = static_cast<uint32_t*>(c->rgba.p);
     const uint32_t pipe0 = c->opts.pipeline == HULK_PIPELINE_QUEUED ? HULK_PIPELINE_QUEUED : HULK_PIPELINE_FUSED;
     std::vector<std::pair<Cand, float>> done;
-    auto measure = [&](const Cand& k, float& ms) -> int {
+    auto measure = [&](const Cand& k, float& ms, uint32_t n_reps, bool record) -> int {
         hulk_opts o = c->opts;
         o.variant = k.variant;
         o.block_size = k.block;
@@ -1173,13 +1173,13 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
         int rr;
         if ((rr = once())) return rr;
         HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-        for (uint32_t i = 0; i < reps; i++)
+        for (uint32_t i = 0; i < n_reps; i++)
             if ((rr = once())) return rr;
         HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        ms /= reps;
-        done.push_back({k, ms});
+        ms /= n_reps;
+        if (record) done.push_back({k, ms});
         return 0;
     };
     // stage 1
@@ -1188,7 +1188,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
         if (!(variant_mask & (1u << v))) continue;
         Cand k{v, 256, 6, 0, 0, pipe0, 1};
         float ms;
-        r = measure(k, ms);
+        r = measure(k, ms, reps, true);
         if (r > 1) return r;
         if (r == 0) firsts.push_back({ms, k});
     }
@@ -1217,7 +1217,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
                             // two records in flight: default stack / LDS depths only
                             if (lb == 2 && (dp != 6 || (ld != 0 && ld != 8))) continue;
                             float ms;
-                            r = measure(k, ms);
+                            r = measure(k, ms, reps, true);
                             if (r > 1) return r;
                         }
                 }
@@ -1230,11 +1230,27 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
         Cand k = done[bi].first;
         k.pipeline = k.pipeline == HULK_PIPELINE_FUSED ? HULK_PIPELINE_QUEUED : HULK_PIPELINE_FUSED;
         float ms;
-        r = measure(k, ms);
+        r = measure(k, ms, reps, true);
         if (r > 1) return r;
-        if (r == 0 && ms < done[bi].second) bi = done.size() - 1;
     }
-    const Cand& b = done[bi].first;
+    // stage 4: the four fastest again, with four times the repetitions, so
+    // that timing noise between near-equal shapes does not decide
+    std::vector<size_t> order(done.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    const size_t top = std::min<size_t>(4, order.size());
+    std::partial_sort(order.begin(), order.begin() + top, order.end(),
+                      [&](size_t a, size_t b) { return done[a].second < done[b].second; });
+    float best_ms = 0.0f;
+    for (size_t t = 0; t < top; t++) {
+        float ms;
+        r = measure(done[order[t]].first, ms, 4 * reps, false);
+        if (r > 1) return r;
+        if (r == 0 && (t == 0 || ms < best_ms)) {
+            best_ms = ms;
+            bi = order[t];
+        }
+    }
+    const Cand b = done[bi].first;
     c->opts.variant = b.variant;
     c->opts.block_size = b.block;
     c->opts.short_stack_depth = b.depth;

@@ -77,10 +77,10 @@ def test_autotune_picks_a_parity_clean_config():
     cam = s.camera(6, 1.0)
     best, table = ctx.autotune(cam, 256, 256, reps=2)
     assert len(table) >= 10
-    assert min(r["ms"] for r in table) == pytest.approx(
-        [r["ms"] for r in table if r["variant"] == best["variant"] and r["block_size"] == best["block_size"]
-         and r["short_stack_depth"] == best["short_stack_depth"] and r["lds_top_levels"] == best["lds_top_levels"]
-         and r["persistent"] == best["persistent"]][0])
+    # the choice is one of the four fastest shapes of the sweep (re-timed with more repetitions)
+    keys = ("variant", "block_size", "short_stack_depth", "lds_top_levels", "persistent", "leaf_batch")
+    fastest = sorted(table, key=lambda r: r["ms"])[:4]
+    assert any(all(r[k] == best[k] for k in keys) for r in fastest)
     got = ctx.trace_camera(cam, 256, 256)
     var = {1: O.V_FULLSTACK, 2: O.V_SHORTSTACK, 3: O.V_LDSSTACK, 4: O.V_RESTART, 5: O.V_ROPES}[best["variant"]]
     ref = oracle_scene(2, 1).trace_camera(var, cam, 256, 256, depth=best["short_stack_depth"])
