@@ -142,7 +142,7 @@ def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, 
     # so a bound is what lets it stop early)
     learned_info = None
     if not args.no_learned:
-        from hulktracer_amd.learned import HitDistanceMLP, calibrate_frames, training_pairs
+        from hulktracer_amd.learned import HitDistanceMLP, select_margin, training_pairs
 
         tl = time.time()
         rays, y = training_pairs(ctx, train_cams, W, H, n_samples=4_000_000)
@@ -154,7 +154,6 @@ def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, 
         torch.cuda.synchronize()
         train_s = time.time() - tt
         del rays, y
-        margin = calibrate_frames(mlp, ctx, train_cams[::3], W, H, quantile=0.95, stride=2)
         tb = torch.empty(nf * n, dtype=torch.float32, device=dev)
 
         def bounds():
@@ -170,13 +169,16 @@ def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, 
         torch.cuda.synchronize()
         infer_ms = e0.elapsed_time(e1) / nf
         learned_info = {"train_pairs": n_pairs, "train_steps": args.train_steps, "train_s": round(train_s, 2),
-                        "data_s": round(tt - tl, 2), "margin": round(margin, 4), "bound_stride": 2,
+                        "data_s": round(tt - tl, 2), "bound_stride": 2,
                         "inference_ms_per_frame": round(infer_ms, 4),
                         "inference_tflops": round(n / 4 * 82176 / (infer_ms * 1e-3) / 1e12, 2),
-                        "loss": [[int(a), round(float(b), 4)] for a, b in losses]}
+                        "loss": [[int(a), round(float(b), 4)] for a, b in losses], "margin_selection": {}}
         opts_now = {k: int(v) for k, v in best.items() if k != "device"}
         for tag, vopts in (("", opts_now), ("_stackless", dict(opts_now, variant=HT.RESTART))):
             ctx.set_options(**vopts)
+            # margin (and whether to bound at all) chosen on training frames for this traversal
+            sel = select_margin(mlp, ctx, train_cams[::3], W, H, stride=2)
+            learned_info["margin_selection"][tag or "tuned"] = sel
             ref_v = [ctx.trace_camera(cam, W, H) for cam in cams] if tag else ref_hits
             if tag:  # the reference's live kernel: plain frame by frame, and batched without a bound
                 methods["plain" + tag] = entry(timed(plain_step, max(1, args.steps // 5)))
@@ -185,15 +187,22 @@ def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, 
                                                        max(1, args.steps // 2)),
                                                  hits_match(pbatch.primary_hits(), ref_v))
                 pbatch.close()
-            lbatch = ctx.batch(frames, None, brgba.data_ptr(), tbound_dev=tb.data_ptr(), use_graph=True)
+            use_bound = mlp.enabled
+            lbatch = ctx.batch(frames, None, brgba.data_ptr(), tbound_dev=tb.data_ptr() if use_bound else None,
+                               use_graph=True)
 
             def learned_step():
-                bounds()
+                if use_bound:  # a bound that saves no work is not computed (select_margin)
+                    bounds()
                 lbatch.launch(L.FLAG_NO_SYNC)
 
+            if use_bound:
+                bounds()
             s_learn = timed(learned_step, args.steps)
             lh = lbatch.primary_hits()
             fb = float(np.mean((lh["flags"] & 1) != 0))
+            bounds()  # node visits with the selected margin, reported also where the bound is not used
+            torch.cuda.synchronize()
             ctx.reset_counters()
             ctx.trace_camera(cams[0], W, H, count=True)
             cp = ctx.counters()["node_visits"]
@@ -201,7 +210,7 @@ def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, 
             ctx.trace_camera(cams[0], W, H, tbound=tb[:n].cpu().numpy(), count=True)
             cb = ctx.counters()["node_visits"]
             methods["learned+batched" + tag] = dict(entry(s_learn, hits_match(lh, ref_v)),
-                                                    fallback_rate=round(fb, 4),
+                                                    bound_used=bool(use_bound), fallback_rate=round(fb, 4),
                                                     node_visits_saved=round(1 - cb / cp, 4))
             lbatch.close()
         ctx.set_options(**opts_now)

@@ -30,6 +30,7 @@ class HitDistanceMLP:
             raise HulkError(rc, "hulk_mlp_create failed")
         self.margin = 0.0
         self.stride = 1
+        self.enabled = True  # select_margin turns the bound off where it saves no traversal work
 
     def _c(self, rc):
         if rc:
@@ -144,6 +145,57 @@ def calibrate_frames(mlp: HitDistanceMLP, ctx: GPUTrace, cams, W: int, H: int, q
     mlp.stride = stride
     mlp.margin = float(torch.quantile(r[: 1 << 24], quantile).item())
     return mlp.margin
+
+
+def select_margin(mlp: HitDistanceMLP, ctx: GPUTrace, cams, W: int, H: int,
+                  quantiles=(0.8, 0.9, 0.95, 0.98, 0.995), stride: int = 2, min_saving: float = 0.02) -> dict:
+    """Margin chosen on training frames by the traversal's own counts: for
+    each quantile of log(t) - log(bound at margin 0), the node visits of the
+    bounded traversal (fallbacks included) with the context's current
+    options; the quantile with the fewest visits wins.  The bound is enabled
+    (mlp.enabled) only if it saves at least `min_saving` of the unbounded
+    traversal's visits: a front-to-back traversal that stops at its first hit
+    gains nothing from an upper bound, a traversal without early exit
+    (kd-restart) does."""
+    import torch
+
+    n = W * H
+    tb = torch.empty(n, dtype=torch.float32, device="cuda")
+    hits = torch.empty(n * 32, dtype=torch.uint8, device="cuda")
+    res = []
+    for cam in cams:
+        _, texit = camera_rays(ctx, cam, W, H)
+        mlp.bound_camera(cam, W, H, tb, margin=0.0, max_log=80.0, stride=stride)
+        ctx.trace_camera_dev(cam, W, H, hits.data_ptr())
+        h = hits.view(torch.float32).view(-1, 8)
+        hit = h[:, 3].contiguous().view(torch.int32) != -1
+        lab = torch.where(hit, h[:, 0], texit)
+        keep = lab > 0
+        res.append(torch.log(lab[keep]) - torch.log(tb[keep]))
+    r = torch.cat(res)[: 1 << 24]
+
+    def visits(margin):
+        ctx.reset_counters()
+        for cam in cams:
+            if margin is None:
+                ctx.trace_camera_dev(cam, W, H, hits.data_ptr(), flags=L.FLAG_COUNT)
+            else:
+                mlp.bound_camera(cam, W, H, tb, margin=margin, stride=stride)
+                ctx.trace_camera_dev(cam, W, H, hits.data_ptr(), tbound_dev=tb.data_ptr(), flags=L.FLAG_COUNT)
+        return ctx.counters()["node_visits"]
+
+    base = visits(None)
+    cands = {}
+    for q in quantiles:
+        m = float(torch.quantile(r, q).item())
+        cands[q] = (m, 1.0 - visits(m) / max(base, 1))
+    q_best = max(cands, key=lambda q: cands[q][1])
+    mlp.stride = stride
+    mlp.margin = cands[q_best][0]
+    mlp.enabled = cands[q_best][1] >= min_saving
+    return {"quantile": q_best, "margin": round(mlp.margin, 4), "node_visits_saved": round(cands[q_best][1], 4),
+            "enabled": bool(mlp.enabled),
+            "candidates": {str(q): [round(m, 4), round(sv, 4)] for q, (m, sv) in cands.items()}}
 
 
 def camera_rays(ctx: GPUTrace, cam, W: int, H: int):
