@@ -72,6 +72,44 @@ def profiled_traffic(kernel: str, opts: dict, W: int, H: int):
     return None, f"no FETCH_SIZE/WRITE_SIZE profile of {kernel} at {W}x{H}"
 
 
+def animated_kernel(args, W: int, H: int) -> int:
+    """--kernel-only on frame 0 of the animated scene (50 tori + ground, 1,000,002
+    triangles, 2 mirror generations): warmup + steps renders, HIP events."""
+    import torch
+
+    import hulktracer_amd as HT
+    from hulktracer_amd import _lib as L
+    from hulktracer_amd import workloads as WL
+
+    opts = {k: int(v) for k, v in (kv.split("=") for kv in args.kernel_only.split(","))}
+    scene = HT.Scene.procedural(L.SCENE_ANIMATED_TORI, seed=7, frame=0)
+    nodes, lp, _ = WL.build(scene)
+    stream = torch.cuda.Stream()
+    ctx = WL.make_ctx(scene, nodes, lp, stream, **opts)
+    mirror = torch.from_numpy(scene.mirror).cuda()
+    rgba = torch.empty(W * H, dtype=torch.int32, device="cuda")
+    hits = torch.empty(W * H * 32, dtype=torch.uint8, device="cuda")
+    cam = scene.camera(0, H / W)
+
+    def frame():
+        ctx.render_frame_dev(cam, W, H, 2, mirror.data_ptr(), rgba.data_ptr(), hits.data_ptr(), flags=L.FLAG_NO_SYNC)
+
+    for _ in range(max(args.warmup, 1)):
+        frame()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        frame()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel": kernel_name(opts), "scene": "animated_1M_frame0", "options": opts,
+                      "kernel_ms": round(e0.elapsed_time(e1) / args.steps, 4), "timed_launches": args.steps,
+                      "launches": args.steps + max(args.warmup, 1), "primary_rays_per_launch": W * H}), flush=True)
+    ctx.close()
+    return 0
+
+
 def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, best: dict, log):
     """plain / batched / learned-bound + batched on `cams` (bounds from an MLP
     trained on `train_cams`), for the tuned options and for kd-restart.

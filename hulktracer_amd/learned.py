@@ -31,10 +31,22 @@ class HitDistanceMLP:
         self.margin = 0.0
         self.stride = 1
         self.enabled = True  # select_margin turns the bound off where it saves no traversal work
+        self._lib_stream = None
 
     def _c(self, rc):
         if rc:
             raise HulkError(rc, (lib().hulk_mlp_last_error(self.h) or b"").decode())
+
+    def _order(self):
+        """Device inputs come from torch's current stream: the library's
+        stream waits for it before reading them."""
+        import torch
+
+        if self._lib_stream is None:
+            self._lib_stream = torch.cuda.ExternalStream(self.ctx.stream)
+        cur = torch.cuda.current_stream()
+        if cur.cuda_stream != self._lib_stream.cuda_stream:
+            self._lib_stream.wait_stream(cur)
 
     def close(self):
         if self.h:
@@ -63,15 +75,18 @@ class HitDistanceMLP:
         self._c(lib().hulk_mlp_set_params(self.h, ptr(p), ptr(f)))
 
     def forward(self, rays_dev, n: int, pred_dev):
+        self._order()
         self._c(lib().hulk_mlp_forward(self.h, ptr(rays_dev), n, ptr(pred_dev)))
 
     def loss_grad(self, rays_dev, y_dev, n: int):
+        self._order()
         loss = C.c_float()
         g = np.zeros(self.param_count(), np.float32)
         self._c(lib().hulk_mlp_loss_grad(self.h, ptr(rays_dev), ptr(y_dev), n, C.byref(loss), ptr(g)))
         return loss.value, g
 
     def train_step(self, rays_dev, y_dev, n: int, lr: float, want_loss: bool = False):
+        self._order()
         loss = C.c_float()
         self._c(lib().hulk_mlp_train_step(self.h, ptr(rays_dev), ptr(y_dev), n, lr,
                                           C.byref(loss) if want_loss else None))
@@ -79,6 +94,7 @@ class HitDistanceMLP:
 
     def bound_camera(self, cam, W: int, H: int, tbound_dev, margin: float | None = None, max_log: float = 60.0,
                      stride: int | None = None, sync: bool = True):
+        self._order()
         m = self.margin if margin is None else margin
         st = self.stride if stride is None else stride
         self._c(lib().hulk_mlp_bound_camera(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, m, max_log,
@@ -87,7 +103,11 @@ class HitDistanceMLP:
     # ------------------------------------------------------------ training --
 
     def fit(self, rays, y, steps: int, batch: int = 16384, lr: float = 2e-3, seed: int = 0, log_every: int = 0):
-        """rays: (N, 8) float32 cuda tensor, y: (N,) float32 cuda tensor."""
+        """rays: (N, 8) float32 cuda tensor, y: (N,) float32 cuda tensor.
+
+        The batches are gathered on the library's stream: the kernels of a
+        step read them there, and the caching allocator may hand a batch's
+        memory to the next gather only once that stream is past the step."""
         import torch
 
         n = rays.shape[0]
@@ -95,15 +115,19 @@ class HitDistanceMLP:
         g = torch.Generator(device=rays.device)
         g.manual_seed(seed)
         losses = []
-        for s in range(steps):
-            idx = torch.randint(0, n, (batch,), device=rays.device, generator=g)
-            rb = rays.index_select(0, idx).contiguous()
-            yb = y.index_select(0, idx).contiguous()
-            cur_lr = lr * 0.5 * (1 + math.cos(math.pi * s / steps))  # cosine decay
-            want = bool(log_every) and (s % log_every == 0 or s == steps - 1)
-            loss = self.train_step(rb, yb, batch, cur_lr, want_loss=want)
-            if want:
-                losses.append((s, loss))
+        lib_stream = torch.cuda.ExternalStream(self.ctx.stream, device=rays.device)
+        lib_stream.wait_stream(torch.cuda.current_stream(rays.device))
+        with torch.cuda.stream(lib_stream):
+            for s in range(steps):
+                idx = torch.randint(0, n, (batch,), device=rays.device, generator=g)
+                rb = rays.index_select(0, idx).contiguous()
+                yb = y.index_select(0, idx).contiguous()
+                cur_lr = lr * 0.5 * (1 + math.cos(math.pi * s / steps))  # cosine decay
+                want = bool(log_every) and (s % log_every == 0 or s == steps - 1)
+                loss = self.train_step(rb, yb, batch, cur_lr, want_loss=want)
+                if want:
+                    losses.append((s, loss))
+        torch.cuda.current_stream(rays.device).wait_stream(lib_stream)
         return losses
 
     def calibrate(self, rays, y, quantile: float = 0.95, chunk: int = 1 << 20) -> float:
@@ -115,7 +139,7 @@ class HitDistanceMLP:
         for a in range(0, rays.shape[0], chunk):
             r = rays[a:a + chunk].contiguous()
             pred = torch.empty(r.shape[0], dtype=torch.float32, device=r.device)
-            self.forward(r, r.shape[0], pred)
+                self.forward(r, r.shape[0], pred)
             res.append(y[a:a + chunk] - pred)
         res = torch.cat(res)
         self.margin = float(torch.quantile(res.float()[:1 << 24], quantile).item())

@@ -171,3 +171,26 @@ def test_margin_selection_by_node_visits():
     savings = [v[1] for v in sel["candidates"].values()]
     assert sel["node_visits_saved"] == max(savings)
     assert sel["enabled"] and mlp.enabled and sel["node_visits_saved"] > 0.05, sel
+
+
+@pytest.mark.gpu
+def test_fit_generalizes_on_held_out_pairs():
+    """Random batches from a large pair set (gathered on the library's
+    stream): the held-out loss falls well below the labels' variance."""
+    import torch
+
+    from hulktracer_amd.learned import training_pairs
+
+    s, ctx, mlp, _, _ = _gpu_setup()
+    cams = [s.camera(k, 1.0) for k in range(0, 10, 2)]
+    rays, y = training_pairs(ctx, cams, 512, 512)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    perm = torch.randperm(rays.shape[0], device="cuda", generator=g)
+    held, train = perm[:16384], perm[16384:]
+    hr, hy = rays[held].contiguous(), y[held].contiguous()
+    tr, ty = rays[train].contiguous(), y[train].contiguous()
+    mlp.fit(tr, ty, steps=400, batch=16384, lr=2e-3)
+    loss, _ = mlp.loss_grad(hr, hy, 16384)
+    var = float(hy.var())
+    assert loss < 0.1 * var, (loss, var)
