@@ -192,6 +192,7 @@ def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, 
         torch.cuda.synchronize()
         train_s = time.time() - tt
         del rays, y
+        mlp.stride = 2  # one prediction per 2x2 pixels (select_margin keeps it)
         tb = torch.empty(nf * n, dtype=torch.float32, device=dev)
 
         def bounds():

@@ -172,7 +172,8 @@ def calibrate_frames(mlp: HitDistanceMLP, ctx: GPUTrace, cams, W: int, H: int, q
 
 
 def select_margin(mlp: HitDistanceMLP, ctx: GPUTrace, cams, W: int, H: int,
-                  quantiles=(0.8, 0.9, 0.95, 0.98, 0.995), stride: int = 2, min_saving: float = 0.02) -> dict:
+                  quantiles=(0.5, 0.65, 0.8, 0.9, 0.95, 0.98, 0.995), stride: int = 2,
+                  min_saving: float = 0.02) -> dict:
     """Margin chosen on training frames by the traversal's own counts: for
     each quantile of log(t) - log(bound at margin 0), the node visits of the
     bounded traversal (fallbacks included) with the context's current
