@@ -139,7 +139,7 @@ class HitDistanceMLP:
         for a in range(0, rays.shape[0], chunk):
             r = rays[a:a + chunk].contiguous()
             pred = torch.empty(r.shape[0], dtype=torch.float32, device=r.device)
-                self.forward(r, r.shape[0], pred)
+            self.forward(r, r.shape[0], pred)
             res.append(y[a:a + chunk] - pred)
         res = torch.cat(res)
         self.margin = float(torch.quantile(res.float()[:1 << 24], quantile).item())

@@ -100,3 +100,12 @@ def test_settings_mirror_reference_defaults():
     assert s["--kernelType"] == "GPUTrace" and s["--treeType"] == "SAHNlogN"
     with pytest.raises(ValueError):
         H.Settings(["--bogus=1"])
+
+
+def test_package_modules_import():
+    """Every module of the package and the bench import without a GPU."""
+    import importlib
+
+    for name in ("hulktracer_amd.engine", "hulktracer_amd.learned", "hulktracer_amd.workloads",
+                 "hulktracer_amd.build", "oracle", "oracle.mlp_ref", "oracle.kdtree_ref"):
+        importlib.import_module(name)
