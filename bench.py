@@ -183,7 +183,9 @@ def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, 
         from hulktracer_amd.learned import HitDistanceMLP, select_margin, training_pairs
 
         tl = time.time()
-        rays, y = training_pairs(ctx, train_cams, W, H, n_samples=4_000_000)
+        # train on every other pose; the rest calibrate the margin (frames the network has not seen)
+        fit_cams, calib_cams = train_cams[0::2], train_cams[1::2]
+        rays, y = training_pairs(ctx, fit_cams, W, H, n_samples=4_000_000)
         n_pairs = int(rays.shape[0])
         mlp = HitDistanceMLP(ctx, seed=1, max_batch=16384)
         tt = time.time()
@@ -216,7 +218,7 @@ def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, 
         for tag, vopts in (("", opts_now), ("_stackless", dict(opts_now, variant=HT.RESTART))):
             ctx.set_options(**vopts)
             # margin (and whether to bound at all) chosen on training frames for this traversal
-            sel = select_margin(mlp, ctx, train_cams[::3], W, H, stride=2)
+            sel = select_margin(mlp, ctx, calib_cams, W, H, stride=2)
             learned_info["margin_selection"][tag or "tuned"] = sel
             ref_v = [ctx.trace_camera(cam, W, H) for cam in cams] if tag else ref_hits
             if tag:  # the reference's live kernel: plain frame by frame, and batched without a bound

@@ -173,22 +173,23 @@ def calibrate_frames(mlp: HitDistanceMLP, ctx: GPUTrace, cams, W: int, H: int, q
 
 def select_margin(mlp: HitDistanceMLP, ctx: GPUTrace, cams, W: int, H: int,
                   quantiles=(0.5, 0.65, 0.8, 0.9, 0.95, 0.98, 0.995), stride: int = 2,
-                  min_saving: float = 0.02) -> dict:
-    """Margin chosen on training frames by the traversal's own counts: for
-    each quantile of log(t) - log(bound at margin 0), the node visits of the
-    bounded traversal (fallbacks included) with the context's current
-    options; the quantile with the fewest visits wins.  The bound is enabled
-    (mlp.enabled) only if it saves at least `min_saving` of the unbounded
-    traversal's visits: a front-to-back traversal that stops at its first hit
-    gains nothing from an upper bound, a traversal without early exit
-    (kd-restart) does."""
+                  min_saving: float = 0.02, reps: int = 3) -> dict:
+    """Margin chosen on calibration frames (frames the network was not
+    trained on) by the traversal itself: for each quantile of
+    log(t) - log(bound at margin 0), the time of the bounded primary traces
+    (fallbacks included) with the context's current options; the fastest
+    quantile wins.  The bound is enabled (mlp.enabled) only if bounded
+    traces plus inference beat the unbounded traces by `min_saving`: a
+    front-to-back traversal that stops at its first hit gains nothing from an
+    upper bound, a traversal without early exit (kd-restart) does."""
     import torch
 
     n = W * H
-    tb = torch.empty(n, dtype=torch.float32, device="cuda")
+    tbs = [torch.empty(n, dtype=torch.float32, device="cuda") for _ in cams]
     hits = torch.empty(n * 32, dtype=torch.uint8, device="cuda")
+    lib_stream = torch.cuda.ExternalStream(ctx.stream)
     res = []
-    for cam in cams:
+    for cam, tb in zip(cams, tbs):
         _, texit = camera_rays(ctx, cam, W, H)
         mlp.bound_camera(cam, W, H, tb, margin=0.0, max_log=80.0, stride=stride)
         ctx.trace_camera_dev(cam, W, H, hits.data_ptr())
@@ -199,28 +200,43 @@ def select_margin(mlp: HitDistanceMLP, ctx: GPUTrace, cams, W: int, H: int,
         res.append(torch.log(lab[keep]) - torch.log(tb[keep]))
     r = torch.cat(res)[: 1 << 24]
 
-    def visits(margin):
-        ctx.reset_counters()
-        for cam in cams:
-            if margin is None:
-                ctx.trace_camera_dev(cam, W, H, hits.data_ptr(), flags=L.FLAG_COUNT)
-            else:
-                mlp.bound_camera(cam, W, H, tb, margin=margin, stride=stride)
-                ctx.trace_camera_dev(cam, W, H, hits.data_ptr(), tbound_dev=tb.data_ptr(), flags=L.FLAG_COUNT)
-        return ctx.counters()["node_visits"]
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(lib_stream)
+        for _ in range(reps):
+            fn()
+        e1.record(lib_stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
 
-    base = visits(None)
+    def traces(bounded):
+        for cam, tb in zip(cams, tbs):
+            ctx.trace_camera_dev(cam, W, H, hits.data_ptr(), tbound_dev=tb.data_ptr() if bounded else None,
+                                 flags=L.FLAG_NO_SYNC)
+
+    def infer(margin):
+        for cam, tb in zip(cams, tbs):
+            mlp.bound_camera(cam, W, H, tb, margin=margin, stride=stride, sync=False)
+
+    base_ms = timed(lambda: traces(False))
     cands = {}
     for q in quantiles:
         m = float(torch.quantile(r, q).item())
-        cands[q] = (m, 1.0 - visits(m) / max(base, 1))
-    q_best = max(cands, key=lambda q: cands[q][1])
+        infer(m)
+        cands[q] = (m, timed(lambda: traces(True)))
+    q_best = min(cands, key=lambda q: cands[q][1])
+    m_best, t_best = cands[q_best]
+    infer_ms = timed(lambda: infer(m_best))
+    saving = 1.0 - (t_best + infer_ms) / base_ms
     mlp.stride = stride
-    mlp.margin = cands[q_best][0]
-    mlp.enabled = cands[q_best][1] >= min_saving
-    return {"quantile": q_best, "margin": round(mlp.margin, 4), "node_visits_saved": round(cands[q_best][1], 4),
+    mlp.margin = m_best
+    mlp.enabled = saving >= min_saving
+    return {"quantile": q_best, "margin": round(m_best, 4), "unbounded_ms": round(base_ms, 4),
+            "bounded_ms": round(t_best, 4), "inference_ms": round(infer_ms, 4), "time_saved": round(saving, 4),
             "enabled": bool(mlp.enabled),
-            "candidates": {str(q): [round(m, 4), round(sv, 4)] for q, (m, sv) in cands.items()}}
+            "candidates": {str(q): [round(m, 4), round(t, 4)] for q, (m, t) in cands.items()}}
 
 
 def camera_rays(ctx: GPUTrace, cam, W: int, H: int):

@@ -157,9 +157,9 @@ def test_learned_bound_keeps_hits_exact(variant):
 
 @pytest.mark.gpu
 def test_margin_selection_by_node_visits():
-    """select_margin picks, per traversal, the quantile with the fewest
-    counted node visits on training frames, and enables the bound only when
-    it saves work: it does for kd-restart (no early exit)."""
+    """select_margin picks, per traversal, the fastest quantile on
+    calibration frames, and enables the bound only when it saves time: it
+    does for kd-restart (no early exit)."""
     from hulktracer_amd.learned import select_margin, training_pairs
 
     s, ctx, mlp, _, _ = _gpu_setup()
@@ -167,10 +167,10 @@ def test_margin_selection_by_node_visits():
     cams = [s.camera(k, 1.0) for k in (0, 4, 8)]
     rays, y = training_pairs(ctx, cams, 256, 256)
     mlp.fit(rays, y, steps=400, batch=16384, lr=2e-3)
-    sel = select_margin(mlp, ctx, cams, 256, 256)
-    savings = [v[1] for v in sel["candidates"].values()]
-    assert sel["node_visits_saved"] == max(savings)
-    assert sel["enabled"] and mlp.enabled and sel["node_visits_saved"] > 0.05, sel
+    sel = select_margin(mlp, ctx, [s.camera(k, 1.0) for k in (2, 6)], 256, 256)
+    times = [v[1] for v in sel["candidates"].values()]
+    assert sel["bounded_ms"] == min(times)
+    assert sel["enabled"] and mlp.enabled and sel["time_saved"] > 0.05, sel
 
 
 @pytest.mark.gpu
