@@ -134,12 +134,14 @@ int check_opts(hulk_ctx* c, const hulk_opts& o) {
     if (o.short_stack_depth < 1 || o.short_stack_depth > kMaxShortStack)
         return fail(c, HULK_E_ILLEGAL_INPUT, "short_stack_depth must be in [1, 16]");
     if (o.lds_top_levels > kTopLevelsMax) return fail(c, HULK_E_ILLEGAL_INPUT, "lds_top_levels must be <= 12");
-    if (o.pipeline > HULK_PIPELINE_FUSED) return fail(c, HULK_E_ILLEGAL_INPUT, "unknown pipeline");
+    if (o.pipeline > HULK_PIPELINE_REFILL) return fail(c, HULK_E_ILLEGAL_INPUT, "unknown pipeline");
+    if (o.pipeline == HULK_PIPELINE_REFILL && (o.variant != HULK_VARIANT_ROPES || o.block_size > 256))
+        return fail(c, HULK_E_ILLEGAL_INPUT, "the refill pipeline needs the ropes variant and blocks of <= 256");
     if (o.leaf_batch > 2) return fail(c, HULK_E_ILLEGAL_INPUT, "leaf_batch must be 0, 1 or 2");
     return HULK_OK;
 }
 
-// Frame launch model of hulk_pipeline (AUTO: fused).
+// Frame launch model of hulk_pipeline (AUTO: fused; REFILL is a fused launch too).
 bool use_fused(const hulk_opts& o, bool /*bounces*/) { return o.pipeline != HULK_PIPELINE_QUEUED; }
 
 Camera to_cam(const hulk_camera& c) {
@@ -188,7 +190,7 @@ int trace_setup(hulk_ctx* c, const hulk_opts& o, TraceParams& P, uint32_t& block
     P.rope_nodes = static_cast<const uint4*>(c->rope_nodes.p);
     P.rope_slots = static_cast<const RopeSlot*>(c->rope_slots.p);
     P.box = c->box;
-    P.guard = 4u * (c->n_nodes + c->n_slots) + 4096u;
+    P.guard = static_cast<uint32_t>(std::min<uint64_t>(4ull * (c->n_nodes + c->n_slots) + 4096u, 0x7FFFFFFFull));
     block = o.block_size;
     uint32_t lds_nodes = 0;
     if (v != HULK_VARIANT_BRUTEFORCE && v != HULK_VARIANT_ROPES)
@@ -339,6 +341,47 @@ int trace_primary(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, 
                      static_cast<uint32_t*>(w.wq.p), timed);
 }
 
+// Whole frame in one launch with lane refill (REFILL pipeline): the same
+// outputs as render_fused.
+int render_refill(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, const float* tbound, hulk_hit* out,
+                  uint32_t max_bounce, const uint8_t* mirror_dev, uint32_t* rgba, bool count, bool timed) {
+    TraceParams P;
+    uint32_t block;
+    size_t lds;
+    int r = trace_setup(c, o, P, block, lds);
+    if (r) return r;
+    if (fs.n == 0) return HULK_OK;
+    if (fs.frames) {
+        P.source = kSourceFrames;
+        P.frames = fs.frames;
+        P.n_frames = fs.n_frames;
+    } else {
+        P.source = kSourceCamera;
+        P.cam = fs.cam;
+        P.W = fs.W;
+        P.H = fs.H;
+    }
+    P.n = fs.n;
+    P.tbound = tbound;
+    P.hits = out;
+    P.lights = static_cast<const float4*>(c->d_lights.p);
+    P.n_lights = static_cast<uint32_t>(c->lights.size());
+    P.mirror = mirror_dev;
+    P.max_bounce = mirror_dev ? max_bounce : 0;
+    P.rgba = rgba;
+    P.work_counter = static_cast<uint32_t*>(w.wq.p);
+    P.persistent = 1;
+    HIP_TRY(c, launch_fill_u32(P.work_counter, kWorkCounterWords, 0u, c->stream));
+    // one resident set of waves (2048 threads per CU), fewer for small frames
+    const uint32_t per_cu = std::max<uint32_t>(1u, 2048u / block);
+    const uint32_t grid =
+        std::max<uint32_t>(1u, std::min<uint32_t>((fs.n + block - 1) / block, c->num_cus * per_cu));
+    if (timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, launch_render_refill(P, count, grid, block, c->stream));
+    if (timed) HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return HULK_OK;
+}
+
 // Whole frame in one launch (FUSED pipeline): primary hit records to `out`,
 // packed pixels to `rgba`.  No allocation, no synchronization.
 int render_fused(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, const float* tbound, hulk_hit* out,
@@ -368,6 +411,15 @@ int render_fused(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, c
     P.rgba = rgba;
     return run_trace(c, o.variant, P, fs.n, block, lds, count, o.persistent != 0, static_cast<uint32_t*>(w.wq.p),
                      timed, true);
+}
+
+// The fused frame launch of the options' pipeline (FUSED or REFILL).
+int render_frame_launch(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, const float* tbound,
+                        hulk_hit* out, uint32_t max_bounce, const uint8_t* mirror_dev, uint32_t* rgba, bool count,
+                        bool timed) {
+    if (o.pipeline == HULK_PIPELINE_REFILL)
+        return render_refill(c, o, fs, w, tbound, out, max_bounce, mirror_dev, rgba, count, timed);
+    return render_fused(c, o, fs, w, tbound, out, max_bounce, mirror_dev, rgba, count, timed);
 }
 
 // Shading of a primary hit buffer: per generation g, point-light shadow
@@ -883,7 +935,7 @@ extern "C" int hulk_render_frame(hulk_ctx* c, const hulk_camera* cam, uint32_t W
     }
     if (count && (r = zero_stats(c))) return r;
     if (use_fused(c->opts, bounces)) {
-        if ((r = render_fused(c, c->opts, fs, c->work, dt, h0, bounces ? max_bounce : 0, dm, out, count, true)))
+        if ((r = render_frame_launch(c, c->opts, fs, c->work, dt, h0, bounces ? max_bounce : 0, dm, out, count, true)))
             return r;
     } else {
         if ((r = trace_primary(c, c->opts, fs, c->work, dt, h0, count, true))) return r;
@@ -966,7 +1018,7 @@ int batch_record(hulk_batch* b, bool count) {
     hulk_hit* h0 = static_cast<hulk_hit*>(b->w.hits.p);
     const uint8_t* dm = b->max_bounce ? static_cast<const uint8_t*>(b->d_mirror.p) : nullptr;
     if (use_fused(b->opts, dm != nullptr))
-        return render_fused(c, b->opts, fs, b->w, b->tbound, h0, dm ? b->max_bounce : 0, dm, b->rgba, count, false);
+        return render_frame_launch(c, b->opts, fs, b->w, b->tbound, h0, dm ? b->max_bounce : 0, dm, b->rgba, count, false);
     if ((r = trace_primary(c, b->opts, fs, b->w, b->tbound, h0, count, false))) return r;
     return shade_pipeline(c, b->opts, fs, b->w, h0, dm ? b->max_bounce : 0, dm, b->rgba, count);
 }
@@ -1166,7 +1218,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
         auto once = [&]() -> int {
             if (!frame) return trace_primary(c, o, fs, c->work, nullptr, out, false, false);
             if (use_fused(o, bounces))
-                return render_fused(c, o, fs, c->work, nullptr, out, max_bounce, dm, px, false, false);
+                return render_frame_launch(c, o, fs, c->work, nullptr, out, max_bounce, dm, px, false, false);
             int q = trace_primary(c, o, fs, c->work, nullptr, out, false, false);
             return q ? q : shade_pipeline(c, o, fs, c->work, out, max_bounce, dm, px, false);
         };
@@ -1226,12 +1278,19 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
     size_t bi = 0;
     for (size_t i = 1; i < done.size(); i++)
         if (done[i].second < done[bi].second) bi = i;
-    if (frame) {  // stage 3: the other frame pipeline on the best shape
+    if (frame) {  // stage 3: the other frame pipeline on the best shape, and lane refill (ropes)
         Cand k = done[bi].first;
         k.pipeline = k.pipeline == HULK_PIPELINE_FUSED ? HULK_PIPELINE_QUEUED : HULK_PIPELINE_FUSED;
         float ms;
         r = measure(k, ms, reps, true);
         if (r > 1) return r;
+        if (variant_mask & (1u << HULK_VARIANT_ROPES))
+            for (uint32_t bs : blocks)
+                for (uint32_t lb = 1; lb <= 2; lb++) {
+                    Cand kr{HULK_VARIANT_ROPES, bs, 6, 0, 1, HULK_PIPELINE_REFILL, lb};
+                    r = measure(kr, ms, reps, true);
+                    if (r > 1) return r;
+                }
     }
     // stage 4: the four fastest again, with four times the repetitions, so
     // that timing noise between near-equal shapes does not decide

@@ -367,63 +367,85 @@ __device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const
 // split > t, else second; test the leaf; leave through the face of minimum
 // slab time (lowest axis on ties); stop when best.t <= t_out, t_out >= te,
 // or there is no rope; else t = max(t, t_out) and follow the rope.
+struct RopeState {
+    uint32_t slot, guard;
+    float t, te;
+};
+
+// Start of a rope traversal; false when the ray misses the scene box.
+__device__ __forceinline__ bool ropes_begin(const TraceParams& P, const Ray& r, float tclip, RopeState& s) {
+    float ts = 0.0f, te = kFltMax;
+    if (!ray_box(P.box, r, ts, te)) return false;
+    if (tclip < te) te = tclip;
+    s.t = ts;
+    s.te = te;
+    s.slot = 0;
+    s.guard = 0;
+    return true;
+}
+
+// One leaf of a rope traversal: descend from the current slot to a leaf,
+// test it, and follow the rope out of it.  False once the traversal is over.
+template <bool COUNT, int LB>
+__device__ __forceinline__ bool ropes_step(const TraceParams& P, const Ray& r, RopeState& s, float tstop, Best& best,
+                                           Cnt& c, uint32_t& flags) {
+    uint4 n = P.rope_nodes[s.slot];
+    if (COUNT) c.nodes++;
+    while (n.x & kNodeJunction) {
+        float split;
+        uint32_t first, second;
+        junction(n, r, split, first, second);
+        s.slot = (split <= 0.0f || split > s.t) ? first : second;
+        n = P.rope_nodes[s.slot];
+        if (COUNT) c.nodes++;
+    }
+    test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
+    if (best.t < tstop) return false;
+    const RopeSlot& rs = P.rope_slots[s.slot];
+    const float4 lo = rs.lo, hi = rs.hi;
+    const float mn[3] = {lo.x, lo.y, lo.z};
+    const float mx[3] = {hi.x, hi.y, hi.z};
+    const float o[3] = {r.o.x, r.o.y, r.o.z};
+    const float d[3] = {r.d.x, r.d.y, r.d.z};
+    const float iv[3] = {r.inv.x, r.inv.y, r.inv.z};
+    float tout = kFltMax;
+    int face = -1;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float ta;
+        int f;
+        if (d[a] > 0.0f) {
+            ta = (mx[a] - o[a]) * iv[a];
+            f = 2 * a + 1;
+        } else if (d[a] < 0.0f) {
+            ta = (mn[a] - o[a]) * iv[a];
+            f = 2 * a;
+        } else {
+            continue;
+        }
+        if (ta < tout) {
+            tout = ta;
+            face = f;
+        }
+    }
+    if (best.t <= tout || tout >= s.te || face < 0) return false;
+    const uint32_t nxt = rs.rope[face];
+    if (nxt == kRopeNone) return false;
+    if (tout > s.t) s.t = tout;
+    s.slot = nxt;
+    if (++s.guard > P.guard) {
+        flags |= kHitFlagGuard;
+        return false;
+    }
+    return true;
+}
+
 template <bool COUNT, int LB>
 __device__ void trav_ropes(const TraceParams& P, const Ray& r, float tclip, float tstop, Best& best, Cnt& c,
                            uint32_t& flags) {
-    float ts = 0.0f, te = kFltMax;
-    if (!ray_box(P.box, r, ts, te)) return;
-    if (tclip < te) te = tclip;
-    float t = ts;
-    uint32_t slot = 0, guard = 0;
-    for (;;) {
-        uint4 n = P.rope_nodes[slot];
-        if (COUNT) c.nodes++;
-        while (n.x & kNodeJunction) {
-            float split;
-            uint32_t first, second;
-            junction(n, r, split, first, second);
-            slot = (split <= 0.0f || split > t) ? first : second;
-            n = P.rope_nodes[slot];
-            if (COUNT) c.nodes++;
-        }
-        test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
-        if (best.t < tstop) break;
-        const RopeSlot& rs = P.rope_slots[slot];
-        const float4 lo = rs.lo, hi = rs.hi;
-        const float mn[3] = {lo.x, lo.y, lo.z};
-        const float mx[3] = {hi.x, hi.y, hi.z};
-        const float o[3] = {r.o.x, r.o.y, r.o.z};
-        const float d[3] = {r.d.x, r.d.y, r.d.z};
-        const float iv[3] = {r.inv.x, r.inv.y, r.inv.z};
-        float tout = kFltMax;
-        int face = -1;
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            float ta;
-            int f;
-            if (d[a] > 0.0f) {
-                ta = (mx[a] - o[a]) * iv[a];
-                f = 2 * a + 1;
-            } else if (d[a] < 0.0f) {
-                ta = (mn[a] - o[a]) * iv[a];
-                f = 2 * a;
-            } else {
-                continue;
-            }
-            if (ta < tout) {
-                tout = ta;
-                face = f;
-            }
-        }
-        if (best.t <= tout || tout >= te || face < 0) break;
-        const uint32_t nxt = rs.rope[face];
-        if (nxt == kRopeNone) break;
-        if (tout > t) t = tout;
-        slot = nxt;
-        if (++guard > P.guard) {
-            flags |= kHitFlagGuard;
-            break;
-        }
+    RopeState s;
+    if (!ropes_begin(P, r, tclip, s)) return;
+    while (ropes_step<COUNT, LB>(P, r, s, tstop, best, c, flags)) {
     }
 }
 
@@ -718,6 +740,248 @@ k_trace(TraceParams P) {
             atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[5]), (unsigned long long)cfb);
         }
     }
+}
+
+// ------------------------------------------------- lane-refill frames --
+
+// Whole frames with the rope traversal, lane by lane: a lane whose ray path
+// (primary hit, each point light's shadow ray, mirror generations) is over
+// takes the next ray of the frame queue while the other lanes of its wave go
+// on, so a wave never waits for its slowest lane.  Every lane runs the same
+// arithmetic in the same order as render_ray + k_trace (bounded pass with
+// fallback, hit record, shading, pixel), so pixels, hit records and counters
+// are identical; only the assignment of rays to lanes differs.  Waves take
+// rays from 8 per-XCD slices of the ray range, a count of idle lanes per
+// atomic, refilling once at least kRefillMin lanes are idle.
+constexpr uint32_t kRefillMin = 16;
+
+enum LanePhase : uint32_t { kLaneIdle = 0, kLanePrimary, kLaneFallback, kLaneShadow, kLaneMirror };
+
+template <bool COUNT, int LB>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LB == 2 ? 6 : 1)))
+k_render_refill(TraceParams P) {
+    const uint32_t lane_id = threadIdx.x & 63u;
+    const uint32_t n_rays = P.n;
+    uint64_t cn = 0, cl = 0, ct = 0, cfb = 0, chits = 0, crays = 0, cshadow = 0;
+
+    // work queue: 8 slices, one counter each, a wave starts at its block's XCD
+    uint32_t slice = blockIdx.x & 7u, tries = 0;
+    bool more = n_rays > 0;
+
+    // lane state
+    uint32_t phase = kLaneIdle, o = 0, mb = 0, g = 0, k = 0, flags = 0, tflags = 0, gtri = kNoHit;
+    float tb = kFltMax, w = 1.0f, ar = 0.0f, ag = 0.0f, ab = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    float dist = 0.0f, intensity = 0.0f;
+    bool clipped = false;
+    Ray tr;       // the ray being traversed (primary, shadow or mirror)
+    V3 Pp = v3(0.0f, 0.0f, 0.0f), nn = Pp, gd = Pp;  // hit point, normal and ray direction of the generation
+    Best bt;      // result of the traversal in progress
+    RopeState st;
+    Cnt c = {0, 0, 0};
+    best_init(bt);
+    st.slot = st.guard = 0;
+    st.t = st.te = 0.0f;
+    tr.o = tr.d = tr.inv = Pp;
+
+    for (;;) {
+        // ---- refill (wave-uniform decision)
+        const uint64_t idle = __ballot(phase == kLaneIdle);
+        const uint32_t n_idle = (uint32_t)__builtin_popcountll(idle);
+        if (more && (n_idle >= kRefillMin || n_idle == 64u)) {
+            const uint32_t lo = (uint32_t)((uint64_t)slice * n_rays / 8u);
+            const uint32_t hi = (uint32_t)((uint64_t)(slice + 1u) * n_rays / 8u);
+            uint32_t got = 0;
+            if (lane_id == 0) got = atomicAdd(P.work_counter + slice * kWorkCounterStride, n_idle);
+            got = __shfl(got, 0, 64);
+            const uint32_t avail = got < hi - lo ? min(n_idle, hi - lo - got) : 0u;
+            if (avail < n_idle) {  // this slice is drained: the next fetch tries the next one
+                slice = (slice + 1u) & 7u;
+                if (++tries == 8u) more = false;
+            }
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            if (phase == kLaneIdle && rank < avail) {
+                const uint32_t i = lo + got + rank;
+                o = i;
+                mb = P.max_bounce;
+                if (P.source == kSourceCamera) {
+                    uint32_t x, y;
+                    tile_pixel(i, P.W, P.H, x, y);
+                    o = y * P.W + x;
+                    cast_ray(P.cam, x, y, P.W, P.H, tr);
+                } else {
+                    uint32_t f;
+                    o = frame_ray_tiled(P.frames, P.n_frames, i, tr, f);
+                    mb = P.frames[f].max_bounce;
+                }
+                set_inv(tr);
+                best_init(bt);
+                flags = 0;
+                g = 0;
+                w = 1.0f;
+                ar = ag = ab = 0.0f;
+                tb = P.tbound ? P.tbound[o] : kFltMax;
+                float bts = 0.0f, bte = 0.0f;
+                const float tclip = tb * (1.0f + 1.0f / 1024.0f);
+                clipped = tb < kFltMax && ray_box(P.box, tr, bts, bte) && tclip < bte;
+                phase = kLanePrimary;
+                if (!ropes_begin(P, tr, clipped ? tclip : kFltMax, st)) st.guard = 0xFFFFFFFFu;  // no leaf
+            }
+        }
+        if (__ballot(phase != kLaneIdle) == 0) {
+            if (!more) break;
+            continue;
+        }
+        if (phase == kLaneIdle) continue;
+
+        // ---- one leaf of this lane's traversal
+        bool done;
+        if (st.guard == 0xFFFFFFFFu) {
+            done = true;  // the ray missed the scene box: nothing to traverse
+        } else {
+            const float tstop = phase == kLaneShadow ? dist : -kFltMax;
+            uint32_t& fl = phase == kLaneShadow || phase == kLaneMirror ? tflags : flags;
+            done = !ropes_step<COUNT, LB>(P, tr, st, tstop, bt, c, fl);
+        }
+        // ---- state transitions of lanes whose traversal is over (a new
+        // traversal may end at once when it misses the box: loop)
+        while (done) {
+            done = false;
+            bool shade = false;  // start shading the generation's hit in bt
+            if (phase == kLanePrimary && clipped && !(bt.tri != kNoHit && bt.t < tb)) {
+                best_init(bt);
+                flags |= HULK_HITF_BOUNDED_FALLBACK;
+                if (COUNT) cfb++;
+                phase = kLaneFallback;
+                if (!ropes_begin(P, tr, kFltMax, st)) done = true;
+                continue;
+            }
+            if (phase == kLanePrimary || phase == kLaneFallback) {
+                float u = 0.0f, v = 0.0f;
+                if (bt.tri != kNoHit) {
+                    u = bt.s1 / bt.det;
+                    v = bt.s2 / bt.det;
+                }
+                float4* h = reinterpret_cast<float4*>(P.hits + o);
+                h[0] = make_float4(bt.t, u, v, __uint_as_float(bt.tri));
+                h[1] = make_float4(bt.nx, bt.ny, bt.nz, __uint_as_float(flags));
+                if (COUNT) {
+                    crays++;
+                    chits += bt.tri != kNoHit ? 1 : 0;
+                }
+                shade = true;
+            } else if (phase == kLaneMirror) {
+                if (COUNT) {
+                    crays++;
+                    chits += bt.tri != kNoHit ? 1 : 0;
+                }
+                shade = true;
+            } else {  // kLaneShadow: point light k of the generation
+                if (COUNT) {
+                    cshadow++;
+                    chits += bt.tri != kNoHit ? 1 : 0;
+                }
+                if (dist > bt.t) intensity *= 0.5f;
+                const float4 col = P.lights[2 * k];
+                const float m = intensity > 0.0f ? intensity : 0.0f;
+                cr += col.x * m;
+                cg += col.y * m;
+                cb += col.z * m;
+                k++;
+            }
+            if (shade) {
+                if (bt.tri == kNoHit) {  // the path ends: pixel
+                    P.rgba[o] = to_u8(ar) | (to_u8(ag) << 8) | (to_u8(ab) << 16);
+                    phase = kLaneIdle;
+                    break;
+                }
+                Pp = add(tr.o, scale(tr.d, bt.t));
+                nn = v3(bt.nx, bt.ny, bt.nz);
+                gd = tr.d;
+                gtri = bt.tri;
+                cr = cg = cb = 0.1f;
+                k = 0;
+            }
+            // lights from k on: directional ones inline, the next point light as a shadow ray
+            bool traced = false;
+            while (k < P.n_lights) {
+                const float4 col = P.lights[2 * k];
+                const float4 dat = P.lights[2 * k + 1];
+                V3 L = v3(0.0f, 0.0f, 0.0f);
+                dist = 0.0f;
+                intensity = light_intensity(col, dat, Pp, nn, L, dist);
+                if (col.w == 1.0f) {
+                    tr.o = add(Pp, scale(L, 0.001f));
+                    tr.d = L;
+                    set_inv(tr);
+                    best_init(bt);
+                    tflags = 0;
+                    phase = kLaneShadow;
+                    if (!ropes_begin(P, tr, kFltMax, st)) done = true;
+                    traced = true;
+                    break;
+                }
+                const float m = intensity > 0.0f ? intensity : 0.0f;
+                cr += col.x * m;
+                cg += col.y * m;
+                cb += col.z * m;
+                k++;
+            }
+            if (traced) continue;
+            // generation complete: accumulate, then a mirror ray or the pixel
+            const bool spawn = P.mirror && g < mb && P.mirror[gtri];
+            const float keep = spawn ? w * 0.2f : w;
+            ar += cr * keep;
+            ag += cg * keep;
+            ab += cb * keep;
+            if (!spawn) {
+                P.rgba[o] = to_u8(ar) | (to_u8(ag) << 8) | (to_u8(ab) << 16);
+                phase = kLaneIdle;
+                break;
+            }
+            const V3 dn = normalize(gd);
+            const float kk = 2.0f * dot(dn, nn);
+            const V3 rr = sub(dn, scale(nn, kk));
+            tr.o = add(Pp, scale(rr, 0.001f));
+            tr.d = rr;
+            set_inv(tr);
+            w = w * 0.8f;
+            g++;
+            best_init(bt);
+            tflags = 0;
+            phase = kLaneMirror;
+            if (!ropes_begin(P, tr, kFltMax, st)) done = true;
+        }
+    }
+    if (COUNT) {
+        cn = wave_sum(cn + c.nodes);
+        cl = wave_sum(cl + c.leaves);
+        ct = wave_sum(ct + c.tris);
+        cfb = wave_sum(cfb);
+        chits = wave_sum(chits);
+        crays = wave_sum(crays);
+        cshadow = wave_sum(cshadow);
+        if (lane_id == 0) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[0]), (unsigned long long)crays);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[6]), (unsigned long long)cshadow);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[1]), (unsigned long long)cn);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[2]), (unsigned long long)cl);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[3]), (unsigned long long)ct);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[4]), (unsigned long long)chits);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&P.counters[5]), (unsigned long long)cfb);
+        }
+    }
+}
+
+hipError_t launch_render_refill(const TraceParams& P, bool count, uint32_t grid, uint32_t block, hipStream_t s) {
+    if (P.leaf_batch == 2) {
+        if (count) hipLaunchKernelGGL((k_render_refill<true, 2>), dim3(grid), dim3(block), 0, s, P);
+        else hipLaunchKernelGGL((k_render_refill<false, 2>), dim3(grid), dim3(block), 0, s, P);
+    } else {
+        if (count) hipLaunchKernelGGL((k_render_refill<true, 1>), dim3(grid), dim3(block), 0, s, P);
+        else hipLaunchKernelGGL((k_render_refill<false, 1>), dim3(grid), dim3(block), 0, s, P);
+    }
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------ record builder --

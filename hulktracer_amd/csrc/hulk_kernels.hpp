@@ -140,6 +140,8 @@ hipError_t launch_camera_rays(const Camera& cam, uint32_t W, uint32_t H, const S
                               float* texit, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 hipError_t launch_selftest_rcp(unsigned long long* out, hipStream_t s);  // out: 4 counters
+// whole frames, rope traversal, lanes refilled from the ray queue (work_counter: 8 slices)
+hipError_t launch_render_refill(const TraceParams& P, bool count, uint32_t grid, uint32_t block, hipStream_t s);
 // rope slots (3 n slots: nodes, then two empty-leaf slots per junction)
 hipError_t launch_build_ropes(const uint4* nodes, uint32_t n, uint32_t* up, uint4* rn, RopeSlot* rs,
                               const SceneBox& box, hipStream_t s);

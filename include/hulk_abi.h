@@ -131,7 +131,10 @@ typedef struct hulk_opts {
 typedef enum hulk_pipeline {
     HULK_PIPELINE_AUTO = 0,
     HULK_PIPELINE_QUEUED = 1,
-    HULK_PIPELINE_FUSED = 2
+    HULK_PIPELINE_FUSED = 2,
+    HULK_PIPELINE_REFILL = 3   /* FUSED with lanes refilled from the ray queue
+                                  as their paths end (ropes variant, blocks
+                                  of at most 256) */
 } hulk_pipeline;
 
 /* Call flags. */

@@ -46,7 +46,7 @@ def test_batch_equals_single_frames(use_graph, pipeline):
     b.close()
 
 
-@pytest.mark.parametrize("pipeline", [1, 2])
+@pytest.mark.parametrize("pipeline", [1, 2, 3])
 def test_batch_matches_oracle_two_scenes(pipeline):
     """Config-4 style: frames of the random-triangle and terrain scenes, one
     batch per scene, uneven sizes, bounce depths 0-3, against orc_render."""
@@ -54,7 +54,8 @@ def test_batch_matches_oracle_two_scenes(pipeline):
 
     for kind in (1, 2):
         s, nodes, lp = scene_and_tree(kind, 1)
-        ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK, pipeline=pipeline)
+        variant = H.ROPES if pipeline == 3 else H.FULLSTACK
+        ctx = gpu_ctx(s, nodes, lp, variant, pipeline=pipeline, block_size=128)
         specs = [(1, 64, 64, 0), (4, 128, 64, 3), (12, 96, 96, 1)]
         frames = _frames(s, specs)
         total = sum(w * h for _, w, h, _ in frames)
@@ -65,7 +66,7 @@ def test_batch_matches_oracle_two_scenes(pipeline):
         osc = oracle_scene(kind, 1)
         off = 0
         for cam, w, h, mb in frames:
-            ref, _ = osc.render(O.V_FULLSTACK, cam, w, h, max_bounce=mb)
+            ref, _ = osc.render(O.V_ROPES if pipeline == 3 else O.V_FULLSTACK, cam, w, h, max_bounce=mb)
             assert np.array_equal(got[off:off + w * h], ref), f"scene {kind} frame {w}x{h}"
             off += w * h
         b.close()

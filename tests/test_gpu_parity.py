@@ -110,7 +110,9 @@ def test_shade_parity(variant, mode):
                                   dict(variant=H.ROPES, persistent=True, block_size=64),
                                   dict(variant=H.FULLSTACK, pipeline=2),
                                   dict(variant=H.ROPES, pipeline=2, persistent=True, block_size=128),
-                                  dict(variant=H.LDSSTACK, pipeline=2, leaf_batch=2, lds_top_levels=8)])
+                                  dict(variant=H.LDSSTACK, pipeline=2, leaf_batch=2, lds_top_levels=8),
+                                  dict(variant=H.ROPES, pipeline=3, block_size=128),
+                                  dict(variant=H.ROPES, pipeline=3, block_size=64, leaf_batch=2)])
 def test_render_bounces_parity(opts):
     """Animated tori (5 meshes, 100,002 triangles): primary + shadow + 2
     mirror generations with per-generation compaction; persistent waves
@@ -128,22 +130,25 @@ def test_render_bounces_parity(opts):
     assert (flat != img).sum() > 100, "mirror generations changed too few pixels"
 
 
-@pytest.mark.parametrize("variant", [H.FULLSTACK, H.LDSSTACK, H.RESTART, H.ROPES])
-def test_fused_pipeline_equals_queued(variant):
-    """One fused launch per frame against the queued launches: pixels,
-    primary hit records and every counter identical, with mirror
-    generations and a learned-style bound on the primary rays."""
+@pytest.mark.parametrize("variant,pipeline", [(H.FULLSTACK, 2), (H.LDSSTACK, 2), (H.RESTART, 2), (H.ROPES, 2),
+                                              (H.ROPES, 3)])
+def test_fused_pipeline_equals_queued(variant, pipeline):
+    """One fused launch per frame (2), or one with lane refill (3), against
+    the queued launches: pixels, primary hit records and every counter
+    identical, with mirror generations and a learned-style bound on the
+    primary rays."""
     s, nodes, lp = scene_and_tree(3, 6, 5, 4)
     cam = s.camera(4, 1.0)
     W = H_ = 128
     plain = gpu_ctx(s, nodes, lp, variant).trace_camera(cam, W, H_)
     tb = np.where(plain["tri"] != H.NO_HIT, plain["t"] * np.float32(1.3), np.float32(0.5)).astype(np.float32)
     out = {}
-    for pipeline in (1, 2):
-        ctx = gpu_ctx(s, nodes, lp, variant, pipeline=pipeline, lds_top_levels=6)
+    for pipe in (1, pipeline):
+        ctx = gpu_ctx(s, nodes, lp, variant, pipeline=pipe, lds_top_levels=0 if pipe == 3 else 6,
+                      block_size=128)
         img, hits = ctx.render_frame(cam, W, H_, max_bounce=2, mirror=s.mirror, tbound=tb, count=True)
-        out[pipeline] = (img, hits, ctx.counters())
-    (i1, h1, c1), (i2, h2, c2) = out[1], out[2]
+        out[pipe] = (img, hits, ctx.counters())
+    (i1, h1, c1), (i2, h2, c2) = out[1], out[pipeline]
     assert np.array_equal(i1, i2)
     assert np.array_equal(h1.view(np.uint32), h2.view(np.uint32))
     for k in ("rays", "node_visits", "leaf_visits", "tri_tests", "hits", "fallbacks", "shadow_rays"):
