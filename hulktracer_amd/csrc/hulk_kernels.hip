@@ -757,31 +757,45 @@ constexpr uint32_t kRefillMin = 16;
 
 enum LanePhase : uint32_t { kLaneIdle = 0, kLanePrimary, kLaneFallback, kLaneShadow, kLaneMirror };
 
+// Path state a lane keeps between traversals lives in per-lane LDS columns
+// (field f of thread t at word f * blockDim + t), so that registers hold
+// only what the traversal step uses.
+enum LaneField : uint32_t {
+    kFPx, kFPy, kFPz, kFNx, kFNy, kFNz, kFDx, kFDy, kFDz,  // generation hit point, normal, ray direction
+    kFW, kFAr, kFAg, kFAb, kFCr, kFCg, kFCb,              // weight, pixel sums, generation sums
+    kFIntensity, kFTb,                                    // current light's term, primary bound
+    kUO, kUMb, kUG, kUK, kUTri, kUFlags,                  // pixel, max bounce, generation, light, hit tri, flags
+    kLaneFields
+};
+constexpr uint32_t kRefillLdsWords = kLaneFields;
+
 template <bool COUNT, int LB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LB == 2 ? 6 : 1)))
 k_render_refill(TraceParams P) {
-    const uint32_t lane_id = threadIdx.x & 63u;
+    extern __shared__ uint32_t lane_mem[];
+    const uint32_t tid = threadIdx.x, block = blockDim.x;
+    const uint32_t lane_id = tid & 63u;
+    auto F = [&](uint32_t f) -> float& { return reinterpret_cast<float*>(lane_mem)[f * block + tid]; };
+    auto U = [&](uint32_t f) -> uint32_t& { return lane_mem[f * block + tid]; };
     const uint32_t n_rays = P.n;
-    uint64_t cn = 0, cl = 0, ct = 0, cfb = 0, chits = 0, crays = 0, cshadow = 0;
+    uint64_t cfb = 0, chits = 0, crays = 0, cshadow = 0;
 
     // work queue: 8 slices, one counter each, a wave starts at its block's XCD
     uint32_t slice = blockIdx.x & 7u, tries = 0;
     bool more = n_rays > 0;
 
-    // lane state
-    uint32_t phase = kLaneIdle, o = 0, mb = 0, g = 0, k = 0, flags = 0, tflags = 0, gtri = kNoHit;
-    float tb = kFltMax, w = 1.0f, ar = 0.0f, ag = 0.0f, ab = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
-    float dist = 0.0f, intensity = 0.0f;
-    bool clipped = false;
-    Ray tr;       // the ray being traversed (primary, shadow or mirror)
-    V3 Pp = v3(0.0f, 0.0f, 0.0f), nn = Pp, gd = Pp;  // hit point, normal and ray direction of the generation
-    Best bt;      // result of the traversal in progress
+    // traversal state (registers)
+    uint32_t phase = kLaneIdle, tfl = 0;
+    float tstop = -kFltMax;
+    bool clipped = false, active = false;  // active: a traversal is in progress
+    Ray tr;
+    Best bt;
     RopeState st;
     Cnt c = {0, 0, 0};
     best_init(bt);
     st.slot = st.guard = 0;
     st.t = st.te = 0.0f;
-    tr.o = tr.d = tr.inv = Pp;
+    tr.o = tr.d = tr.inv = v3(0.0f, 0.0f, 0.0f);
 
     for (;;) {
         // ---- refill (wave-uniform decision)
@@ -802,8 +816,7 @@ k_render_refill(TraceParams P) {
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
             if (phase == kLaneIdle && rank < avail) {
                 const uint32_t i = lo + got + rank;
-                o = i;
-                mb = P.max_bounce;
+                uint32_t o, mb = P.max_bounce;
                 if (P.source == kSourceCamera) {
                     uint32_t x, y;
                     tile_pixel(i, P.W, P.H, x, y);
@@ -816,16 +829,21 @@ k_render_refill(TraceParams P) {
                 }
                 set_inv(tr);
                 best_init(bt);
-                flags = 0;
-                g = 0;
-                w = 1.0f;
-                ar = ag = ab = 0.0f;
-                tb = P.tbound ? P.tbound[o] : kFltMax;
+                U(kUO) = o;
+                U(kUMb) = mb;
+                U(kUG) = 0;
+                U(kUFlags) = 0;
+                F(kFW) = 1.0f;
+                F(kFAr) = F(kFAg) = F(kFAb) = 0.0f;
+                const float tb = P.tbound ? P.tbound[o] : kFltMax;
+                F(kFTb) = tb;
                 float bts = 0.0f, bte = 0.0f;
                 const float tclip = tb * (1.0f + 1.0f / 1024.0f);
                 clipped = tb < kFltMax && ray_box(P.box, tr, bts, bte) && tclip < bte;
                 phase = kLanePrimary;
-                if (!ropes_begin(P, tr, clipped ? tclip : kFltMax, st)) st.guard = 0xFFFFFFFFu;  // no leaf
+                tstop = -kFltMax;
+                tfl = 0;
+                active = ropes_begin(P, tr, clipped ? tclip : kFltMax, st);
             }
         }
         if (__ballot(phase != kLaneIdle) == 0) {
@@ -835,34 +853,30 @@ k_render_refill(TraceParams P) {
         if (phase == kLaneIdle) continue;
 
         // ---- one leaf of this lane's traversal
-        bool done;
-        if (st.guard == 0xFFFFFFFFu) {
-            done = true;  // the ray missed the scene box: nothing to traverse
-        } else {
-            const float tstop = phase == kLaneShadow ? dist : -kFltMax;
-            uint32_t& fl = phase == kLaneShadow || phase == kLaneMirror ? tflags : flags;
-            done = !ropes_step<COUNT, LB>(P, tr, st, tstop, bt, c, fl);
-        }
+        bool done = !active || !ropes_step<COUNT, LB>(P, tr, st, tstop, bt, c, tfl);
         // ---- state transitions of lanes whose traversal is over (a new
         // traversal may end at once when it misses the box: loop)
         while (done) {
             done = false;
             bool shade = false;  // start shading the generation's hit in bt
-            if (phase == kLanePrimary && clipped && !(bt.tri != kNoHit && bt.t < tb)) {
+            if (phase == kLanePrimary && clipped && !(bt.tri != kNoHit && bt.t < F(kFTb))) {
                 best_init(bt);
-                flags |= HULK_HITF_BOUNDED_FALLBACK;
+                U(kUFlags) |= tfl | HULK_HITF_BOUNDED_FALLBACK;
                 if (COUNT) cfb++;
                 phase = kLaneFallback;
-                if (!ropes_begin(P, tr, kFltMax, st)) done = true;
+                tfl = 0;
+                active = ropes_begin(P, tr, kFltMax, st);
+                done = !active;
                 continue;
             }
             if (phase == kLanePrimary || phase == kLaneFallback) {
+                const uint32_t flags = U(kUFlags) | tfl;
                 float u = 0.0f, v = 0.0f;
                 if (bt.tri != kNoHit) {
                     u = bt.s1 / bt.det;
                     v = bt.s2 / bt.det;
                 }
-                float4* h = reinterpret_cast<float4*>(P.hits + o);
+                float4* h = reinterpret_cast<float4*>(P.hits + U(kUO));
                 h[0] = make_float4(bt.t, u, v, __uint_as_float(bt.tri));
                 h[1] = make_float4(bt.nx, bt.ny, bt.nz, __uint_as_float(flags));
                 if (COUNT) {
@@ -881,82 +895,99 @@ k_render_refill(TraceParams P) {
                     cshadow++;
                     chits += bt.tri != kNoHit ? 1 : 0;
                 }
-                if (dist > bt.t) intensity *= 0.5f;
-                const float4 col = P.lights[2 * k];
+                float intensity = F(kFIntensity);
+                if (tstop > bt.t) intensity *= 0.5f;  // tstop holds the light distance
+                const float4 col = P.lights[2 * U(kUK)];
                 const float m = intensity > 0.0f ? intensity : 0.0f;
-                cr += col.x * m;
-                cg += col.y * m;
-                cb += col.z * m;
-                k++;
+                F(kFCr) += col.x * m;
+                F(kFCg) += col.y * m;
+                F(kFCb) += col.z * m;
+                U(kUK) += 1;
             }
             if (shade) {
                 if (bt.tri == kNoHit) {  // the path ends: pixel
-                    P.rgba[o] = to_u8(ar) | (to_u8(ag) << 8) | (to_u8(ab) << 16);
+                    P.rgba[U(kUO)] = to_u8(F(kFAr)) | (to_u8(F(kFAg)) << 8) | (to_u8(F(kFAb)) << 16);
                     phase = kLaneIdle;
+                    active = false;
                     break;
                 }
-                Pp = add(tr.o, scale(tr.d, bt.t));
-                nn = v3(bt.nx, bt.ny, bt.nz);
-                gd = tr.d;
-                gtri = bt.tri;
-                cr = cg = cb = 0.1f;
-                k = 0;
+                const V3 hp = add(tr.o, scale(tr.d, bt.t));
+                F(kFPx) = hp.x;
+                F(kFPy) = hp.y;
+                F(kFPz) = hp.z;
+                F(kFNx) = bt.nx;
+                F(kFNy) = bt.ny;
+                F(kFNz) = bt.nz;
+                F(kFDx) = tr.d.x;
+                F(kFDy) = tr.d.y;
+                F(kFDz) = tr.d.z;
+                U(kUTri) = bt.tri;
+                F(kFCr) = F(kFCg) = F(kFCb) = 0.1f;
+                U(kUK) = 0;
             }
             // lights from k on: directional ones inline, the next point light as a shadow ray
+            const V3 Pp = v3(F(kFPx), F(kFPy), F(kFPz)), nn = v3(F(kFNx), F(kFNy), F(kFNz));
             bool traced = false;
-            while (k < P.n_lights) {
+            for (uint32_t k = U(kUK); k < P.n_lights; k++) {
                 const float4 col = P.lights[2 * k];
                 const float4 dat = P.lights[2 * k + 1];
                 V3 L = v3(0.0f, 0.0f, 0.0f);
-                dist = 0.0f;
-                intensity = light_intensity(col, dat, Pp, nn, L, dist);
+                float dist = 0.0f;
+                const float intensity = light_intensity(col, dat, Pp, nn, L, dist);
                 if (col.w == 1.0f) {
+                    U(kUK) = k;
+                    F(kFIntensity) = intensity;
                     tr.o = add(Pp, scale(L, 0.001f));
                     tr.d = L;
                     set_inv(tr);
                     best_init(bt);
-                    tflags = 0;
+                    tfl = 0;
+                    tstop = dist;
                     phase = kLaneShadow;
-                    if (!ropes_begin(P, tr, kFltMax, st)) done = true;
+                    active = ropes_begin(P, tr, kFltMax, st);
+                    done = !active;
                     traced = true;
                     break;
                 }
                 const float m = intensity > 0.0f ? intensity : 0.0f;
-                cr += col.x * m;
-                cg += col.y * m;
-                cb += col.z * m;
-                k++;
+                F(kFCr) += col.x * m;
+                F(kFCg) += col.y * m;
+                F(kFCb) += col.z * m;
             }
             if (traced) continue;
             // generation complete: accumulate, then a mirror ray or the pixel
-            const bool spawn = P.mirror && g < mb && P.mirror[gtri];
+            const uint32_t g = U(kUG);
+            const bool spawn = P.mirror && g < U(kUMb) && P.mirror[U(kUTri)];
+            const float w = F(kFW);
             const float keep = spawn ? w * 0.2f : w;
-            ar += cr * keep;
-            ag += cg * keep;
-            ab += cb * keep;
+            F(kFAr) += F(kFCr) * keep;
+            F(kFAg) += F(kFCg) * keep;
+            F(kFAb) += F(kFCb) * keep;
             if (!spawn) {
-                P.rgba[o] = to_u8(ar) | (to_u8(ag) << 8) | (to_u8(ab) << 16);
+                P.rgba[U(kUO)] = to_u8(F(kFAr)) | (to_u8(F(kFAg)) << 8) | (to_u8(F(kFAb)) << 16);
                 phase = kLaneIdle;
+                active = false;
                 break;
             }
-            const V3 dn = normalize(gd);
+            const V3 dn = normalize(v3(F(kFDx), F(kFDy), F(kFDz)));
             const float kk = 2.0f * dot(dn, nn);
             const V3 rr = sub(dn, scale(nn, kk));
             tr.o = add(Pp, scale(rr, 0.001f));
             tr.d = rr;
             set_inv(tr);
-            w = w * 0.8f;
-            g++;
+            F(kFW) = w * 0.8f;
+            U(kUG) = g + 1;
             best_init(bt);
-            tflags = 0;
+            tfl = 0;
+            tstop = -kFltMax;
             phase = kLaneMirror;
-            if (!ropes_begin(P, tr, kFltMax, st)) done = true;
+            active = ropes_begin(P, tr, kFltMax, st);
+            done = !active;
         }
     }
     if (COUNT) {
-        cn = wave_sum(cn + c.nodes);
-        cl = wave_sum(cl + c.leaves);
-        ct = wave_sum(ct + c.tris);
+        const uint64_t cn = wave_sum((uint64_t)c.nodes), cl = wave_sum((uint64_t)c.leaves),
+                       ct = wave_sum((uint64_t)c.tris);
         cfb = wave_sum(cfb);
         chits = wave_sum(chits);
         crays = wave_sum(crays);
@@ -974,12 +1005,13 @@ k_render_refill(TraceParams P) {
 }
 
 hipError_t launch_render_refill(const TraceParams& P, bool count, uint32_t grid, uint32_t block, hipStream_t s) {
+    const size_t lds = (size_t)kRefillLdsWords * block * 4;
     if (P.leaf_batch == 2) {
-        if (count) hipLaunchKernelGGL((k_render_refill<true, 2>), dim3(grid), dim3(block), 0, s, P);
-        else hipLaunchKernelGGL((k_render_refill<false, 2>), dim3(grid), dim3(block), 0, s, P);
+        if (count) hipLaunchKernelGGL((k_render_refill<true, 2>), dim3(grid), dim3(block), lds, s, P);
+        else hipLaunchKernelGGL((k_render_refill<false, 2>), dim3(grid), dim3(block), lds, s, P);
     } else {
-        if (count) hipLaunchKernelGGL((k_render_refill<true, 1>), dim3(grid), dim3(block), 0, s, P);
-        else hipLaunchKernelGGL((k_render_refill<false, 1>), dim3(grid), dim3(block), 0, s, P);
+        if (count) hipLaunchKernelGGL((k_render_refill<true, 1>), dim3(grid), dim3(block), lds, s, P);
+        else hipLaunchKernelGGL((k_render_refill<false, 1>), dim3(grid), dim3(block), lds, s, P);
     }
     return hipGetLastError();
 }
