@@ -42,8 +42,8 @@ struct DevBuf {
 // accumulation).  The context owns one set; every batch owns its own, so a
 // captured graph never sees a buffer that a later call reallocates.
 struct Work {
-    DevBuf hits, hits2, hits3, shadow_rays, shadow_slot, shadow_t, qrays[2], qpix[2], qw[2], acc, ctr, wq, wcount,
-        woffset;
+    DevBuf hits, hits2, hits3, shadow_rays, shadow_slot, shadow_t, qrays[2], qpix[2], qw[2], acc, ctr, wq, bcount,
+        boffset;
 };
 
 struct hulk_ctx {
@@ -275,8 +275,8 @@ int finish(hulk_ctx* c, uint32_t flags, bool timed) {
 
 void release_work(Work& w) {
     DevBuf* bufs[] = {&w.hits, &w.hits2, &w.hits3, &w.shadow_rays, &w.shadow_slot, &w.shadow_t, &w.qrays[0],
-                      &w.qrays[1], &w.qpix[0], &w.qpix[1], &w.qw[0], &w.qw[1], &w.acc, &w.ctr, &w.wq, &w.wcount,
-                      &w.woffset};
+                      &w.qrays[1], &w.qpix[0], &w.qpix[1], &w.qw[0], &w.qw[1], &w.acc, &w.ctr, &w.wq, &w.bcount,
+                      &w.boffset};
     for (DevBuf* b : bufs) release(*b);
 }
 
@@ -293,9 +293,9 @@ int ensure_work(hulk_ctx* c, Work& w, uint32_t n, uint32_t np, bool bounces, boo
     if ((r = ensure(c, w.shadow_rays, (size_t)n * m * sizeof(hulk_ray)))) return r;
     if ((r = ensure(c, w.shadow_slot, (size_t)n * m * 4))) return r;
     if ((r = ensure(c, w.shadow_t, (size_t)n * m * 4))) return r;
-    const size_t waves = ((size_t)n + 255) / 256 * 4;
-    if ((r = ensure(c, w.wcount, waves * 8))) return r;
-    if ((r = ensure(c, w.woffset, waves * 8))) return r;
+    const size_t blocks = ((size_t)n + 255) / 256;  // 2 counts per 256-ray block of the compaction
+    if ((r = ensure(c, w.bcount, blocks * 8))) return r;
+    if ((r = ensure(c, w.boffset, blocks * 8))) return r;
     if (bounces) {
         if ((r = ensure(c, w.hits2, (size_t)n * sizeof(hulk_hit)))) return r;
         if ((r = ensure(c, w.hits3, (size_t)n * sizeof(hulk_hit)))) return r;
@@ -473,8 +473,8 @@ int shade_pipeline(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w,
             G.weight_out = static_cast<float*>(w.qw[qo].p);
             G.next_count = ctr + 2 + qo;
         }
-        G.block_counts = static_cast<uint32_t*>(w.wcount.p);
-        G.block_offsets = static_cast<uint32_t*>(w.woffset.p);
+        G.block_counts = static_cast<uint32_t*>(w.bcount.p);
+        G.block_offsets = static_cast<uint32_t*>(w.boffset.p);
         HIP_TRY(c, launch_gen_secondary(G, n, c->stream));
         if (np) {
             TraceParams S = P;

@@ -10,10 +10,11 @@
 //            breadth-first order (the LDS top tree is a prefix), the rest
 //            in depth-first order per subtree.  Root = 0, empty child = 0.
 //   recs   : TriRec (64 B) per leaf reference, leaf-contiguous.
-//   ropes  : RopeNode (16 B) + RopeSlot (48 B) per rope slot.
+//   ropes  : rope node (16 B) + RopeSlot (64 B) per rope slot, built on the
+//            device (3 slots per node: the node and two empty-leaf slots).
 // Work distribution: chunks of 64 rays per wave, either statically mapped
-// with an XCD-aware block order or fetched with one atomic per wave
-// (persistent mode).
+// with an XCD-aware block order or fetched from 8 per-XCD slices by
+// persistent waves; the refill kernel fetches rays per idle lane.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -1041,9 +1042,10 @@ __global__ void k_build_records(const float4* __restrict__ verts, const uint4* _
 // ------------------------------------------------- secondary generation --
 
 // Secondary rays of generation g, compacted without atomics in three
-// passes: k_gen_secondary<false> counts per wave (ballots), k_scan turns the
-// wave counts into offsets (and the queue lengths), k_gen_secondary<true>
-// writes each ray at its wave offset + lane rank.  Queue order is therefore
+// passes: k_gen_secondary<false> counts per 256-ray block (wave ballots),
+// k_scan_pairs turns the block counts into offsets (and the queue lengths),
+// k_gen_secondary<true> writes each ray at its block offset + the counts of
+// the earlier waves of its block + its lane rank.  Queue order is therefore
 // ray order, deterministic and coherent.
 //
 // For every ray of generation g with a hit: one shadow ray per point light

@@ -156,7 +156,7 @@ def test_learned_bound_keeps_hits_exact(variant):
 
 
 @pytest.mark.gpu
-def test_margin_selection_by_node_visits():
+def test_margin_selection_by_time():
     """select_margin picks, per traversal, the fastest quantile on
     calibration frames, and enables the bound only when it saves time: it
     does for kd-restart (no early exit)."""
