@@ -481,6 +481,8 @@ def main() -> int:
 
         for name, fn in (("methods_1M", methods_1m),
                          ("animated_1M", lambda: WL.animated(frames=20, stream=stream, log=log)),
+                         ("animated_1M_device_build",
+                          lambda: WL.animated_device_build(frames=20, stream=stream, log=log)),
                          ("batched_64_frames", lambda: WL.batched(stream=stream, log=log))):
             try:
                 extras[name] = fn()

@@ -6,9 +6,9 @@ the reference's renderer interface.
 """
 from ._lib import (BRUTEFORCE, FULLSTACK, HIT_DTYPE, LDSSTACK, NO_HIT, RAY_DTYPE, RESTART, ROPES, SHORTSTACK,
                    VARIANTS, HulkError, lib)
-from .engine import (GLVisitorGPUTrace, GPUTrace, KDTreeCPU, Scene, Settings, make_camera, make_camera_lookat,
+from .engine import (GLVisitorGPUTrace, GPUTrace, KDTreeCPU, KDTreeGPU, Scene, Settings, make_camera, make_camera_lookat,
                      scene_box)
 
 __all__ = ["BRUTEFORCE", "FULLSTACK", "SHORTSTACK", "LDSSTACK", "RESTART", "ROPES", "VARIANTS", "HIT_DTYPE",
-           "RAY_DTYPE", "NO_HIT", "HulkError", "lib", "GLVisitorGPUTrace", "GPUTrace", "KDTreeCPU", "Scene",
+           "RAY_DTYPE", "NO_HIT", "HulkError", "lib", "GLVisitorGPUTrace", "GPUTrace", "KDTreeCPU", "KDTreeGPU", "Scene",
            "Settings", "make_camera", "make_camera_lookat", "scene_box"]

@@ -133,6 +133,10 @@ EXPORTS = {
     "hulk_selftest_reciprocal": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hulk_camera_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                    C.c_uint32]),
+    "hulk_build_kdtree_device": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "hulk_download_kdtree": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_void_p,
+                                       C.c_uint32, C.POINTER(C.c_uint32)]),
+    "hulk_device_build_info": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hulk_kdtree_build": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                     C.POINTER(C.c_void_p)]),
     "hulk_kdtree_build_threads": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,

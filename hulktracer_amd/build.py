@@ -17,7 +17,7 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libhulk.so"
 
-HIP_SOURCES = ["hulk_kernels.hip", "hulk_api.hip", "hulk_mlp.hip"]
+HIP_SOURCES = ["hulk_kernels.hip", "hulk_api.hip", "hulk_mlp.hip", "hulk_kdbuild.hip"]
 CPP_SOURCES = ["kdtree_sah.cpp", "scenes.cpp"]
 
 # -ffp-contract=off: no implicit multiply-add fusion, so the kernels keep

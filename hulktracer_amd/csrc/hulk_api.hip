@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "hulk_abi.h"
+#include "hulk_kdbuild.hpp"
 #include "hulk_kernels.hpp"
 
 using namespace hulk;
@@ -60,9 +61,12 @@ struct hulk_ctx {
     SceneBox box{};
     bool have_scene = false;
 
-    // tree
-    DevBuf nodes, lp, recs;
+    // tree: device layout (nodes), reference layout (ref_nodes), leaf
+    // polygon references (lp) and their triangle records (recs)
+    DevBuf nodes, ref_nodes, lp, recs;
     uint32_t n_nodes = 0, n_lp = 0, max_jdepth = 0;
+    KdBuilder* kdb = nullptr;
+    KdBuildResult kdb_last{};
     uint32_t level_prefix[kTopLevelsMax + 1] = {0};
     bool have_tree = false;
     bool ropes_ready = false;
@@ -606,10 +610,11 @@ extern "C" int hulk_create(hulk_ctx** out, const hulk_opts* opts) {
 extern "C" void hulk_destroy(hulk_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DevBuf* bufs[] = {&c->verts, &c->tris, &c->nodes, &c->lp, &c->recs, &c->rope_nodes, &c->rope_slots, &c->rope_links, &c->selftest,
+    DevBuf* bufs[] = {&c->verts, &c->tris, &c->nodes, &c->ref_nodes, &c->lp, &c->recs, &c->rope_nodes, &c->rope_slots, &c->rope_links, &c->selftest,
                       &c->d_lights, &c->d_point, &c->rays, &c->tbound, &c->rgba, &c->mirror, &c->stats, &c->hits_io};
     for (DevBuf* b : bufs) release(*b);
     release_work(c->work);
+    kdb_destroy(c->kdb);
     if (c->host_stats) (void)hipHostFree(c->host_stats);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -757,9 +762,11 @@ extern "C" int hulk_upload_kdtree(hulk_ctx* c, const hulk_kd_node* nodes, uint32
     }
     int r;
     if ((r = ensure(c, c->nodes, (size_t)n_nodes * 16))) return r;
+    if ((r = ensure(c, c->ref_nodes, (size_t)n_nodes * 16))) return r;
     if ((r = ensure(c, c->lp, (size_t)std::max<uint32_t>(n_lp, 1) * 4))) return r;
     if ((r = ensure(c, c->recs, (size_t)std::max<uint32_t>(n_lp, 1) * sizeof(TriRec)))) return r;
     HIP_TRY(c, hipMemcpyAsync(c->nodes.p, dev.data(), (size_t)n_nodes * 16, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->ref_nodes.p, nodes, (size_t)n_nodes * 16, hipMemcpyHostToDevice, c->stream));
     if (n_lp) HIP_TRY(c, hipMemcpyAsync(c->lp.p, leaf_polys, (size_t)n_lp * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, launch_build_records(static_cast<const float4*>(c->verts.p), static_cast<const uint4*>(c->tris.p),
                                     static_cast<const uint32_t*>(c->lp.p), n_lp, static_cast<TriRec*>(c->recs.p),
@@ -975,6 +982,74 @@ extern "C" int hulk_ctx_scene_box(const hulk_ctx* c, float out[6]) {
         out[i] = c->box.mn[i];
         out[3 + i] = c->box.mx[i];
     }
+    return HULK_OK;
+}
+
+// ------------------------------------------------------ device KD build --
+
+extern "C" int hulk_build_kdtree_device(hulk_ctx* c, uint32_t capacity) {
+    if (!c) return HULK_E_ILLEGAL_INPUT;
+    if (!c->have_scene) return fail(c, HULK_E_NO_SCENE, "upload the scene before building the tree");
+    if (!c->kdb && !(c->kdb = kdb_create())) return fail(c, HULK_E_ALLOC, "device KD builder allocation failed");
+    KdBuildResult R;
+    std::string err;
+    int r = kdb_build(c->kdb, static_cast<const float4*>(c->verts.p), c->n_verts, static_cast<const uint4*>(c->tris.p),
+                      c->n_tris, c->box, capacity, c->stream, &R, err);
+    if (r) {
+        c->err = err;
+        return r;
+    }
+    if (R.max_jdepth > kMaxFullStack) return fail(c, HULK_E_ILLEGAL_INPUT, "tree deeper than 64 junctions");
+    // the previous tree stays installed until here
+    c->have_tree = false;
+    c->ropes_ready = false;
+    if ((r = ensure(c, c->nodes, (size_t)R.n_nodes * 16))) return r;
+    if ((r = ensure(c, c->ref_nodes, (size_t)R.n_nodes * 16))) return r;
+    if ((r = ensure(c, c->lp, (size_t)std::max<uint32_t>(R.n_lp, 1) * 4))) return r;
+    if ((r = ensure(c, c->recs, (size_t)std::max<uint32_t>(R.n_lp, 1) * sizeof(TriRec)))) return r;
+    HIP_TRY(c, hipMemcpyAsync(c->nodes.p, kdb_dev_nodes(c->kdb), (size_t)R.n_nodes * 16, hipMemcpyDeviceToDevice,
+                              c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->ref_nodes.p, kdb_ref_nodes(c->kdb), (size_t)R.n_nodes * 16,
+                              hipMemcpyDeviceToDevice, c->stream));
+    if (R.n_lp)
+        HIP_TRY(c, hipMemcpyAsync(c->lp.p, kdb_leaf_polys(c->kdb), (size_t)R.n_lp * 4, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    HIP_TRY(c, launch_build_records(static_cast<const float4*>(c->verts.p), static_cast<const uint4*>(c->tris.p),
+                                    static_cast<const uint32_t*>(c->lp.p), R.n_lp, static_cast<TriRec*>(c->recs.p),
+                                    c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->n_nodes = R.n_nodes;
+    c->n_lp = R.n_lp;
+    c->max_jdepth = R.max_jdepth;
+    static_assert(kKdTopLevels == kTopLevelsMax, "device layout depth");
+    std::memcpy(c->level_prefix, R.level_prefix, sizeof(c->level_prefix));
+    c->kdb_last = R;
+    c->have_tree = true;
+    return HULK_OK;
+}
+
+extern "C" int hulk_download_kdtree(hulk_ctx* c, hulk_kd_node* nodes, uint32_t node_cap, uint32_t* n_nodes,
+                                    uint32_t* leaf_polys, uint32_t lp_cap, uint32_t* n_lp) {
+    if (!c) return HULK_E_ILLEGAL_INPUT;
+    if (!c->have_tree) return fail(c, HULK_E_NO_TREE, "no KD tree uploaded or built");
+    if (n_nodes) *n_nodes = c->n_nodes;
+    if (n_lp) *n_lp = c->n_lp;
+    if ((nodes && node_cap < c->n_nodes) || (leaf_polys && lp_cap < c->n_lp))
+        return fail(c, HULK_E_ILLEGAL_INPUT, "download buffer smaller than the tree");
+    if (nodes)
+        HIP_TRY(c, hipMemcpyAsync(nodes, c->ref_nodes.p, (size_t)c->n_nodes * 16, hipMemcpyDeviceToHost, c->stream));
+    if (leaf_polys && c->n_lp)
+        HIP_TRY(c, hipMemcpyAsync(leaf_polys, c->lp.p, (size_t)c->n_lp * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HULK_OK;
+}
+
+extern "C" int hulk_device_build_info(const hulk_ctx* c, uint32_t out[4]) {
+    if (!c || !out) return HULK_E_ILLEGAL_INPUT;
+    out[0] = c->kdb_last.capacity;
+    out[1] = c->kdb_last.retries;
+    out[2] = c->kdb_last.levels;
+    out[3] = c->kdb_last.max_jdepth;
     return HULK_OK;
 }
 

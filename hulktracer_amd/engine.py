@@ -44,6 +44,9 @@ class Settings:
         # MI355X knobs of this library (autotuner space)
         "--blockSize": "256", "--shortStackDepth": "6", "--ldsTopLevels": "0", "--persistent": "0",
         "--shadeMode": "fused",
+        # where buildTree builds the SAH-NlogN tree: "cpu" (KDTreeCPU, then
+        # upload) or "gpu" (KDTreeGPU, built in HBM); same arrays either way
+        "--treeBuild": "cpu",
     }
 
     def __init__(self, argv: list[str] | None = None):
@@ -180,6 +183,40 @@ class KDTreeCPU:
         return self.m_leafPolygons
 
 
+class KDTreeGPU(KDTreeCPU):
+    """KD tree built on the GPU (KDTreeGPU, KDTreeGPU.hpp:12-23, a subclass
+    of the CPU tree that the reference leaves empty).  buildKDTree uploads
+    the scene to a context, builds there and keeps the tree installed in that
+    context; the arrays are downloaded only when asked for."""
+
+    def __init__(self, ctx: "GPUTrace | None" = None):
+        super().__init__()
+        self.ctx = ctx
+        self._stale = True
+
+    def buildKDTree(self, scene: Scene, upload: bool = True, capacity: int = 0) -> int:
+        if self.ctx is None:
+            self.ctx = GPUTrace()
+        if upload:
+            self.ctx.upload_scene(scene)
+        self.ctx.build_kdtree_device(capacity)
+        self._stale = True
+        return 0
+
+    def _fetch(self):
+        if self._stale:
+            self.m_treeNodes, self.m_leafPolygons = self.ctx.download_kdtree()
+            self._stale = False
+
+    def getNodeArray(self) -> np.ndarray:
+        self._fetch()
+        return self.m_treeNodes
+
+    def getPolygonArray(self) -> np.ndarray:
+        self._fetch()
+        return self.m_leafPolygons
+
+
 # -------------------------------------------------------------- context --
 
 class GPUTrace:
@@ -235,6 +272,28 @@ class GPUTrace:
         nodes = np.ascontiguousarray(nodes, np.uint32)
         lp = np.ascontiguousarray(leaf_polys, np.uint32)
         self._c(lib().hulk_upload_kdtree(self.h, ptr(nodes), len(nodes), ptr(lp) if len(lp) else None, len(lp)))
+
+    def build_kdtree_device(self, capacity: int = 0):
+        """SAH-NlogN KD tree built on the GPU from the uploaded scene and
+        installed as this context's tree (hulk_build_kdtree_device); same
+        arrays as KDTreeCPU.buildKDTree."""
+        self._c(lib().hulk_build_kdtree_device(self.h, int(capacity)))
+
+    def download_kdtree(self) -> tuple[np.ndarray, np.ndarray]:
+        """The current tree in the reference layout: (nodes (N,4) uint32,
+        leaf polygon references (L,) uint32)."""
+        nn, nl = C.c_uint32(), C.c_uint32()
+        self._c(lib().hulk_download_kdtree(self.h, None, 0, C.byref(nn), None, 0, C.byref(nl)))
+        nodes = np.zeros((nn.value, 4), np.uint32)
+        lp = np.zeros(nl.value, np.uint32)
+        self._c(lib().hulk_download_kdtree(self.h, ptr(nodes), nn.value, C.byref(nn),
+                                           ptr(lp) if nl.value else None, nl.value, C.byref(nl)))
+        return nodes, lp
+
+    def device_build_info(self) -> dict:
+        out = np.zeros(4, np.uint32)
+        self._c(lib().hulk_device_build_info(self.h, ptr(out)))
+        return dict(capacity=int(out[0]), retries=int(out[1]), levels=int(out[2]), max_jdepth=int(out[3]))
 
     def set_lights(self, lights: np.ndarray):
         lt = np.ascontiguousarray(lights, np.float32)
@@ -385,12 +444,17 @@ class GLVisitorGPUTrace:
 
     def initializeVisitor(self):
         s = self.settings
-        self.m_pKdtree = KDTreeCPU()
         self.ctx = GPUTrace(variant=s["--kernelType"], block_size=int(s["--blockSize"]),
                             short_stack_depth=int(s["--shortStackDepth"]), lds_top_levels=int(s["--ldsTopLevels"]),
                             persistent=s["--persistent"] == "1")
+        if s["--treeBuild"] not in ("cpu", "gpu"):
+            raise ValueError("--treeBuild must be cpu or gpu")
+        self.m_pKdtree = KDTreeGPU(self.ctx) if s["--treeBuild"] == "gpu" else KDTreeCPU()
 
     def buildTree(self):
+        if isinstance(self.m_pKdtree, KDTreeGPU):
+            self.m_pKdtree.buildKDTree(self.m_pScene)  # uploads the scene, builds in place
+            return
         self.m_pKdtree.buildKDTree(self.m_pScene)
         self.ctx.upload_scene(self.m_pScene)
         self.ctx.upload_kdtree(self.m_pKdtree.getNodeArray(), self.m_pKdtree.getPolygonArray())

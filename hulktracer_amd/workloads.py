@@ -3,6 +3,8 @@
   animated        1,000,002-triangle animated tori, 20 frames, CPU KD
                   rebuild per frame overlapped with the previous frame's
                   GPU render, primary + shadow + 2 mirror generations
+  animated_device_build
+                  the same frames with the KD tree built on the GPU
   batched         64 frames of the random-triangle and terrain scenes,
                   resolutions 256^2-1024^2, bounce depth 0-3, one
                   graph-captured pipeline per scene, autotuned per scene
@@ -110,6 +112,70 @@ def animated(frames: int = 20, W: int = 1024, H: int = 1024, max_bounce: int = 2
         "gpu_mrays_s_total": round((rays_frame + c0["shadow_rays"]) / np.mean(gpu_ms) / 1e3, 1),
     }
     log(f"animated: {res}")
+    return res
+
+
+def animated_device_build(frames: int = 20, W: int = 1024, H: int = 1024, max_bounce: int = 2, seed: int = 7,
+                          stream=None, autotune: bool = True, log=print):
+    """The animated workload with the KD tree built on the GPU every frame
+    (hulk_build_kdtree_device, same arrays as the CPU build): per frame the
+    moved scene is uploaded, the tree is built in HBM and the frame is
+    rendered.  The frames' geometry is synthesised on the host before the
+    timed loop (the animation itself is input, not part of the path)."""
+    torch = _torch()
+    stream = stream or torch.cuda.Stream()
+    rgba = torch.empty(W * H, dtype=torch.int32, device="cuda")
+    hits = torch.empty(W * H * 32, dtype=torch.uint8, device="cuda")
+    t_syn = time.perf_counter()
+    scenes = [Scene.procedural(L.SCENE_ANIMATED_TORI, seed=seed, frame=f) for f in range(frames)]
+    syn_ms = (time.perf_counter() - t_syn) * 1e3 / frames
+    ctx = GPUTrace(stream=stream.cuda_stream, variant=L.FULLSTACK)
+    ctx.upload_scene(scenes[0])
+    ctx.build_kdtree_device()
+    mirror = torch.from_numpy(scenes[0].mirror).cuda()
+    cam = scenes[0].camera(0, H / W)
+    tuned = None
+    if autotune:
+        tuned, _ = ctx.autotune(cam, W, H, reps=2, frame=True, max_bounce=max_bounce, mirror=scenes[0].mirror)
+    ctx.reset_counters()
+    ctx.render_frame_dev(cam, W, H, max_bounce, mirror.data_ptr(), rgba.data_ptr(), hits.data_ptr(),
+                         flags=L.FLAG_COUNT)
+    c0 = ctx.counters()
+    upload_ms, build_ms, gpu_ms = [], [], []
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    for f in range(frames):
+        t0 = time.perf_counter()
+        ctx.upload_scene(scenes[f])
+        t1 = time.perf_counter()
+        ctx.build_kdtree_device()  # returns once the tree is installed
+        t2 = time.perf_counter()
+        e0, e1 = _events(stream)
+        e0.record(stream)
+        ctx.render_frame_dev(scenes[f].camera(f, H / W), W, H, max_bounce, mirror.data_ptr(), rgba.data_ptr(),
+                             hits.data_ptr(), flags=L.FLAG_NO_SYNC)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        upload_ms.append((t1 - t0) * 1e3)
+        build_ms.append((t2 - t1) * 1e3)
+        gpu_ms.append(e0.elapsed_time(e1))
+    wall = time.perf_counter() - t_wall
+    info = ctx.device_build_info()
+    ctx.close()
+    rays_frame = c0["rays"]
+    res = {
+        "triangles": int(scenes[0].n_tris), "frames": frames, "res": f"{W}x{H}", "max_bounce": max_bounce,
+        "tuned": tuned, "tree_build": "device",
+        "nearest_hit_rays_frame0": int(rays_frame), "shadow_rays_frame0": int(c0["shadow_rays"]),
+        "device_kd_build_ms_per_frame": round(float(np.mean(build_ms)), 2),
+        "upload_ms_per_frame": round(float(np.mean(upload_ms)), 2),
+        "gpu_ms_per_frame": round(float(np.mean(gpu_ms)), 3),
+        "wall_ms_per_frame": round(wall * 1e3 / frames, 2),
+        "host_scene_synthesis_ms_per_frame": round(syn_ms, 1),
+        "build_capacity": info["capacity"], "build_depths": info["levels"],
+        "gpu_mrays_s_nearest": round(rays_frame / np.mean(gpu_ms) / 1e3, 1),
+    }
+    log(f"animated (device build): {res}")
     return res
 
 

@@ -349,6 +349,32 @@ int hulk_selftest_reciprocal(hulk_ctx* ctx, unsigned long long out[4]);
 /* Scene box of a context (min xyz, max xyz). */
 int hulk_ctx_scene_box(const hulk_ctx* ctx, float out[6]);
 
+/* ------------------------------------------------ device KD tree build -- */
+
+/* SAH-NlogN KD tree built on the GPU from the scene uploaded to the
+ * context, installed as the context's tree (no host round trip).  Same
+ * node and leaf-polygon arrays as KDTreeCPU::buildKDTree
+ * (KDTreeCPU.cpp:61-94; KDTreeSAH.cpp:247-521) and hulk_kdtree_build, so
+ * every trace gives the same hits as with the uploaded CPU tree.  The
+ * reference's device builder is an empty stub (KDTreeGPU,
+ * KDTreeGPU.hpp:12-23).  `capacity`: polygon references per depth for the
+ * first attempt (0 = 6 per triangle); a depth that overflows doubles it and
+ * the build starts again.  Returns HULK_E_ILLEGAL_INPUT for a NaN vertex
+ * coordinate of a triangle. */
+int hulk_build_kdtree_device(hulk_ctx* ctx, uint32_t capacity);
+
+/* The context's current tree (uploaded or built on the device) in the
+ * reference layout (m_treeNodes / m_leafPolygons, KDTreeCPU.hpp:128-132).
+ * Writes the counts; copies the arrays when the buffers are non-null and
+ * large enough (else HULK_E_ILLEGAL_INPUT). */
+int hulk_download_kdtree(hulk_ctx* ctx, hulk_kd_node* nodes, uint32_t node_cap, uint32_t* n_nodes,
+                         uint32_t* leaf_polys, uint32_t lp_cap, uint32_t* n_lp);
+
+/* Last device build (no reference counterpart): out[0] = capacity it ran
+ * with, out[1] = capacity doublings, out[2] = depths holding nodes,
+ * out[3] = deepest junction depth + 1. */
+int hulk_device_build_info(const hulk_ctx* ctx, uint32_t out[4]);
+
 /* ------------------------------------------ CPU KD tree (input producer) -- */
 
 /* SAH-NlogN KD tree, the reference's default --treeType
