@@ -175,6 +175,14 @@ def lib() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: run `python -m hulktracer_amd.build` (hipcc, gfx950) first")
+    # PyTorch ships its own HIP / HSA runtime (same sonames as /opt/rocm's).
+    # Loaded after libhulk, it would map a second runtime next to the one
+    # libhulk pulled in, and that second runtime finds no GPU.  Loading torch
+    # first lets libhulk bind to torch's runtime: one runtime per process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     for name, (res, args) in {**EXPORTS, **OPTIONAL_EXPORTS}.items():
         fn = getattr(L, name)
