@@ -72,6 +72,36 @@ def profiled_traffic(kernel: str, opts: dict, W: int, H: int):
     return None, f"no FETCH_SIZE/WRITE_SIZE profile of {kernel} at {W}x{H}"
 
 
+def kd_build_only(args) -> int:
+    """--kd-build-only: warmup + steps device KD builds of one scene in one
+    context; prints the wall time per build (the call returns once the tree
+    is installed) and the host builder's time on the same scene."""
+    import hulktracer_amd as H
+    from hulktracer_amd import _lib as L
+    from hulktracer_amd import workloads as WL
+
+    if args.kernel_scene == "animated":
+        scene = H.Scene.procedural(L.SCENE_ANIMATED_TORI, seed=7, frame=0)
+    else:
+        scene = H.Scene.procedural(L.SCENE_TERRAIN_SPHERES, seed=1)
+    ctx = H.GPUTrace()
+    ctx.upload_scene(scene)
+    for _ in range(args.warmup):
+        ctx.build_kdtree_device()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.build_kdtree_device()
+    dev_ms = (time.perf_counter() - t) * 1e3 / max(1, args.steps)
+    info = ctx.device_build_info()
+    nodes, lp = ctx.download_kdtree()
+    ctx.close()
+    _, _, host_ms = WL.build(scene)
+    print(json.dumps({"kd_build": "device", "scene": args.kernel_scene, "triangles": scene.n_tris,
+                      "nodes": int(len(nodes)), "leaf_refs": int(len(lp)), "device_ms_per_build": round(dev_ms, 3),
+                      "host_builder_ms": round(host_ms, 1), "builds": args.steps, **info}), flush=True)
+    return 0
+
+
 def animated_kernel(args, W: int, H: int) -> int:
     """--kernel-only on frame 0 of the animated scene (50 tori + ground, 1,000,002
     triangles, 2 mirror generations): warmup + steps renders, HIP events."""
@@ -276,6 +306,9 @@ def main() -> int:
                     help="only time the primary trace kernel of pose 10 with these options "
                          "(e.g. variant=5,block_size=64,short_stack_depth=6,lds_top_levels=0,persistent=0): "
                          "warmup + steps launches, for profiler runs")
+    ap.add_argument("--kd-build-only", action="store_true",
+                    help="only time the device KD-tree build of --kernel-scene (terrain, or frame 0 of the "
+                         "animated scene): warmup + steps builds, wall clock per build, for profiler runs")
     ap.add_argument("--kernel-scene", default="terrain", choices=("terrain", "animated"),
                     help="--kernel-only workload: the 10-frame terrain batch, or frame 0 of the animated "
                          "1M-triangle scene (1024x1024, 2 mirror generations, one render per launch)")
@@ -304,6 +337,8 @@ def main() -> int:
 
     W = Hh = args.res
     n = W * Hh
+    if args.kd_build_only:
+        return kd_build_only(args)
     if args.kernel_only and args.kernel_scene == "animated":
         return animated_kernel(args, W, Hh)
     t0 = time.time()

@@ -26,7 +26,7 @@ def test_more_than_one_gpu_is_refused():
 
 def test_modes_and_helpers_exist():
     b = _bench()
-    for name in ("main", "compare_methods", "animated_kernel", "kernel_name", "profiled_traffic"):
+    for name in ("main", "compare_methods", "animated_kernel", "kd_build_only", "kernel_name", "profiled_traffic"):
         assert callable(getattr(b, name)), name
     assert b.kernel_name({"variant": 3, "pipeline": 2, "leaf_batch": 2}) == "k_trace<3,false,true,2>"
     assert b.kernel_name({"variant": 5, "pipeline": 0}) == "k_trace<5,false,true,1>"
