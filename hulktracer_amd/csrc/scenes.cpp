@@ -18,6 +18,7 @@
 
 #include "hulk_abi.h"
 #include "hulk_scenes.h"
+#include "scene_data.hpp"
 
 namespace {
 
@@ -118,14 +119,8 @@ double value_noise(uint64_t seed, double x, double z) {
 
 }  // namespace
 
-struct hulk_scene {
-    int kind = 0;
-    uint64_t seed = 0;
-    std::vector<hulk_float4> verts;
-    std::vector<hulk_tri> tris;
-    std::vector<hulk_light> lights;
-    std::vector<uint8_t> mirror;
-};
+double hulk_host_sin(double x) { return dsin(x); }
+double hulk_host_cos(double x) { return dcos(x); }
 
 namespace {
 

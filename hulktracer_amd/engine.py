@@ -84,6 +84,23 @@ class Scene:
     def procedural(cls, kind: int, seed: int = 1, param: int = 0, frame: int = 0) -> "Scene":
         h = C.c_void_p()
         _check(None, lib().hulk_scene_create(kind, seed, param, frame, C.byref(h)))
+        s = cls._from_handle(h, kind, seed)
+        s._frame = frame
+        return s
+
+    @classmethod
+    def from_obj(cls, path, time: float = 0.0) -> "Scene":
+        """An OBJ file as the reference's OBJLoader reads it (OBJLoader.cpp:
+        15-174), at scene time `time` (rotation about y, OBJTransformVisitor.cpp:
+        22-41), with the loader's directional light."""
+        h = C.c_void_p()
+        _check(None, lib().hulk_scene_load_obj(str(path).encode(), float(time), C.byref(h)))
+        s = cls._from_handle(h, 0, 0)
+        s.name = str(path)
+        return s
+
+    @classmethod
+    def _from_handle(cls, h, kind: int, seed: int) -> "Scene":
         try:
             nv = lib().hulk_scene_vertex_count(h)
             nt = lib().hulk_scene_triangle_count(h)
@@ -96,9 +113,7 @@ class Scene:
             _check(None, lib().hulk_scene_get_box(h, ptr(box)))
         finally:
             lib().hulk_scene_free(h)
-        s = cls(verts, tris, lights, box, mirror, kind, seed)
-        s._frame = frame
-        return s
+        return cls(verts, tris, lights, box, mirror, kind, seed)
 
     @classmethod
     def from_arrays(cls, vertices, triangles, lights=None) -> "Scene":

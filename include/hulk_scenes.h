@@ -1,5 +1,6 @@
 /*
- * hulk_scenes.h - seeded procedural scenes for the headless frame driver.
+ * hulk_scenes.h - seeded procedural scenes and reference scene files for the
+ * headless frame driver.
  *
  * The reference fills Scene::m_vertices / m_vertexIndices / m_lights from
  * its BART and OBJ loaders (LoaderInterface, HulkEngine/public/
@@ -43,6 +44,14 @@ int hulk_scene_get_box(const hulk_scene* s, hulk_box* out);
 int hulk_scene_camera(const hulk_scene* s, uint32_t frame, float aspect, hulk_camera* out);
 /* The same camera from the scene kind alone. */
 int hulk_scene_kind_camera(int kind, uint32_t frame, float aspect, hulk_camera* out);
+
+/* Scene of an OBJ file as OBJLoader::viParseFile reads it
+ * (OBJLoader/OBJLoader.cpp:15-174): triangles "f a b c" over vertices "v",
+ * at scene time `time` rotated about y by `time` (OBJTransformVisitor.cpp:
+ * 22-41), with the loader's directional light.  HULK_E_FILE_OPEN when the
+ * file cannot be read; HULK_E_ILLEGAL_INPUT for a face corner outside the
+ * vertex list. */
+int hulk_scene_load_obj(const char* path, float time, hulk_scene** out);
 
 /* Look-at camera: basis in double rounded to float, then the float steps of
  * GLVisitorCPUTrace::updateCamera (GLVisitorCPUTrace.cpp:136-145). */

@@ -1,0 +1,68 @@
+"""GPU parity on the reference's own models (OBJLoader/Models: teapot,
+box; copied to tests/golden): every traversal variant, shading and the
+device KD build, through the C ABI, against the CPU oracle."""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import hulktracer_amd as H
+import oracle as O
+from hulktracer_amd import _lib as L
+from tests.helpers import describe, hit_mismatch
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+VARIANTS = [H.BRUTEFORCE, H.FULLSTACK, H.SHORTSTACK, H.LDSSTACK, H.RESTART, H.ROPES]
+ORACLE_OF = {H.BRUTEFORCE: O.V_BRUTE, H.FULLSTACK: O.V_FULLSTACK, H.SHORTSTACK: O.V_SHORTSTACK,
+             H.LDSSTACK: O.V_LDSSTACK, H.RESTART: O.V_RESTART, H.ROPES: O.V_ROPES}
+VIEWS = {
+    "teapot.obj": ((0.9, 0.7, 1.1), (0.0, 0.15, 0.0)),
+    "Box.obj": ((0.8, 0.9, 1.0), (0.0, 0.2, 0.0)),
+}
+
+
+def _setup(name, time=0.0):
+    s = H.Scene.from_obj(GOLDEN / name, time)
+    kd = H.KDTreeCPU()
+    kd.buildKDTree(s)
+    nodes, lp = kd.getNodeArray(), kd.getPolygonArray()
+    frm, tgt = VIEWS[name]
+    cam = H.make_camera_lookat(frm, tgt, fov=50.0, aspect=1.0)
+    osc = O.OracleScene(s.vertices, s.triangles, s.box, nodes, lp, s.lights, s.mirror)
+    return s, nodes, lp, cam, osc
+
+
+@pytest.mark.parametrize("name", sorted(VIEWS))
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_reference_model_trace_and_shade(name, variant):
+    s, nodes, lp, cam, osc = _setup(name, 0.7)
+    print(f"reference model {name} variant {variant}", flush=True)
+    ctx = H.GPUTrace(variant=variant, lds_top_levels=6)
+    ctx.upload_scene(s)
+    ctx.upload_kdtree(nodes, lp)
+    got = ctx.trace_camera(cam, 256, 256)
+    ref = osc.trace_camera(ORACLE_OF[variant], cam, 256, 256, depth=6)
+    bad = hit_mismatch(got, ref)
+    assert len(bad) == 0, f"{len(bad)} rays differ\n{describe(got, ref, bad)}"
+    assert (got["tri"] != H.NO_HIT).sum() > 3000
+    for mode in (L.SHADE_FUSED, L.SHADE_PICTURE):
+        img = ctx.shade(cam, 256, 256, got, mode)
+        rimg = osc.shade(ORACLE_OF[variant], cam, 256, 256, ref, mode=mode)
+        assert np.array_equal(img, rimg), f"shade mode {mode}: {(img != rimg).sum()} pixels differ"
+
+
+@pytest.mark.parametrize("name", sorted(VIEWS))
+def test_reference_model_device_build(name):
+    s, nodes, lp, cam, osc = _setup(name, 0.7)
+    ctx = H.GPUTrace(variant=H.FULLSTACK)
+    ctx.upload_scene(s)
+    ctx.build_kdtree_device()
+    dn, dlp = ctx.download_kdtree()
+    assert np.array_equal(dn, nodes) and np.array_equal(dlp, lp)
+    img, hits = ctx.render_frame(cam, 200, 200)
+    rimg, _ = osc.render(O.V_FULLSTACK, cam, 200, 200)
+    assert np.array_equal(img, rimg)
