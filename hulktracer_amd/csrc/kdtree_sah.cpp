@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "hulk_abi.h"
+#include "kdtree_data.hpp"
 
 namespace {
 
@@ -328,10 +329,6 @@ private:
 
 }  // namespace
 
-struct hulk_kdtree {
-    std::vector<hulk_kd_node> nodes;
-    std::vector<uint32_t> leaf_polys;
-};
 
 extern "C" int hulk_kdtree_build_threads(const hulk_float4* verts, uint32_t n_verts, const hulk_tri* tris,
                                          uint32_t n_tris, const hulk_box* scene_box, uint32_t max_threads,

@@ -100,17 +100,39 @@ class Scene:
         return s
 
     @classmethod
+    def from_dataset(cls, path) -> tuple["Scene", np.ndarray, np.ndarray]:
+        """A paris.bin KD dataset (GLVisitorGPUTraceSSD::buildTree,
+        GLVisitorGPUTraceSSD.cpp:43-107) as (scene, nodes, leaf polygons): the
+        per-leaf vertices and triangles (triangle k = the k-th triangle of
+        the leaves in node order; no lights) and the tree over them."""
+        hs, ht = C.c_void_p(), C.c_void_p()
+        _check(None, lib().hulk_dataset_load(str(path).encode(), C.byref(hs), C.byref(ht)))
+        try:
+            nodes, lp = KDTreeCPU._arrays(ht)
+        finally:
+            lib().hulk_kdtree_free(ht)
+        s = cls._from_handle(hs, 0, 0)
+        s.name = str(path)
+        return s, nodes, lp
+
+    @classmethod
     def _from_handle(cls, h, kind: int, seed: int) -> "Scene":
+        def view(ctype, n, addr, shape, dtype):
+            if n == 0:
+                return np.zeros(shape, dtype)
+            return np.ctypeslib.as_array((ctype * (int(np.prod(shape)))).from_address(addr)).reshape(shape).copy()
+
         try:
             nv = lib().hulk_scene_vertex_count(h)
             nt = lib().hulk_scene_triangle_count(h)
             nl = lib().hulk_scene_light_count(h)
-            verts = np.ctypeslib.as_array((C.c_float * (4 * nv)).from_address(lib().hulk_scene_vertices(h))).reshape(nv, 4).copy()
-            tris = np.ctypeslib.as_array((C.c_uint32 * (4 * nt)).from_address(lib().hulk_scene_triangles(h))).reshape(nt, 4).copy()
-            lights = np.ctypeslib.as_array((C.c_float * (8 * nl)).from_address(lib().hulk_scene_lights(h))).reshape(nl, 8).copy()
-            mirror = np.ctypeslib.as_array((C.c_uint8 * nt).from_address(lib().hulk_scene_mirror_mask(h))).copy()
+            verts = view(C.c_float, nv, lib().hulk_scene_vertices(h), (nv, 4), np.float32)
+            tris = view(C.c_uint32, nt, lib().hulk_scene_triangles(h), (nt, 4), np.uint32)
+            lights = view(C.c_float, nl, lib().hulk_scene_lights(h), (nl, 8), np.float32)
+            mirror = view(C.c_uint8, nt, lib().hulk_scene_mirror_mask(h), (nt,), np.uint8)
             box = np.zeros(8, np.float32)
-            _check(None, lib().hulk_scene_get_box(h, ptr(box)))
+            if nv:
+                _check(None, lib().hulk_scene_get_box(h, ptr(box)))
         finally:
             lib().hulk_scene_free(h)
         return cls(verts, tris, lights, box, mirror, kind, seed)
@@ -180,16 +202,29 @@ class KDTreeCPU:
                                                      ptr(scene.triangles), scene.n_tris, ptr(scene.box),
                                                      threads, C.byref(h)))
         try:
-            nn = lib().hulk_kdtree_node_count(h)
-            nl = lib().hulk_kdtree_leaf_poly_count(h)
-            self.m_treeNodes = np.ctypeslib.as_array(
-                (C.c_uint32 * (4 * nn)).from_address(lib().hulk_kdtree_nodes(h))).reshape(nn, 4).copy()
-            self.m_leafPolygons = (np.ctypeslib.as_array(
-                (C.c_uint32 * nl).from_address(lib().hulk_kdtree_leaf_polys(h))).copy()
-                if nl else np.zeros(0, np.uint32))
+            self.m_treeNodes, self.m_leafPolygons = self._arrays(h)
         finally:
             lib().hulk_kdtree_free(h)
         return 0
+
+    @staticmethod
+    def _arrays(h) -> tuple[np.ndarray, np.ndarray]:
+        nn = lib().hulk_kdtree_node_count(h)
+        nl = lib().hulk_kdtree_leaf_poly_count(h)
+        nodes = np.ctypeslib.as_array(
+            (C.c_uint32 * (4 * nn)).from_address(lib().hulk_kdtree_nodes(h))).reshape(nn, 4).copy()
+        lp = (np.ctypeslib.as_array((C.c_uint32 * nl).from_address(lib().hulk_kdtree_leaf_polys(h))).copy()
+              if nl else np.zeros(0, np.uint32))
+        return nodes, lp
+
+    def saveDataSetToFile(self, scene: Scene, path="paris.bin"):
+        """The tree and its per-leaf data as the reference's KD dataset
+        (GLVisitorGPUTrace::saveDataSetToFile, GLVisitorGPUTrace.cpp:638-674)."""
+        nodes = np.ascontiguousarray(self.getNodeArray(), np.uint32)
+        lp = np.ascontiguousarray(self.getPolygonArray(), np.uint32)
+        _check(None, lib().hulk_dataset_write(str(path).encode(), ptr(scene.vertices), len(scene.vertices),
+                                              ptr(scene.triangles), scene.n_tris, ptr(nodes), len(nodes),
+                                              ptr(lp) if len(lp) else None, len(lp)))
 
     def getNodeArray(self) -> np.ndarray:
         return self.m_treeNodes

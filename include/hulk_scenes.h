@@ -53,6 +53,19 @@ int hulk_scene_kind_camera(int kind, uint32_t frame, float aspect, hulk_camera* 
  * vertex list. */
 int hulk_scene_load_obj(const char* path, float time, hulk_scene** out);
 
+/* The reference's on-disk KD dataset ("paris.bin"):
+ * hulk_dataset_write stores a scene's tree in the layout of
+ * GLVisitorGPUTrace::saveDataSetToFile (GLVisitorGPUTrace.cpp:638-674)
+ * with the per-leaf vertex / index lists of buildPerLeafData (:163-234);
+ * hulk_dataset_load reads one (GLVisitorGPUTraceSSD::buildTree,
+ * GLVisitorGPUTraceSSD.cpp:43-107) back into a scene (per-leaf vertices and
+ * triangles, triangle k = the k-th triangle of the leaves in node order, no
+ * lights) and its tree (free with hulk_scene_free / hulk_kdtree_free). */
+int hulk_dataset_write(const char* path, const hulk_float4* verts, uint32_t n_verts, const hulk_tri* tris,
+                       uint32_t n_tris, const hulk_kd_node* nodes, uint32_t n_nodes, const uint32_t* leaf_polys,
+                       uint32_t n_lp);
+int hulk_dataset_load(const char* path, hulk_scene** scene_out, hulk_kdtree** tree_out);
+
 /* Look-at camera: basis in double rounded to float, then the float steps of
  * GLVisitorCPUTrace::updateCamera (GLVisitorCPUTrace.cpp:136-145). */
 int hulk_make_camera_lookat(const float from[3], const float target[3], const float up[3],

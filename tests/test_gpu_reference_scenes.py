@@ -66,3 +66,28 @@ def test_reference_model_device_build(name):
     img, hits = ctx.render_frame(cam, 200, 200)
     rimg, _ = osc.render(O.V_FULLSTACK, cam, 200, 200)
     assert np.array_equal(img, rimg)
+
+
+def test_dataset_round_trip_on_the_gpu(tmp_path):
+    """Terrain + spheres written as a paris.bin dataset, loaded back and
+    traced on the GPU: the same hits as the original scene, triangle k of the
+    dataset being the original leaf reference k."""
+    from tests.helpers import scene_and_tree
+
+    s, nodes, lp = scene_and_tree(2, 1)
+    kd = H.KDTreeCPU()
+    kd.m_treeNodes, kd.m_leafPolygons = nodes, lp
+    kd.saveDataSetToFile(s, tmp_path / "terrain.bin")
+    s2, nodes2, lp2 = H.Scene.from_dataset(tmp_path / "terrain.bin")
+    cam = s.camera(13, 1.0)
+    out = []
+    for sc, nd, l in ((s, nodes, lp), (s2, nodes2, lp2)):
+        ctx = H.GPUTrace(variant=H.LDSSTACK, lds_top_levels=8)
+        ctx.upload_scene(sc)
+        ctx.upload_kdtree(nd, l)
+        out.append(ctx.trace_camera(cam, 256, 256))
+    a, b = out
+    for f in ("t", "u", "v", "nx", "ny", "nz"):
+        assert np.array_equal(a[f].view(np.uint32), b[f].view(np.uint32)), f
+    hit = b["tri"] != H.NO_HIT
+    assert np.array_equal(lp[b["tri"][hit]], a["tri"][hit]) and hit.sum() > 10000
