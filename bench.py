@@ -514,7 +514,33 @@ def main() -> int:
             return {"workload": f"animated_tori_{sc.n_tris}tris_frame0_poses10-19_{W}x{Hh}_primary+shadow+shade",
                     "kd_build_ms": round(bms, 1), "tuned": b1, "methods": m1, "learned": li1}
 
-        for name, fn in (("methods_1M", methods_1m),
+        def streamed():
+            # out-of-core: terrain pose 10 at 1024^2, kd-restart, record pool
+            # of 100 / 25 / 5 % of the leaf references against the in-core kernel
+            sc = scene
+            cx = WL.make_ctx(sc, nodes, lp, stream, variant=H.RESTART)
+            cam = cams[0]
+            cx.trace_camera(cam, W, Hh)
+            t0 = time.perf_counter()
+            ref = cx.trace_camera(cam, W, Hh)
+            core_ms = (time.perf_counter() - t0) * 1e3
+            out = {"workload": f"terrain_{sc.n_tris}tris_pose10_{W}x{Hh}_primary_kd-restart",
+                   "leaf_refs": int(len(lp)), "in_core_ms": round(core_ms, 2)}
+            for frac in (1.0, 0.25, 0.05):
+                pool = max(int(frac * len(lp)), int(nodes[nodes[:, 0] == 4, 2].max()))
+                got, st = cx.trace_camera_streamed(cam, W, Hh, pool)
+                same = bool(np.array_equal(got.view(np.uint32).reshape(-1, 8)[:, :7],
+                                           ref.view(np.uint32).reshape(-1, 8)[:, :7]))
+                out[f"pool_{int(frac * 100)}pct"] = {
+                    "pool_records": pool, "ms": round(st["ms"], 2), "launches": st["launches"],
+                    "leaves_loaded": st["leaves_loaded"], "leaves_evicted": st["leaves_evicted"],
+                    "mb_loaded": round(st["bytes_loaded"] / 1e6, 1), "mrays_s": round(n / st["ms"] / 1e3, 1),
+                    "hits_equal_in_core": same}
+            cx.close()
+            log(f"streamed: {out}")
+            return out
+
+        for name, fn in (("streamed_out_of_core", streamed), ("methods_1M", methods_1m),
                          ("animated_1M", lambda: WL.animated(frames=20, stream=stream, log=log)),
                          ("animated_1M_device_build",
                           lambda: WL.animated_device_build(frames=20, stream=stream, log=log)),

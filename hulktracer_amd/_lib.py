@@ -87,6 +87,12 @@ class TuneRequest(C.Structure):
                 ("mirror_mask", C.c_void_p)]
 
 
+class StreamStats(C.Structure):
+    _fields_ = [("launches", C.c_uint32), ("leaves_loaded", C.c_uint32), ("leaves_evicted", C.c_uint32),
+                ("max_queue", C.c_uint32), ("bytes_loaded", C.c_uint64), ("ms", C.c_float),
+                ("pool_records", C.c_uint32)]
+
+
 EXPORTS = {
     # name: (restype, argtypes)
     "hulk_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(Opts)]),
@@ -133,6 +139,8 @@ EXPORTS = {
     "hulk_selftest_reciprocal": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hulk_camera_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                    C.c_uint32]),
+    "hulk_trace_camera_streamed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                             C.c_void_p, C.c_uint32, C.POINTER(StreamStats)]),
     "hulk_build_kdtree_device": (C.c_int, [C.c_void_p, C.c_uint32]),
     "hulk_download_kdtree": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_void_p,
                                        C.c_uint32, C.POINTER(C.c_uint32)]),

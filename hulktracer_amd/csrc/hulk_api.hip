@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -67,6 +68,17 @@ struct hulk_ctx {
     uint32_t n_nodes = 0, n_lp = 0, max_jdepth = 0;
     KdBuilder* kdb = nullptr;
     KdBuildResult kdb_last{};
+    uint32_t tree_gen = 0;  // bumped whenever the tree or the vertices change
+
+    // out-of-core streaming: per-leaf extents of the records, their pinned
+    // host copy (the backing store) and the device pool / tables
+    uint32_t stream_gen = ~0u;
+    std::vector<uint2> leaf_extent;  // per leaf id: (first record, count)
+    uint32_t max_leaf = 0;
+    TriRec* backing = nullptr;  // pinned
+    size_t backing_cap = 0;
+    uint32_t* stream_host = nullptr;  // pinned: (stopped pixels, requested leaves)
+    DevBuf s_pool, s_table, s_request, s_list, s_queue[2], s_state, s_best, s_hits, s_count;
     uint32_t level_prefix[kTopLevelsMax + 1] = {0};
     bool have_tree = false;
     bool ropes_ready = false;
@@ -615,6 +627,11 @@ extern "C" void hulk_destroy(hulk_ctx* c) {
     for (DevBuf* b : bufs) release(*b);
     release_work(c->work);
     kdb_destroy(c->kdb);
+    for (DevBuf* b : {&c->s_pool, &c->s_table, &c->s_request, &c->s_list, &c->s_queue[0], &c->s_queue[1],
+                      &c->s_state, &c->s_best, &c->s_hits, &c->s_count})
+        release(*b);
+    if (c->backing) (void)hipHostFree(c->backing);
+    if (c->stream_host) (void)hipHostFree(c->stream_host);
     if (c->host_stats) (void)hipHostFree(c->host_stats);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -668,6 +685,7 @@ extern "C" int hulk_upload_scene(hulk_ctx* c, const hulk_float4* verts, uint32_t
     c->box.mx[0] = scene_box->max.x; c->box.mx[1] = scene_box->max.y; c->box.mx[2] = scene_box->max.z;
     c->have_scene = true;
     c->have_tree = false;  // records depend on the vertices
+    c->tree_gen++;
     c->ropes_ready = false;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return HULK_OK;
@@ -778,6 +796,7 @@ extern "C" int hulk_upload_kdtree(hulk_ctx* c, const hulk_kd_node* nodes, uint32
     std::memcpy(c->level_prefix, prefix, sizeof(prefix));
     c->ropes_ready = false;
     c->have_tree = true;
+    c->tree_gen++;
     return HULK_OK;
 }
 
@@ -985,6 +1004,229 @@ extern "C" int hulk_ctx_scene_box(const hulk_ctx* c, float out[6]) {
     return HULK_OK;
 }
 
+// ------------------------------------------------- out-of-core streaming --
+
+namespace {
+
+// Pool allocator of the streaming trace: contiguous runs of records handed
+// out in a ring, freed oldest first (the FIFO eviction of
+// GLVisitorGPUTrace::clearLeaf, GLVisitorGPUTrace.cpp:491-502, over the
+// index / vertex RingBuffers of RingBuffer.h:6-82, here one ring of triangle
+// records).  A run never wraps: it starts at the write position or, when
+// the tail of the pool is too short, at 0.
+struct RecordRing {
+    uint32_t cap = 0, head = 0, tail = 0;  // head: oldest run's start; tail: next free
+    struct Run {
+        uint32_t leaf, off, size, launch;  // launch: the relaunch round that loaded it
+    };
+    std::deque<Run> runs;
+    bool fits(uint32_t size, uint32_t& off) const {
+        if (runs.empty()) {
+            off = 0;
+            return size <= cap;
+        }
+        if (tail > head) {  // free: [tail, cap) and [0, head)
+            if (size <= cap - tail) {
+                off = tail;
+                return true;
+            }
+            if (size <= head) {
+                off = 0;
+                return true;
+            }
+            return false;
+        }
+        if (size <= head - tail) {  // wrapped: free [tail, head)
+            off = tail;
+            return true;
+        }
+        return false;
+    }
+    void push(uint32_t leaf, uint32_t off, uint32_t size, uint32_t launch) {
+        if (runs.empty()) head = off;
+        runs.push_back({leaf, off, size, launch});
+        tail = off + size;
+    }
+    Run pop() {
+        Run r = runs.front();
+        runs.pop_front();
+        head = runs.empty() ? tail : runs.front().off;
+        return r;
+    }
+};
+
+}  // namespace
+
+extern "C" int hulk_trace_camera_streamed(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H,
+                                          uint32_t pool_records, hulk_hit* hits, uint32_t flags,
+                                          hulk_stream_stats* stats) {
+    if (!c || !cam || !hits || W == 0 || H == 0) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument or empty image");
+    if (!c->have_scene) return fail(c, HULK_E_NO_SCENE, "no scene uploaded");
+    if (!c->have_tree) return fail(c, HULK_E_NO_TREE, "no KD tree uploaded");
+    const uint64_t n64 = (uint64_t)W * H;
+    if (n64 >= (1ull << 31)) return fail(c, HULK_E_ILLEGAL_INPUT, "image too large");
+    const uint32_t n = static_cast<uint32_t>(n64);
+    const auto t_start = std::chrono::steady_clock::now();
+    int r;
+    // per-tree tables: leaf extents from the node array, records -> pinned host memory
+    if (c->stream_gen != c->tree_gen) {
+        std::vector<uint4> nodes(c->n_nodes);
+        HIP_TRY(c, hipMemcpyAsync(nodes.data(), c->nodes.p, (size_t)c->n_nodes * 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        uint32_t n_leaves = 0, max_leaf = 0;
+        for (const uint4& nd : nodes)
+            if (nd.x & HULK_NODE_LEAF) n_leaves = std::max(n_leaves, nd.w + 1);
+        if (n_leaves > (1u << 30)) return fail(c, HULK_E_ILLEGAL_INPUT, "leaf ids out of range");
+        c->leaf_extent.assign(n_leaves, make_uint2(0, 0));
+        for (const uint4& nd : nodes)
+            if (nd.x & HULK_NODE_LEAF) {
+                c->leaf_extent[nd.w] = make_uint2(nd.y, nd.z);
+                max_leaf = std::max(max_leaf, nd.z);
+            }
+        c->max_leaf = max_leaf;
+        const size_t bytes = std::max<size_t>((size_t)c->n_lp * sizeof(TriRec), 64);
+        if (bytes > c->backing_cap) {
+            if (c->backing) HIP_TRY(c, hipHostFree(c->backing));
+            c->backing = nullptr;
+            c->backing_cap = 0;
+            HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->backing), bytes, hipHostMallocDefault));
+            c->backing_cap = bytes;
+        }
+        if (c->n_lp)
+            HIP_TRY(c, hipMemcpyAsync(c->backing, c->recs.p, (size_t)c->n_lp * sizeof(TriRec), hipMemcpyDeviceToHost,
+                                      c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        c->stream_gen = c->tree_gen;
+    }
+    if (!c->stream_host)
+        HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->stream_host), 64, hipHostMallocDefault));
+    const uint32_t n_leaves = static_cast<uint32_t>(c->leaf_extent.size());
+    const uint32_t pool = pool_records ? pool_records : std::max<uint32_t>(c->n_lp, 1);
+    if (pool < c->max_leaf) {
+        c->err = "pool of " + std::to_string(pool) + " records is smaller than the largest leaf (" +
+                 std::to_string(c->max_leaf) + " records)";
+        return HULK_E_ILLEGAL_INPUT;
+    }
+    const bool dev_out = (flags & HULK_FLAG_DEVICE_PTRS) != 0;
+    if ((r = ensure(c, c->s_pool, (size_t)pool * sizeof(TriRec)))) return r;
+    if ((r = ensure(c, c->s_table, (size_t)std::max<uint32_t>(n_leaves, 1) * 8))) return r;
+    if ((r = ensure(c, c->s_request, (size_t)std::max<uint32_t>(n_leaves, 1) * 4))) return r;
+    if ((r = ensure(c, c->s_list, (size_t)std::max<uint32_t>(n_leaves, 1) * 4))) return r;
+    for (DevBuf* b : {&c->s_queue[0], &c->s_queue[1]})
+        if ((r = ensure(c, *b, (size_t)n * 4))) return r;
+    if ((r = ensure(c, c->s_state, (size_t)n * 16))) return r;
+    if ((r = ensure(c, c->s_best, (size_t)n * sizeof(Best)))) return r;
+    if (!dev_out && (r = ensure(c, c->s_hits, (size_t)n * sizeof(hulk_hit)))) return r;
+    if ((r = ensure(c, c->s_count, 64))) return r;
+    uint32_t* d_count = static_cast<uint32_t*>(c->s_count.p);
+    HIP_TRY(c, launch_fill_u32(static_cast<uint32_t*>(c->s_request.p), std::max<uint32_t>(n_leaves, 1), 0u, c->stream));
+    std::vector<uint2> table(std::max<uint32_t>(n_leaves, 1), make_uint2(0, 0));
+    HIP_TRY(c, hipMemcpyAsync(c->s_table.p, table.data(), table.size() * 8, hipMemcpyHostToDevice, c->stream));
+
+    StreamParams S;
+    std::memset(&S, 0, sizeof(S));
+    S.nodes = static_cast<const uint4*>(c->nodes.p);
+    S.pool = static_cast<const TriRec*>(c->s_pool.p);
+    S.leaf_table = static_cast<const uint2*>(c->s_table.p);
+    S.request = static_cast<uint32_t*>(c->s_request.p);
+    S.box = c->box;
+    S.cam = to_cam(*cam);
+    S.W = W;
+    S.H = H;
+    S.queue_in = nullptr;
+    S.n_in = n;
+    S.state = static_cast<float4*>(c->s_state.p);
+    S.best = static_cast<Best*>(c->s_best.p);
+    S.hits = dev_out ? hits : static_cast<hulk_hit*>(c->s_hits.p);
+    S.guard = static_cast<uint32_t>(std::min<uint64_t>(4ull * c->n_nodes + 4096u, 0x7FFFFFFFull));
+
+    hulk_stream_stats st;
+    std::memset(&st, 0, sizeof(st));
+    st.pool_records = pool;
+    RecordRing ring;
+    ring.cap = pool;
+    std::vector<uint32_t> req;
+    const uint32_t max_launches = 1u << 20;
+    uint32_t cur = 0;
+    for (;;) {
+        if (st.launches >= max_launches) return fail(c, HULK_E_HIP, "streaming trace made no progress");
+        S.queue_out = static_cast<uint32_t*>(c->s_queue[cur ^ 1].p);
+        S.n_out = d_count;
+        HIP_TRY(c, launch_fill_u32(d_count, 2, 0u, c->stream));
+        HIP_TRY(c, launch_trace_streamed(S, c->stream));
+        HIP_TRY(c, launch_collect_requests(S.request, n_leaves, static_cast<uint32_t*>(c->s_list.p), d_count + 1,
+                                           c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->stream_host, d_count, 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        st.launches++;
+        const uint32_t n_out = c->stream_host[0], n_req = c->stream_host[1];
+        if (n_out == 0) break;
+        st.max_queue = std::max(st.max_queue, n_out);
+        if (n_req == 0 || n_req > n_leaves) return fail(c, HULK_E_HIP, "stopped rays without a leaf request");
+        req.resize(n_req);
+        HIP_TRY(c, hipMemcpyAsync(req.data(), c->s_list.p, (size_t)n_req * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        std::sort(req.begin(), req.end());  // the device appends in any order
+        // admit requested leaves, evicting the oldest; at most half the pool
+        // is cleared per launch (GLVisitorGPUTrace.cpp:445-462)
+        uint32_t cleared = 0, admitted = 0;
+        const uint32_t max_clear = pool / 2;
+        auto admit = [&](uint32_t leaf, uint32_t off) -> int {
+            const uint2 e = c->leaf_extent[leaf];
+            if (e.y)
+                HIP_TRY(c, hipMemcpyAsync(static_cast<TriRec*>(c->s_pool.p) + off, c->backing + e.x,
+                                          (size_t)e.y * sizeof(TriRec), hipMemcpyHostToDevice, c->stream));
+            ring.push(leaf, off, e.y, st.launches);
+            table[leaf] = make_uint2(off, 1u);
+            st.leaves_loaded++;
+            st.bytes_loaded += (uint64_t)e.y * sizeof(TriRec);
+            admitted++;
+            return HULK_OK;
+        };
+        for (uint32_t leaf : req) {
+            if (table[leaf].y) continue;
+            const uint32_t size = c->leaf_extent[leaf].y;
+            uint32_t off = 0;
+            bool ok = ring.fits(size, off);
+            // never evict a leaf loaded in this round: its rays have not used it yet
+            while (!ok && !ring.runs.empty() && ring.runs.front().launch != st.launches && cleared <= max_clear) {
+                const RecordRing::Run old = ring.pop();
+                table[old.leaf] = make_uint2(0, 0);
+                cleared += old.size;
+                st.leaves_evicted++;
+                ok = ring.fits(size, off);
+            }
+            if (!ok || cleared > max_clear) break;
+            if ((r = admit(leaf, off))) return r;
+        }
+        if (admitted == 0) {  // progress: empty the pool for the first request
+            while (!ring.runs.empty()) {
+                const RecordRing::Run old = ring.pop();
+                table[old.leaf] = make_uint2(0, 0);
+                st.leaves_evicted++;
+            }
+            uint32_t off = 0;
+            for (uint32_t leaf : req) {
+                if (table[leaf].y) continue;
+                if (!ring.fits(c->leaf_extent[leaf].y, off)) break;
+                if ((r = admit(leaf, off))) return r;
+                break;
+            }
+            if (admitted == 0) return fail(c, HULK_E_HIP, "no requested leaf fits the pool");
+        }
+        HIP_TRY(c, hipMemcpyAsync(c->s_table.p, table.data(), table.size() * 8, hipMemcpyHostToDevice, c->stream));
+        S.queue_in = static_cast<const uint32_t*>(c->s_queue[cur ^ 1].p);
+        S.n_in = n_out;
+        cur ^= 1;
+    }
+    if (!dev_out)
+        HIP_TRY(c, hipMemcpyAsync(hits, c->s_hits.p, (size_t)n * sizeof(hulk_hit), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    st.ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    if (stats) *stats = st;
+    return HULK_OK;
+}
+
 // ------------------------------------------------------ device KD build --
 
 extern "C" int hulk_build_kdtree_device(hulk_ctx* c, uint32_t capacity) {
@@ -1025,6 +1267,7 @@ extern "C" int hulk_build_kdtree_device(hulk_ctx* c, uint32_t capacity) {
     std::memcpy(c->level_prefix, R.level_prefix, sizeof(c->level_prefix));
     c->kdb_last = R;
     c->have_tree = true;
+    c->tree_gen++;
     return HULK_OK;
 }
 

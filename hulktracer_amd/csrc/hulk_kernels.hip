@@ -1523,4 +1523,153 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
+// ------------------------------------------------ out-of-core streaming --
+
+// kd-restart with leaves paged into a record pool (the residency protocol of
+// GPUTrace_stackless.cl:27-115 and the relaunch loop of GLVisitorGPUTrace::
+// launchTraceIterations, GLVisitorGPUTrace.cpp:412-489).  A lane traces its
+// pixel as trav_restart does, but a leaf's records are read from the pool at
+// the leaf's base, and only while the leaf is resident.  At a leaf that is
+// not, the lane flags the leaf as requested, saves (tmin, tmax, the leaf's
+// node) and its best hit so far, and queues its pixel for the next launch,
+// which tests that leaf with the same interval and goes on.  The sequence of
+// leaf tests is therefore the in-core kernel's, and so is the hit.  (The
+// reference's relaunch returns early once its hit lies below tmax; the
+// in-core kernel has no such exit, and neither has this one, so the two
+// agree bit for bit.)
+__global__ void __launch_bounds__(256) k_trace_streamed(StreamParams S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane_id = threadIdx.x & 63u;
+    bool queued = false;
+    uint32_t o = 0;
+    if (i < S.n_in) {
+        uint32_t x, y;
+        if (S.queue_in) {
+            o = S.queue_in[i];
+            y = o / S.W;
+            x = o - y * S.W;
+        } else {
+            tile_pixel(i, S.W, S.H, x, y);
+            o = y * S.W + x;
+        }
+        Ray r;
+        cast_ray(S.cam, x, y, S.W, S.H, r);
+        set_inv(r);
+        Best best;
+        float tmin = 0.0f, tmax = 0.0f;
+        uint32_t pending = 0, flags = 0;
+        if (S.queue_in) {
+            best = S.best[o];
+            const float4 st = S.state[o];
+            tmin = st.x;
+            tmax = st.y;
+            pending = __float_as_uint(st.z);
+        } else {
+            best_init(best);
+        }
+        Cnt c = {0, 0, 0};
+        float ts = 0.0f, te = kFltMax;
+        if (ray_box(S.box, r, ts, te)) {
+            uint32_t guard = 0;
+            if (pending) {
+                const uint4 n = S.nodes[pending];
+                const uint2 lt = S.leaf_table[n.w];
+                if (lt.y) {
+                    test_leaf<false, 1>(S.pool, make_uint4(n.x, lt.x, n.z, n.w), r, best, c, -kFltMax);
+                    pending = 0;
+                } else {
+                    S.request[n.w] = 1u;
+                    queued = true;
+                }
+            }
+            while (!queued && tmax < te) {
+                tmin = tmax;
+                tmax = te;
+                uint32_t node = 0;
+                uint4 n = S.nodes[0];
+                while (n.x & kNodeJunction) {
+                    float split;
+                    uint32_t first, second;
+                    junction(n, r, split, first, second);
+                    if (split >= tmax || split <= 0.0f) {
+                        node = first;
+                    } else if (split <= tmin) {
+                        node = second;
+                    } else {
+                        node = first;
+                        tmax = split;
+                    }
+                    n = S.nodes[node];
+                    if (node == 0) break;
+                }
+                if (n.x & kNodeLeaf) {
+                    const uint2 lt = S.leaf_table[n.w];
+                    if (lt.y) {
+                        test_leaf<false, 1>(S.pool, make_uint4(n.x, lt.x, n.z, n.w), r, best, c, -kFltMax);
+                    } else {
+                        S.request[n.w] = 1u;
+                        pending = node;
+                        queued = true;
+                    }
+                }
+                if (++guard > S.guard) {
+                    flags |= kHitFlagGuard;
+                    break;
+                }
+            }
+        }
+        if (queued) {
+            S.state[o] = make_float4(tmin, tmax, __uint_as_float(pending), 0.0f);
+            S.best[o] = best;
+        } else {
+            float u = 0.0f, v = 0.0f;
+            if (best.tri != kNoHit) {
+                u = best.s1 / best.det;
+                v = best.s2 / best.det;
+            }
+            float4* h = reinterpret_cast<float4*>(S.hits + o);
+            h[0] = make_float4(best.t, u, v, __uint_as_float(best.tri));
+            h[1] = make_float4(best.nx, best.ny, best.nz, __uint_as_float(flags));
+        }
+    }
+    // queue the stopped pixels: one atomic per wave
+    const unsigned long long m = __ballot(queued);
+    if (m) {
+        const int leader = __ffsll(static_cast<long long>(m)) - 1;
+        uint32_t base = 0;
+        if (static_cast<int>(lane_id) == leader) base = atomicAdd(S.n_out, static_cast<uint32_t>(__popcll(m)));
+        base = __shfl(base, leader, 64);
+        if (queued) S.queue_out[base + __popcll(m & ((1ull << lane_id) - 1ull))] = o;
+    }
+}
+
+// Requested leaves (flag set by a lane this launch) -> list; flags cleared.
+__global__ void __launch_bounds__(256) k_collect_requests(uint32_t* request, uint32_t n, uint32_t* list,
+                                                          uint32_t* n_list) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane_id = threadIdx.x & 63u;
+    const bool req = i < n && request[i] != 0u;
+    if (req) request[i] = 0u;
+    const unsigned long long m = __ballot(req);
+    if (m) {
+        const int leader = __ffsll(static_cast<long long>(m)) - 1;
+        uint32_t base = 0;
+        if (static_cast<int>(lane_id) == leader) base = atomicAdd(n_list, static_cast<uint32_t>(__popcll(m)));
+        base = __shfl(base, leader, 64);
+        if (req) list[base + __popcll(m & ((1ull << lane_id) - 1ull))] = i;
+    }
+}
+
+hipError_t launch_trace_streamed(const StreamParams& S, hipStream_t s) {
+    if (S.n_in == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_trace_streamed, dim3((S.n_in + 255) / 256), dim3(256), 0, s, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_collect_requests(uint32_t* request, uint32_t n, uint32_t* list, uint32_t* n_list, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_collect_requests, dim3((n + 255) / 256), dim3(256), 0, s, request, n, list, n_list);
+    return hipGetLastError();
+}
+
 }  // namespace hulk

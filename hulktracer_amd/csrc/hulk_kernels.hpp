@@ -126,6 +126,28 @@ struct ShadeParams {
     float* acc;
 };
 
+// Out-of-core trace (kd-restart over a paged record pool).  Per pixel:
+// state = (tmin, tmax, pending leaf node, -) and the best hit so far, kept
+// between launches; queue_in = pixels to resume (null: every pixel, first
+// launch); pixels that stop at a non-resident leaf go to queue_out.
+struct StreamParams {
+    const uint4* nodes;
+    const TriRec* pool;
+    const uint2* leaf_table;  // per leaf id: (pool base, resident)
+    uint32_t* request;        // per leaf id: set to 1 by a lane that needs the leaf
+    SceneBox box;
+    Camera cam;
+    uint32_t W, H;
+    const uint32_t* queue_in;
+    uint32_t n_in;
+    uint32_t* queue_out;
+    uint32_t* n_out;
+    float4* state;
+    Best* best;
+    hulk_hit* hits;
+    uint32_t guard;
+};
+
 // leaf_batch: triangle records in flight per lane in the leaf loop (1 or 2)
 hipError_t launch_trace(int variant, const TraceParams& P, bool count, bool render, uint32_t leaf_batch,
                         uint32_t grid, uint32_t block, size_t lds, hipStream_t s);
@@ -143,6 +165,8 @@ hipError_t launch_selftest_rcp(unsigned long long* out, hipStream_t s);  // out:
 // whole frames, rope traversal, lanes refilled from the ray queue (work_counter: 8 slices)
 hipError_t launch_render_refill(const TraceParams& P, bool count, uint32_t grid, uint32_t block, hipStream_t s);
 // rope slots (3 n slots: nodes, then two empty-leaf slots per junction)
+hipError_t launch_trace_streamed(const StreamParams& S, hipStream_t s);
+hipError_t launch_collect_requests(uint32_t* request, uint32_t n, uint32_t* list, uint32_t* n_list, hipStream_t s);
 hipError_t launch_build_ropes(const uint4* nodes, uint32_t n, uint32_t* up, uint4* rn, RopeSlot* rs,
                               const SceneBox& box, hipStream_t s);
 

@@ -323,6 +323,17 @@ class GPUTrace:
         lp = np.ascontiguousarray(leaf_polys, np.uint32)
         self._c(lib().hulk_upload_kdtree(self.h, ptr(nodes), len(nodes), ptr(lp) if len(lp) else None, len(lp)))
 
+    def trace_camera_streamed(self, cam: np.ndarray, W: int, H: int, pool_records: int = 0) -> tuple[np.ndarray, dict]:
+        """Primary rays with the leaves' records paged into a device pool of
+        `pool_records` records (hulk_trace_camera_streamed): kd-restart
+        traversal, the reference's residency protocol and relaunch loop.
+        Returns (hits, stats)."""
+        hits = np.zeros(W * H, HIT_DTYPE)
+        st = L.StreamStats()
+        self._c(lib().hulk_trace_camera_streamed(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H,
+                                                 int(pool_records), ptr(hits), 0, C.byref(st)))
+        return hits, {k: getattr(st, k) for k, _ in L.StreamStats._fields_}
+
     def build_kdtree_device(self, capacity: int = 0):
         """SAH-NlogN KD tree built on the GPU from the uploaded scene and
         installed as this context's tree (hulk_build_kdtree_device); same

@@ -349,6 +349,32 @@ int hulk_selftest_reciprocal(hulk_ctx* ctx, unsigned long long out[4]);
 /* Scene box of a context (min xyz, max xyz). */
 int hulk_ctx_scene_box(const hulk_ctx* ctx, float out[6]);
 
+/* --------------------------------------------- out-of-core streaming -- */
+
+typedef struct hulk_stream_stats {
+    uint32_t launches;        /* trace launches until every ray finished */
+    uint32_t leaves_loaded;   /* leaf uploads into the pool */
+    uint32_t leaves_evicted;  /* leaves dropped from the pool, oldest first */
+    uint32_t max_queue;       /* most rays resumed by one launch */
+    uint64_t bytes_loaded;    /* triangle-record bytes copied host -> device */
+    float    ms;              /* wall time of the call */
+    uint32_t pool_records;    /* pool size used */
+} hulk_stream_stats;
+
+/* Primary rays of a camera with the leaves' triangle records paged into a
+ * device pool of `pool_records` 64-B records (0 = every record) from a
+ * pinned host copy, as rays request them: the residency protocol of
+ * GPUTrace_stackless.cl:27-115 (a lane that reaches a non-resident leaf
+ * flags it, saves its traversal state and stops) and the relaunch loop of
+ * GLVisitorGPUTrace::launchTraceIterations (GLVisitorGPUTrace.cpp:412-489:
+ * load the requested leaves, evicting the oldest, at most half the pool per
+ * launch; launch again until no ray stops).  Traversal is kd-restart and the
+ * hits equal HULK_VARIANT_RESTART's with every leaf resident.  The pool must
+ * hold the largest leaf (else HULK_E_ILLEGAL_INPUT).  flags:
+ * HULK_FLAG_DEVICE_PTRS (hits is a device pointer). */
+int hulk_trace_camera_streamed(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height,
+                               uint32_t pool_records, hulk_hit* hits, uint32_t flags, hulk_stream_stats* stats);
+
 /* ------------------------------------------------ device KD tree build -- */
 
 /* SAH-NlogN KD tree built on the GPU from the scene uploaded to the
