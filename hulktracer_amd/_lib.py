@@ -159,6 +159,7 @@ EXPORTS = {
     "hulk_make_camera_lookat": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_void_p]),
     "hulk_scene_create": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "hulk_scene_load_obj": (C.c_int, [C.c_char_p, C.c_float, C.POINTER(C.c_void_p)]),
+    "hulk_scene_load_aff": (C.c_int, [C.c_char_p, C.c_float, C.POINTER(C.c_void_p)]),
     "hulk_dataset_write": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                      C.c_uint32, C.c_void_p, C.c_uint32]),
     "hulk_dataset_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),

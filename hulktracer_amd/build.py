@@ -18,7 +18,7 @@ LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libhulk.so"
 
 HIP_SOURCES = ["hulk_kernels.hip", "hulk_api.hip", "hulk_mlp.hip", "hulk_kdbuild.hip"]
-CPP_SOURCES = ["kdtree_sah.cpp", "scenes.cpp", "loaders.cpp", "dataset.cpp"]
+CPP_SOURCES = ["kdtree_sah.cpp", "scenes.cpp", "loaders.cpp", "loader_aff.cpp", "dataset.cpp"]
 
 # -ffp-contract=off: no implicit multiply-add fusion, so the kernels keep
 # the reference's operation order bit for bit.  Packed FP32 ops are turned

@@ -14,6 +14,10 @@ struct hulk_scene {
     std::vector<hulk_tri> tris;
     std::vector<hulk_light> lights;
     std::vector<uint8_t> mirror;
+    // camera of a scene read from a file (GLVisitorCPUTrace::updateCamera
+    // inputs): position, rotation axis and angle, fov (degrees), aspect
+    bool has_camera = false;
+    float cam_from[3] = {0, 0, 0}, cam_axis[3] = {0, 0, 0}, cam_angle = 0, cam_fov = 45.0f, cam_aspect = 1.0f;
 };
 
 // sin / cos of the scene code (double, host-independent polynomial).

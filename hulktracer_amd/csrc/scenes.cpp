@@ -351,6 +351,10 @@ extern "C" int hulk_scene_get_box(const hulk_scene* s, hulk_box* out) {
 
 extern "C" int hulk_scene_camera(const hulk_scene* s, uint32_t frame, float aspect, hulk_camera* out) {
     if (!s || !out) return HULK_E_ILLEGAL_INPUT;
+    if (s->has_camera) {  // a scene file's camera at its load time; aspect <= 0: the file's
+        const float aa[4] = {s->cam_axis[0], s->cam_axis[1], s->cam_axis[2], s->cam_angle};
+        return hulk_make_camera(s->cam_from, aa, s->cam_fov, aspect > 0.0f ? aspect : s->cam_aspect, out);
+    }
     return hulk_scene_kind_camera(s->kind, frame, aspect, out);
 }
 

@@ -100,6 +100,26 @@ class Scene:
         return s
 
     @classmethod
+    def from_aff(cls, path, time: float = 0.0) -> "Scene":
+        """A BART scene file as the reference's AFFParser reads it
+        (AFFParser.cpp:1469-1539) at scene time `time`: meshes under static and
+        key-framed transforms, point lights; camera() gives the camera of its
+        "camera" transform (aspect None: the file's resolution)."""
+        h = C.c_void_p()
+        _check(None, lib().hulk_scene_load_aff(str(path).encode(), float(time), C.byref(h)))
+        cam = np.zeros(16, np.float32)
+        try:
+            _check(None, lib().hulk_scene_camera(h, 0, 0.0, ptr(cam)))
+        except Exception:
+            lib().hulk_scene_free(h)
+            raise
+        s = cls._from_handle(h, 0, 0)
+        s.name = str(path)
+        s._file_camera = (str(path), float(time))
+        s._file_cam0 = cam
+        return s
+
+    @classmethod
     def from_dataset(cls, path) -> tuple["Scene", np.ndarray, np.ndarray]:
         """A paris.bin KD dataset (GLVisitorGPUTraceSSD::buildTree,
         GLVisitorGPUTraceSSD.cpp:43-107) as (scene, nodes, leaf polygons): the
@@ -148,8 +168,21 @@ class Scene:
         lt = np.zeros((0, 8), np.float32) if lights is None else np.ascontiguousarray(lights, np.float32)
         return cls(v, np.ascontiguousarray(t), lt, scene_box(v))
 
-    def camera(self, frame: int = 0, aspect: float = 1.0) -> np.ndarray:
-        """CameraPos (64 B) of a procedural scene's pose / frame."""
+    def camera(self, frame: int = 0, aspect: float | None = 1.0) -> np.ndarray:
+        """CameraPos (64 B) of a procedural scene's pose / frame, or of a
+        BART file's camera transform (aspect None: the file's)."""
+        if getattr(self, "_file_camera", None):
+            if aspect is None:
+                return self._file_cam0.copy()
+            h = C.c_void_p()
+            path, t = self._file_camera
+            _check(None, lib().hulk_scene_load_aff(path.encode(), t, C.byref(h)))
+            try:
+                cam = np.zeros(16, np.float32)
+                _check(None, lib().hulk_scene_camera(h, 0, float(aspect), ptr(cam)))
+            finally:
+                lib().hulk_scene_free(h)
+            return cam
         if not self.kind:
             raise ValueError("camera poses exist only for procedural scenes; use make_camera")
         cam = np.zeros(16, np.float32)

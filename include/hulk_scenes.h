@@ -53,6 +53,16 @@ int hulk_scene_kind_camera(int kind, uint32_t frame, float aspect, hulk_camera* 
  * vertex list. */
 int hulk_scene_load_obj(const char* path, float time, hulk_scene** out);
 
+/* Scene of a BART AFF file as AFFParser::viParseFile reads it
+ * (BartLoader/src/AFFParser.cpp:1469-1539): its triangle meshes at scene
+ * time `time` under static and key-framed transforms
+ * (AFFTransformVisitor::preAct, AFFTransformVisitor.cpp:24-104), its point
+ * lights, and the camera of its "camera" transform and viewpoint fov
+ * (hulk_scene_camera with aspect <= 0 uses the file's resolution).  Include
+ * files are read relative to the file's directory.  HULK_E_FILE_OPEN when
+ * the file cannot be read; HULK_E_ILLEGAL_INPUT for a malformed entity. */
+int hulk_scene_load_aff(const char* path, float time, hulk_scene** out);
+
 /* The reference's on-disk KD dataset ("paris.bin"):
  * hulk_dataset_write stores a scene's tree in the layout of
  * GLVisitorGPUTrace::saveDataSetToFile (GLVisitorGPUTrace.cpp:638-674)

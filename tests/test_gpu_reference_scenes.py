@@ -91,3 +91,30 @@ def test_dataset_round_trip_on_the_gpu(tmp_path):
         assert np.array_equal(a[f].view(np.uint32), b[f].view(np.uint32)), f
     hit = b["tri"] != H.NO_HIT
     assert np.array_equal(lp[b["tri"][hit]], a["tri"][hit]) and hit.sum() > 10000
+
+
+@pytest.mark.parametrize("time", [0.5, 3.0])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_bart_museum_trace_and_device_build(variant, time):
+    """The BART museum scene (museum3, from the reference's BartLoader) through
+    its own camera path: every variant against the oracle, shading, and the
+    device-built tree against the host builder."""
+    s = H.Scene.from_aff(GOLDEN / "museum" / "museum3.aff", time)
+    cam = s.camera(0, 1.0)
+    kd = H.KDTreeCPU()
+    kd.buildKDTree(s)
+    nodes, lp = kd.getNodeArray(), kd.getPolygonArray()
+    osc = O.OracleScene(s.vertices, s.triangles, s.box, nodes, lp, s.lights, s.mirror)
+    print(f"museum3 t={time} variant {variant}", flush=True)
+    ctx = H.GPUTrace(variant=variant, lds_top_levels=6)
+    ctx.upload_scene(s)
+    ctx.build_kdtree_device()
+    dn, dlp = ctx.download_kdtree()
+    assert np.array_equal(dn, nodes) and np.array_equal(dlp, lp)
+    got = ctx.trace_camera(cam, 192, 192)
+    ref = osc.trace_camera(ORACLE_OF[variant], cam, 192, 192, depth=6)
+    bad = hit_mismatch(got, ref)
+    assert len(bad) == 0, f"{len(bad)} rays differ\n{describe(got, ref, bad)}"
+    assert (got["tri"] != H.NO_HIT).sum() > 500
+    img = ctx.shade(cam, 192, 192, got, L.SHADE_FUSED)
+    assert np.array_equal(img, osc.shade(ORACLE_OF[variant], cam, 192, 192, ref, mode=L.SHADE_FUSED))

@@ -125,3 +125,118 @@ def test_host_tree_of_the_teapot_matches_the_restatement_oracle():
     kd.buildKDTree(s)
     nodes, lp = K.build(s.vertices, s.triangles, s.box)
     assert np.array_equal(kd.getNodeArray(), nodes) and np.array_equal(kd.getPolygonArray(), lp)
+
+
+# ---------------------------------------------------------------- BART (AFF) --
+
+MUSEUM = GOLDEN / "museum" / "museum3.aff"
+
+
+def test_museum_scene_loads_its_mesh_and_lights():
+    """museum3: the only mesh within the detail level is the pillar (5,025
+    vertices, 10,000 triangles, no transform); the room, paintings and
+    animated art are polygons and tpa triangles, which the reference parses
+    and drops; two point lights."""
+    s = H.Scene.from_aff(MUSEUM)
+    assert s.vertices.shape == (5025, 4) and s.triangles.shape == (10000, 4)
+    text = (GOLDEN / "museum" / "pillar.aff").read_text().split()
+    i = text.index("vertices")
+    V = np.array([strtof(x) for x in text[i + 2:i + 2 + 3 * 5025]], np.float32).reshape(-1, 3)
+    assert np.array_equal(s.vertices[:, :3].view(np.uint32), V.view(np.uint32)) and np.all(s.vertices[:, 3] == 1)
+    j = text.index("triangles")
+    per = 3 * (1 + ("normals" in text[:j]) + ("texturecoords" in text[:j]))  # vertex [normal] [texture] indices
+    T = np.array([int(x) for x in text[j + 2:j + 2 + per * 10000]]).reshape(-1, per)[:, :3]
+    assert np.array_equal(s.triangles[:, :3], T)
+    assert np.array_equal(s.lights, np.array([[0.5, 0.5, 0.5, 1.0, -4.0, 4.5, 0.0, 1.0],
+                                              [0.5, 0.5, 0.5, 1.0, 4.0, 4.5, 0.0, 1.0]], np.float32))
+
+
+def test_museum_camera_passes_through_its_keys():
+    """The camera transform is a Kochanek-Bartels spline through its keys:
+    at a key time its position is the key's translation and its rotation the
+    key's, so the camera equals one built from the key."""
+    for t, pos, axis, deg in ((0.5, (3.0, 2.4, -1.0), (0.0, 1.0, 0.0), 90.0),
+                              (2.5, (-4.0, 1.8, 0.0), (0.0, 1.0, 0.0), -90.0)):
+        cam = H.Scene.from_aff(MUSEUM, t).camera(0, 1.0)
+        # updateCamera: rotateMatrix(axis, -angle); fov of the file's viewpoint
+        want = H.make_camera(pos, (*axis, -math.radians(deg)), fov=_museum_fov(), aspect=1.0)
+        assert np.allclose(cam[:3], pos, atol=1e-6)
+        assert np.allclose(cam, want, atol=2e-5), (t, cam, want)
+
+
+def _museum_fov():
+    words = MUSEUM.read_text().split()
+    return float(words[words.index("angle") + 1])
+
+
+def _write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return p
+
+
+def test_static_transform_scale_rotate_translate(tmp_path):
+    """xs: scale, then rotation about the axis, then translation (row
+    vectors times S R T), checked against the closed form."""
+    p = _write(tmp_path, "xs.aff", "xs 2 3 1  0 0 1 90  1 0.5 0\n{\nm vertices 3\n 1 0 0\n 0 1 0\n 0 0 1\n"
+                                   "triangles 1\n 0 1 2\n}\n")
+    s = H.Scene.from_aff(p)
+    v = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float64) * [2, 3, 1]
+    c, sn = math.cos(math.pi / 2), math.sin(math.pi / 2)
+    rot = np.array([[c, -sn, 0], [sn, c, 0], [0, 0, 1]])  # rows of rotateMatrix(0, 0, 1, angle)
+    want = v @ rot + [1, 0.5, 0]
+    assert np.allclose(s.vertices[:, :3], want, atol=1e-6)
+
+
+def test_keyframed_translation_hits_its_keys_and_clamps(tmp_path):
+    keys = [(-1, 0), (0, 0), (1, 2), (2, 5), (3, 4), (4, 4)]
+    body = "\n".join(f"  {t} {x} 0 0  0 0 0" for t, x in keys)
+    p = _write(tmp_path, "k.aff", "x mover\n{\nm vertices 3\n 0 0 0\n 1 0 0\n 0 1 0\ntriangles 1\n 0 1 2\n}\n"
+                                  f"k mover\n{{\n transl {len(keys)}\n{body}\n}}\n")
+    for t, x in ((0, 0), (1, 2), (2, 5), (3, 4)):  # interior keys: segment ends
+        s = H.Scene.from_aff(p, t)
+        assert np.allclose(s.vertices[:, 0], np.array([0, 1, 0]) + x, atol=1e-6), t
+    # before the first segment / after the last: clamped to its ends
+    assert np.allclose(H.Scene.from_aff(p, -5).vertices[0, 0], 0, atol=1e-6)
+    assert np.allclose(H.Scene.from_aff(p, 9).vertices[0, 0], 4, atol=1e-6)
+    # between keys the curve is smooth and stays between neighbouring values here
+    mid = H.Scene.from_aff(p, 1.5).vertices[0, 0]
+    assert 2 < mid < 5.5
+
+
+def test_entities_the_reference_drops_and_includes(tmp_path):
+    _write(tmp_path, "extra.aff", "m vertices 3\n 5 5 5\n 6 5 5\n 5 6 5\ntriangles 1\n 0 1 2\n")
+    p = _write(tmp_path, "main.aff", "\n".join([
+        "# comment", "% comment", "b 0.1 0.2 0.3",
+        "v", "from 0 0 5", "at 0 0 0", "up 0 1 0", "angle 40", "hither 1", "resolution 640 480",
+        "am 1 1 1", "a 0 10 100", "f 1 0 0 1 0 10 0 1", "fm 0 0 0 1 1 1 0 0 0 10 0 1",
+        "c 0 0 0 1 0 1 0 0.5", "s 0 0 0 1", "p 3\n 0 0 0\n 1 0 0\n 0 1 0",
+        "pp 3\n 0 0 0 0 0 1\n 1 0 0 0 0 1\n 0 1 0 0 0 1",
+        "tt tex 0 0 0 0 0  1 0 0 1 0  0 1 0 0 1",
+        "tpa 1\n 0\n 0 0 0 0 0 1\n 1 0 0 0 0 1\n 0 1 0 0 0 1",
+        "l 1 2 3", "la lamp 4 5 6 0.2 0.3 0.4",
+        "i 1 extra.aff",  # above the detail level: skipped
+        "m vertices 3\n 0 0 0\n 1 0 0\n 0 1 0\nnormals 1\n 0 0 1\ntexturecoords 3 tex\n 0 0 1 0 0 1\n"
+        "triangles 1\n 0 1 2  0 0 0  0 1 2",
+        "d 1", "i 1 extra.aff",  # now read
+    ]) + "\n")
+    s = H.Scene.from_aff(p)
+    assert s.triangles[:, :3].tolist() == [[0, 1, 2], [3, 4, 5]]
+    assert s.vertices[3:, :3].tolist() == [[5, 5, 5], [6, 5, 5], [5, 6, 5]]
+    assert np.array_equal(s.lights, np.array([[1, 1, 1, 1, 1, 2, 3, 1], [0.2, 0.3, 0.4, 1, 4, 5, 6, 1]], np.float32))
+
+
+@pytest.mark.parametrize("text", ["q 1 2 3\n", "xs 1 1 1 0 1 0 0 0 0 0\nm\n", "i 0 missing.aff\n",
+                                  "m vertices 3\n 0 0 0\n 1 0 0\n 0 1 0\ntriangles 1\n 0 1 7\n",
+                                  "k a\n{\n transl 2\n 0 0 0 0 0 0 0\n 1 0 0 0 0 0 0\n}\n"])
+def test_malformed_aff_is_rejected(tmp_path, text):
+    p = _write(tmp_path, "bad.aff", text)
+    with pytest.raises(H.HulkError) as e:
+        H.Scene.from_aff(p)
+    assert e.value.code == 4
+
+
+def test_missing_aff_file():
+    with pytest.raises(H.HulkError) as e:
+        H.Scene.from_aff(GOLDEN / "museum" / "none.aff")
+    assert e.value.code == 2
