@@ -77,6 +77,8 @@ struct hulk_ctx {
     uint32_t max_leaf = 0;
     TriRec* backing = nullptr;  // pinned
     size_t backing_cap = 0;
+    TriRec* staging = nullptr;  // pinned mirror of the device pool
+    size_t staging_cap = 0;
     uint32_t* stream_host = nullptr;  // pinned: (stopped pixels, requested leaves)
     DevBuf s_pool, s_table, s_request, s_list, s_queue[2], s_state, s_best, s_hits, s_count;
     uint32_t level_prefix[kTopLevelsMax + 1] = {0};
@@ -631,6 +633,7 @@ extern "C" void hulk_destroy(hulk_ctx* c) {
                       &c->s_state, &c->s_best, &c->s_hits, &c->s_count})
         release(*b);
     if (c->backing) (void)hipHostFree(c->backing);
+    if (c->staging) (void)hipHostFree(c->staging);
     if (c->stream_host) (void)hipHostFree(c->stream_host);
     if (c->host_stats) (void)hipHostFree(c->host_stats);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1118,7 +1121,16 @@ extern "C" int hulk_trace_camera_streamed(hulk_ctx* c, const hulk_camera* cam, u
     if ((r = ensure(c, c->s_best, (size_t)n * sizeof(Best)))) return r;
     if (!dev_out && (r = ensure(c, c->s_hits, (size_t)n * sizeof(hulk_hit)))) return r;
     if ((r = ensure(c, c->s_count, 64))) return r;
+    if ((size_t)pool * sizeof(TriRec) > c->staging_cap) {
+        if (c->staging) HIP_TRY(c, hipHostFree(c->staging));
+        c->staging = nullptr;
+        c->staging_cap = 0;
+        HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->staging), (size_t)pool * sizeof(TriRec),
+                                 hipHostMallocDefault));
+        c->staging_cap = (size_t)pool * sizeof(TriRec);
+    }
     uint32_t* d_count = static_cast<uint32_t*>(c->s_count.p);
+    std::vector<uint2> loaded;  // (pool offset, records) of this round's leaves
     HIP_TRY(c, launch_fill_u32(static_cast<uint32_t*>(c->s_request.p), std::max<uint32_t>(n_leaves, 1), 0u, c->stream));
     std::vector<uint2> table(std::max<uint32_t>(n_leaves, 1), make_uint2(0, 0));
     HIP_TRY(c, hipMemcpyAsync(c->s_table.p, table.data(), table.size() * 8, hipMemcpyHostToDevice, c->stream));
@@ -1171,11 +1183,15 @@ extern "C" int hulk_trace_camera_streamed(hulk_ctx* c, const hulk_camera* cam, u
         // is cleared per launch (GLVisitorGPUTrace.cpp:445-462)
         uint32_t cleared = 0, admitted = 0;
         const uint32_t max_clear = pool / 2;
+        // a leaf's records go to the pinned mirror of the pool; the pool is
+        // updated per contiguous run of this round's leaves, not per leaf
+        loaded.clear();
         auto admit = [&](uint32_t leaf, uint32_t off) -> int {
             const uint2 e = c->leaf_extent[leaf];
-            if (e.y)
-                HIP_TRY(c, hipMemcpyAsync(static_cast<TriRec*>(c->s_pool.p) + off, c->backing + e.x,
-                                          (size_t)e.y * sizeof(TriRec), hipMemcpyHostToDevice, c->stream));
+            if (e.y) {
+                std::memcpy(c->staging + off, c->backing + e.x, (size_t)e.y * sizeof(TriRec));
+                loaded.push_back(make_uint2(off, e.y));
+            }
             ring.push(leaf, off, e.y, st.launches);
             table[leaf] = make_uint2(off, 1u);
             st.leaves_loaded++;
@@ -1213,6 +1229,13 @@ extern "C" int hulk_trace_camera_streamed(hulk_ctx* c, const hulk_camera* cam, u
                 break;
             }
             if (admitted == 0) return fail(c, HULK_E_HIP, "no requested leaf fits the pool");
+        }
+        std::sort(loaded.begin(), loaded.end(), [](const uint2& a, const uint2& b) { return a.x < b.x; });
+        for (size_t k = 0; k < loaded.size();) {
+            uint32_t lo = loaded[k].x, hi = loaded[k].x + loaded[k].y;
+            for (k++; k < loaded.size() && loaded[k].x <= hi; k++) hi = std::max(hi, loaded[k].x + loaded[k].y);
+            HIP_TRY(c, hipMemcpyAsync(static_cast<TriRec*>(c->s_pool.p) + lo, c->staging + lo,
+                                      (size_t)(hi - lo) * sizeof(TriRec), hipMemcpyHostToDevice, c->stream));
         }
         HIP_TRY(c, hipMemcpyAsync(c->s_table.p, table.data(), table.size() * 8, hipMemcpyHostToDevice, c->stream));
         S.queue_in = static_cast<const uint32_t*>(c->s_queue[cur ^ 1].p);
