@@ -581,8 +581,10 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
 // device).  Output: a full hit record, or only t scattered to t_slot[i].
 // RENDER: camera / frame rays only; after the primary hit record each ray
 // runs its whole shading path (render_ray) and writes its packed pixel.
+// Two records in flight (LB = 2) want about 100 VGPRs; a cap of 5 waves per
+// SIMD (96 VGPRs, a few bytes of spill) measured fastest for this kernel.
 template <int VARIANT, bool COUNT, bool RENDER, int LB>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(LB == 2 ? 6 : 1)))
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(LB == 2 ? 5 : 1)))
 k_trace(TraceParams P) {
     extern __shared__ uint4 smem[];
     const uint32_t tid = threadIdx.x;
@@ -770,9 +772,9 @@ enum LaneField : uint32_t {
 };
 constexpr uint32_t kRefillLdsWords = kLaneFields;
 
+// No wave cap: the refill loop's lane state measured fastest unspilled.
 template <bool COUNT, int LB>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LB == 2 ? 6 : 1)))
-k_render_refill(TraceParams P) {
+__global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
     extern __shared__ uint32_t lane_mem[];
     const uint32_t tid = threadIdx.x, block = blockDim.x;
     const uint32_t lane_id = tid & 63u;
