@@ -104,6 +104,7 @@ EXPORTS = {
     "hulk_sync": (C.c_int, [C.c_void_p]),
     "hulk_upload_scene": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
     "hulk_upload_kdtree": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
+    "hulk_update_vertices": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
     "hulk_set_lights": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "hulk_trace_camera": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                     C.c_uint32]),

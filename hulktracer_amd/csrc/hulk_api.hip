@@ -694,6 +694,24 @@ extern "C" int hulk_upload_scene(hulk_ctx* c, const hulk_float4* verts, uint32_t
     return HULK_OK;
 }
 
+extern "C" int hulk_update_vertices(hulk_ctx* c, const hulk_float4* verts, uint32_t n_verts,
+                                    const hulk_box* scene_box, uint32_t flags) {
+    if (!c || !verts || !scene_box) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    if (!c->have_scene) return fail(c, HULK_E_NO_SCENE, "upload the scene before updating its vertices");
+    if (n_verts != c->n_verts) return fail(c, HULK_E_ILLEGAL_INPUT, "vertex count differs from the uploaded scene");
+    if (n_verts)
+        HIP_TRY(c, hipMemcpyAsync(c->verts.p, verts, (size_t)n_verts * 16,
+                                  (flags & HULK_FLAG_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                  c->stream));
+    c->box.mn[0] = scene_box->min.x; c->box.mn[1] = scene_box->min.y; c->box.mn[2] = scene_box->min.z;
+    c->box.mx[0] = scene_box->max.x; c->box.mx[1] = scene_box->max.y; c->box.mx[2] = scene_box->max.z;
+    c->have_tree = false;  // records and tree depend on the vertices
+    c->ropes_ready = false;
+    c->tree_gen++;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HULK_OK;
+}
+
 extern "C" int hulk_upload_kdtree(hulk_ctx* c, const hulk_kd_node* nodes, uint32_t n_nodes,
                                   const uint32_t* leaf_polys, uint32_t n_lp) {
     if (!c) return HULK_E_ILLEGAL_INPUT;

@@ -351,6 +351,12 @@ class GPUTrace:
         if scene.lights is not None:
             self.set_lights(scene.lights)
 
+    def update_vertices(self, vertices: np.ndarray, box: np.ndarray):
+        """New vertex positions of the uploaded scene (same count and
+        triangles); the tree must be rebuilt or uploaded afterwards."""
+        v = np.ascontiguousarray(vertices, np.float32)
+        self._c(lib().hulk_update_vertices(self.h, ptr(v), len(v), ptr(np.ascontiguousarray(box, np.float32)), 0))
+
     def upload_kdtree(self, nodes: np.ndarray, leaf_polys: np.ndarray):
         nodes = np.ascontiguousarray(nodes, np.uint32)
         lp = np.ascontiguousarray(leaf_polys, np.uint32)

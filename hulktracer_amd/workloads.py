@@ -4,7 +4,8 @@
                   rebuild per frame overlapped with the previous frame's
                   GPU render, primary + shadow + 2 mirror generations
   animated_device_build
-                  the same frames with the KD tree built on the GPU
+                  the same frames with the KD tree built on the GPU from the moved
+                  vertices
   batched         64 frames of the random-triangle and terrain scenes,
                   resolutions 256^2-1024^2, bounce depth 0-3, one
                   graph-captured pipeline per scene, autotuned per scene
@@ -121,7 +122,8 @@ def animated_device_build(frames: int = 20, W: int = 1024, H: int = 1024, max_bo
     (hulk_build_kdtree_device, same arrays as the CPU build): per frame the
     moved scene is uploaded, the tree is built in HBM and the frame is
     rendered.  The frames' geometry is synthesised on the host before the
-    timed loop (the animation itself is input, not part of the path)."""
+    timed loop (the animation itself is input, not part of the path); per
+    frame only the moved vertices go to the device."""
     torch = _torch()
     stream = stream or torch.cuda.Stream()
     rgba = torch.empty(W * H, dtype=torch.int32, device="cuda")
@@ -146,7 +148,7 @@ def animated_device_build(frames: int = 20, W: int = 1024, H: int = 1024, max_bo
     t_wall = time.perf_counter()
     for f in range(frames):
         t0 = time.perf_counter()
-        ctx.upload_scene(scenes[f])
+        ctx.update_vertices(scenes[f].vertices, scenes[f].box)  # the triangles do not change
         t1 = time.perf_counter()
         ctx.build_kdtree_device()  # returns once the tree is installed
         t2 = time.perf_counter()

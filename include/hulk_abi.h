@@ -194,6 +194,15 @@ int hulk_sync(hulk_ctx* ctx);
 int hulk_upload_scene(hulk_ctx* ctx, const hulk_float4* verts, uint32_t n_verts,
                       const hulk_tri* tris, uint32_t n_tris, const hulk_box* scene_box);
 
+/* New positions for the uploaded scene's vertices (same count, same
+ * triangles), e.g. the next frame of an animation: the vertex write of
+ * the per-frame buildTree (GLVisitorGPUTrace.cpp:256-294) without the
+ * triangle upload and its index check.  The tree must be uploaded or built
+ * again before tracing.  flags: HULK_FLAG_DEVICE_PTRS (verts on the
+ * device). */
+int hulk_update_vertices(hulk_ctx* ctx, const hulk_float4* verts, uint32_t n_verts, const hulk_box* scene_box,
+                         uint32_t flags);
+
 /* KD tree as produced by KDTreeCPU::buildKDTree (m_treeNodes, m_leafPolygons;
  * KDTreeCPU.hpp:128-132).  The library derives its own device layout
  * (per-leaf triangle records, LDS top tree, ropes).  Replaces

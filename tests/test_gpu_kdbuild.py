@@ -235,3 +235,27 @@ def test_visitor_builds_tree_on_the_device():
         v.traverseBegin()
         frames.append(np.array(v.framebuffer))
     assert np.array_equal(frames[0], frames[1])
+
+
+def test_update_vertices_then_device_build():
+    """Animated frames through hulk_update_vertices (vertices only) and a
+    device build: each frame's tree and hits equal a fresh upload's."""
+    s0, _, _ = scene_and_tree(3, 7, 0, 5)
+    ctx = H.GPUTrace(variant=H.ROPES)
+    ctx.upload_scene(s0)
+    for frame in (6, 9):
+        s, nodes, lp = scene_and_tree(3, 7, 0, frame)
+        ctx.update_vertices(s.vertices, s.box)
+        with pytest.raises(H.HulkError):
+            ctx.trace_camera(s.camera(frame, 1.0), 32, 32)  # the old tree was dropped
+        ctx.build_kdtree_device()
+        n2, lp2 = ctx.download_kdtree()
+        assert_same_tree(n2, lp2, nodes, lp, f"frame {frame}")
+        ref = H.GPUTrace(variant=H.ROPES)
+        ref.upload_scene(s)
+        ref.upload_kdtree(nodes, lp)
+        cam = s.camera(frame, 1.0)
+        assert len(hit_mismatch(ctx.trace_camera(cam, 128, 128), ref.trace_camera(cam, 128, 128))) == 0
+    bad = np.zeros((10, 4), np.float32)
+    with pytest.raises(H.HulkError):
+        ctx.update_vertices(bad, s0.box)  # vertex count differs
