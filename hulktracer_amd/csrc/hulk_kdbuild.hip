@@ -42,8 +42,8 @@
 // start of the reference arrays, and into the device layout the traversal
 // kernels read (breadth-first above depth 12, depth-first below).
 //
-// Sorting the root candidates (one radix sort per axis, rocPRIM) replaces
-// the three std::sort calls of KDTreeSAH.cpp:303-306.  Only the order of
+// Sorting the root candidates (one rocPRIM radix sort of (axis, value)
+// keys) replaces the three std::sort calls of KDTreeSAH.cpp:303-306.  Only the order of
 // values matters: candidates of equal value form one sweep step.  -0.0 and
 // +0.0 sort as equal keys, as under std::sort; which of them comes first in
 // such a run is left open by std::sort, so a split plane at zero may carry
@@ -239,6 +239,43 @@ __device__ __forceinline__ uint32_t block_excl(uint32_t v, Op op, uint32_t* lds,
     return op(pre, ex);
 }
 
+// Three scans in one pass (two sums and a sum or max in the third lane):
+// the same steps as block_excl with one barrier pair for all three.
+template <uint32_t NT, bool MAX3>
+__device__ __forceinline__ uint3 block_excl3(uint3 v, uint3* lds, uint3& total) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint3 inc = v;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t ta = __shfl_up(inc.x, d, 64), tb = __shfl_up(inc.y, d, 64), tc = __shfl_up(inc.z, d, 64);
+        if (lane >= d) {
+            inc.x += ta;
+            inc.y += tb;
+            inc.z = MAX3 ? (inc.z > tc ? inc.z : tc) : inc.z + tc;
+        }
+    }
+    if (lane == 63u) lds[wave] = inc;
+    __syncthreads();
+    uint3 pre = make_uint3(0, 0, 0), tot = make_uint3(0, 0, 0);
+#pragma unroll
+    for (uint32_t w = 0; w < NT / 64; w++) {
+        const uint3 q = lds[w];
+        if (w < wave) {
+            pre.x += q.x;
+            pre.y += q.y;
+            pre.z = MAX3 ? (pre.z > q.z ? pre.z : q.z) : pre.z + q.z;
+        }
+        tot.x += q.x;
+        tot.y += q.y;
+        tot.z = MAX3 ? (tot.z > q.z ? tot.z : q.z) : tot.z + q.z;
+    }
+    __syncthreads();
+    uint3 ex = make_uint3(__shfl_up(inc.x, 1, 64), __shfl_up(inc.y, 1, 64), __shfl_up(inc.z, 1, 64));
+    if (lane == 0) ex = make_uint3(0, 0, 0);
+    total = tot;
+    return make_uint3(pre.x + ex.x, pre.y + ex.y, MAX3 ? (pre.z > ex.z ? pre.z : ex.z) : pre.z + ex.z);
+}
+
 // ------------------------------------------------------------------ root --
 
 __global__ void k_kdb_init(KdDev D, SceneBox box, uint32_t n_tris, uint32_t ns) {
@@ -276,8 +313,9 @@ __global__ void k_kdb_init(KdDev D, SceneBox box, uint32_t n_tris, uint32_t ns) 
 
 // Root polygon list, input check and the unsorted candidates: every 5th
 // triangle, per axis a BEGIN at its min and an END at its max
-// (KDTreeSAH.cpp:273-295).  keys / vals: 3 runs of 2 * ns.
-__global__ void k_kdb_root(KdDev D, uint32_t n_tris, uint32_t ns, uint32_t* keys, uint32_t* vals) {
+// (KDTreeSAH.cpp:273-295).  Key = axis << 32 | ordered value, so one sort
+// of all three axes leaves them as 3 runs of 2 * ns.
+__global__ void k_kdb_root(KdDev D, uint32_t n_tris, uint32_t ns, unsigned long long* keys, uint32_t* vals) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n_tris) {
         D.poly[0][g] = g;
@@ -302,21 +340,22 @@ __global__ void k_kdb_root(KdDev D, uint32_t n_tris, uint32_t ns, uint32_t* keys
             const float mn = sse_min(sse_min(lane3(p1, j), lane3(p2, j)), lane3(p3, j));
             const float mx = sse_max(sse_max(lane3(p1, j), lane3(p2, j)), lane3(p3, j));
             const uint32_t o = j * 2u * ns + 2u * g;
-            keys[o] = order_key(mn);
+            const unsigned long long axis = static_cast<unsigned long long>(j) << 32;
+            keys[o] = axis | order_key(mn);
             vals[o] = p | (__float_as_uint(mn) == 0x80000000u ? kNegZeroBit : 0u);
-            keys[o + 1] = order_key(mx);
+            keys[o + 1] = axis | order_key(mx);
             vals[o + 1] = p | kEndBit | (__float_as_uint(mx) == 0x80000000u ? kNegZeroBit : 0u);
         }
     }
 }
 
-__global__ void k_kdb_root_cands(KdDev D, uint32_t ns, const uint32_t* keys, const uint32_t* vals) {
+__global__ void k_kdb_root_cands(KdDev D, uint32_t ns, const unsigned long long* keys, const uint32_t* vals) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= 6u * ns) return;
     const uint32_t v = vals[e];
-    const uint32_t bits = (v & kNegZeroBit) ? 0x80000000u : key_bits(keys[e]);
+    const uint32_t bits = (v & kNegZeroBit) ? 0x80000000u : key_bits(static_cast<uint32_t>(keys[e]));
     D.cand[0][e] = make_uint2(bits, v & ~kNegZeroBit);
-    D.cseg[0][e] = e / (2u * ns);
+    D.cseg[0][e] = static_cast<uint32_t>(keys[e] >> 32);  // axis run e / (2 ns)
 }
 
 // ------------------------------------------------------------ group scan --
@@ -345,7 +384,7 @@ __device__ __forceinline__ void group_item(const KdDev& D, uint32_t L, uint32_t 
 }
 
 __global__ void __launch_bounds__(kThreads) k_kdb_group_reduce(KdDev D, uint32_t L) {
-    __shared__ uint32_t s0[kThreads / 64], s1[kThreads / 64], s2[kThreads / 64];
+    __shared__ uint3 s3[kThreads / 64];
     if (D.st->overflow) return;
     const uint32_t n = D.lvl[L].n_cands;
     const uint32_t tiles = (n + kTile - 1) / kTile;
@@ -361,11 +400,9 @@ __global__ void __launch_bounds__(kThreads) k_kdb_group_reduce(KdDev D, uint32_t
             e += ei;
             h = max(h, hi);
         }
-        uint32_t tb, te, th;
-        block_excl<kThreads>(b, SumOp(), s0, tb);
-        block_excl<kThreads>(e, SumOp(), s1, te);
-        block_excl<kThreads>(h, MaxOp(), s2, th);
-        if (threadIdx.x == 0) D.part_c[t] = make_uint4(tb, te, th, 0);
+        uint3 tot;
+        block_excl3<kThreads, true>(make_uint3(b, e, h), s3, tot);
+        if (threadIdx.x == 0) D.part_c[t] = make_uint4(tot.x, tot.y, tot.z, 0);
     }
 }
 
@@ -426,7 +463,7 @@ __global__ void __launch_bounds__(1024) k_kdb_scan_parts(KdDev D, uint32_t L, ui
 }
 
 __global__ void __launch_bounds__(kThreads) k_kdb_group_apply(KdDev D, uint32_t L) {
-    __shared__ uint32_t s0[kThreads / 64], s1[kThreads / 64], s2[kThreads / 64];
+    __shared__ uint3 s3[kThreads / 64];
     if (D.st->overflow) return;
     const uint32_t n = D.lvl[L].n_cands;
     const uint32_t tiles = (n + kTile - 1) / kTile;
@@ -443,11 +480,12 @@ __global__ void __launch_bounds__(kThreads) k_kdb_group_apply(KdDev D, uint32_t 
             e += ek[k];
             h = max(h, hk[k]);
         }
-        uint32_t tb, te, th;
         const uint4 P = D.part_c[t];
-        uint32_t B = P.x + block_excl<kThreads>(b, SumOp(), s0, tb);
-        uint32_t E = P.y + block_excl<kThreads>(e, SumOp(), s1, te);
-        uint32_t H = max(P.z, block_excl<kThreads>(h, MaxOp(), s2, th));
+        uint3 tot;
+        const uint3 ex = block_excl3<kThreads, true>(make_uint3(b, e, h), s3, tot);
+        uint32_t B = P.x + ex.x;
+        uint32_t E = P.y + ex.y;
+        uint32_t H = max(P.z, ex.z);
 #pragma unroll
         for (uint32_t k = 0; k < kItems; k++) {
             const uint32_t i = i0 + k;
@@ -517,8 +555,8 @@ __device__ __forceinline__ bool run_sah(const KdDev& D, uint32_t L, uint32_t i, 
 //
 // Depths up to kSahLdsDepth have at most 64 nodes: there the block also
 // folds the wave results into an LDS slot table and flushes it once, and
-// the host launches fewer, longer blocks, so a node sees one global atomic
-// per block.
+// the host launches at most 1,024 blocks, so a node sees at most 1,024
+// global atomics.
 constexpr uint32_t kSahLdsDepth = 6;
 
 __global__ void __launch_bounds__(kThreads) k_kdb_sah(KdDev D, uint32_t L) {
@@ -621,7 +659,7 @@ __device__ __forceinline__ void split_flags(uint8_t r, bool poly, uint32_t& a, u
 }
 
 __global__ void __launch_bounds__(kThreads) k_kdb_split_reduce(KdDev D, uint32_t L) {
-    __shared__ uint32_t s0[kThreads / 64], s1[kThreads / 64], s2[kThreads / 64];
+    __shared__ uint3 s3[kThreads / 64];
     if (D.st->overflow) return;
     const uint32_t np = D.lvl[L].n_polys, nc = D.lvl[L].n_cands;
     const uint32_t tp = (np + kTile - 1) / kTile, tc = (nc + kTile - 1) / kTile;
@@ -658,16 +696,14 @@ __global__ void __launch_bounds__(kThreads) k_kdb_split_reduce(KdDev D, uint32_t
             b += fb;
             l += fl;
         }
-        uint32_t ta, tb, tl;
-        block_excl<kThreads>(a, SumOp(), s0, ta);
-        block_excl<kThreads>(b, SumOp(), s1, tb);
-        block_excl<kThreads>(l, SumOp(), s2, tl);
-        if (threadIdx.x == 0) (is_poly ? D.part_p : D.part_c)[tt] = make_uint4(ta, tb, tl, 0);
+        uint3 tot;
+        block_excl3<kThreads, false>(make_uint3(a, b, l), s3, tot);
+        if (threadIdx.x == 0) (is_poly ? D.part_p : D.part_c)[tt] = make_uint4(tot.x, tot.y, tot.z, 0);
     }
 }
 
 __global__ void __launch_bounds__(kThreads) k_kdb_split_apply(KdDev D, uint32_t L) {
-    __shared__ uint32_t s0[kThreads / 64], s1[kThreads / 64], s2[kThreads / 64];
+    __shared__ uint3 s3[kThreads / 64];
     if (D.st->overflow) return;
     const uint32_t np = D.lvl[L].n_polys, nc = D.lvl[L].n_cands;
     const uint32_t tp = (np + kTile - 1) / kTile, tc = (nc + kTile - 1) / kTile;
@@ -688,10 +724,11 @@ __global__ void __launch_bounds__(kThreads) k_kdb_split_apply(KdDev D, uint32_t 
             l += lk[k];
         }
         const uint4 P = (is_poly ? D.part_p : D.part_c)[tt];
-        uint32_t ta, tb, tl;
-        uint32_t A = P.x + block_excl<kThreads>(a, SumOp(), s0, ta);
-        uint32_t B = P.y + block_excl<kThreads>(b, SumOp(), s1, tb);
-        uint32_t C = P.z + block_excl<kThreads>(l, SumOp(), s2, tl);
+        uint3 tot;
+        const uint3 ex = block_excl3<kThreads, false>(make_uint3(a, b, l), s3, tot);
+        uint32_t A = P.x + ex.x;
+        uint32_t B = P.y + ex.y;
+        uint32_t C = P.z + ex.z;
 #pragma unroll
         for (uint32_t k = 0; k < kItems; k++) {
             const uint32_t i = i0 + k;
@@ -941,7 +978,7 @@ struct KdBuilder {
     Status* host_status = nullptr;  // pinned
     KdDev D{};
     uint32_t ns_cap = 0;  // sampled triangles the sort buffers hold
-    uint32_t* keys[2] = {nullptr, nullptr};
+    unsigned long long* keys[2] = {nullptr, nullptr};
     uint32_t* vals[2] = {nullptr, nullptr};
 };
 
@@ -1012,7 +1049,7 @@ hipError_t layout(KdBuilder* b, const Caps& cap, uint32_t n_tris, uint32_t ns, b
     D.dev_nodes = static_cast<uint4*>(take(16 * size_t(kSlots)));
     D.lp = static_cast<uint32_t*>(take(4 * S));
     for (int k = 0; k < 2; k++) {
-        b->keys[k] = static_cast<uint32_t*>(take(4 * 6 * size_t(ns)));
+        b->keys[k] = static_cast<unsigned long long*>(take(8 * 6 * size_t(ns)));
         b->vals[k] = static_cast<uint32_t*>(take(4 * 6 * size_t(ns)));
     }
     D.cap = cap;
@@ -1075,9 +1112,10 @@ int kdb_build(KdBuilder* b, const float4* verts, uint32_t n_verts, const uint4* 
         if (n_tris) {
             hipLaunchKernelGGL(k_kdb_root, dim3(blocks_for(n_tris, kThreads)), dim3(kThreads), 0, s, D, n_tris, ns,
                                b->keys[0], b->vals[0]);
+            // one sort of the three axes: 34 key bits (axis, ordered value)
             size_t need = 0;
             KDB_TRY(rocprim::radix_sort_pairs(nullptr, need, b->keys[0], b->keys[1], b->vals[0], b->vals[1],
-                                              2u * ns, 0, 32, s));
+                                              6u * ns, 0, 34, s));
             if (need > b->sort_cap) {
                 if (b->sort_tmp) KDB_TRY(hipFree(b->sort_tmp));
                 b->sort_tmp = nullptr;
@@ -1085,12 +1123,9 @@ int kdb_build(KdBuilder* b, const float4* verts, uint32_t n_verts, const uint4* 
                 KDB_TRY(hipMalloc(&b->sort_tmp, need));
                 b->sort_cap = need;
             }
-            for (uint32_t j = 0; j < 3; j++) {
-                size_t sz = b->sort_cap;
-                const size_t o = size_t(j) * 2u * ns;
-                KDB_TRY(rocprim::radix_sort_pairs(b->sort_tmp, sz, b->keys[0] + o, b->keys[1] + o, b->vals[0] + o,
-                                                  b->vals[1] + o, 2u * ns, 0, 32, s));
-            }
+            size_t sz = b->sort_cap;
+            KDB_TRY(rocprim::radix_sort_pairs(b->sort_tmp, sz, b->keys[0], b->keys[1], b->vals[0], b->vals[1], 6u * ns,
+                                              0, 34, s));
             hipLaunchKernelGGL(k_kdb_root_cands, dim3(blocks_for(6ull * ns, kThreads)), dim3(kThreads), 0, s, D, ns,
                                b->keys[1], b->vals[1]);
         }
@@ -1101,7 +1136,7 @@ int kdb_build(KdBuilder* b, const float4* verts, uint32_t n_verts, const uint4* 
             hipLaunchKernelGGL(k_kdb_group_reduce, dim3(grid_c), dim3(kThreads), 0, s, D, L);
             hipLaunchKernelGGL(k_kdb_scan_parts, dim3(1), dim3(1024), 0, s, D, L, 0u);
             hipLaunchKernelGGL(k_kdb_group_apply, dim3(grid_c), dim3(kThreads), 0, s, D, L);
-            hipLaunchKernelGGL(k_kdb_sah, dim3(L <= kSahLdsDepth ? std::min<uint32_t>(grid_c, 256) : grid_c),
+            hipLaunchKernelGGL(k_kdb_sah, dim3(L <= kSahLdsDepth ? std::min<uint32_t>(grid_c, 1024) : grid_c),
                                dim3(kThreads), 0, s, D, L);
             hipLaunchKernelGGL(k_kdb_decide, dim3(slots), dim3(kThreads), 0, s, D, L);
             hipLaunchKernelGGL(k_kdb_split_reduce, dim3(grid_pc), dim3(kThreads), 0, s, D, L);
