@@ -8,8 +8,9 @@ the timed region; outputs stay in HBM.
 
 Options (traversal variant, block size, stack / LDS depths, persistent
 waves, fused or queued frame pipeline) come from the autotuner, timing
-whole frames of pose 10 at the workload resolution.  Methods on the same
-10 frames:
+whole frames of pose 10 at the workload resolution; for the batched methods
+its six fastest shapes are timed again on the 10-frame batched launch
+(workloads.tune_batch).  Methods on the same 10 frames:
   plain    - frame after frame (hulk_render_frame)
   batched  - the 10 frames in one graph-captured launch (hulk_batch)
   learned  - batched, with per-pixel bounds from the MLP trained on poses
@@ -140,10 +141,12 @@ def animated_kernel(args, W: int, H: int) -> int:
     return 0
 
 
-def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, best: dict, log):
+def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, best: dict, bbest: dict, log):
     """plain / batched / learned-bound + batched on `cams` (bounds from an MLP
     trained on `train_cams`), for the tuned options and for kd-restart.
-    Returns (methods, learned info, batched-launch ms of the tuned options)."""
+    plain runs with `best` (tuned on single frames), the batched methods with
+    `bbest` (tuned on the batched launch).  Returns (methods, learned info,
+    batched-launch ms of the tuned options)."""
     import numpy as np
     import torch
 
@@ -197,9 +200,21 @@ def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, 
 
     methods["plain"] = entry(timed(plain_step, args.steps))
     # batched: one graph for the frames
+    bopts = {k: int(v) for k, v in bbest.items() if k != "device"}
+    ctx.set_options(**bopts)
+    cross = None
+    if bopts["variant"] != int(best["variant"]):
+        # another traversal variant: its own plain trace is the reference
+        # (variants may break exact ties differently); the agreement of the
+        # two variants' triangle ids is reported beside it
+        plain_hits = ref_hits
+        ref_hits = [ctx.trace_camera(cam, W, H) for cam in cams]
+        cross = float(np.mean([np.mean(a["tri"] == b["tri"]) for a, b in zip(plain_hits, ref_hits)]))
     batch = ctx.batch(frames, None, brgba.data_ptr(), use_graph=True)
     methods["batched"] = entry(timed(lambda: batch.launch(L.FLAG_NO_SYNC), args.steps),
                                hits_match(batch.primary_hits()))
+    if cross is not None:
+        methods["batched"]["tri_agreement_with_plain_variant"] = round(cross, 7)
     # the batched launch is the dominant kernel (the whole step when the
     # pipeline is fused): live HIP events on its stream
     k_ms = time_batch(batch, max(args.steps, 10))
@@ -244,7 +259,7 @@ def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, 
                         "inference_ms_per_frame": round(infer_ms, 4),
                         "inference_tflops": round(n / 4 * 82176 / (infer_ms * 1e-3) / 1e12, 2),
                         "loss": [[int(a), round(float(b), 4)] for a, b in losses], "margin_selection": {}}
-        opts_now = {k: int(v) for k, v in best.items() if k != "device"}
+        opts_now = bopts
         for tag, vopts in (("", opts_now), ("_stackless", dict(opts_now, variant=HT.RESTART))):
             ctx.set_options(**vopts)
             # margin (and whether to bound at all) chosen on training frames for this traversal
@@ -314,8 +329,8 @@ def main() -> int:
                          "1M-triangle scene (1024x1024, 2 mirror generations, one render per launch)")
     args = ap.parse_args()
     if args.gpus != 1:
-        log("bench.py measures one GPU only: this build has no multi-GPU path (rays are split per GPU only "
-            "in the design, see DESIGN.md section 8). Run with --gpus 1.")
+        log("bench.py measures one GPU only: the library has no multi-GPU path (rays would be split "
+            "per GPU by image tiles or frames, with the scene replicated). Run with --gpus 1.")
         return 2
 
     import numpy as np
@@ -411,6 +426,11 @@ def main() -> int:
     tt = time.time()
     best, table = ctx.autotune(cams[0], W, Hh, reps=2, frame=True)
     log(f"autotune {time.time() - tt:.1f}s over {len(table)} shapes -> {best}")
+    # batch stage: the fastest shapes again, on the 10-frame graph launch itself
+    tt = time.time()
+    bbest, btimes = WL.tune_batch(ctx, frames, None, brgba.data_ptr(), table, stream)
+    log(f"batch tune {time.time() - tt:.1f}s over {len(btimes)} shapes {btimes} -> {bbest}")
+    ctx.set_options(**{k: int(v) for k, v in best.items() if k != "device"})
 
     # -- algorithmic bytes of one batched launch (10 frames): full-stack
     # counts of every traced ray (primary and shadow), 32 B ray + 16 B
@@ -429,7 +449,8 @@ def main() -> int:
     ctx.render_frame(cams[0], W, Hh, count=True)
     shadow_per_frame = ctx.counters()["shadow_rays"]
     train_cams = [scene.camera(k, Hh / W) for k in range(10)]
-    methods, learned_info, k_ms = compare_methods(ctx, scene, cams, train_cams, W, Hh, stream, args, best, log)
+    methods, learned_info, k_ms = compare_methods(ctx, scene, cams, train_cams, W, Hh, stream, args, best, bbest,
+                                                  log)
     for k, v in methods.items():
         log(f"method {k:16s} {v}")
 
@@ -440,9 +461,11 @@ def main() -> int:
     achieved = bytes_launch / (k_ms * 1e-3) / 1e9
     log(f"batched launch {k_ms:.3f} ms, {bytes_launch / 1e9:.2f} GB algorithmic ({traced} rays, "
         f"nodes {n_node:.2f} / tris {n_tri:.2f} per ray), {achieved:.0f} GB/s")
-    tuned_name = L.VARIANT_NAMES[best["variant"]]
-    kname = kernel_name(best)
-    traffic, traffic_src = profiled_traffic(kname, best, W, Hh)
+    bat = best_m != "plain"
+    tuned = bbest if bat else best
+    tuned_name = L.VARIANT_NAMES[tuned["variant"]]
+    kname = kernel_name(bbest)
+    traffic, traffic_src = profiled_traffic(kname, bbest, W, Hh)
 
     result = {
         "metric": "Mrays/s nearest-hit @1024^2, learned-bound+batched vs plain, 100% hit match",
@@ -459,7 +482,9 @@ def main() -> int:
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": f"terrain_spheres_163650tris_poses10-19_{W}x{Hh}_primary+shadow+shade",
                    "frames_per_step": nf, "method": best_m, "variant": tuned_name,
-                   "options": {k: int(v) for k, v in best.items()}, "kd_build_ms": round(build_ms, 1)},
+                   "options": {k: int(v) for k, v in tuned.items()},
+                   "options_single_frame": {k: int(v) for k, v in best.items()},
+                   "batch_tune_ms": btimes, "kd_build_ms": round(build_ms, 1)},
         "methods": methods,
         "speedup_vs_plain": round(methods[best_m]["mrays_s"] / methods["plain"]["mrays_s"], 3),
         "shadow_rays_per_frame": int(shadow_per_frame),
@@ -505,14 +530,17 @@ def main() -> int:
             nd, lpp, bms = WL.build(sc)
             cx = WL.make_ctx(sc, nd, lpp, stream, variant=H.FULLSTACK)
             cams1 = [sc.camera(k, Hh / W) for k in range(10, 20)]
-            b1, _ = cx.autotune(cams1[0], W, Hh, reps=2, frame=True)
+            b1, t1 = cx.autotune(cams1[0], W, Hh, reps=2, frame=True)
+            fr1 = [(cam, W, Hh, 0) for cam in cams1]
+            bb1, _ = WL.tune_batch(cx, fr1, None, brgba.data_ptr(), t1, stream)
+            cx.set_options(**{k: int(v) for k, v in b1.items() if k != "device"})
             m1, li1, _ = compare_methods(cx, sc, cams1, [sc.camera(k, Hh / W) for k in range(10)], W, Hh, stream,
-                                         args, b1, log)
+                                         args, b1, bb1, log)
             cx.close()
             for k, v in m1.items():
                 log(f"1M method {k:16s} {v}")
             return {"workload": f"animated_tori_{sc.n_tris}tris_frame0_poses10-19_{W}x{Hh}_primary+shadow+shade",
-                    "kd_build_ms": round(bms, 1), "tuned": b1, "methods": m1, "learned": li1}
+                    "kd_build_ms": round(bms, 1), "tuned": b1, "tuned_batch": bb1, "methods": m1, "learned": li1}
 
         def streamed():
             # out-of-core: terrain pose 10 at 1024^2, kd-restart, record pool

@@ -43,6 +43,54 @@ def make_ctx(scene, nodes, lp, stream, **opts) -> GPUTrace:
     return ctx
 
 
+_TUNE_KEYS = ("variant", "block_size", "short_stack_depth", "lds_top_levels", "persistent", "pipeline",
+              "leaf_batch")
+
+
+def tune_batch(ctx: GPUTrace, frames, mirror, rgba_dev: int, table, stream, top: int = 6, reps: int = 5,
+               rounds: int = 2):
+    """Batch stage of the autotuner: the `top` fastest distinct shapes of a
+    per-frame autotune table are re-timed on the graph-captured batch of
+    `frames` itself (a launch over every frame's rays behaves unlike one
+    frame: more waves in flight, other tails), `rounds` times each in turn,
+    keeping each shape's best time.  The fastest shape becomes the
+    context's options; returns (options dict, {shape: ms})."""
+    torch = _torch()
+    seen, cands = set(), []
+    for row in sorted(table, key=lambda r: r["ms"]):
+        k = tuple(int(row[f]) for f in _TUNE_KEYS)
+        if k not in seen:
+            seen.add(k)
+            cands.append(k)
+        if len(cands) == top:
+            break
+    if not cands:
+        raise ValueError("empty autotune table")
+    batches = []
+    for k in cands:
+        ctx.set_options(**dict(zip(_TUNE_KEYS, k)))
+        b = ctx.batch(frames, mirror, rgba_dev, use_graph=True)
+        b.launch(L.FLAG_NO_SYNC)
+        batches.append(b)
+    torch.cuda.synchronize()
+    times = {k: float("inf") for k in cands}
+    for _ in range(rounds):
+        for k, b in zip(cands, batches):
+            e0, e1 = _events(stream)
+            e0.record(stream)
+            for _ in range(reps):
+                b.launch(L.FLAG_NO_SYNC)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            times[k] = min(times[k], e0.elapsed_time(e1) / reps)
+    for b in batches:
+        b.close()
+    best = min(cands, key=lambda k: times[k])
+    opts = dict(zip(_TUNE_KEYS, best))
+    ctx.set_options(**opts)
+    return dict(opts, device=int(ctx.opts.device)), {",".join(map(str, k)): round(v, 4) for k, v in times.items()}
+
+
 def build(scene) -> tuple[np.ndarray, np.ndarray, float]:
     kd = KDTreeCPU()
     t = time.perf_counter()
@@ -228,8 +276,12 @@ def batched(stream=None, specs=None, reps: int = 3, autotune: bool = True, log=p
         e1.record(stream)
         torch.cuda.synchronize()
         plain_ms = e0.elapsed_time(e1) / reps
-        # batched: one graph-captured pipeline for all frames of this scene
+        # batched: one graph-captured pipeline for all frames of this scene,
+        # with the shape re-tuned on that launch
         brgba = torch.empty(rays, dtype=torch.int32, device="cuda")
+        btuned = None
+        if autotune:
+            btuned, _ = tune_batch(ctx, frames, mirror, brgba.data_ptr(), table, stream)
         b = ctx.batch(frames, mirror, brgba.data_ptr(), use_graph=True)
         b.launch(L.FLAG_NO_SYNC)
         torch.cuda.synchronize()
@@ -243,7 +295,7 @@ def batched(stream=None, specs=None, reps: int = 3, autotune: bool = True, log=p
         b.close()
         ctx.close()
         out["scenes"][kind] = {"frames": len(mine), "primary_rays": rays, "plain_ms": round(plain_ms, 3),
-                               "batched_ms": round(batch_ms, 3), "tuned": tuned}
+                               "batched_ms": round(batch_ms, 3), "tuned": tuned, "tuned_batch": btuned}
         total_plain += plain_ms
         total_batch += batch_ms
         total_rays += rays

@@ -26,7 +26,7 @@ How each oracle function is pinned (parity status):
       are rsqrt-normalized, so t is in other units).
 The reference holds no golden hit / image files for this path, so the hit
 records at scale are pinned only by these properties: parity unpinned
-beyond them, as recorded in DESIGN.md.
+beyond them.
 """
 from __future__ import annotations
 
