@@ -40,15 +40,20 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-OPT_KEYS = ("variant", "block_size", "short_stack_depth", "lds_top_levels", "persistent", "pipeline", "leaf_batch")
+OPT_KEYS = ("variant", "block_size", "short_stack_depth", "lds_top_levels", "persistent", "pipeline", "leaf_batch",
+            "coop_lanes")
 
 
 def kernel_name(opts: dict) -> str:
-    """The batched launch's dominant kernel: the fused frame kernel, or the
-    queued pipeline's graph of trace, compaction and shading kernels."""
+    """The batched launch's dominant kernel: the fused frame kernel, the
+    lane-refill frame kernel, or the queued pipeline's graph of trace,
+    compaction and shading kernels."""
+    lb = 2 if int(opts.get("leaf_batch", 1)) == 2 else 1
     if int(opts.get("pipeline", 0)) == 1:
         return "queued pipeline graph"
-    return f"k_trace<{int(opts['variant'])},false,true,{2 if int(opts.get('leaf_batch', 1)) == 2 else 1}>"
+    if int(opts.get("pipeline", 0)) == 3:
+        return f"k_render_refill<false,{lb}>"
+    return f"k_trace<{int(opts['variant'])},false,true,{lb}>"
 
 
 def profiled_traffic(kernel: str, opts: dict, W: int, H: int):

@@ -61,7 +61,7 @@ class Opts(C.Structure):
     _fields_ = [("device", C.c_int32), ("variant", C.c_int32), ("block_size", C.c_uint32),
                 ("short_stack_depth", C.c_uint32), ("lds_top_levels", C.c_uint32),
                 ("persistent", C.c_uint32), ("pipeline", C.c_uint32), ("leaf_batch", C.c_uint32),
-                ("stream", C.c_void_p)]
+                ("coop_lanes", C.c_uint32), ("reserved", C.c_uint32), ("stream", C.c_void_p)]
 
 
 class Counters(C.Structure):
@@ -78,7 +78,7 @@ class FrameDesc(C.Structure):
 class TuneEntry(C.Structure):
     _fields_ = [("variant", C.c_int32), ("block_size", C.c_uint32), ("short_stack_depth", C.c_uint32),
                 ("lds_top_levels", C.c_uint32), ("persistent", C.c_uint32), ("pipeline", C.c_uint32),
-                ("leaf_batch", C.c_uint32), ("ms", C.c_float)]
+                ("leaf_batch", C.c_uint32), ("coop_lanes", C.c_uint32), ("ms", C.c_float)]
 
 
 class TuneRequest(C.Structure):

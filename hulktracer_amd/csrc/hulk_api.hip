@@ -27,7 +27,7 @@ using namespace hulk;
 static_assert(sizeof(hulk_float4) == 16 && sizeof(hulk_tri) == 16 && sizeof(hulk_kd_node) == 16, "layout");
 static_assert(sizeof(hulk_box) == 32 && sizeof(hulk_camera) == 64 && sizeof(hulk_light) == 32, "layout");
 static_assert(sizeof(hulk_ray) == 32 && sizeof(hulk_hit) == 32, "layout");
-static_assert(sizeof(hulk_opts) == 40 && sizeof(hulk_tune_entry) == 32 && sizeof(hulk_tune_request) == 96, "layout");
+static_assert(sizeof(hulk_opts) == 48 && sizeof(hulk_tune_entry) == 36 && sizeof(hulk_tune_request) == 96, "layout");
 
 namespace {
 
@@ -156,6 +156,8 @@ int check_opts(hulk_ctx* c, const hulk_opts& o) {
     if (o.pipeline == HULK_PIPELINE_REFILL && (o.variant != HULK_VARIANT_ROPES || o.block_size > 256))
         return fail(c, HULK_E_ILLEGAL_INPUT, "the refill pipeline needs the ropes variant and blocks of <= 256");
     if (o.leaf_batch > 2) return fail(c, HULK_E_ILLEGAL_INPUT, "leaf_batch must be 0, 1 or 2");
+    if (o.coop_lanes > 64) return fail(c, HULK_E_ILLEGAL_INPUT, "coop_lanes must be in [0, 64]");
+    if (o.reserved) return fail(c, HULK_E_ILLEGAL_INPUT, "hulk_opts.reserved must be 0");
     return HULK_OK;
 }
 
@@ -200,12 +202,15 @@ int trace_setup(hulk_ctx* c, const hulk_opts& o, TraceParams& P, uint32_t& block
     }
     std::memset(&P, 0, sizeof(P));
     P.leaf_batch = o.leaf_batch == 2 ? 2u : 1u;
+    P.coop_lanes = o.coop_lanes;
     P.nodes = static_cast<const uint4*>(c->nodes.p);
     P.recs = static_cast<const TriRec*>(c->recs.p);
     P.verts = static_cast<const float4*>(c->verts.p);
     P.tris = static_cast<const uint4*>(c->tris.p);
     P.n_tris = c->n_tris;
     P.rope_nodes = static_cast<const uint4*>(c->rope_nodes.p);
+    P.node_bytes = c->n_nodes * 16u;
+    P.rope_node_bytes = c->n_slots * 16u;
     P.rope_slots = static_cast<const RopeSlot*>(c->rope_slots.p);
     P.box = c->box;
     P.guard = static_cast<uint32_t>(std::min<uint64_t>(4ull * (c->n_nodes + c->n_slots) + 4096u, 0x7FFFFFFFull));
@@ -717,7 +722,8 @@ extern "C" int hulk_upload_kdtree(hulk_ctx* c, const hulk_kd_node* nodes, uint32
     if (!c) return HULK_E_ILLEGAL_INPUT;
     if (!c->have_scene) return fail(c, HULK_E_NO_SCENE, "upload the scene before the tree");
     if (!nodes || n_nodes == 0 || (n_lp && !leaf_polys)) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
-    if (n_nodes >= (1u << 30)) return fail(c, HULK_E_ILLEGAL_INPUT, "more than 2^30 nodes");
+    // nodes and rope slots (3 per node) are addressed by 32-bit byte offsets
+    if (n_nodes >= (1u << 26)) return fail(c, HULK_E_ILLEGAL_INPUT, "more than 2^26 nodes");
     // Validate: children in range, every node reached once from the root
     // (no cycles / sharing), leaf ranges inside leaf_polys, references < n_tris.
     std::vector<uint8_t> seen(n_nodes, 0);
@@ -1519,7 +1525,7 @@ namespace {
 
 struct Cand {
     int32_t variant;
-    uint32_t block, depth, lds, persistent, pipeline, leaf_batch;
+    uint32_t block, depth, lds, persistent, pipeline, leaf_batch, coop;
 };
 
 }  // namespace
@@ -1570,6 +1576,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
         o.persistent = k.persistent;
         o.pipeline = k.pipeline;
         o.leaf_batch = k.leaf_batch;
+        o.coop_lanes = k.coop;
         TraceParams P;
         uint32_t block;
         size_t lds;
@@ -1597,7 +1604,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
     std::vector<std::pair<float, Cand>> firsts;
     for (int v = HULK_VARIANT_FULLSTACK; v < HULK_VARIANT_COUNT; v++) {
         if (!(variant_mask & (1u << v))) continue;
-        Cand k{v, 256, 6, 0, 0, pipe0, 1};
+        Cand k{v, 256, 6, 0, 0, pipe0, 1, 0};
         float ms;
         r = measure(k, ms, reps, true);
         if (r > 1) return r;
@@ -1620,7 +1627,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
                     if (!ldsv && ld) continue;
                     for (uint32_t ps = 0; ps < 2; ps++)
                         for (uint32_t lb = 1; lb <= 2; lb++) {
-                            Cand k{v, bs, dp, ld, ps, pipe0, lb};
+                            Cand k{v, bs, dp, ld, ps, pipe0, lb, 0};
                             if (bs == 256 && dp == 6 && ld == 0 && ps == 0 && lb == 1) continue;  // stage 1
                             // persistent + LDS top tree pays off; skip the
                             // non-persistent LDS shapes beyond depth 8
@@ -1643,13 +1650,15 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
         float ms;
         r = measure(k, ms, reps, true);
         if (r > 1) return r;
+        const uint32_t coops[] = {0, 4, 16};
         if (variant_mask & (1u << HULK_VARIANT_ROPES))
             for (uint32_t bs : blocks)
-                for (uint32_t lb = 1; lb <= 2; lb++) {
-                    Cand kr{HULK_VARIANT_ROPES, bs, 6, 0, 1, HULK_PIPELINE_REFILL, lb};
-                    r = measure(kr, ms, reps, true);
-                    if (r > 1) return r;
-                }
+                for (uint32_t lb = 1; lb <= 2; lb++)
+                    for (uint32_t cl : coops) {
+                        Cand kr{HULK_VARIANT_ROPES, bs, 6, 0, 1, HULK_PIPELINE_REFILL, lb, cl};
+                        r = measure(kr, ms, reps, true);
+                        if (r > 1) return r;
+                    }
     }
     // stage 4: the four fastest again, with four times the repetitions, so
     // that timing noise between near-equal shapes does not decide
@@ -1675,6 +1684,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
     c->opts.lds_top_levels = b.lds;
     c->opts.persistent = b.persistent;
     c->opts.leaf_batch = b.leaf_batch;
+    c->opts.coop_lanes = b.coop;
     if (frame) c->opts.pipeline = b.pipeline;
     if (best_out) *best_out = c->opts;
     if (table) {
@@ -1686,6 +1696,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
             table[i].persistent = done[i].first.persistent;
             table[i].pipeline = done[i].first.pipeline;
             table[i].leaf_batch = done[i].first.leaf_batch;
+            table[i].coop_lanes = done[i].first.coop;
             table[i].ms = done[i].second;
         }
     }

@@ -26,9 +26,22 @@ namespace hulk {
 
 // ------------------------------------------------------------ utilities --
 
-__device__ __forceinline__ uint4 fetch_node(const uint4* __restrict__ g, const uint4* lds, uint32_t m, uint32_t i) {
+// A 16-B node from HBM / L2 by one raw buffer load (word 3 of the
+// resource: 32-bit data format, the gfx9 value).  One instruction for the
+// whole node: a plain load of the select between the LDS and the global
+// copy compiles to flat loads, with the plane fetched by a second, dependent
+// load after the junction test.  The buffer's range check (num_bytes)
+// returns zeros past the end: flags 0, neither junction nor leaf.
+__device__ __forceinline__ uint4 load_node(const uint4* g, uint32_t num_bytes, uint32_t i) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4*>(g), (short)0, (int)num_bytes, 0x00020000);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, i * 16u, 0, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
+__device__ __forceinline__ uint4 fetch_node(const TraceParams& P, const uint4* lds, uint32_t m, uint32_t i) {
     if (i < m) return lds[i];
-    return g[i];
+    return load_node(P.nodes, P.node_bytes, i);
 }
 
 __device__ __forceinline__ void junction(const uint4& n, const Ray& r, float& split, uint32_t& first,
@@ -151,7 +164,7 @@ __device__ void trav_fullstack(const TraceParams& P, const uint4* lds_nodes, uin
     uint32_t sp = 0, node = 0, guard = 0;
     do {
         if (best.t < ts) break;
-        const uint4 n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
+        const uint4 n = fetch_node(P, lds_nodes, P.lds_node_count, node);
         if (COUNT) c.nodes++;
         if (n.x & kNodeLeaf) {
             test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
@@ -220,7 +233,7 @@ __device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, ui
             tmax = __uint_as_float(stk[(3 * idx + 1) * stride]);
             node = stk[(3 * idx + 2) * stride];
         }
-        uint4 n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
+        uint4 n = fetch_node(P, lds_nodes, P.lds_node_count, node);
         if (COUNT) c.nodes++;
         while (n.x & kNodeJunction) {
             float split;
@@ -239,7 +252,7 @@ __device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, ui
                 node = first;
                 tmax = split;
             }
-            n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
+            n = fetch_node(P, lds_nodes, P.lds_node_count, node);
             if (COUNT) c.nodes++;
             if (node == 0) break;
         }
@@ -280,7 +293,7 @@ __device__ void trav_shortstack_private(const TraceParams& P, const uint4* lds_n
             tmax = f_te[idx];
             node = f_node[idx];
         }
-        uint4 n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
+        uint4 n = fetch_node(P, lds_nodes, P.lds_node_count, node);
         if (COUNT) c.nodes++;
         while (n.x & kNodeJunction) {
             float split;
@@ -299,7 +312,7 @@ __device__ void trav_shortstack_private(const TraceParams& P, const uint4* lds_n
                 node = first;
                 tmax = split;
             }
-            n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
+            n = fetch_node(P, lds_nodes, P.lds_node_count, node);
             if (COUNT) c.nodes++;
             if (node == 0) break;
         }
@@ -331,7 +344,7 @@ __device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const
         tmin = tmax;
         tmax = te;
         uint32_t node = 0;
-        uint4 n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, 0);
+        uint4 n = fetch_node(P, lds_nodes, P.lds_node_count, 0);
         if (COUNT) c.nodes++;
         while (n.x & kNodeJunction) {
             float split;
@@ -345,7 +358,7 @@ __device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const
                 node = first;
                 tmax = split;
             }
-            n = fetch_node(P.nodes, lds_nodes, P.lds_node_count, node);
+            n = fetch_node(P, lds_nodes, P.lds_node_count, node);
             if (COUNT) c.nodes++;
             if (node == 0) break;
         }
@@ -385,22 +398,27 @@ __device__ __forceinline__ bool ropes_begin(const TraceParams& P, const Ray& r, 
     return true;
 }
 
-// One leaf of a rope traversal: descend from the current slot to a leaf,
-// test it, and follow the rope out of it.  False once the traversal is over.
-template <bool COUNT, int LB>
-__device__ __forceinline__ bool ropes_step(const TraceParams& P, const Ray& r, RopeState& s, float tstop, Best& best,
-                                           Cnt& c, uint32_t& flags) {
-    uint4 n = P.rope_nodes[s.slot];
+// Descent of a rope traversal from the current slot to a leaf; returns the
+// leaf's rope node (s.slot is its slot).
+template <bool COUNT>
+__device__ __forceinline__ uint4 ropes_descend(const TraceParams& P, const Ray& r, RopeState& s, Cnt& c) {
+    uint4 n = load_node(P.rope_nodes, P.rope_node_bytes, s.slot);
     if (COUNT) c.nodes++;
     while (n.x & kNodeJunction) {
         float split;
         uint32_t first, second;
         junction(n, r, split, first, second);
         s.slot = (split <= 0.0f || split > s.t) ? first : second;
-        n = P.rope_nodes[s.slot];
+        n = load_node(P.rope_nodes, P.rope_node_bytes, s.slot);
         if (COUNT) c.nodes++;
     }
-    test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
+    return n;
+}
+
+// After the leaf of s.slot was tested: follow the rope out of it.  False
+// once the traversal is over.
+__device__ __forceinline__ bool ropes_exit(const TraceParams& P, const Ray& r, RopeState& s, float tstop,
+                                           const Best& best, uint32_t& flags) {
     if (best.t < tstop) return false;
     const RopeSlot& rs = P.rope_slots[s.slot];
     const float4 lo = rs.lo, hi = rs.hi;
@@ -439,6 +457,84 @@ __device__ __forceinline__ bool ropes_step(const TraceParams& P, const Ray& r, R
         return false;
     }
     return true;
+}
+
+// One leaf of a rope traversal: descend from the current slot to a leaf,
+// test it, and follow the rope out of it.  False once the traversal is over.
+template <bool COUNT, int LB>
+__device__ __forceinline__ bool ropes_step(const TraceParams& P, const Ray& r, RopeState& s, float tstop, Best& best,
+                                           Cnt& c, uint32_t& flags) {
+    const uint4 n = ropes_descend<COUNT>(P, r, s, c);
+    test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
+    return ropes_exit(P, r, s, tstop, best, flags);
+}
+
+// The leaves of the lanes in `want` (a wave-uniform ballot; every lane of
+// the wave executes this call) tested by the whole wave, one owner lane at a
+// time, 64 records per pass: lane k tests record k of the pass against the
+// owner's ray and the owner's best t before the pass.  The pass keeps the
+// smallest t, and of equal t the lowest leaf position, which is the hit the
+// owner's own loop keeps (strict '<' in leaf order: the first of equal t
+// wins, GPU_common.cl:62-68), with the same per-test arithmetic
+// (test_record).  A shadow ray (best.t < tstop ends it) stops after the pass
+// in which that happens, where its own loop stops inside the pass: its hit
+// may then be another triangle closer than the light, which leaves the
+// shading decision (light distance > t) unchanged.  Used where few lanes of
+// a wave are still at work, so that a long leaf costs one memory latency per
+// 64 records instead of one per record.
+__device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, const uint4& n, const Ray& r,
+                                            Best& best, float tstop, uint64_t want, uint32_t lane_id) {
+    while (want) {
+        const uint32_t L = (uint32_t)__builtin_ctzll(want);
+        want &= want - 1;
+        Ray q;
+        q.o = v3(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.o.x), L)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.o.y), L)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.o.z), L)));
+        q.d = v3(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.d.x), L)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.d.y), L)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.d.z), L)));
+        const float ts = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tstop), L));
+        const uint32_t start = (uint32_t)__builtin_amdgcn_readlane((int)n.y, L);
+        const uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane((int)n.z, L);
+        float bt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(best.t), L));
+        for (uint32_t base = 0; base < cnt; base += 64u) {
+            Best b;
+            b.t = bt;
+            bool hit = false;
+            if (base + lane_id < cnt) {
+                const TriRec rec = recs[start + base + lane_id];
+                hit = test_record(q, rec, b);
+            }
+            // t >= 0 here (or -0): the bits without the sign order like the value
+            uint32_t key = hit ? (__float_as_uint(b.t) & 0x7FFFFFFFu) : 0xFFFFFFFFu;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) key = min(key, (uint32_t)__shfl_xor((int)key, off, 64));
+            key = (uint32_t)__builtin_amdgcn_readfirstlane((int)key);
+            if (key == 0xFFFFFFFFu) continue;
+            const uint64_t at = __ballot(hit && (__float_as_uint(b.t) & 0x7FFFFFFFu) == key);
+            const uint32_t w = (uint32_t)__builtin_ctzll(at);
+            bt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.t), w));
+            const float s1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.s1), w));
+            const float s2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.s2), w));
+            const float det = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.det), w));
+            const uint32_t tri = (uint32_t)__builtin_amdgcn_readlane((int)b.tri, w);
+            const float nx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.nx), w));
+            const float ny = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.ny), w));
+            const float nz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.nz), w));
+            if (lane_id == L) {
+                best.t = bt;
+                best.s1 = s1;
+                best.s2 = s2;
+                best.det = det;
+                best.tri = tri;
+                best.nx = nx;
+                best.ny = ny;
+                best.nz = nz;
+            }
+            if (bt < ts) break;
+        }
+    }
 }
 
 template <bool COUNT, int LB>
@@ -853,10 +949,26 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             if (!more) break;
             continue;
         }
-        if (phase == kLaneIdle) continue;
-
-        // ---- one leaf of this lane's traversal
-        bool done = !active || !ropes_step<COUNT, LB>(P, tr, st, tstop, bt, c, tfl);
+        // ---- one leaf of this lane's traversal.  Idle lanes stay in step
+        // (no early continue), so that the wave can test the leaves of its
+        // few busy lanes together when at most P.coop_lanes lanes are at one
+        // (the counted instance keeps every test per lane, so its counters
+        // equal the other kernels').
+        const bool busy = phase != kLaneIdle;
+        bool done = false;
+        if (COUNT || P.coop_lanes == 0) {
+            if (busy) done = !active || !ropes_step<COUNT, LB>(P, tr, st, tstop, bt, c, tfl);
+        } else {
+            const bool walk = busy && active;
+            uint4 ln = make_uint4(0u, 0u, 0u, 0u);
+            if (walk) ln = ropes_descend<COUNT>(P, tr, st, c);
+            const uint64_t at_leaf = __ballot(walk && ln.z != 0u);
+            if (at_leaf && (uint32_t)__builtin_popcountll(at_leaf) <= P.coop_lanes)
+                coop_leaves(P.recs, ln, tr, bt, tstop, at_leaf, lane_id);
+            else if (walk)
+                test_leaf<COUNT, LB>(P.recs, ln, tr, bt, c, tstop);
+            if (busy) done = !walk || !ropes_exit(P, tr, st, tstop, bt, tfl);
+        }
         // ---- state transitions of lanes whose traversal is over (a new
         // traversal may end at once when it misses the box: loop)
         while (done) {

@@ -35,6 +35,8 @@ struct FrameDev {
 
 struct TraceParams {
     const uint4* nodes;
+    uint32_t node_bytes;       // 16 * node count (range of the node buffer loads)
+    uint32_t rope_node_bytes;  // 16 * rope slot count
     uint32_t lds_node_count;
     uint32_t stack_depth;
     const TriRec* recs;
@@ -73,6 +75,7 @@ struct TraceParams {
     uint32_t max_bounce;    // camera source; frames carry their own
     uint32_t* rgba;
     uint32_t leaf_batch;    // host side: picks the kernel instance (1 or 2 records in flight)
+    uint32_t coop_lanes;    // refill kernel: wave-cooperative leaf tests at <= this many busy lanes
     uint32_t stop_at_w;  // shadow queue: stop once best.t < ray.start.w (the light distance)
 };
 

@@ -44,7 +44,7 @@ def make_ctx(scene, nodes, lp, stream, **opts) -> GPUTrace:
 
 
 _TUNE_KEYS = ("variant", "block_size", "short_stack_depth", "lds_top_levels", "persistent", "pipeline",
-              "leaf_batch")
+              "leaf_batch", "coop_lanes")
 
 
 def tune_batch(ctx: GPUTrace, frames, mirror, rgba_dev: int, table, stream, top: int = 6, reps: int = 5,

@@ -120,6 +120,11 @@ typedef struct hulk_opts {
     uint32_t leaf_batch;        /* triangle records in flight per lane in a
                                    leaf: 0 or 1 = one, 2 = two (for scenes
                                    whose records do not stay in L1)          */
+    uint32_t coop_lanes;        /* lane-refill pipeline: when at most this
+                                   many lanes of a wave are at a leaf, the
+                                   whole wave tests their leaves, 64 records
+                                   per pass; 0 = never, 1..64               */
+    uint32_t reserved;          /* must be 0                                 */
     void*    stream;            /* hipStream_t to use; NULL = own stream     */
 } hulk_opts;
 
@@ -281,7 +286,7 @@ void hulk_batch_destroy(hulk_batch* batch);
 
 typedef struct hulk_tune_entry {
     int32_t variant;
-    uint32_t block_size, short_stack_depth, lds_top_levels, persistent, pipeline, leaf_batch;
+    uint32_t block_size, short_stack_depth, lds_top_levels, persistent, pipeline, leaf_batch, coop_lanes;
     float ms;                 /* time per launch of the tuned objective */
 } hulk_tune_entry;
 

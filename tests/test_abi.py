@@ -52,9 +52,9 @@ def test_struct_sizes_match_reference_layout():
     # CameraPos 64 B, Light 32 B, Ray 32 B
     assert H.HIT_DTYPE.itemsize == 32
     assert H.RAY_DTYPE.itemsize == 32
-    assert C.sizeof(_lib.Opts) == 40  # 8 x 4 B + stream pointer
+    assert C.sizeof(_lib.Opts) == 48  # 10 x 4 B + stream pointer
     assert C.sizeof(_lib.TuneRequest) == 96  # camera 64 B + 6 x 4 B + mirror pointer
-    assert C.sizeof(_lib.TuneEntry) == 32
+    assert C.sizeof(_lib.TuneEntry) == 36
 
 
 def test_procedural_scenes_and_counts():

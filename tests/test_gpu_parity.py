@@ -112,7 +112,9 @@ def test_shade_parity(variant, mode):
                                   dict(variant=H.ROPES, pipeline=2, persistent=True, block_size=128),
                                   dict(variant=H.LDSSTACK, pipeline=2, leaf_batch=2, lds_top_levels=8),
                                   dict(variant=H.ROPES, pipeline=3, block_size=128),
-                                  dict(variant=H.ROPES, pipeline=3, block_size=64, leaf_batch=2)])
+                                  dict(variant=H.ROPES, pipeline=3, block_size=64, leaf_batch=2),
+                                  dict(variant=H.ROPES, pipeline=3, block_size=128, coop_lanes=64),
+                                  dict(variant=H.ROPES, pipeline=3, block_size=256, leaf_batch=2, coop_lanes=8)])
 def test_render_bounces_parity(opts):
     """Animated tori (5 meshes, 100,002 triangles): primary + shadow + 2
     mirror generations with per-generation compaction; persistent waves
@@ -154,6 +156,28 @@ def test_fused_pipeline_equals_queued(variant, pipeline):
     for k in ("rays", "node_visits", "leaf_visits", "tri_tests", "hits", "fallbacks", "shadow_rays"):
         assert c1[k] == c2[k], (k, c1[k], c2[k])
     assert c1["shadow_rays"] > 0 and c1["rays"] > W * H_
+
+
+@pytest.mark.parametrize("coop", [1, 8, 64])
+def test_refill_cooperative_leaves(coop):
+    """Lane refill with the wave testing the leaves of its few busy lanes
+    together (coop_lanes = 64: on every step): pixels and primary hit
+    records equal the queued pipeline's, with mirror generations and a
+    bound on the primary rays, at two block sizes and both record batches."""
+    s, nodes, lp = scene_and_tree(3, 6, 5, 4)
+    cam = s.camera(4, 1.0)
+    W = H_ = 128
+    plain = gpu_ctx(s, nodes, lp, H.ROPES).trace_camera(cam, W, H_)
+    tb = np.where(plain["tri"] != H.NO_HIT, plain["t"] * np.float32(0.9), np.float32(0.5)).astype(np.float32)
+    ref = gpu_ctx(s, nodes, lp, H.ROPES, pipeline=1).render_frame(cam, W, H_, max_bounce=2, mirror=s.mirror,
+                                                                  tbound=tb)
+    for block, lb in ((64, 1), (256, 2)):
+        ctx = gpu_ctx(s, nodes, lp, H.ROPES, pipeline=3, block_size=block, leaf_batch=lb, coop_lanes=coop)
+        img, hits = ctx.render_frame(cam, W, H_, max_bounce=2, mirror=s.mirror, tbound=tb)
+        assert np.array_equal(img, ref[0]), f"block {block}: pixels differ"
+        assert np.array_equal(hits.view(np.uint32), ref[1].view(np.uint32)), f"block {block}: hits differ"
+    with pytest.raises(H.HulkError):
+        gpu_ctx(s, nodes, lp, H.ROPES, pipeline=3, coop_lanes=65)
 
 
 @pytest.mark.parametrize("n", [1, 63, 65, 1000, 40000])
