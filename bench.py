@@ -265,7 +265,10 @@ def compare_methods(ctx, scene, cams, train_cams, W: int, H: int, stream, args, 
                         "inference_tflops": round(n / 4 * 82176 / (infer_ms * 1e-3) / 1e12, 2),
                         "loss": [[int(a), round(float(b), 4)] for a, b in losses], "margin_selection": {}}
         opts_now = bopts
-        for tag, vopts in (("", opts_now), ("_stackless", dict(opts_now, variant=HT.RESTART))):
+        # kd-restart runs in the fused frame kernel (lane refill is a rope-traversal pipeline)
+        restart_opts = dict(opts_now, variant=HT.RESTART,
+                            pipeline=2 if opts_now.get("pipeline") == 3 else opts_now.get("pipeline", 2))
+        for tag, vopts in (("", opts_now), ("_stackless", restart_opts)):
             ctx.set_options(**vopts)
             # margin (and whether to bound at all) chosen on training frames for this traversal
             sel = select_margin(mlp, ctx, calib_cams, W, H, stride=2)
