@@ -550,6 +550,63 @@ def main() -> int:
             return {"workload": f"animated_tori_{sc.n_tris}tris_frame0_poses10-19_{W}x{Hh}_primary+shadow+shade",
                     "kd_build_ms": round(bms, 1), "tuned": b1, "tuned_batch": bb1, "methods": m1, "learned": li1}
 
+        def mt_compare():
+            # the optional Moller-Trumbore test against the reference's on the
+            # headline frames: batched frame time, and triangle ids of the
+            # primary hits; every ray whose ids differ is classified with the
+            # FP64 twin of the reference test: a tie (the two triangles' t
+            # within 1e-4 relative, the FP32 error of t at grazing angles), or
+            # a hit point within 1e-4 of an edge of either triangle
+            import numpy as np
+
+            import oracle as O
+
+            opts = {k: int(v) for k, v in bbest.items() if k != "device"}
+            if opts.get("leaf_batch") == 2:
+                opts["leaf_batch"] = 1
+            out = {}
+            hits_by = {}
+            for tag, tt in (("reference", H.TRI_REFERENCE), ("moller_trumbore", H.TRI_MOLLER_TRUMBORE)):
+                cx = WL.make_ctx(scene, nodes, lp, stream, **dict(opts, tri_test=tt))
+                b = cx.batch(frames, None, brgba.data_ptr(), use_graph=True)
+                ms = time_batch(b, max(args.steps, 5))
+                hits_by[tag] = b.primary_hits()
+                b.close()
+                cx.close()
+                out[tag] = {"batch_ms": round(ms, 4), "mrays_s": round(nf * n / ms / 1e3, 1)}
+            a, m = hits_by["reference"], hits_by["moller_trumbore"]
+            diff = np.nonzero(a["tri"] != m["tri"])[0]
+            V = scene.vertices[:, :3].astype(np.float32)
+            T = scene.triangles
+            tie = edge = other = 0
+            rays_all = {}
+            for i in diff:
+                f, px = divmod(int(i), n)
+                if f not in rays_all:
+                    rays_all[f] = O.cast_rays(cams[f], W, Hh).reshape(-1, 8)
+                r = rays_all[f][px]
+                dd = []
+                for tri in (a["tri"][i], m["tri"][i]):
+                    if tri == H.NO_HIT:
+                        dd.append(None)
+                        continue
+                    p = V[T[tri, :3]]
+                    dd.append(O.tri_test_f64(r, p[0], p[1], p[2]))
+                if all(d is not None for d in dd) and abs(dd[0][0] - dd[1][0]) <= 1e-4 * abs(dd[0][0]):
+                    tie += 1
+                elif any(d is not None and min(d[1], d[2], 1 - d[1] - d[2]) < 1e-4 for d in dd):
+                    edge += 1
+                else:
+                    other += 1
+            out["primary_rays"] = int(len(a))
+            out["tri_id_differences"] = {"total": int(len(diff)), "tie_in_t": tie, "edge_graze": edge,
+                                         "unclassified": other}
+            both = (a["tri"] == m["tri"]) & (a["tri"] != H.NO_HIT)
+            out["t_rel_diff_max_same_tri"] = float(np.max(np.abs(a["t"][both] - m["t"][both]) / a["t"][both]))
+            out["options"] = opts
+            log(f"triangle test: {out}")
+            return out
+
         def streamed():
             # out-of-core: terrain pose 10 at 1024^2, kd-restart, record pool
             # of 100 / 25 / 5 % of the leaf references against the in-core kernel
@@ -576,7 +633,8 @@ def main() -> int:
             log(f"streamed: {out}")
             return out
 
-        for name, fn in (("streamed_out_of_core", streamed), ("methods_1M", methods_1m),
+        for name, fn in (("triangle_test_mt", mt_compare), ("streamed_out_of_core", streamed),
+                         ("methods_1M", methods_1m),
                          ("animated_1M", lambda: WL.animated(frames=20, stream=stream, log=log)),
                          ("animated_1M_device_build",
                           lambda: WL.animated_device_build(frames=20, stream=stream, log=log)),

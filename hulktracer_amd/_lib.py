@@ -24,6 +24,7 @@ VARIANTS = {
 }
 VARIANT_NAMES = {v: k for k, v in VARIANTS.items()}
 BRUTEFORCE, FULLSTACK, SHORTSTACK, LDSSTACK, RESTART, ROPES = range(6)
+TRI_REFERENCE, TRI_MOLLER_TRUMBORE = 0, 1  # hulk_tri_test
 
 FLAG_DEVICE_PTRS = 1
 FLAG_ANYHIT = 2
@@ -61,7 +62,7 @@ class Opts(C.Structure):
     _fields_ = [("device", C.c_int32), ("variant", C.c_int32), ("block_size", C.c_uint32),
                 ("short_stack_depth", C.c_uint32), ("lds_top_levels", C.c_uint32),
                 ("persistent", C.c_uint32), ("pipeline", C.c_uint32), ("leaf_batch", C.c_uint32),
-                ("coop_lanes", C.c_uint32), ("reserved", C.c_uint32), ("stream", C.c_void_p)]
+                ("coop_lanes", C.c_uint32), ("tri_test", C.c_uint32), ("stream", C.c_void_p)]
 
 
 class Counters(C.Structure):

@@ -157,7 +157,9 @@ int check_opts(hulk_ctx* c, const hulk_opts& o) {
         return fail(c, HULK_E_ILLEGAL_INPUT, "the refill pipeline needs the ropes variant and blocks of <= 256");
     if (o.leaf_batch > 2) return fail(c, HULK_E_ILLEGAL_INPUT, "leaf_batch must be 0, 1 or 2");
     if (o.coop_lanes > 64) return fail(c, HULK_E_ILLEGAL_INPUT, "coop_lanes must be in [0, 64]");
-    if (o.reserved) return fail(c, HULK_E_ILLEGAL_INPUT, "hulk_opts.reserved must be 0");
+    if (o.tri_test > HULK_TRI_MOLLER_TRUMBORE) return fail(c, HULK_E_ILLEGAL_INPUT, "unknown tri_test");
+    if (o.tri_test == HULK_TRI_MOLLER_TRUMBORE && o.leaf_batch == 2)
+        return fail(c, HULK_E_ILLEGAL_INPUT, "the Moller-Trumbore test runs with one record in flight");
     return HULK_OK;
 }
 
@@ -201,7 +203,7 @@ int trace_setup(hulk_ctx* c, const hulk_opts& o, TraceParams& P, uint32_t& block
         if (r) return r;
     }
     std::memset(&P, 0, sizeof(P));
-    P.leaf_batch = o.leaf_batch == 2 ? 2u : 1u;
+    P.leaf_batch = o.tri_test == HULK_TRI_MOLLER_TRUMBORE ? 3u : (o.leaf_batch == 2 ? 2u : 1u);
     P.coop_lanes = o.coop_lanes;
     P.nodes = static_cast<const uint4*>(c->nodes.p);
     P.recs = static_cast<const TriRec*>(c->recs.p);
@@ -1090,6 +1092,8 @@ extern "C" int hulk_trace_camera_streamed(hulk_ctx* c, const hulk_camera* cam, u
     if (!c || !cam || !hits || W == 0 || H == 0) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument or empty image");
     if (!c->have_scene) return fail(c, HULK_E_NO_SCENE, "no scene uploaded");
     if (!c->have_tree) return fail(c, HULK_E_NO_TREE, "no KD tree uploaded");
+    if (c->opts.tri_test != HULK_TRI_REFERENCE)
+        return fail(c, HULK_E_ILLEGAL_INPUT, "the streamed trace uses the reference triangle test");
     const uint64_t n64 = (uint64_t)W * H;
     if (n64 >= (1ull << 31)) return fail(c, HULK_E_ILLEGAL_INPUT, "image too large");
     const uint32_t n = static_cast<uint32_t>(n64);

@@ -122,8 +122,60 @@ __device__ __forceinline__ bool test_record(const Ray& r, const TriRec& rec, Bes
     return false;
 }
 
+// Optional Moller-Trumbore test (T. Moller, B. Trumbore, "Fast, minimum
+// storage ray-triangle intersection", JGT 2(1), 1997), the non-culling
+// branch, on the record's p1, e0 = p2 - p1 and e1 = p3 - p1.  It is not the
+// reference's test (checkRayToPoly solves the plane and a 2x2 Gram system),
+// so its t, u, v round differently and its hits agree with the reference's
+// up to ties and edge grazes; the hit keeps the reference's unit normal for
+// shading and u, v directly (det = 1).  The paper's parallel cut,
+// |det| < EPSILON, scales with the triangle's area (on 0.004-wide terrain
+// triangles it drops rays up to 3 degrees off the plane), so the cut here is
+// the reference's, |n.d| < FLT_EPSILON, written without a normal:
+// det = -|e0 x e1| (n.d) and |e0 x e1|^2 = e00 e11 - e01^2 (the Gram
+// determinant, from the record's terms), so the ray is cut when
+// det^2 < FLT_EPSILON^2 (e00 e11 - e01^2).  The conditions of a hit are a
+// conjunction of functions of the inputs, so their order changes no result;
+// the cut is tested after u and v, where few triangles are left.  Fixed
+// operation order: cross and dot as above, every 1 / x by rcp_exact.
+__device__ __forceinline__ bool test_record_mt(const Ray& r, const TriRec& rec, Best& best) {
+    const V3 e0 = v3(rec.c.x, rec.c.y, rec.c.z);
+    const V3 e1 = v3(rec.d.x, rec.d.y, rec.d.z);
+    const V3 pvec = cross(r.d, e1);
+    const float det = dot(e0, pvec);
+    if (det == 0.0f) return false;
+    const float inv = rcp_exact(det);
+    const V3 tvec = sub(r.o, v3(rec.b.x, rec.b.y, rec.b.z));
+    const float u = dot(tvec, pvec) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    const V3 qvec = cross(tvec, e0);
+    const float v = dot(r.d, qvec) * inv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    const float gram = __builtin_fmaf(rec.b.w, rec.d.w, -(rec.c.w * rec.c.w));
+    if (det * det < (kFltEpsilon * kFltEpsilon) * gram) return false;
+    const float t = dot(e1, qvec) * inv;
+    if (t < 0.0f) return false;
+    if (!(t < best.t)) return false;
+    best.t = t;
+    best.s1 = u;
+    best.s2 = v;
+    best.det = 1.0f;
+    best.tri = __float_as_uint(rec.a.w);
+    best.nx = rec.a.x;
+    best.ny = rec.a.y;
+    best.nz = rec.a.z;
+    return true;
+}
+
+// The record test of leaf mode MT (Moller-Trumbore) or the reference test.
+template <bool MT>
+__device__ __forceinline__ bool test_any(const Ray& r, const TriRec& rec, Best& best) {
+    return MT ? test_record_mt(r, rec, best) : test_record(r, rec, best);
+}
+
 // The same test straight from three vertices (brute force variant,
 // GPUTrace_bruteforce.cl:30-42).
+template <bool MT = false>
 __device__ __forceinline__ void test_vertices(const Ray& r, V3 p1, V3 p2, V3 p3, uint32_t id, Best& best) {
     const V3 e0 = sub(p2, p1), e1 = sub(p3, p1), e2 = sub(p3, p2);
     const V3 n = normalize(add(add(cross(e0, e1), cross(e1, e2)), cross(e2, e0)));
@@ -132,7 +184,7 @@ __device__ __forceinline__ void test_vertices(const Ray& r, V3 p1, V3 p2, V3 p3,
     rec.b = make_float4(p1.x, p1.y, p1.z, dot(e0, e0));
     rec.c = make_float4(e0.x, e0.y, e0.z, dot(e0, e1));
     rec.d = make_float4(e1.x, e1.y, e1.z, dot(e1, e1));
-    test_record(r, rec, best);
+    test_any<MT>(r, rec, best);
 }
 
 // isRayToAbovePlane / isRayToBelowPlane / checkRayToBoundingBoxGPU

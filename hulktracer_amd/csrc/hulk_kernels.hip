@@ -110,6 +110,11 @@ __device__ __forceinline__ uint32_t frame_ray(const FrameDev* F, uint32_t n, uin
     return f;
 }
 
+// Leaf modes (template parameter LB of the traversal kernels): one record
+// in flight, two records in flight, or one record with the Moller-Trumbore
+// test instead of the reference's.
+constexpr int kLeafMt = 3;
+
 // Leaf loop.  tstop: a shadow ray only needs to know whether its nearest
 // hit lies below the light distance; once best.t < tstop that answer is
 // fixed (best.t only decreases), so the ray stops.  Primary and mirror rays
@@ -143,7 +148,7 @@ __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, const
         rec.b = recs[i].b;
         rec.c = recs[i].c;
         rec.d = recs[i].d;
-        if (test_record(r, rec, best) && best.t < tstop) break;
+        if (test_any<LB == kLeafMt>(r, rec, best) && best.t < tstop) break;
     }
 }
 
@@ -482,6 +487,7 @@ __device__ __forceinline__ bool ropes_step(const TraceParams& P, const Ray& r, R
 // shading decision (light distance > t) unchanged.  Used where few lanes of
 // a wave are still at work, so that a long leaf costs one memory latency per
 // 64 records instead of one per record.
+template <bool MT>
 __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, const uint4& n, const Ray& r,
                                             Best& best, float tstop, uint64_t want, uint32_t lane_id) {
     while (want) {
@@ -504,7 +510,7 @@ __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, con
             bool hit = false;
             if (base + lane_id < cnt) {
                 const TriRec rec = recs[start + base + lane_id];
-                hit = test_record(q, rec, b);
+                hit = test_any<MT>(q, rec, b);
             }
             // t >= 0 here (or -0): the bits without the sign order like the value
             uint32_t key = hit ? (__float_as_uint(b.t) & 0x7FFFFFFFu) : 0xFFFFFFFFu;
@@ -551,7 +557,7 @@ __device__ void trav_brute(const TraceParams& P, const Ray& r, float tstop, Best
     for (uint32_t i = 0; i < P.n_tris; i++) {
         const uint4 t = P.tris[i];
         const float4 a = P.verts[t.x], b = P.verts[t.y], cc = P.verts[t.z];
-        test_vertices(r, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), i, best);
+        test_vertices<LB == kLeafMt>(r, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), i, best);
         if (best.t < tstop) break;
     }
     if (COUNT) c.tris += P.n_tris;
@@ -964,7 +970,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             if (walk) ln = ropes_descend<COUNT>(P, tr, st, c);
             const uint64_t at_leaf = __ballot(walk && ln.z != 0u);
             if (at_leaf && (uint32_t)__builtin_popcountll(at_leaf) <= P.coop_lanes)
-                coop_leaves(P.recs, ln, tr, bt, tstop, at_leaf, lane_id);
+                coop_leaves<LB == kLeafMt>(P.recs, ln, tr, bt, tstop, at_leaf, lane_id);
             else if (walk)
                 test_leaf<COUNT, LB>(P.recs, ln, tr, bt, c, tstop);
             if (busy) done = !walk || !ropes_exit(P, tr, st, tstop, bt, tfl);
@@ -1124,6 +1130,9 @@ hipError_t launch_render_refill(const TraceParams& P, bool count, uint32_t grid,
     if (P.leaf_batch == 2) {
         if (count) hipLaunchKernelGGL((k_render_refill<true, 2>), dim3(grid), dim3(block), lds, s, P);
         else hipLaunchKernelGGL((k_render_refill<false, 2>), dim3(grid), dim3(block), lds, s, P);
+    } else if (P.leaf_batch == kLeafMt) {
+        if (count) hipLaunchKernelGGL((k_render_refill<true, kLeafMt>), dim3(grid), dim3(block), lds, s, P);
+        else hipLaunchKernelGGL((k_render_refill<false, kLeafMt>), dim3(grid), dim3(block), lds, s, P);
     } else {
         if (count) hipLaunchKernelGGL((k_render_refill<true, 1>), dim3(grid), dim3(block), lds, s, P);
         else hipLaunchKernelGGL((k_render_refill<false, 1>), dim3(grid), dim3(block), lds, s, P);
@@ -1550,6 +1559,9 @@ static hipError_t launch_trace_t(const TraceParams& P, bool render, uint32_t lea
     if (leaf_batch == 2) {
         if (render) hipLaunchKernelGGL((k_trace<V, C, true, 2>), dim3(grid), dim3(block), lds, s, P);
         else hipLaunchKernelGGL((k_trace<V, C, false, 2>), dim3(grid), dim3(block), lds, s, P);
+    } else if (leaf_batch == kLeafMt) {
+        if (render) hipLaunchKernelGGL((k_trace<V, C, true, kLeafMt>), dim3(grid), dim3(block), lds, s, P);
+        else hipLaunchKernelGGL((k_trace<V, C, false, kLeafMt>), dim3(grid), dim3(block), lds, s, P);
     } else {
         if (render) hipLaunchKernelGGL((k_trace<V, C, true, 1>), dim3(grid), dim3(block), lds, s, P);
         else hipLaunchKernelGGL((k_trace<V, C, false, 1>), dim3(grid), dim3(block), lds, s, P);

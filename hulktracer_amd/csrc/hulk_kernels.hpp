@@ -74,7 +74,8 @@ struct TraceParams {
     const uint8_t* mirror;  // mirror flag per triangle, or null (no bounces)
     uint32_t max_bounce;    // camera source; frames carry their own
     uint32_t* rgba;
-    uint32_t leaf_batch;    // host side: picks the kernel instance (1 or 2 records in flight)
+    uint32_t leaf_batch;    // host side: picks the kernel instance (1 or 2 records in flight, 3 = one
+                            // record with the Moller-Trumbore test)
     uint32_t coop_lanes;    // refill kernel: wave-cooperative leaf tests at <= this many busy lanes
     uint32_t stop_at_w;  // shadow queue: stop once best.t < ray.start.w (the light distance)
 };

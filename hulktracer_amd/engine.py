@@ -307,19 +307,20 @@ class GPUTrace:
 
     def __init__(self, variant: int | str = L.FULLSTACK, block_size: int = 256, short_stack_depth: int = 6,
                  lds_top_levels: int = 0, persistent: bool = False, device: int = 0, stream: int | None = None,
-                 pipeline: int = 0, leaf_batch: int = 1, coop_lanes: int = 0):
+                 pipeline: int = 0, leaf_batch: int = 1, coop_lanes: int = 0, tri_test: int = 0):
         """pipeline: 0 = automatic, 1 = queued shadow / mirror launches,
         2 = one fused launch per frame, 3 = fused with lane refill
         (hulk_pipeline).  leaf_batch: triangle records in flight per lane
         (1 or 2).  coop_lanes: lane-refill pipeline only; when at most this
         many lanes of a wave are at a leaf, the whole wave tests their
-        leaves (0 = never)."""
+        leaves (0 = never).  tri_test: 0 = the reference's triangle test,
+        1 = Moller-Trumbore (hulk_tri_test)."""
         if isinstance(variant, str):
             variant = L.VARIANTS[variant]
         self.opts = L.Opts(device=device, variant=variant, block_size=block_size,
                            short_stack_depth=short_stack_depth, lds_top_levels=lds_top_levels,
                            persistent=int(persistent), pipeline=int(pipeline), leaf_batch=int(leaf_batch),
-                           coop_lanes=int(coop_lanes), stream=stream)
+                           coop_lanes=int(coop_lanes), tri_test=int(tri_test), stream=stream)
         self.h = C.c_void_p()
         _check(None, lib().hulk_create(C.byref(self.h), C.byref(self.opts)))
 
@@ -484,7 +485,7 @@ class GPUTrace:
         self._c(lib().hulk_autotune(self.h, C.byref(req), table, cap, C.byref(n), C.byref(best)))
         self.opts = best
         rows = [{f: getattr(table[i], f) for f, _ in L.TuneEntry._fields_} for i in range(n.value)]
-        return {f: getattr(best, f) for f, _ in L.Opts._fields_ if f not in ("stream", "reserved")}, rows
+        return {f: getattr(best, f) for f, _ in L.Opts._fields_ if f != "stream"}, rows
 
     def counters(self) -> dict:
         c = L.Counters()

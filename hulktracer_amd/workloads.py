@@ -173,7 +173,7 @@ def _device_build_pipelined(ctx, stream, scenes, tuned, W, H, max_bounce, mirror
     previous render.  Returns the wall time per frame, and whether the last
     frame's pixels equal `last_rgba` (the one-context run's last frame)."""
     torch = _torch()
-    opts = {k: int(v) for k, v in (tuned or {}).items() if k not in ("device", "stream", "reserved")}
+    opts = {k: int(v) for k, v in (tuned or {}).items() if k not in ("device", "stream")}
     streams = [stream, torch.cuda.Stream()]
     other = GPUTrace(stream=streams[1].cuda_stream, **opts)
     other.upload_scene(scenes[0])

@@ -124,9 +124,25 @@ typedef struct hulk_opts {
                                    many lanes of a wave are at a leaf, the
                                    whole wave tests their leaves, 64 records
                                    per pass; 0 = never, 1..64               */
-    uint32_t reserved;          /* must be 0                                 */
+    uint32_t tri_test;          /* hulk_tri_test: HULK_TRI_REFERENCE (the
+                                   reference's checkRayToPoly, bit-exact) or
+                                   HULK_TRI_MOLLER_TRUMBORE (optional; one
+                                   record in flight, not with the streamed
+                                   trace)                                    */
     void*    stream;            /* hipStream_t to use; NULL = own stream     */
 } hulk_opts;
+
+/* Triangle test.  REFERENCE restates checkRayToPoly
+ * (SharedFunctions.hpp:47-102: plane, then a 2x2 Gram system) bit for bit.
+ * MOLLER_TRUMBORE (Moller and Trumbore, JGT 1997, non-culling branch,
+ * with the reference's parallel cut |n.d| < FLT_EPSILON in place of the
+ * paper's absolute EPSILON) is an option with no counterpart in the
+ * reference: its nearest hits agree with the reference test's except at
+ * ties and edge grazes, and its t, u, v round differently. */
+typedef enum hulk_tri_test {
+    HULK_TRI_REFERENCE = 0,
+    HULK_TRI_MOLLER_TRUMBORE = 1
+} hulk_tri_test;
 
 /* Launch model of a frame (render_frame, batches).  QUEUED: primary trace,
  * then per generation compacted shadow and mirror queues, each traced by

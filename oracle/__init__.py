@@ -22,6 +22,7 @@ HERE = Path(__file__).resolve().parent
 LIB = HERE / "_build" / "liboracle.so"
 
 V_BRUTE, V_FULLSTACK, V_SHORTSTACK, V_LDSSTACK, V_RESTART, V_ROPES = range(6)
+MT = 0x100  # or-ed into a variant: trace with the Moller-Trumbore test (ORC_MT)
 NO_HIT = 0xFFFFFFFF
 FLT_MAX = np.float32(3.4028234663852886e38)
 
@@ -56,6 +57,7 @@ def lib() -> C.CDLL:
         L.orc_cast_rays.argtypes = [vp, u32, u32, vp]
         L.orc_ray_box.argtypes = [vp, vp, vp, vp]
         L.orc_tri_test_ex.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_tri_test_mt_ex.argtypes = [vp, vp, vp, vp, vp]
         L.orc_tri_test_f64.argtypes = [vp, vp, vp, vp, vp]
         L.orc_build_ropes.argtypes = [vp, u32, vp, vp, vp, vp]
         L.orc_build_ropes.restype = u32
@@ -164,6 +166,16 @@ def tri_test(ray, p1, p2, p3):
     ps = [np.ascontiguousarray(np.r_[np.asarray(p, np.float32)[:3], np.float32(1)], np.float32) for p in (p1, p2, p3)]
     out = np.zeros(7, np.float32)
     if lib().orc_tri_test_ex(_p(r), *[_p(p) for p in ps], _p(out)):
+        return out[0], out[1], out[2], out[3], out[4:7].copy()
+    return None
+
+
+def tri_test_mt(ray, p1, p2, p3):
+    """Moller-Trumbore (ORC_MT mode): returns None or (t, u, v, 1, n)."""
+    r = np.ascontiguousarray(ray, np.float32).reshape(8)
+    ps = [np.ascontiguousarray(np.r_[np.asarray(p, np.float32)[:3], np.float32(1)], np.float32) for p in (p1, p2, p3)]
+    out = np.zeros(7, np.float32)
+    if lib().orc_tri_test_mt_ex(_p(r), *[_p(p) for p in ps], _p(out)):
         return out[0], out[1], out[2], out[3], out[4:7].copy()
     return None
 

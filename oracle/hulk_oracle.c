@@ -158,6 +158,47 @@ int orc_tri_test_ex(const ray_t* ray, const f4* p1, const f4* p2, const f4* p3, 
     return r;
 }
 
+/* Moller-Trumbore test (T. Moller, B. Trumbore, "Fast, minimum storage
+ * ray-triangle intersection", JGT 2(1), 1997), non-culling branch.  No
+ * counterpart in the reference (its test is checkRayToPoly above); the
+ * library's optional HULK_TRI_MOLLER_TRUMBORE mode.  e0 = p2 - p1,
+ * e1 = p3 - p1; the hit keeps the reference's unit normal and reports
+ * s1 = u, s2 = v, det = 1.  Division is 1.0f / det.  Parallel cut: the
+ * reference's |n.d| < FLT_EPSILON, as det^2 < FLT_EPSILON^2 * gram with
+ * gram = |e0 x e1|^2 = fma(e00, e11, -(e01 * e01)) (the paper's absolute
+ * EPSILON would scale with the triangle's area). */
+int orc_tri_test_mt(const ray_t* ray, f4 p1, f4 p2, f4 p3, tri_hit* h) {
+    f4 e0 = v_sub(p2, p1);
+    f4 e1 = v_sub(p3, p1);
+    f4 pvec = v_cross(ray->d, e1);
+    float det = v_dot(e0, pvec);
+    float gram = fmaf(v_dot(e0, e0), v_dot(e1, e1), -(v_dot(e0, e1) * v_dot(e0, e1)));
+    if (det == 0.0f || det * det < (FLT_EPSILON * FLT_EPSILON) * gram) return 0;
+    float inv = 1.0f / det;
+    f4 tvec = v_sub(ray->o, p1);
+    float u = v_dot(tvec, pvec) * inv;
+    if (u < 0.0f || u > 1.0f) return 0;
+    f4 qvec = v_cross(tvec, e0);
+    float v = v_dot(ray->d, qvec) * inv;
+    if (v < 0.0f || u + v > 1.0f) return 0;
+    float t = v_dot(e1, qvec) * inv;
+    if (t < 0.0f) return 0;
+    f4 e2 = v_sub(p3, p2);
+    f4 n = v_add(v_add(v_cross(e0, e1), v_cross(e1, e2)), v_cross(e2, e0));
+    h->t = t; h->s1 = u; h->s2 = v; h->det = 1.0f; h->n = v_normalize(n);
+    return 1;
+}
+
+int orc_tri_test_mt_ex(const ray_t* ray, const f4* p1, const f4* p2, const f4* p3, float out[7]) {
+    tri_hit h;
+    int r = orc_tri_test_mt(ray, *p1, *p2, *p3, &h);
+    if (r) {
+        out[0] = h.t; out[1] = h.s1; out[2] = h.s2; out[3] = h.det;
+        out[4] = h.n.x; out[5] = h.n.y; out[6] = h.n.z;
+    }
+    return r;
+}
+
 /* Double-precision twin of the test: used to state the FP32 error of t, u, v. */
 int orc_tri_test_f64(const ray_t* ray, const f4* p1, const f4* p2, const f4* p3, double out[3]) {
     double a[3] = {p1->x, p1->y, p1->z}, b[3] = {p2->x, p2->y, p2->z}, c[3] = {p3->x, p3->y, p3->z};
@@ -196,7 +237,12 @@ typedef struct {
     const box_t* cells;
     const uint32_t* kids;
     uint32_t n_slots;
+    int mt; /* 1: the Moller-Trumbore test instead of checkRayToPoly */
 } scene_t;
+
+/* Variant argument flag of the public entry points: trace with the
+ * Moller-Trumbore test. */
+#define ORC_MT 0x100
 
 typedef struct {
     float t, s1, s2, det;
@@ -210,7 +256,8 @@ static inline void test_tri_into(const scene_t* s, const ray_t* ray, uint32_t tr
     const tri_t* T = &s->tris[tri];
     tri_hit h;
     c->tris++;
-    if (orc_tri_test(ray, s->verts[T->idx[0]], s->verts[T->idx[1]], s->verts[T->idx[2]], &h)) {
+    const f4 p1 = s->verts[T->idx[0]], p2 = s->verts[T->idx[1]], p3 = s->verts[T->idx[2]];
+    if (s->mt ? orc_tri_test_mt(ray, p1, p2, p3, &h) : orc_tri_test(ray, p1, p2, p3, &h)) {
         if (h.t < best->t) {
             best->t = h.t; best->s1 = h.s1; best->s2 = h.s2; best->det = h.det;
             best->n = h.n; best->tri = tri;
@@ -530,6 +577,7 @@ static void make_scene(scene_t* s, const f4* verts, const tri_t* tris, uint32_t 
     s->nodes = nodes; s->n_nodes = n_nodes; s->lp = lp;
     s->ropes = rv ? rv->ropes : NULL; s->cells = rv ? rv->cells : NULL;
     s->kids = rv ? rv->kids : NULL; s->n_slots = rv ? rv->n_slots : 0;
+    s->mt = 0;
 }
 
 /* Trace arbitrary rays.  counts (nullable): 3 uint32 per ray (nodes, leaves,
@@ -542,6 +590,8 @@ void orc_trace_rays(int variant, uint32_t depth,
                     int nthreads) {
     scene_t s;
     make_scene(&s, verts, tris, n_tris, box, nodes, n_nodes, lp, rv);
+    s.mt = (variant & ORC_MT) != 0;
+    variant &= ~ORC_MT;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 256)
@@ -563,6 +613,8 @@ void orc_trace_camera(int variant, uint32_t depth,
                       uint32_t* counts, int nthreads) {
     scene_t s;
     make_scene(&s, verts, tris, n_tris, box, nodes, n_nodes, lp, rv);
+    s.mt = (variant & ORC_MT) != 0;
+    variant &= ~ORC_MT;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 4)
@@ -646,6 +698,8 @@ void orc_shade(int variant, uint32_t depth,
                const light_t* lights, uint32_t n_lights, int mode, uint32_t* out, int nthreads) {
     scene_t s;
     make_scene(&s, verts, tris, n_tris, box, nodes, n_nodes, lp, rv);
+    s.mt = (variant & ORC_MT) != 0;
+    variant &= ~ORC_MT;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 4)
@@ -690,6 +744,8 @@ void orc_render(int variant, uint32_t depth,
                 int nthreads) {
     scene_t s;
     make_scene(&s, verts, tris, n_tris, box, nodes, n_nodes, lp, rv);
+    s.mt = (variant & ORC_MT) != 0;
+    variant &= ~ORC_MT;
     uint64_t n_near = 0, n_shadow = 0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);

@@ -257,3 +257,60 @@ def test_reciprocal_selftest():
     assert in_range == 2 * 252 * (1 << 23)
     assert diff_in == 0 and diff_exact == 0, (diff_in, diff_exact)
     assert diff_out > 0  # the refinement alone is not exact outside the range: the fallback is needed
+
+
+# --------------------------------------------- Moller-Trumbore test mode --
+
+@pytest.mark.parametrize("variant", [H.BRUTEFORCE] + VARIANTS)
+def test_mt_primary_parity(variant):
+    """tri_test = Moller-Trumbore: every hit field equals the oracle's MT
+    mode bit for bit (config-1 scene, and arbitrary rays with degenerate
+    directions); triangle ids agree with the reference test's except at a
+    few grazes of shared edges."""
+    s, nodes, lp = scene_and_tree(1, 1)
+    cam = s.camera(0, 1.0)
+    ctx = gpu_ctx(s, nodes, lp, variant, tri_test=H.TRI_MOLLER_TRUMBORE)
+    got = ctx.trace_camera(cam, 256, 256)
+    osc = oracle_scene(1, 1)
+    _check(got, osc.trace_camera(ORACLE_OF[variant] | O.MT, cam, 256, 256, depth=6), f"MT variant {variant}")
+    ref = osc.trace_camera(ORACLE_OF[variant], cam, 256, 256, depth=6)
+    assert np.mean(got["tri"] == ref["tri"]) > 0.999
+    s3, n3, l3 = scene_and_tree(1, 3, 300)
+    rays = random_rays(20000, s3.box, seed=variant + 50)
+    ctx3 = gpu_ctx(s3, n3, l3, variant, tri_test=H.TRI_MOLLER_TRUMBORE)
+    _check(ctx3.trace_rays(rays), oracle_scene(1, 3, 300).trace_rays(ORACLE_OF[variant] | O.MT, rays, depth=6),
+           f"MT rays variant {variant}")
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_mt_terrain_parity(variant):
+    """Terrain scene, MT mode, LDS top tree and persistent waves."""
+    s, nodes, lp = scene_and_tree(2, 1)
+    cam = s.camera(13, 1.0)
+    ctx = gpu_ctx(s, nodes, lp, variant, tri_test=H.TRI_MOLLER_TRUMBORE, lds_top_levels=8, persistent=True)
+    got = ctx.trace_camera(cam, 256, 256)
+    _check(got, oracle_scene(2, 1).trace_camera(ORACLE_OF[variant] | O.MT, cam, 256, 256, depth=6),
+           f"MT terrain variant {variant}")
+
+
+@pytest.mark.parametrize("opts", [dict(variant=H.FULLSTACK, pipeline=1), dict(variant=H.LDSSTACK, pipeline=2),
+                                  dict(variant=H.ROPES, pipeline=3, block_size=128),
+                                  dict(variant=H.ROPES, pipeline=3, block_size=256, coop_lanes=64)])
+def test_mt_render_parity(opts):
+    """Whole frames with shadow rays and two mirror generations in MT mode
+    (queued, fused, lane refill, cooperative leaves): pixels equal the
+    oracle's MT render; the options that MT cannot take are refused."""
+    s, nodes, lp = scene_and_tree(3, 6, 5, 4)
+    cam = s.camera(4, 1.0)
+    opts = dict(opts)
+    v = opts.pop("variant")
+    ctx = gpu_ctx(s, nodes, lp, v, tri_test=H.TRI_MOLLER_TRUMBORE, **opts)
+    img, _ = ctx.render_frame(cam, 160, 160, max_bounce=2, mirror=s.mirror)
+    rimg, _ = oracle_scene(3, 6, 5, 4).render(ORACLE_OF[v] | O.MT, cam, 160, 160, max_bounce=2)
+    assert np.array_equal(img, rimg)
+    with pytest.raises(H.HulkError):
+        gpu_ctx(s, nodes, lp, v, tri_test=H.TRI_MOLLER_TRUMBORE, leaf_batch=2)
+    with pytest.raises(H.HulkError):
+        gpu_ctx(s, nodes, lp, v, tri_test=2)
+    with pytest.raises(H.HulkError):
+        gpu_ctx(s, nodes, lp, H.RESTART, tri_test=H.TRI_MOLLER_TRUMBORE).trace_camera_streamed(cam, 64, 64)

@@ -15,6 +15,11 @@ How each oracle function is pinned (parity status):
       identical node arrays on several scenes + structural invariants; the
       only golden vectors of the reference tests that touch the path
       (getBoxSurface / getBoxVolume, test/sharedFunctionTest.cpp:7-56).
+  orc_tri_test_mt (Moller-Trumbore, JGT 1997; optional mode, no reference
+      counterpart) known answers on dyadic inputs (t, u, v exact, both
+      determinant signs, edges inclusive, the parallel cut and behind cases), hit /
+      miss agreement with checkRayToPoly except within 1e-5 of an edge, the
+      FP64 twin, and brute-force agreement of every traversal in MT mode.
   orc_shade / orc_pack_rgb (GPUTrace.cl:193-228, FrameBuffer.hpp:196-217)
       closed-form Lambert + shadow halving on a one-triangle scene;
       round-half-even packing.
@@ -113,6 +118,90 @@ def test_tri_against_fp64():
     assert len(et) > 500
     assert np.quantile(et, 0.999) < 1e-5 and et.max() < 1e-4, (np.quantile(et, 0.999), et.max())
     assert np.quantile(eu, 0.99) < 1e-5 and eu.max() < 1e-3, (np.quantile(eu, 0.99), eu.max())
+
+
+def test_mt_known_answers():
+    """Moller-Trumbore (optional mode): on the unit right triangle with the
+    ray along -z, u = x and v = y of the crossing point and t the height,
+    exactly for dyadic inputs; edges and vertices are inside (u, v >= 0,
+    u + v <= 1); just outside, parallel (|n.d| < FLT_EPSILON, the reference's
+    cut), behind: miss;
+    t == 0 is a hit; the normal is the reference's."""
+    for (x, y, z) in ((0.25, 0.5, 1.0), (0.0, 0.0, 2.0), (1.0, 0.0, 0.5), (0.0, 1.0, 4.0), (0.5, 0.5, 1.0),
+                      (0.125, 0.75, 3.0)):
+        h = O.tri_test_mt(ray((x, y, z), (0, 0, -1)), *TRI)
+        assert h is not None, (x, y)
+        t, u, v, det, n = h
+        assert (t, u, v, det) == (z, x, y, 1.0)
+        assert np.array_equal(n, np.array([0, 0, 1], F32))
+    for o in ((-1e-6, 0.3, 1), (0.3, -1e-6, 1), (0.5 + 1e-6, 0.5 + 1e-6, 1)):
+        assert O.tri_test_mt(ray(o, (0, 0, -1)), *TRI) is None
+    assert O.tri_test_mt(ray((0.2, 0.2, 1), (1, 0, 0)), *TRI) is None          # det = 0
+    assert O.tri_test_mt(ray((0.2, 0.2, 1), (1, 0, -1e-7)), *TRI) is None      # |n.d| < FLT_EPSILON
+    # the cut is the reference's (scale-free): a tiny triangle seen at a
+    # shallow angle (|n.d| = 1e-3) is a hit in both tests
+    tiny = [np.array(p, F32) * F32(1.0 / 1024) for p in ((0, 0, 0), (1, 0, 0), (0, 1, 0))]
+    r = ray((0.25 / 1024 - 1.0, 0.25 / 1024, 1e-3), (1, 0, -1e-3))
+    a, b = O.tri_test(r, *tiny), O.tri_test_mt(r, *tiny)
+    assert a is not None and b is not None and abs(a[0] - b[0]) <= 1e-6 * a[0]
+    assert O.tri_test_mt(ray((0.2, 0.2, -1), (0, 0, -1)), *TRI) is None       # behind
+    h = O.tri_test_mt(ray((0.2, 0.2, 0), (0, 0, 1)), *TRI)
+    assert h is not None and h[0] == 0.0
+    # unnormalized direction: t in units of |d|; a wrong sign of qvec or of
+    # the determinant would flip t or u here
+    h = O.tri_test_mt(ray((0.25, 0.5, 4), (0, 0, -2)), *TRI)
+    assert (h[0], h[1], h[2]) == (2.0, 0.25, 0.5)
+    h = O.tri_test_mt(ray((0.25, 0.5, -4), (0, 0, 2)), *TRI)                   # from below: det < 0
+    assert (h[0], h[1], h[2]) == (2.0, 0.25, 0.5)
+
+
+def test_mt_agrees_with_reference_and_fp64():
+    """On random rays through random triangles, Moller-Trumbore and the
+    reference test agree on hit / miss except within 1e-5 of an edge (FP64
+    barycentrics), and both are within the FP32 error of the FP64 twin."""
+    rng = np.random.default_rng(11)
+    disagree, et, eu = 0, [], []
+    for _ in range(4000):
+        p = [rng.random(3).astype(F32) for _ in range(3)]
+        c = (p[0] + p[1] + p[2]) / 3
+        o = (rng.random(3) * 2 - 0.5).astype(F32)
+        tgt = c + (rng.random(3) - 0.5).astype(F32) * 0.6
+        r = ray(o, (tgt - o).astype(F32))
+        a, b, d = O.tri_test(r, *p), O.tri_test_mt(r, *p), O.tri_test_f64(r, *p)
+        if (a is None) != (b is None):
+            disagree += 1
+            assert d is not None
+            u, v = d[1], d[2]
+            assert min(u, v, 1 - u - v) > -1e-5 and min(u, v, 1 - u - v) < 1e-5, (u, v)
+            continue
+        if b is None:
+            continue
+        et.append(abs(b[0] - d[0]) / abs(d[0]))
+        eu.append(max(abs(b[1] - d[1]), abs(b[2] - d[2])))
+    assert len(et) > 500 and disagree < 20
+    assert np.percentile(et, 99.9) < 1e-5 and np.percentile(eu, 99) < 1e-5
+
+
+@pytest.mark.parametrize("variant", [O.V_FULLSTACK, O.V_SHORTSTACK, O.V_RESTART, O.V_ROPES])
+def test_mt_variants_match_brute_force(variant):
+    """Traversals with the Moller-Trumbore test equal brute force with it on
+    every config-1 ray (nearest hit is independent of the tree), and agree
+    with the reference test's triangle ids except at grazes of shared edges."""
+    s, nodes, lp = scene_and_tree(1, 1)
+    osc = oracle_scene(1, 1)
+    cam = s.camera(0, 1.0)
+    bf = osc.trace_camera(O.V_BRUTE | O.MT, cam, 128, 128)
+    got = osc.trace_camera(variant | O.MT, cam, 128, 128, depth=6)
+    assert np.array_equal(got["tri"], bf["tri"])
+    assert np.array_equal(got["t"].view(np.uint32), bf["t"].view(np.uint32))
+    ref = osc.trace_camera(variant, cam, 128, 128, depth=6)
+    hit = ref["tri"] != O.NO_HIT
+    assert (bf["tri"] != O.NO_HIT).sum() > 1000
+    assert np.mean(got["tri"] == ref["tri"]) > 0.999
+    both = hit & (got["tri"] == ref["tri"])
+    rel = np.abs(got["t"][both] - ref["t"][both]) / ref["t"][both]
+    # the two tests round differently; t is ill-conditioned at grazing angles
+    assert np.quantile(rel, 0.999) < 1e-5 and rel.max() < 1e-3, (np.quantile(rel, 0.999), rel.max())
 
 
 # ------------------------------------------------------------ box test --
