@@ -86,7 +86,7 @@ struct alignas(16) LNode {
     uint32_t cstart[3], ccount[3];
     unsigned long long key;    // (ordered SAH << 32) | candidate index, min over the node
     uint32_t g1[3];            // first candidate of the second run of equal values, per axis
-    uint32_t flags;            // kSlotLeaf / kSlotSplit | axis << 4; 0 = no node
+    uint32_t flags;            // kSlotLeaf / kSlotSplit | axis << 4 (valid for the nodes of the lists)
     float plane;
     uint32_t stage;            // leaf: first staging entry
     uint32_t size, nleaf, nlp; // subtree: nodes, leaves, leaf polygon references
@@ -97,7 +97,8 @@ struct alignas(16) LNode {
 static_assert(sizeof(LNode) == 128, "LNode layout");
 
 struct LevelInfo {
-    uint32_t n_polys, n_cands, stage_base, pad;
+    uint32_t n_polys, n_cands, stage_base;
+    uint32_t n_nodes;  // nodes of this depth (entries of its node list)
 };
 
 struct Status {
@@ -127,6 +128,7 @@ struct KdDev {
     uint32_t *ca, *cb;
     uint8_t *rp, *rc;
     uint4 *part_p, *part_c;
+    uint32_t* nlist;  // per depth L at slot_base(L): the slots of the depth's nodes, in no order
     uint32_t *stage_poly, *stage_owner;
     uint4* ref_nodes;
     uint4* dev_nodes;
@@ -177,13 +179,6 @@ __device__ __forceinline__ void faces(const LNode& n, uint32_t j, float value, f
     const float lo = j == 0 ? n.bmin[0] : (j == 1 ? n.bmin[1] : n.bmin[2]);
     upper = area + (hi - value) * xpy;
     lower = area + (value - lo) * xpy;
-}
-
-__device__ __forceinline__ bool slot_exists(const KdDev& D, uint32_t L, uint32_t i) {
-    if (L == 0) return true;
-    const LNode& p = D.nodes[slot_base(L - 1) + (i >> 1)];
-    if (!(p.flags & kSlotSplit)) return false;
-    return D.nodes[slot_base(L) + i].pcount > 0;
 }
 
 __device__ __forceinline__ bool level_empty(const KdDev& D, uint32_t L) { return L > 0 && D.lvl[L].n_polys == 0; }
@@ -284,10 +279,11 @@ __global__ void k_kdb_init(KdDev D, SceneBox box, uint32_t n_tris, uint32_t ns) 
         D.lvl[t].n_polys = t == 0 ? n_tris : 0u;
         D.lvl[t].n_cands = t == 0 ? 6u * ns : 0u;
         D.lvl[t].stage_base = 0;
-        D.lvl[t].pad = 0;
+        D.lvl[t].n_nodes = t == 0 ? 1u : 0u;
     }
     if (t < sizeof(Status) / 4) reinterpret_cast<uint32_t*>(D.st)[t] = 0;
     if (t == 0) {
+        D.nlist[0] = 0;
         LNode& r = D.nodes[0];
         for (int a = 0; a < 3; a++) {
             r.bmin[a] = box.mn[a];
@@ -603,14 +599,20 @@ __global__ void __launch_bounds__(kThreads) k_kdb_sah(KdDev D, uint32_t L) {
 
 // ---------------------------------------------------------------- decide --
 
-__global__ void __launch_bounds__(kThreads) k_kdb_decide(KdDev D, uint32_t L) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (1u << L) || D.st->overflow || level_empty(D, L)) return;
+// The nodes of a depth, one thread each, from the depth's node list, so that
+// a pass costs the depth's nodes, not its 2^L heap slots.  The list holds
+// both children of every split of the depth above (decide appends them);
+// a child that got no polygons is no node (the root always is one).
+__device__ __forceinline__ bool is_node(const KdDev& D, uint32_t L, uint32_t i) {
+    return in_range(D, i, 1u << L) && (L == 0 || D.nodes[slot_base(L) + i].pcount > 0);
+}
+#define KDB_FOR_NODES(D, L, i)                                                                          \
+    for (uint32_t k_ = blockIdx.x * blockDim.x + threadIdx.x, n_ = (D).lvl[L].n_nodes; k_ < n_;         \
+         k_ += gridDim.x * blockDim.x)                                                                   \
+        if (const uint32_t i = (D).nlist[slot_base(L) + k_]; is_node((D), (L), i))
+
+__device__ bool decide_node(const KdDev& D, uint32_t L, uint32_t i) {
     LNode& nd = D.nodes[slot_base(L) + i];
-    if (!slot_exists(D, L, i)) {
-        nd.flags = 0;
-        return;
-    }
     uint32_t flags = kSlotLeaf;
     if (L < kMaxDepth && nd.pcount >= kMinLeafPolys) {
         const unsigned long long key = nd.key;
@@ -618,7 +620,7 @@ __global__ void __launch_bounds__(kThreads) k_kdb_decide(KdDev D, uint32_t L) {
         const float leaf_cost = 5.0f * static_cast<float>(nd.ev);
         if (!(best >= leaf_cost)) {
             const uint32_t f = static_cast<uint32_t>(key);
-            if (!in_range(D, f, D.lvl[L].n_cands)) return;
+            if (!in_range(D, f, D.lvl[L].n_cands)) return false;
             const uint32_t j = D.cseg[L & 1][f] & 3u;
             const float value = cval(D.cand[L & 1][f]);
             float area, upper, lower;
@@ -642,6 +644,37 @@ __global__ void __launch_bounds__(kThreads) k_kdb_decide(KdDev D, uint32_t L) {
         }
     }
     nd.flags = flags;
+    return (flags & kSlotSplit) != 0;
+}
+
+// Decide every node of the depth; both children of each split go to the
+// next depth's node list, one atomic per wave.
+__global__ void __launch_bounds__(kThreads) k_kdb_decide(KdDev D, uint32_t L) {
+    if (D.st->overflow) return;
+    const uint32_t n = D.lvl[L].n_nodes;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t k = base + threadIdx.x;
+        uint32_t i = 0;
+        bool split = false;
+        if (k < n) {
+            i = D.nlist[slot_base(L) + k];
+            if (is_node(D, L, i)) split = decide_node(D, L, i);
+        }
+        const unsigned long long m = __ballot(split);
+        if (m == 0ull) continue;
+        const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(m));
+        uint32_t first = 0;
+        if (lane == leader) first = atomicAdd(&D.lvl[L + 1].n_nodes, 2u * static_cast<uint32_t>(__builtin_popcountll(m)));
+        first = static_cast<uint32_t>(__shfl(static_cast<int>(first), static_cast<int>(leader), 64));
+        if (split) {
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+            uint32_t* next = D.nlist + slot_base(L + 1) + first + 2u * rank;
+            next[0] = 2u * i;
+            next[1] = 2u * i + 1u;
+        }
+    }
 }
 
 // ------------------------------------------------------------ split scan --
@@ -843,23 +876,21 @@ __global__ void __launch_bounds__(kThreads) k_kdb_scatter(KdDev D, uint32_t L) {
 // -------------------------------------------------------------- assembly --
 
 __global__ void __launch_bounds__(kThreads) k_kdb_bottom_up(KdDev D, uint32_t L) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (1u << L) || D.st->overflow || level_empty(D, L)) return;
-    LNode& nd = D.nodes[slot_base(L) + i];
-    if (!slot_exists(D, L, i)) {
-        nd.size = nd.nleaf = nd.nlp = 0;
-        return;
-    }
-    if (nd.flags & kSlotSplit) {
-        const LNode& a = D.nodes[slot_base(L + 1) + 2u * i];
-        const LNode& b = D.nodes[slot_base(L + 1) + 2u * i + 1u];
-        nd.size = 1u + a.size + b.size;
-        nd.nleaf = a.nleaf + b.nleaf;
-        nd.nlp = a.nlp + b.nlp;
-    } else {
-        nd.size = 1u;
-        nd.nleaf = 1u;
-        nd.nlp = nd.pcount;
+    if (D.st->overflow) return;
+    KDB_FOR_NODES(D, L, i) {
+        LNode& nd = D.nodes[slot_base(L) + i];
+        if (nd.flags & kSlotSplit) {
+            // a child without polygons is no node (its record holds no counts)
+            const LNode& a = D.nodes[slot_base(L + 1) + 2u * i];
+            const LNode& b = D.nodes[slot_base(L + 1) + 2u * i + 1u];
+            nd.size = 1u + (a.pcount ? a.size : 0u) + (b.pcount ? b.size : 0u);
+            nd.nleaf = (a.pcount ? a.nleaf : 0u) + (b.pcount ? b.nleaf : 0u);
+            nd.nlp = (a.pcount ? a.nlp : 0u) + (b.pcount ? b.nlp : 0u);
+        } else {
+            nd.size = 1u;
+            nd.nleaf = 1u;
+            nd.nlp = nd.pcount;
+        }
     }
 }
 
@@ -868,16 +899,24 @@ __global__ void __launch_bounds__(kThreads) k_kdb_bottom_up(KdDev D, uint32_t L)
 // subtrees of depth-12 nodes follow in slot order, each depth first.
 __global__ void __launch_bounds__(1024) k_kdb_top_order(KdDev D) {
     __shared__ uint32_t s0[16];
+    __shared__ uint8_t exists[1u << kKdTopLevels];
     if (D.st->overflow) return;
     uint32_t prefix = 0;
     for (uint32_t L = 0; L <= kKdTopLevels; L++) {
         if (threadIdx.x == 0) D.st->prefix[L] = prefix;
         if (level_empty(D, L)) continue;
         const uint32_t n = 1u << L;
+        for (uint32_t i = threadIdx.x; i < n; i += 1024) exists[i] = 0;
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < D.lvl[L].n_nodes; k += 1024) {
+            const uint32_t i = D.nlist[slot_base(L) + k];
+            if (is_node(D, L, i)) exists[i] = 1;
+        }
+        __syncthreads();
         uint32_t running = prefix;
         for (uint32_t c = 0; c < n; c += 1024) {
             const uint32_t i = c + threadIdx.x;
-            const bool ex = i < n && slot_exists(D, L, i);
+            const bool ex = i < n && exists[i];
             LNode* nd = ex ? &D.nodes[slot_base(L) + i] : nullptr;
             const uint32_t v = ex ? (L == kKdTopLevels ? nd->size : 1u) : 0u;
             uint32_t tot;
@@ -905,49 +944,42 @@ __global__ void __launch_bounds__(1024) k_kdb_top_order(KdDev D) {
     }
 }
 
+// Top down, a depth per launch: the children's reference index, leaf id,
+// leaf start and (below the breadth-first top) device index, then the
+// node's own records in both layouts (its indices are final: the parent's
+// launch, or the top order, wrote them).
 __global__ void __launch_bounds__(kThreads) k_kdb_top_down(KdDev D, uint32_t L) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (1u << L) || D.st->overflow || level_empty(D, L)) return;
-    if (!slot_exists(D, L, i)) return;
-    const LNode& nd = D.nodes[slot_base(L) + i];
-    if (!(nd.flags & kSlotSplit)) return;
-    LNode& a = D.nodes[slot_base(L + 1) + 2u * i];
-    LNode& b = D.nodes[slot_base(L + 1) + 2u * i + 1u];
-    const bool deep = L + 1 > kKdTopLevels;
-    if (a.pcount) {
-        a.pre = nd.pre + 1u;
-        a.lid = nd.lid;
-        a.lps = nd.lps;
-        if (deep) a.dev = nd.dev + 1u;
-    }
-    if (b.pcount) {
-        b.pre = nd.pre + 1u + a.size;
-        b.lid = nd.lid + a.nleaf;
-        b.lps = nd.lps + a.nlp;
-        if (deep) b.dev = nd.dev + 1u + a.size;
-    }
-}
-
-__global__ void __launch_bounds__(kThreads) k_kdb_emit(KdDev D) {
-    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= kSlots || D.st->overflow) return;
-    const uint32_t L = 31u - __clz(h + 1u);
-    const uint32_t i = h - slot_base(L);
-    if (level_empty(D, L) || !slot_exists(D, L, i)) return;
-    const LNode& nd = D.nodes[h];
+    if (D.st->overflow) return;
     const uint32_t n_nodes = D.st->n_nodes;
-    if (!in_range(D, nd.pre, n_nodes) || !in_range(D, nd.dev, n_nodes)) return;
-    if (nd.flags & kSlotSplit) {
-        const LNode& a = D.nodes[slot_base(L + 1) + 2u * i];
-        const LNode& b = D.nodes[slot_base(L + 1) + 2u * i + 1u];
-        const uint32_t fl = HULK_NODE_JUNCTION | ((nd.flags >> 4) & 3u);
-        const uint32_t pl = __float_as_uint(nd.plane);
-        D.ref_nodes[nd.pre] = make_uint4(fl, a.pcount ? a.pre : 0u, b.pcount ? b.pre : 0u, pl);
-        D.dev_nodes[nd.dev] = make_uint4(fl, a.pcount ? a.dev : 0u, b.pcount ? b.dev : 0u, pl);
-    } else {
-        const uint4 leaf = make_uint4(HULK_NODE_LEAF, nd.lps, nd.pcount, nd.lid);
-        D.ref_nodes[nd.pre] = leaf;
-        D.dev_nodes[nd.dev] = leaf;
+    KDB_FOR_NODES(D, L, i) {
+        const LNode& nd = D.nodes[slot_base(L) + i];
+        if (!in_range(D, nd.pre, n_nodes) || !in_range(D, nd.dev, n_nodes)) continue;
+        if (nd.flags & kSlotSplit) {
+            LNode& a = D.nodes[slot_base(L + 1) + 2u * i];
+            LNode& b = D.nodes[slot_base(L + 1) + 2u * i + 1u];
+            const bool deep = L + 1 > kKdTopLevels;
+            const uint32_t a_size = a.pcount ? a.size : 0u;
+            if (a.pcount) {
+                a.pre = nd.pre + 1u;
+                a.lid = nd.lid;
+                a.lps = nd.lps;
+                if (deep) a.dev = nd.dev + 1u;
+            }
+            if (b.pcount) {
+                b.pre = nd.pre + 1u + a_size;
+                b.lid = nd.lid + (a.pcount ? a.nleaf : 0u);
+                b.lps = nd.lps + (a.pcount ? a.nlp : 0u);
+                if (deep) b.dev = nd.dev + 1u + a_size;
+            }
+            const uint32_t fl = HULK_NODE_JUNCTION | ((nd.flags >> 4) & 3u);
+            const uint32_t pl = __float_as_uint(nd.plane);
+            D.ref_nodes[nd.pre] = make_uint4(fl, a.pcount ? a.pre : 0u, b.pcount ? b.pre : 0u, pl);
+            D.dev_nodes[nd.dev] = make_uint4(fl, a.pcount ? a.dev : 0u, b.pcount ? b.dev : 0u, pl);
+        } else {
+            const uint4 leaf = make_uint4(HULK_NODE_LEAF, nd.lps, nd.pcount, nd.lid);
+            D.ref_nodes[nd.pre] = leaf;
+            D.dev_nodes[nd.dev] = leaf;
+        }
     }
 }
 
@@ -967,6 +999,11 @@ __global__ void __launch_bounds__(kThreads) k_kdb_leaf_copy(KdDev D) {
 uint32_t blocks_for(uint64_t n, uint32_t per) {
     return static_cast<uint32_t>(std::max<uint64_t>(1, (n + per - 1) / per));
 }
+
+// Blocks for a pass over the nodes of depth L (at most 2^L, counted on the
+// device): enough for every node of the shallow depths, a grid-stride loop
+// below.
+uint32_t node_grid(uint32_t L) { return std::min<uint32_t>(blocks_for(1ull << L, kThreads), 512u); }
 
 }  // namespace
 
@@ -1043,6 +1080,7 @@ hipError_t layout(KdBuilder* b, const Caps& cap, uint32_t n_tris, uint32_t ns, b
     D.rc = static_cast<uint8_t*>(take(C));
     D.part_p = static_cast<uint4*>(take(16 * (P / kTile + 2)));
     D.part_c = static_cast<uint4*>(take(16 * (C / kTile + 2)));
+    D.nlist = static_cast<uint32_t*>(take(4 * size_t(kSlots)));
     D.stage_poly = static_cast<uint32_t*>(take(4 * S));
     D.stage_owner = static_cast<uint32_t*>(take(4 * S));
     D.ref_nodes = static_cast<uint4*>(take(16 * size_t(kSlots)));
@@ -1132,24 +1170,22 @@ int kdb_build(KdBuilder* b, const float4* verts, uint32_t n_verts, const uint4* 
         const uint32_t grid_c = std::min<uint32_t>(kMaxGrid, blocks_for(cap.cands, kTile));
         const uint32_t grid_pc = std::min<uint32_t>(kMaxGrid, blocks_for(uint64_t(cap.polys) + cap.cands, kTile));
         for (uint32_t L = 0; L < kKdLevels; L++) {
-            const uint32_t slots = blocks_for(1ull << L, kThreads);
             hipLaunchKernelGGL(k_kdb_group_reduce, dim3(grid_c), dim3(kThreads), 0, s, D, L);
             hipLaunchKernelGGL(k_kdb_scan_parts, dim3(1), dim3(1024), 0, s, D, L, 0u);
             hipLaunchKernelGGL(k_kdb_group_apply, dim3(grid_c), dim3(kThreads), 0, s, D, L);
             hipLaunchKernelGGL(k_kdb_sah, dim3(L <= kSahLdsDepth ? std::min<uint32_t>(grid_c, 1024) : grid_c),
                                dim3(kThreads), 0, s, D, L);
-            hipLaunchKernelGGL(k_kdb_decide, dim3(slots), dim3(kThreads), 0, s, D, L);
+            hipLaunchKernelGGL(k_kdb_decide, dim3(node_grid(L)), dim3(kThreads), 0, s, D, L);
             hipLaunchKernelGGL(k_kdb_split_reduce, dim3(grid_pc), dim3(kThreads), 0, s, D, L);
             hipLaunchKernelGGL(k_kdb_scan_parts, dim3(2), dim3(1024), 0, s, D, L, 1u);
             hipLaunchKernelGGL(k_kdb_split_apply, dim3(grid_pc), dim3(kThreads), 0, s, D, L);
             hipLaunchKernelGGL(k_kdb_scatter, dim3(grid_pc * 4), dim3(kThreads), 0, s, D, L);
         }
         for (uint32_t L = kKdLevels; L-- > 0;)
-            hipLaunchKernelGGL(k_kdb_bottom_up, dim3(blocks_for(1ull << L, kThreads)), dim3(kThreads), 0, s, D, L);
+            hipLaunchKernelGGL(k_kdb_bottom_up, dim3(node_grid(L)), dim3(kThreads), 0, s, D, L);
         hipLaunchKernelGGL(k_kdb_top_order, dim3(1), dim3(1024), 0, s, D);
-        for (uint32_t L = 0; L + 1 < kKdLevels; L++)
-            hipLaunchKernelGGL(k_kdb_top_down, dim3(blocks_for(1ull << L, kThreads)), dim3(kThreads), 0, s, D, L);
-        hipLaunchKernelGGL(k_kdb_emit, dim3(blocks_for(kSlots, kThreads)), dim3(kThreads), 0, s, D);
+        for (uint32_t L = 0; L < kKdLevels; L++)
+            hipLaunchKernelGGL(k_kdb_top_down, dim3(node_grid(L)), dim3(kThreads), 0, s, D, L);
         hipLaunchKernelGGL(k_kdb_leaf_copy, dim3(std::min<uint32_t>(kMaxGrid * 4, blocks_for(cap.stage, kThreads))),
                            dim3(kThreads), 0, s, D);
         KDB_TRY(hipGetLastError());
