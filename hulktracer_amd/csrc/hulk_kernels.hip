@@ -605,13 +605,25 @@ __device__ __forceinline__ float light_intensity(const float4& col, const float4
     return intensity;
 }
 
+// Direction and distance of a point light at pos (w = 1 light) from P, with
+// light_intensity's operations; true when the surface faces it (n.L > 0),
+// the only case that needs a shadow ray.
+__device__ __forceinline__ bool point_light_dir(const float4& pos, const V3& Pp, const V3& nn, V3& L, float& dist) {
+    L = sub(v3(pos.x, pos.y, pos.z), Pp);
+    const float dd = dot(L, L);
+    dist = __builtin_sqrtf(dd);
+    L = scale(L, rcp_exact(__builtin_sqrtf(dd)));
+    return dot(nn, L) > 0.0f;
+}
+
 // Whole-frame path of one ray (render kernel): the same values, in the same
 // order, as the queued pipeline (k_gen_secondary, the shadow trace,
 // k_shade_accumulate and the mirror trace per generation, then k_pack),
 // with every intermediate kept in registers.  Generation g shades its hit:
 // c = 0.1 + sum_k color_k * max(I_k, 0), a point light's I halved when its
 // shadow ray (origin P + 0.001 L, nearest hit, stopped once t < distance)
-// is blocked; acc += c * keep, keep = 0.2 w for a mirror hit that spawns a
+// is blocked; no shadow ray where I <= 0 (the term is 0 either way, NaN
+// included); acc += c * keep, keep = 0.2 w for a mirror hit that spawns a
 // reflection (r = d - 2 (d.n) n, origin P + 0.001 r, weight 0.8 w), else w.
 template <int VARIANT, bool COUNT, int LB>
 __device__ void render_ray(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk, uint32_t stride, Ray r,
@@ -629,7 +641,9 @@ __device__ void render_ray(const TraceParams& P, const uint4* lds_nodes, uint32_
             V3 L = v3(0.0f, 0.0f, 0.0f);
             float dist = 0.0f;
             float intensity = light_intensity(col, dat, Pp, nn, L, dist);
-            if (col.w == 1.0f) {
+            // a point light the surface faces away from adds max(I, 0) = 0
+            // whether or not it is blocked: no shadow ray
+            if (col.w == 1.0f && intensity > 0.0f) {
                 Ray sr;
                 sr.o = add(Pp, scale(L, 0.001f));
                 sr.d = L;
@@ -1055,7 +1069,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
                 V3 L = v3(0.0f, 0.0f, 0.0f);
                 float dist = 0.0f;
                 const float intensity = light_intensity(col, dat, Pp, nn, L, dist);
-                if (col.w == 1.0f) {
+                if (col.w == 1.0f && intensity > 0.0f) {  // facing away: no shadow ray, term 0
                     U(kUK) = k;
                     F(kFIntensity) = intensity;
                     tr.o = add(Pp, scale(L, 0.001f));
@@ -1210,10 +1224,21 @@ __global__ void __launch_bounds__(256) k_gen_secondary(GenParams G) {
     }
     const bool hit = active && h.tri != kNoHit;
     const bool mirror = hit && G.next_rays && may_spawn && G.mirror && G.mirror[h.tri];
-    const uint64_t hit_mask = __ballot(hit);
     const uint64_t mir_mask = __ballot(mirror);
+    // shadow rays: point light k of a hit whose surface faces it (I > 0;
+    // elsewhere the light adds 0 whether blocked or not); light-major
+    // within the wave
+    const V3 P = hit ? add(r.o, scale(r.d, h.t)) : v3(0.0f, 0.0f, 0.0f);
+    const V3 nn = hit ? v3(h.nx, h.ny, h.nz) : v3(0.0f, 0.0f, 0.0f);
+    uint32_t n_shadow = 0;
+    for (uint32_t k = 0; k < G.n_point; k++) {
+        V3 L;
+        float dist;
+        const bool lit = hit && point_light_dir(G.point_pos[k], P, nn, L, dist);
+        n_shadow += (uint32_t)__builtin_popcountll(__ballot(lit));
+    }
     if (lane_id == 0) {
-        wsum[0][wave] = (uint32_t)__builtin_popcountll(hit_mask) * G.n_point;
+        wsum[0][wave] = n_shadow;
         wsum[1][wave] = (uint32_t)__builtin_popcountll(mir_mask);
     }
     __syncthreads();
@@ -1230,28 +1255,27 @@ __global__ void __launch_bounds__(256) k_gen_secondary(GenParams G) {
         shadow_base += wsum[0][k];
         mirror_base += wsum[1][k];
     }
-    const uint32_t hit_rank =
-        __builtin_amdgcn_mbcnt_hi((uint32_t)(hit_mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hit_mask, 0u));
     const uint32_t mir_rank =
         __builtin_amdgcn_mbcnt_hi((uint32_t)(mir_mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mir_mask, 0u));
-    const uint32_t n_hit = (uint32_t)__builtin_popcountll(hit_mask);
-    if (!hit) return;
-    const V3 P = add(r.o, scale(r.d, h.t));
-    const uint32_t sbase = shadow_base;
+    uint32_t j = shadow_base;
     for (uint32_t k = 0; k < G.n_point; k++) {
-        const float4 lp = G.point_pos[k];
-        const uint32_t j = sbase + k * n_hit + hit_rank;
-        V3 L = sub(v3(lp.x, lp.y, lp.z), P);
-        const float dd = dot(L, L);
-        const float dist = __builtin_sqrtf(dd);
-        L = scale(L, rcp_exact(__builtin_sqrtf(dd)));
-        hulk_ray sr;
-        const V3 so = add(P, scale(L, 0.001f));
-        sr.start = hulk_float4{so.x, so.y, so.z, dist};
-        sr.dir = hulk_float4{L.x, L.y, L.z, 0.0f};
-        G.shadow_rays[j] = sr;
-        G.shadow_slot[j] = i * G.n_point + k;
+        V3 L;
+        float dist = 0.0f;
+        const bool lit = hit && point_light_dir(G.point_pos[k], P, nn, L, dist);
+        const uint64_t m = __ballot(lit);
+        if (lit) {
+            const uint32_t rank =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            hulk_ray sr;
+            const V3 so = add(P, scale(L, 0.001f));
+            sr.start = hulk_float4{so.x, so.y, so.z, dist};
+            sr.dir = hulk_float4{L.x, L.y, L.z, 0.0f};
+            G.shadow_rays[j + rank] = sr;
+            G.shadow_slot[j + rank] = i * G.n_point + k;
+        }
+        j += (uint32_t)__builtin_popcountll(m);
     }
+    if (!hit) return;
     if (mirror) {
         const uint32_t j = mirror_base + mir_rank;
         const V3 nn = v3(h.nx, h.ny, h.nz);
@@ -1351,8 +1375,10 @@ __global__ void k_shade_accumulate(ShadeParams S) {
         float dist = 0.0f;
         float intensity = light_intensity(col, dat, P, nn, L, dist);
         if (col.w == 1.0f) {
-            const float tsh = S.shadow_t[i * S.n_point + kp];
-            if (dist > tsh) intensity *= 0.5f;
+            if (intensity > 0.0f) {  // only these had a shadow ray
+                const float tsh = S.shadow_t[i * S.n_point + kp];
+                if (dist > tsh) intensity *= 0.5f;
+            }
             kp++;
         }
         const float m = intensity > 0.0f ? intensity : 0.0f;
