@@ -314,3 +314,37 @@ def test_mt_render_parity(opts):
         gpu_ctx(s, nodes, lp, v, tri_test=2)
     with pytest.raises(H.HulkError):
         gpu_ctx(s, nodes, lp, H.RESTART, tri_test=H.TRI_MOLLER_TRUMBORE).trace_camera_streamed(cam, 64, 64)
+
+
+@pytest.mark.parametrize("pipeline", [1, 2])
+def test_shadow_rays_only_toward_lit_lights(pipeline):
+    """A point light the surface faces away from (n.L <= 0) adds 0 whether
+    it is blocked or not, so no shadow ray is traced for it: the counted
+    shadow rays equal the primary hits with n.L > 0, recomputed here with
+    the kernel's float32 operations (P = o + d t, L = (pos - P) * (1 / |pos - P|),
+    n.L = (nx Lx + ny Ly) + nz Lz), and the pixels still equal the oracle's,
+    which traces every shadow ray."""
+    s, nodes, lp = scene_and_tree(2, 1)
+    cam = s.camera(11, 1.0)
+    ctx = gpu_ctx(s, nodes, lp, H.LDSSTACK, pipeline=pipeline, lds_top_levels=8)
+    img, hits = ctx.render_frame(cam, 192, 192, count=True)
+    cnt = ctx.counters()
+    rays = O.cast_rays(cam, 192, 192)
+    f32 = np.float32
+    hit = hits["tri"] != H.NO_HIT
+    t = hits["t"].astype(f32)
+    P = [rays[:, a] + rays[:, 4 + a] * t for a in range(3)]
+    lit_total = 0
+    for light in s.lights:
+        if light[3] != 1.0:
+            continue
+        Lv = [f32(light[4 + a]) - P[a] for a in range(3)]
+        dd = (Lv[0] * Lv[0] + Lv[1] * Lv[1]) + Lv[2] * Lv[2]
+        inv = f32(1.0) / np.sqrt(dd)
+        Ln = [Lv[a] * inv for a in range(3)]
+        I = (hits["nx"] * Ln[0] + hits["ny"] * Ln[1]) + hits["nz"] * Ln[2]
+        lit_total += int(np.sum(hit & (I > 0)))
+    assert 0 < lit_total < int(hit.sum()) * sum(1 for l in s.lights if l[3] == 1.0)
+    assert cnt["shadow_rays"] == lit_total
+    rimg, _ = oracle_scene(2, 1).render(O.V_LDSSTACK, cam, 192, 192, max_bounce=0)
+    assert np.array_equal(img, rimg)
