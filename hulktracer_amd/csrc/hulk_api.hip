@@ -207,6 +207,7 @@ int trace_setup(hulk_ctx* c, const hulk_opts& o, TraceParams& P, uint32_t& block
     P.coop_lanes = o.coop_lanes;
     P.nodes = static_cast<const uint4*>(c->nodes.p);
     P.recs = static_cast<const TriRec*>(c->recs.p);
+    P.rec_bytes = c->n_lp * static_cast<uint32_t>(sizeof(TriRec));
     P.verts = static_cast<const float4*>(c->verts.p);
     P.tris = static_cast<const uint4*>(c->tris.p);
     P.n_tris = c->n_tris;
@@ -726,6 +727,8 @@ extern "C" int hulk_upload_kdtree(hulk_ctx* c, const hulk_kd_node* nodes, uint32
     if (!nodes || n_nodes == 0 || (n_lp && !leaf_polys)) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
     // nodes and rope slots (3 per node) are addressed by 32-bit byte offsets
     if (n_nodes >= (1u << 26)) return fail(c, HULK_E_ILLEGAL_INPUT, "more than 2^26 nodes");
+    // triangle records (64 B per leaf reference) are addressed by 32-bit byte offsets
+    if (n_lp >= kMaxLeafRefs) return fail(c, HULK_E_ILLEGAL_INPUT, "2^26 or more leaf polygon references");
     // Validate: children in range, every node reached once from the root
     // (no cycles / sharing), leaf ranges inside leaf_polys, references < n_tris.
     std::vector<uint8_t> seen(n_nodes, 0);
@@ -742,6 +745,8 @@ extern "C" int hulk_upload_kdtree(hulk_ctx* c, const hulk_kd_node* nodes, uint32
     uint32_t prefix[kTopLevelsMax + 1] = {0};
     auto visit_children = [&](uint32_t i, auto&& push) -> int {
         const hulk_kd_node& n = nodes[i];
+        if ((n.flags & (HULK_NODE_LEAF | HULK_NODE_JUNCTION)) == (HULK_NODE_LEAF | HULK_NODE_JUNCTION))
+            return fail(c, HULK_E_ILLEGAL_INPUT, "node is both leaf and junction");
         if (n.flags & HULK_NODE_LEAF) {
             if ((uint64_t)n.u.leaf.start + n.u.leaf.count > n_lp)
                 return fail(c, HULK_E_ILLEGAL_INPUT, "leaf range outside leaf_polys");
@@ -794,17 +799,20 @@ extern "C" int hulk_upload_kdtree(hulk_ctx* c, const hulk_kd_node* nodes, uint32
         prefix[d] = k;
     }
     if (max_jdepth > kMaxFullStack) return fail(c, HULK_E_ILLEGAL_INPUT, "tree deeper than 64 junctions");
+    // Device word x is normalised: exactly HULK_NODE_LEAF for a leaf and
+    // HULK_NODE_JUNCTION | axis for a junction (other flag bits dropped), the
+    // form the traversal kernels test with one compare.
     std::vector<uint4> dev(n_nodes);
     for (uint32_t k = 0; k < n_nodes; k++) {
         const hulk_kd_node& n = nodes[order[k]];
         if (n.flags & HULK_NODE_LEAF) {
-            dev[k] = make_uint4(n.flags, n.u.leaf.start, n.u.leaf.count, n.u.leaf.leaf_id);
+            dev[k] = make_uint4(HULK_NODE_LEAF, n.u.leaf.start, n.u.leaf.count, n.u.leaf.leaf_id);
         } else {
             uint32_t plane;
             std::memcpy(&plane, &n.u.junction.plane, 4);
             const uint32_t a = n.u.junction.above ? perm[n.u.junction.above] : 0;
             const uint32_t b = n.u.junction.below ? perm[n.u.junction.below] : 0;
-            dev[k] = make_uint4(n.flags, a, b, plane);
+            dev[k] = make_uint4(HULK_NODE_JUNCTION | (n.flags & 3u), a, b, plane);
         }
     }
     int r;
@@ -1133,6 +1141,7 @@ extern "C" int hulk_trace_camera_streamed(hulk_ctx* c, const hulk_camera* cam, u
         HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->stream_host), 64, hipHostMallocDefault));
     const uint32_t n_leaves = static_cast<uint32_t>(c->leaf_extent.size());
     const uint32_t pool = pool_records ? pool_records : std::max<uint32_t>(c->n_lp, 1);
+    if (pool >= kMaxLeafRefs) return fail(c, HULK_E_ILLEGAL_INPUT, "leaf pool of 2^26 or more records");
     if (pool < c->max_leaf) {
         c->err = "pool of " + std::to_string(pool) + " records is smaller than the largest leaf (" +
                  std::to_string(c->max_leaf) + " records)";
@@ -1167,6 +1176,7 @@ extern "C" int hulk_trace_camera_streamed(hulk_ctx* c, const hulk_camera* cam, u
     std::memset(&S, 0, sizeof(S));
     S.nodes = static_cast<const uint4*>(c->nodes.p);
     S.pool = static_cast<const TriRec*>(c->s_pool.p);
+    S.pool_bytes = pool * static_cast<uint32_t>(sizeof(TriRec));
     S.leaf_table = static_cast<const uint2*>(c->s_table.p);
     S.request = static_cast<uint32_t*>(c->s_request.p);
     S.box = c->box;
@@ -1293,6 +1303,8 @@ extern "C" int hulk_build_kdtree_device(hulk_ctx* c, uint32_t capacity) {
         return r;
     }
     if (R.max_jdepth > kMaxFullStack) return fail(c, HULK_E_ILLEGAL_INPUT, "tree deeper than 64 junctions");
+    if (R.n_nodes >= (1u << 26)) return fail(c, HULK_E_ILLEGAL_INPUT, "more than 2^26 nodes");
+    if (R.n_lp >= kMaxLeafRefs) return fail(c, HULK_E_ILLEGAL_INPUT, "2^26 or more leaf polygon references");
     // the previous tree stays installed until here
     c->have_tree = false;
     c->ropes_ready = false;

@@ -17,8 +17,12 @@
 
 namespace hulk {
 
+// Word x of a device node is exactly kNodeLeaf for a leaf and
+// kNodeJunction | axis (axis 0-2) for a junction: the host upload and the
+// device build write it so.  A junction is then x >= kNodeJunction.
 constexpr uint32_t kNodeLeaf = 4u;
 constexpr uint32_t kNodeJunction = 8u;
+__device__ __forceinline__ bool is_junction(uint32_t x) { return x >= kNodeJunction; }
 constexpr uint32_t kNoHit = 0xFFFFFFFFu;
 constexpr uint32_t kRopeNone = 0xFFFFFFFFu;
 constexpr float kFltMax = 3.402823466e+38f;
@@ -45,6 +49,15 @@ __device__ __forceinline__ V3 cross(V3 a, V3 b) {
 __device__ __forceinline__ float rcp_exact(float x) {
     const float ax = __builtin_fabsf(x);
     if (ax >= 0x1p-126f && ax < 0x1p126f) {
+        const float y0 = __builtin_amdgcn_rcpf(x);
+        return __builtin_fmaf(__builtin_fmaf(-x, y0, 1.0f), y0, y0);
+    }
+    return 1.0f / x;
+}
+// rcp_exact for an x already known not to lie below FLT_EPSILON in
+// magnitude (or NaN): only the upper end of the fast range is tested.
+__device__ __forceinline__ float rcp_exact_above_eps(float x) {
+    if (__builtin_fabsf(x) < 0x1p126f) {
         const float y0 = __builtin_amdgcn_rcpf(x);
         return __builtin_fmaf(__builtin_fmaf(-x, y0, 1.0f), y0, y0);
     }
@@ -93,7 +106,7 @@ __device__ __forceinline__ bool test_record(const Ray& r, const TriRec& rec, Bes
     const V3 n = v3(rec.a.x, rec.a.y, rec.a.z);
     float den = dot(n, r.d);
     if (__builtin_fabsf(den) < kFltEpsilon) return false;
-    den = rcp_exact(den);
+    den = rcp_exact_above_eps(den);
     const V3 p1 = v3(rec.b.x, rec.b.y, rec.b.z);
     const float t = dot(n, sub(p1, r.o)) * den;
     if (t < 0.0f) return false;

@@ -39,6 +39,11 @@ __device__ __forceinline__ uint4 load_node(const uint4* g, uint32_t num_bytes, u
     return make_uint4(v[0], v[1], v[2], v[3]);
 }
 
+__device__ __forceinline__ float4 load_f4(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+
 __device__ __forceinline__ uint4 fetch_node(const TraceParams& P, const uint4* lds, uint32_t m, uint32_t i) {
     if (i < m) return lds[i];
     return load_node(P.nodes, P.node_bytes, i);
@@ -48,14 +53,13 @@ __device__ __forceinline__ void junction(const uint4& n, const Ray& r, float& sp
                                          uint32_t& second) {
     const uint32_t axis = n.x & 3u;
     const float plane = __uint_as_float(n.w);
-    const float ro = lane(r.o, axis), rd = lane(r.d, axis), iv = lane(r.inv, axis);
+    const float ro = lane(r.o, axis), iv = lane(r.inv, axis), rd = lane(r.d, axis);
     split = (plane - ro) * iv;
-    first = n.y;
-    second = n.z;
-    if (ro < plane || (ro == plane && rd <= 0.0f)) {
-        first = n.z;
-        second = n.y;
-    }
+    // below child first when ro < plane, or ro == plane and rd <= 0
+    // (bitwise, so that the lane masks combine without a branch)
+    const bool below = (ro < plane) | ((ro == plane) & (rd <= 0.0f));
+    first = below ? n.z : n.y;
+    second = below ? n.y : n.z;
 }
 
 struct Cnt {
@@ -110,6 +114,9 @@ __device__ __forceinline__ uint32_t frame_ray(const FrameDev* F, uint32_t n, uin
     return f;
 }
 
+constexpr uint32_t kRecBytes = sizeof(TriRec);
+static_assert(kRecBytes == 64, "triangle records are 64 B");
+
 // Leaf modes (template parameter LB of the traversal kernels): one record
 // in flight, two records in flight, or one record with the Moller-Trumbore
 // test instead of the reference's.
@@ -120,8 +127,8 @@ constexpr int kLeafMt = 3;
 // fixed (best.t only decreases), so the ray stops.  Primary and mirror rays
 // pass tstop = -FLT_MAX, which never stops them.
 template <bool COUNT, int LB>
-__device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, const uint4& n, const Ray& r, Best& best,
-                                          Cnt& c, float tstop) {
+__device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, uint32_t rec_bytes, const uint4& n,
+                                          const Ray& r, Best& best, Cnt& c, float tstop) {
     const uint32_t start = n.y, end = n.y + n.z;
     if (COUNT) {
         c.leaves++;
@@ -142,12 +149,19 @@ __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, const
         }
         return;
     }
-    for (uint32_t i = start; i < end; i++) {
+    // A record array holds fewer than 2^26 records (the upload, the device
+    // build and the leaf pool refuse more), so buffer loads at a 32-bit byte
+    // offset reach every record: one add per record, the four 16-B parts at
+    // immediate offsets.  Past rec_bytes the loads return zeros (normal 0:
+    // no hit).
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<TriRec*>(recs), (short)0, (int)rec_bytes, 0x00020000);
+    for (uint32_t off = start * kRecBytes, off_end = end * kRecBytes; off < off_end; off += kRecBytes) {
         TriRec rec;
-        rec.a = recs[i].a;
-        rec.b = recs[i].b;
-        rec.c = recs[i].c;
-        rec.d = recs[i].d;
+        rec.a = load_f4(rs, off);
+        rec.b = load_f4(rs, off + 16u);
+        rec.c = load_f4(rs, off + 32u);
+        rec.d = load_f4(rs, off + 48u);
         if (test_any<LB == kLeafMt>(r, rec, best) && best.t < tstop) break;
     }
 }
@@ -172,7 +186,7 @@ __device__ void trav_fullstack(const TraceParams& P, const uint4* lds_nodes, uin
         const uint4 n = fetch_node(P, lds_nodes, P.lds_node_count, node);
         if (COUNT) c.nodes++;
         if (n.x & kNodeLeaf) {
-            test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
+            test_leaf<COUNT, LB>(P.recs, P.rec_bytes, n, r, best, c, tstop);
             if (best.t < tstop) break;
             node = 0;
         } else {
@@ -222,7 +236,10 @@ __device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, ui
     if (!ray_box(P.box, r, ts, te)) return;
     if (tclip < te) te = tclip;
     const int depth = (int)P.stack_depth;
-    int size = 0, idx = 0;
+    // entry idx starts at word off = 3 idx stride: kept premultiplied, so
+    // that a push or pop is an add and a compare
+    const int step = 3 * (int)stride, wrap = depth * step;
+    int size = 0, off = 0;
     uint32_t node = 0, guard = 0;
     float tmax = ts, tmin = ts;
     while (tmax < te) {
@@ -232,15 +249,15 @@ __device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, ui
             node = 0;
         } else {
             --size;
-            --idx;
-            if (idx == -1) idx += depth;
-            tmin = __uint_as_float(stk[(3 * idx) * stride]);
-            tmax = __uint_as_float(stk[(3 * idx + 1) * stride]);
-            node = stk[(3 * idx + 2) * stride];
+            off -= step;
+            if (off < 0) off += wrap;
+            tmin = __uint_as_float(stk[off]);
+            tmax = __uint_as_float(stk[off + stride]);
+            node = stk[off + 2 * stride];
         }
         uint4 n = fetch_node(P, lds_nodes, P.lds_node_count, node);
         if (COUNT) c.nodes++;
-        while (n.x & kNodeJunction) {
+        while (is_junction(n.x)) {
             float split;
             uint32_t first, second;
             junction(n, r, split, first, second);
@@ -249,10 +266,10 @@ __device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, ui
             } else if (split <= tmin) {
                 node = second;
             } else {
-                stk[(3 * idx) * stride] = __float_as_uint(split);
-                stk[(3 * idx + 1) * stride] = __float_as_uint(tmax);
-                stk[(3 * idx + 2) * stride] = second;
-                idx = idx + 1 == depth ? 0 : idx + 1;
+                stk[off] = __float_as_uint(split);
+                stk[off + stride] = __float_as_uint(tmax);
+                stk[off + 2 * stride] = second;
+                off = off + step == wrap ? 0 : off + step;
                 size = size + 1 > depth ? depth : size + 1;
                 node = first;
                 tmax = split;
@@ -262,7 +279,7 @@ __device__ void trav_shortstack(const TraceParams& P, const uint4* lds_nodes, ui
             if (node == 0) break;
         }
         if (n.x & kNodeLeaf) {
-            test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
+            test_leaf<COUNT, LB>(P.recs, P.rec_bytes, n, r, best, c, tstop);
             if (best.t < tmax || best.t < tstop) break;
         }
         if (++guard > P.guard) {
@@ -300,7 +317,7 @@ __device__ void trav_shortstack_private(const TraceParams& P, const uint4* lds_n
         }
         uint4 n = fetch_node(P, lds_nodes, P.lds_node_count, node);
         if (COUNT) c.nodes++;
-        while (n.x & kNodeJunction) {
+        while (is_junction(n.x)) {
             float split;
             uint32_t first, second;
             junction(n, r, split, first, second);
@@ -322,7 +339,7 @@ __device__ void trav_shortstack_private(const TraceParams& P, const uint4* lds_n
             if (node == 0) break;
         }
         if (n.x & kNodeLeaf) {
-            test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
+            test_leaf<COUNT, LB>(P.recs, P.rec_bytes, n, r, best, c, tstop);
             if (best.t < tmax || best.t < tstop) break;
         }
         if (++guard > P.guard) {
@@ -351,7 +368,7 @@ __device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const
         uint32_t node = 0;
         uint4 n = fetch_node(P, lds_nodes, P.lds_node_count, 0);
         if (COUNT) c.nodes++;
-        while (n.x & kNodeJunction) {
+        while (is_junction(n.x)) {
             float split;
             uint32_t first, second;
             junction(n, r, split, first, second);
@@ -368,7 +385,7 @@ __device__ void trav_restart(const TraceParams& P, const uint4* lds_nodes, const
             if (node == 0) break;
         }
         if (n.x & kNodeLeaf) {
-            test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
+            test_leaf<COUNT, LB>(P.recs, P.rec_bytes, n, r, best, c, tstop);
             if (best.t < tstop) break;
         }
         if (++guard > P.guard) {
@@ -409,7 +426,7 @@ template <bool COUNT>
 __device__ __forceinline__ uint4 ropes_descend(const TraceParams& P, const Ray& r, RopeState& s, Cnt& c) {
     uint4 n = load_node(P.rope_nodes, P.rope_node_bytes, s.slot);
     if (COUNT) c.nodes++;
-    while (n.x & kNodeJunction) {
+    while (is_junction(n.x)) {
         float split;
         uint32_t first, second;
         junction(n, r, split, first, second);
@@ -470,7 +487,7 @@ template <bool COUNT, int LB>
 __device__ __forceinline__ bool ropes_step(const TraceParams& P, const Ray& r, RopeState& s, float tstop, Best& best,
                                            Cnt& c, uint32_t& flags) {
     const uint4 n = ropes_descend<COUNT>(P, r, s, c);
-    test_leaf<COUNT, LB>(P.recs, n, r, best, c, tstop);
+    test_leaf<COUNT, LB>(P.recs, P.rec_bytes, n, r, best, c, tstop);
     return ropes_exit(P, r, s, tstop, best, flags);
 }
 
@@ -699,8 +716,12 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
 // runs its whole shading path (render_ray) and writes its packed pixel.
 // Two records in flight (LB = 2) want about 100 VGPRs; a cap of 5 waves per
 // SIMD (96 VGPRs, a few bytes of spill) measured fastest for this kernel.
+// The LDS short stack with one record in flight takes 86 VGPRs (5 waves);
+// at 6 waves (80 VGPRs, a few spills outside the traversal loops) its
+// 10-frame terrain batch runs 0.6-1.4 % faster.
 template <int VARIANT, bool COUNT, bool RENDER, int LB>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(LB == 2 ? 5 : 1)))
+__global__ void __launch_bounds__(1024)
+    __attribute__((amdgpu_waves_per_eu(LB == 2 ? 5 : (VARIANT == HULK_VARIANT_LDSSTACK ? 6 : 1))))
 k_trace(TraceParams P) {
     extern __shared__ uint4 smem[];
     const uint32_t tid = threadIdx.x;
@@ -986,7 +1007,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             if (at_leaf && (uint32_t)__builtin_popcountll(at_leaf) <= P.coop_lanes)
                 coop_leaves<LB == kLeafMt>(P.recs, ln, tr, bt, tstop, at_leaf, lane_id);
             else if (walk)
-                test_leaf<COUNT, LB>(P.recs, ln, tr, bt, c, tstop);
+                test_leaf<COUNT, LB>(P.recs, P.rec_bytes, ln, tr, bt, c, tstop);
             if (busy) done = !walk || !ropes_exit(P, tr, st, tstop, bt, tfl);
         }
         // ---- state transitions of lanes whose traversal is over (a new
@@ -1727,7 +1748,7 @@ __global__ void __launch_bounds__(256) k_trace_streamed(StreamParams S) {
                 const uint4 n = S.nodes[pending];
                 const uint2 lt = S.leaf_table[n.w];
                 if (lt.y) {
-                    test_leaf<false, 1>(S.pool, make_uint4(n.x, lt.x, n.z, n.w), r, best, c, -kFltMax);
+                    test_leaf<false, 1>(S.pool, S.pool_bytes, make_uint4(n.x, lt.x, n.z, n.w), r, best, c, -kFltMax);
                     pending = 0;
                 } else {
                     S.request[n.w] = 1u;
@@ -1739,7 +1760,7 @@ __global__ void __launch_bounds__(256) k_trace_streamed(StreamParams S) {
                 tmax = te;
                 uint32_t node = 0;
                 uint4 n = S.nodes[0];
-                while (n.x & kNodeJunction) {
+                while (is_junction(n.x)) {
                     float split;
                     uint32_t first, second;
                     junction(n, r, split, first, second);
@@ -1757,7 +1778,7 @@ __global__ void __launch_bounds__(256) k_trace_streamed(StreamParams S) {
                 if (n.x & kNodeLeaf) {
                     const uint2 lt = S.leaf_table[n.w];
                     if (lt.y) {
-                        test_leaf<false, 1>(S.pool, make_uint4(n.x, lt.x, n.z, n.w), r, best, c, -kFltMax);
+                        test_leaf<false, 1>(S.pool, S.pool_bytes, make_uint4(n.x, lt.x, n.z, n.w), r, best, c, -kFltMax);
                     } else {
                         S.request[n.w] = 1u;
                         pending = node;

@@ -12,6 +12,9 @@ namespace hulk {
 
 constexpr uint32_t kMaxShortStack = 16;
 constexpr uint32_t kMaxFullStack = 64;
+// Leaf references (64-B triangle records) per tree or leaf pool: records are
+// addressed by 32-bit byte offsets.
+constexpr uint32_t kMaxLeafRefs = 1u << 26;
 constexpr uint32_t kSourceCamera = 0;
 constexpr uint32_t kSourceRays = 1;
 constexpr uint32_t kSourceFrames = 2;  // batched primary rays of a device frame table
@@ -40,6 +43,7 @@ struct TraceParams {
     uint32_t lds_node_count;
     uint32_t stack_depth;
     const TriRec* recs;
+    uint32_t rec_bytes;        // 64 * leaf reference count (range of the record buffer loads)
     const float4* verts;
     const uint4* tris;
     uint32_t n_tris;
@@ -137,6 +141,7 @@ struct ShadeParams {
 struct StreamParams {
     const uint4* nodes;
     const TriRec* pool;
+    uint32_t pool_bytes;      // 64 * pool records
     const uint2* leaf_table;  // per leaf id: (pool base, resident)
     uint32_t* request;        // per leaf id: set to 1 by a lane that needs the leaf
     SceneBox box;

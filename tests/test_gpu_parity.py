@@ -348,3 +348,25 @@ def test_shadow_rays_only_toward_lit_lights(pipeline):
     assert cnt["shadow_rays"] == lit_total
     rimg, _ = oracle_scene(2, 1).render(O.V_LDSSTACK, cam, 192, 192, max_bounce=0)
     assert np.array_equal(img, rimg)
+
+
+def test_node_flag_bits():
+    """Bits of a node's flags word above the axis, leaf and junction bits
+    change no hit (the device copy keeps only those), and a node flagged both
+    leaf and junction, which the reference's kernels read differently
+    (GPUTrace.cl:34 tests the leaf bit first, GPUTrace_shortStack.cl:51 the
+    junction bit), is refused."""
+    s, nodes, lp = scene_and_tree(1, 3, 300)
+    rays = random_rays(5000, s.box, seed=7)
+    high = nodes.copy()
+    high[:, 0] |= np.uint32(0xFFF0)
+    for v in VARIANTS:
+        ref = gpu_ctx(s, nodes, lp, v).trace_rays(rays)
+        got = gpu_ctx(s, high, lp, v).trace_rays(rays)
+        _check(got, ref, f"variant {v}, high flag bits")
+    node_leaf, node_junction = 4, 8  # HULK_NODE_LEAF, HULK_NODE_JUNCTION
+    both = nodes.copy()
+    leaf = int(np.nonzero(nodes[:, 0] & node_leaf)[0][0])
+    both[leaf, 0] |= np.uint32(node_junction)
+    with pytest.raises(H.HulkError, match="both leaf and junction"):
+        gpu_ctx(s, both, lp, H.LDSSTACK)
