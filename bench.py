@@ -52,7 +52,7 @@ def kernel_name(opts: dict) -> str:
     if int(opts.get("pipeline", 0)) == 1:
         return "queued pipeline graph"
     if int(opts.get("pipeline", 0)) == 3:
-        return f"k_render_refill<false,{lb}>"
+        return f"k_render_refill<{int(opts['variant'])},false,{lb}>"
     return f"k_trace<{int(opts['variant'])},false,true,{lb}>"
 
 

@@ -153,8 +153,10 @@ int check_opts(hulk_ctx* c, const hulk_opts& o) {
         return fail(c, HULK_E_ILLEGAL_INPUT, "short_stack_depth must be in [1, 16]");
     if (o.lds_top_levels > kTopLevelsMax) return fail(c, HULK_E_ILLEGAL_INPUT, "lds_top_levels must be <= 12");
     if (o.pipeline > HULK_PIPELINE_REFILL) return fail(c, HULK_E_ILLEGAL_INPUT, "unknown pipeline");
-    if (o.pipeline == HULK_PIPELINE_REFILL && (o.variant != HULK_VARIANT_ROPES || o.block_size > 256))
-        return fail(c, HULK_E_ILLEGAL_INPUT, "the refill pipeline needs the ropes variant and blocks of <= 256");
+    if (o.pipeline == HULK_PIPELINE_REFILL &&
+        ((o.variant != HULK_VARIANT_ROPES && o.variant != HULK_VARIANT_LDSSTACK) || o.block_size > 256))
+        return fail(c, HULK_E_ILLEGAL_INPUT,
+                    "the refill pipeline needs the ropes or LDS short-stack variant and blocks of <= 256");
     if (o.leaf_batch > 2) return fail(c, HULK_E_ILLEGAL_INPUT, "leaf_batch must be 0, 1 or 2");
     if (o.coop_lanes > 64) return fail(c, HULK_E_ILLEGAL_INPUT, "coop_lanes must be in [0, 64]");
     if (o.tri_test > HULK_TRI_MOLLER_TRUMBORE) return fail(c, HULK_E_ILLEGAL_INPUT, "unknown tri_test");
@@ -376,6 +378,8 @@ int render_refill(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, 
     size_t lds;
     int r = trace_setup(c, o, P, block, lds);
     if (r) return r;
+    if (refill_lds_bytes(lds, block) > c->max_lds)
+        return fail(c, HULK_E_ILLEGAL_INPUT, "refill LDS (lane fields + short stack + top levels) exceeds the block limit");
     if (fs.n == 0) return HULK_OK;
     if (fs.frames) {
         P.source = kSourceFrames;
@@ -403,7 +407,7 @@ int render_refill(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, 
     const uint32_t grid =
         std::max<uint32_t>(1u, std::min<uint32_t>((fs.n + block - 1) / block, c->num_cus * per_cu));
     if (timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-    HIP_TRY(c, launch_render_refill(P, count, grid, block, c->stream));
+    HIP_TRY(c, launch_render_refill(o.variant, P, count, grid, block, lds, c->stream));
     if (timed) HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return HULK_OK;
 }
@@ -1597,6 +1601,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
         uint32_t block;
         size_t lds;
         if (check_opts(c, o) || trace_setup(c, o, P, block, lds)) return 1;  // shape not allowed: skip
+        if (o.pipeline == HULK_PIPELINE_REFILL && refill_lds_bytes(lds, block) > c->max_lds) return 1;
         auto once = [&]() -> int {
             if (!frame) return trace_primary(c, o, fs, c->work, nullptr, out, false, false);
             if (use_fused(o, bounces))
@@ -1660,7 +1665,7 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
     size_t bi = 0;
     for (size_t i = 1; i < done.size(); i++)
         if (done[i].second < done[bi].second) bi = i;
-    if (frame) {  // stage 3: the other frame pipeline on the best shape, and lane refill (ropes)
+    if (frame) {  // stage 3: the other frame pipeline on the best shape, and lane refill (ropes, LDS short stack)
         Cand k = done[bi].first;
         k.pipeline = k.pipeline == HULK_PIPELINE_FUSED ? HULK_PIPELINE_QUEUED : HULK_PIPELINE_FUSED;
         float ms;
@@ -1675,6 +1680,20 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
                         r = measure(kr, ms, reps, true);
                         if (r > 1) return r;
                     }
+        if (variant_mask & (1u << HULK_VARIANT_LDSSTACK)) {
+            // stack depth and top levels of the best shape when it is the LDS short stack
+            const Cand& bk = done[bi].first;
+            const bool lds_best = bk.variant == HULK_VARIANT_LDSSTACK;
+            const uint32_t depth = lds_best ? bk.depth : 6u, top = lds_best ? bk.lds : 6u;
+            const uint32_t coops_s[] = {0, 16};
+            for (uint32_t bs : blocks)
+                for (uint32_t lb = 1; lb <= 2; lb++)
+                    for (uint32_t cl : coops_s) {
+                        Cand kr{HULK_VARIANT_LDSSTACK, bs, depth, top, 1, HULK_PIPELINE_REFILL, lb, cl};
+                        r = measure(kr, ms, reps, true);
+                        if (r > 1) return r;
+                    }
+        }
     }
     // stage 4: the four fastest again, with four times the repetitions, so
     // that timing noise between near-equal shapes does not decide

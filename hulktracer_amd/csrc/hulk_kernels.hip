@@ -491,6 +491,108 @@ __device__ __forceinline__ bool ropes_step(const TraceParams& P, const Ray& r, R
     return ropes_exit(P, r, s, tstop, best, flags);
 }
 
+// The LDS short stack of trav_shortstack split at its leaves, so that a
+// lane-refill loop can interleave it with shading: ss_begin enters the
+// scene box, ss_descend runs one pass of the outer loop up to its leaf (a
+// zero node when the pass ends without one), ss_exit applies the loop's
+// tests after the leaf.  The state below is the loop's locals.
+struct SsState {
+    float tmin, tmax, te;
+    int size, off;
+    uint32_t guard;
+};
+
+__device__ __forceinline__ bool ss_begin(const TraceParams& P, const Ray& r, float tclip, SsState& s) {
+    float ts = 0.0f, te = kFltMax;
+    if (!ray_box(P.box, r, ts, te)) return false;
+    if (tclip < te) te = tclip;
+    s.tmin = s.tmax = ts;
+    s.te = te;
+    s.size = s.off = 0;
+    s.guard = 0;
+    return s.tmax < s.te;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ uint4 ss_descend(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk,
+                                            uint32_t stride, const Ray& r, SsState& s, Cnt& c) {
+    const int step = 3 * (int)stride, wrap = (int)P.stack_depth * step;
+    uint32_t node;
+    if (s.size == 0) {
+        s.tmin = s.tmax;
+        s.tmax = s.te;
+        node = 0;
+    } else {
+        --s.size;
+        s.off -= step;
+        if (s.off < 0) s.off += wrap;
+        s.tmin = __uint_as_float(stk[s.off]);
+        s.tmax = __uint_as_float(stk[s.off + stride]);
+        node = stk[s.off + 2 * stride];
+    }
+    uint4 n = fetch_node(P, lds_nodes, P.lds_node_count, node);
+    if (COUNT) c.nodes++;
+    while (is_junction(n.x)) {
+        float split;
+        uint32_t first, second;
+        junction(n, r, split, first, second);
+        if (split >= s.tmax || split <= 0.0f) {
+            node = first;
+        } else if (split <= s.tmin) {
+            node = second;
+        } else {
+            stk[s.off] = __float_as_uint(split);
+            stk[s.off + stride] = __float_as_uint(s.tmax);
+            stk[s.off + 2 * stride] = second;
+            s.off = s.off + step == wrap ? 0 : s.off + step;
+            s.size = s.size + 1 > (int)P.stack_depth ? (int)P.stack_depth : s.size + 1;
+            node = first;
+            s.tmax = split;
+        }
+        n = fetch_node(P, lds_nodes, P.lds_node_count, node);
+        if (COUNT) c.nodes++;
+        if (node == 0) break;
+    }
+    return (n.x & kNodeLeaf) ? n : make_uint4(0u, 0u, 0u, 0u);
+}
+
+__device__ __forceinline__ bool ss_exit(const TraceParams& P, SsState& s, const uint4& leaf, float tstop,
+                                        const Best& best, uint32_t& flags) {
+    if ((leaf.x & kNodeLeaf) && (best.t < s.tmax || best.t < tstop)) return false;
+    if (++s.guard > P.guard) {
+        flags |= kHitFlagGuard;
+        return false;
+    }
+    return s.tmax < s.te;
+}
+
+// Traversal of the refill kernel: ropes (TV = HULK_VARIANT_ROPES) or the
+// LDS short stack (TV = HULK_VARIANT_LDSSTACK).
+struct TravState {
+    RopeState rope;
+    SsState ss;
+};
+
+template <int TV>
+__device__ __forceinline__ bool trav_begin(const TraceParams& P, const Ray& r, float tclip, TravState& s) {
+    if (TV == HULK_VARIANT_LDSSTACK) return ss_begin(P, r, tclip, s.ss);
+    return ropes_begin(P, r, tclip, s.rope);
+}
+
+template <int TV, bool COUNT>
+__device__ __forceinline__ uint4 trav_descend(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk,
+                                              uint32_t stride, const Ray& r, TravState& s, Cnt& c) {
+    if (TV == HULK_VARIANT_LDSSTACK) return ss_descend<COUNT>(P, lds_nodes, stk, stride, r, s.ss, c);
+    return ropes_descend<COUNT>(P, r, s.rope, c);
+}
+
+template <int TV>
+__device__ __forceinline__ bool trav_exit(const TraceParams& P, const Ray& r, TravState& s, const uint4& leaf,
+                                          float tstop, const Best& best, uint32_t& flags) {
+    if (TV == HULK_VARIANT_LDSSTACK) return ss_exit(P, s.ss, leaf, tstop, best, flags);
+    return ropes_exit(P, r, s.rope, tstop, best, flags);
+}
+
 // The leaves of the lanes in `want` (a wave-uniform ballot; every lane of
 // the wave executes this call) tested by the whole wave, one owner lane at a
 // time, 64 records per pass: lane k tests record k of the pass against the
@@ -910,11 +1012,18 @@ enum LaneField : uint32_t {
 constexpr uint32_t kRefillLdsWords = kLaneFields;
 
 // No wave cap: the refill loop's lane state measured fastest unspilled.
-template <bool COUNT, int LB>
+// LDS: the top tree levels (LDS short stack), the lane fields, then the
+// short-stack columns.
+template <int TV, bool COUNT, int LB>
 __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
-    extern __shared__ uint32_t lane_mem[];
+    extern __shared__ uint4 smem[];
     const uint32_t tid = threadIdx.x, block = blockDim.x;
     const uint32_t lane_id = tid & 63u;
+    uint4* lds_nodes = smem;
+    uint32_t* lane_mem = reinterpret_cast<uint32_t*>(smem + P.lds_node_count);
+    uint32_t* stk = lane_mem + kLaneFields * block + tid;
+    for (uint32_t i = tid; i < P.lds_node_count; i += block) lds_nodes[i] = P.nodes[i];
+    if (P.lds_node_count) __syncthreads();
     auto F = [&](uint32_t f) -> float& { return reinterpret_cast<float*>(lane_mem)[f * block + tid]; };
     auto U = [&](uint32_t f) -> uint32_t& { return lane_mem[f * block + tid]; };
     const uint32_t n_rays = P.n;
@@ -930,11 +1039,14 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
     bool clipped = false, active = false;  // active: a traversal is in progress
     Ray tr;
     Best bt;
-    RopeState st;
+    TravState st;
     Cnt c = {0, 0, 0};
     best_init(bt);
-    st.slot = st.guard = 0;
-    st.t = st.te = 0.0f;
+    st.rope.slot = st.rope.guard = 0;
+    st.rope.t = st.rope.te = 0.0f;
+    st.ss.tmin = st.ss.tmax = st.ss.te = 0.0f;
+    st.ss.size = st.ss.off = 0;
+    st.ss.guard = 0;
     tr.o = tr.d = tr.inv = v3(0.0f, 0.0f, 0.0f);
 
     for (;;) {
@@ -983,7 +1095,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
                 phase = kLanePrimary;
                 tstop = -kFltMax;
                 tfl = 0;
-                active = ropes_begin(P, tr, clipped ? tclip : kFltMax, st);
+                active = trav_begin<TV>(P, tr, clipped ? tclip : kFltMax, st);
             }
         }
         if (__ballot(phase != kLaneIdle) == 0) {
@@ -998,17 +1110,24 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
         const bool busy = phase != kLaneIdle;
         bool done = false;
         if (COUNT || P.coop_lanes == 0) {
-            if (busy) done = !active || !ropes_step<COUNT, LB>(P, tr, st, tstop, bt, c, tfl);
+            if (busy) {
+                done = !active;
+                if (active) {
+                    const uint4 ln = trav_descend<TV, COUNT>(P, lds_nodes, stk, block, tr, st, c);
+                    if (ln.x & kNodeLeaf) test_leaf<COUNT, LB>(P.recs, P.rec_bytes, ln, tr, bt, c, tstop);
+                    done = !trav_exit<TV>(P, tr, st, ln, tstop, bt, tfl);
+                }
+            }
         } else {
             const bool walk = busy && active;
             uint4 ln = make_uint4(0u, 0u, 0u, 0u);
-            if (walk) ln = ropes_descend<COUNT>(P, tr, st, c);
+            if (walk) ln = trav_descend<TV, COUNT>(P, lds_nodes, stk, block, tr, st, c);
             const uint64_t at_leaf = __ballot(walk && ln.z != 0u);
             if (at_leaf && (uint32_t)__builtin_popcountll(at_leaf) <= P.coop_lanes)
                 coop_leaves<LB == kLeafMt>(P.recs, ln, tr, bt, tstop, at_leaf, lane_id);
-            else if (walk)
+            else if (walk && (ln.x & kNodeLeaf))
                 test_leaf<COUNT, LB>(P.recs, P.rec_bytes, ln, tr, bt, c, tstop);
-            if (busy) done = !walk || !ropes_exit(P, tr, st, tstop, bt, tfl);
+            if (busy) done = !walk || !trav_exit<TV>(P, tr, st, ln, tstop, bt, tfl);
         }
         // ---- state transitions of lanes whose traversal is over (a new
         // traversal may end at once when it misses the box: loop)
@@ -1021,7 +1140,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
                 if (COUNT) cfb++;
                 phase = kLaneFallback;
                 tfl = 0;
-                active = ropes_begin(P, tr, kFltMax, st);
+                active = trav_begin<TV>(P, tr, kFltMax, st);
                 done = !active;
                 continue;
             }
@@ -1100,7 +1219,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
                     tfl = 0;
                     tstop = dist;
                     phase = kLaneShadow;
-                    active = ropes_begin(P, tr, kFltMax, st);
+                    active = trav_begin<TV>(P, tr, kFltMax, st);
                     done = !active;
                     traced = true;
                     break;
@@ -1137,7 +1256,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             tfl = 0;
             tstop = -kFltMax;
             phase = kLaneMirror;
-            active = ropes_begin(P, tr, kFltMax, st);
+            active = trav_begin<TV>(P, tr, kFltMax, st);
             done = !active;
         }
     }
@@ -1160,18 +1279,28 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
     }
 }
 
-hipError_t launch_render_refill(const TraceParams& P, bool count, uint32_t grid, uint32_t block, hipStream_t s) {
-    const size_t lds = (size_t)kRefillLdsWords * block * 4;
+template <int TV>
+static void launch_refill_tv(const TraceParams& P, bool count, uint32_t grid, uint32_t block, size_t lds,
+                             hipStream_t s) {
     if (P.leaf_batch == 2) {
-        if (count) hipLaunchKernelGGL((k_render_refill<true, 2>), dim3(grid), dim3(block), lds, s, P);
-        else hipLaunchKernelGGL((k_render_refill<false, 2>), dim3(grid), dim3(block), lds, s, P);
+        if (count) hipLaunchKernelGGL((k_render_refill<TV, true, 2>), dim3(grid), dim3(block), lds, s, P);
+        else hipLaunchKernelGGL((k_render_refill<TV, false, 2>), dim3(grid), dim3(block), lds, s, P);
     } else if (P.leaf_batch == kLeafMt) {
-        if (count) hipLaunchKernelGGL((k_render_refill<true, kLeafMt>), dim3(grid), dim3(block), lds, s, P);
-        else hipLaunchKernelGGL((k_render_refill<false, kLeafMt>), dim3(grid), dim3(block), lds, s, P);
+        if (count) hipLaunchKernelGGL((k_render_refill<TV, true, kLeafMt>), dim3(grid), dim3(block), lds, s, P);
+        else hipLaunchKernelGGL((k_render_refill<TV, false, kLeafMt>), dim3(grid), dim3(block), lds, s, P);
     } else {
-        if (count) hipLaunchKernelGGL((k_render_refill<true, 1>), dim3(grid), dim3(block), lds, s, P);
-        else hipLaunchKernelGGL((k_render_refill<false, 1>), dim3(grid), dim3(block), lds, s, P);
+        if (count) hipLaunchKernelGGL((k_render_refill<TV, true, 1>), dim3(grid), dim3(block), lds, s, P);
+        else hipLaunchKernelGGL((k_render_refill<TV, false, 1>), dim3(grid), dim3(block), lds, s, P);
     }
+}
+
+size_t refill_lds_bytes(size_t trace_lds, uint32_t block) { return trace_lds + (size_t)kRefillLdsWords * block * 4; }
+
+hipError_t launch_render_refill(int variant, const TraceParams& P, bool count, uint32_t grid, uint32_t block,
+                                size_t trace_lds, hipStream_t s) {
+    const size_t lds = refill_lds_bytes(trace_lds, block);
+    if (variant == HULK_VARIANT_LDSSTACK) launch_refill_tv<HULK_VARIANT_LDSSTACK>(P, count, grid, block, lds, s);
+    else launch_refill_tv<HULK_VARIANT_ROPES>(P, count, grid, block, lds, s);
     return hipGetLastError();
 }
 

@@ -172,7 +172,11 @@ hipError_t launch_camera_rays(const Camera& cam, uint32_t W, uint32_t H, const S
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 hipError_t launch_selftest_rcp(unsigned long long* out, hipStream_t s);  // out: 4 counters
 // whole frames, rope traversal, lanes refilled from the ray queue (work_counter: 8 slices)
-hipError_t launch_render_refill(const TraceParams& P, bool count, uint32_t grid, uint32_t block, hipStream_t s);
+// variant: HULK_VARIANT_ROPES or HULK_VARIANT_LDSSTACK; trace_lds: the LDS of
+// the traversal (top tree levels, short-stack columns) before the lane fields
+size_t refill_lds_bytes(size_t trace_lds, uint32_t block);
+hipError_t launch_render_refill(int variant, const TraceParams& P, bool count, uint32_t grid, uint32_t block,
+                                size_t trace_lds, hipStream_t s);
 // rope slots (3 n slots: nodes, then two empty-leaf slots per junction)
 hipError_t launch_trace_streamed(const StreamParams& S, hipStream_t s);
 hipError_t launch_collect_requests(uint32_t* request, uint32_t n, uint32_t* list, uint32_t* n_list, hipStream_t s);

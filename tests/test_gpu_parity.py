@@ -114,7 +114,10 @@ def test_shade_parity(variant, mode):
                                   dict(variant=H.ROPES, pipeline=3, block_size=128),
                                   dict(variant=H.ROPES, pipeline=3, block_size=64, leaf_batch=2),
                                   dict(variant=H.ROPES, pipeline=3, block_size=128, coop_lanes=64),
-                                  dict(variant=H.ROPES, pipeline=3, block_size=256, leaf_batch=2, coop_lanes=8)])
+                                  dict(variant=H.ROPES, pipeline=3, block_size=256, leaf_batch=2, coop_lanes=8),
+                                  dict(variant=H.LDSSTACK, pipeline=3, block_size=128, lds_top_levels=8),
+                                  dict(variant=H.LDSSTACK, pipeline=3, block_size=64, leaf_batch=2, coop_lanes=16,
+                                       short_stack_depth=2)])
 def test_render_bounces_parity(opts):
     """Animated tori (5 meshes, 100,002 triangles): primary + shadow + 2
     mirror generations with per-generation compaction; persistent waves
@@ -133,7 +136,7 @@ def test_render_bounces_parity(opts):
 
 
 @pytest.mark.parametrize("variant,pipeline", [(H.FULLSTACK, 2), (H.LDSSTACK, 2), (H.RESTART, 2), (H.ROPES, 2),
-                                              (H.ROPES, 3)])
+                                              (H.ROPES, 3), (H.LDSSTACK, 3)])
 def test_fused_pipeline_equals_queued(variant, pipeline):
     """One fused launch per frame (2), or one with lane refill (3), against
     the queued launches: pixels, primary hit records and every counter
@@ -146,8 +149,7 @@ def test_fused_pipeline_equals_queued(variant, pipeline):
     tb = np.where(plain["tri"] != H.NO_HIT, plain["t"] * np.float32(1.3), np.float32(0.5)).astype(np.float32)
     out = {}
     for pipe in (1, pipeline):
-        ctx = gpu_ctx(s, nodes, lp, variant, pipeline=pipe, lds_top_levels=0 if pipe == 3 else 6,
-                      block_size=128)
+        ctx = gpu_ctx(s, nodes, lp, variant, pipeline=pipe, lds_top_levels=6, block_size=128)
         img, hits = ctx.render_frame(cam, W, H_, max_bounce=2, mirror=s.mirror, tbound=tb, count=True)
         out[pipe] = (img, hits, ctx.counters())
     (i1, h1, c1), (i2, h2, c2) = out[1], out[pipeline]
@@ -160,10 +162,11 @@ def test_fused_pipeline_equals_queued(variant, pipeline):
 
 @pytest.mark.parametrize("coop", [1, 8, 64])
 def test_refill_cooperative_leaves(coop):
-    """Lane refill with the wave testing the leaves of its few busy lanes
-    together (coop_lanes = 64: on every step): pixels and primary hit
-    records equal the queued pipeline's, with mirror generations and a
-    bound on the primary rays, at two block sizes and both record batches."""
+    """Lane refill (ropes and LDS short stack) with the wave testing the
+    leaves of its few busy lanes together (coop_lanes = 64: on every step):
+    pixels and primary hit records equal the queued pipeline's, with mirror
+    generations and a bound on the primary rays, at two block sizes and both
+    record batches."""
     s, nodes, lp = scene_and_tree(3, 6, 5, 4)
     cam = s.camera(4, 1.0)
     W = H_ = 128
@@ -171,11 +174,12 @@ def test_refill_cooperative_leaves(coop):
     tb = np.where(plain["tri"] != H.NO_HIT, plain["t"] * np.float32(0.9), np.float32(0.5)).astype(np.float32)
     ref = gpu_ctx(s, nodes, lp, H.ROPES, pipeline=1).render_frame(cam, W, H_, max_bounce=2, mirror=s.mirror,
                                                                   tbound=tb)
-    for block, lb in ((64, 1), (256, 2)):
-        ctx = gpu_ctx(s, nodes, lp, H.ROPES, pipeline=3, block_size=block, leaf_batch=lb, coop_lanes=coop)
+    for v, block, lb in ((H.ROPES, 64, 1), (H.ROPES, 256, 2), (H.LDSSTACK, 128, 1), (H.LDSSTACK, 256, 2)):
+        ctx = gpu_ctx(s, nodes, lp, v, pipeline=3, block_size=block, leaf_batch=lb, coop_lanes=coop,
+                      lds_top_levels=6)
         img, hits = ctx.render_frame(cam, W, H_, max_bounce=2, mirror=s.mirror, tbound=tb)
-        assert np.array_equal(img, ref[0]), f"block {block}: pixels differ"
-        assert np.array_equal(hits.view(np.uint32), ref[1].view(np.uint32)), f"block {block}: hits differ"
+        assert np.array_equal(img, ref[0]), f"variant {v} block {block}: pixels differ"
+        assert np.array_equal(hits.view(np.uint32), ref[1].view(np.uint32)), f"variant {v} block {block}: hits differ"
     with pytest.raises(H.HulkError):
         gpu_ctx(s, nodes, lp, H.ROPES, pipeline=3, coop_lanes=65)
 
