@@ -154,8 +154,8 @@ typedef enum hulk_pipeline {
     HULK_PIPELINE_QUEUED = 1,
     HULK_PIPELINE_FUSED = 2,
     HULK_PIPELINE_REFILL = 3   /* FUSED with lanes refilled from the ray queue
-                                  as their paths end (ropes variant, blocks
-                                  of at most 256) */
+                                  as their paths end (ropes or LDS short-stack
+                                  variant, blocks of at most 256) */
 } hulk_pipeline;
 
 /* Call flags. */
@@ -227,7 +227,11 @@ int hulk_update_vertices(hulk_ctx* ctx, const hulk_float4* verts, uint32_t n_ver
 /* KD tree as produced by KDTreeCPU::buildKDTree (m_treeNodes, m_leafPolygons;
  * KDTreeCPU.hpp:128-132).  The library derives its own device layout
  * (per-leaf triangle records, LDS top tree, ropes).  Replaces
- * buildPerLeafData + the node upload (GLVisitorGPUTrace.cpp:163-294). */
+ * buildPerLeafData + the node upload (GLVisitorGPUTrace.cpp:163-294).
+ * HULK_E_ILLEGAL_INPUT for 2^26 or more nodes or leaf polygon references, a
+ * node flagged both leaf and junction, a cycle, a shared or unreachable node,
+ * or an index out of range; flag bits above the axis / leaf / junction bits
+ * are ignored. */
 int hulk_upload_kdtree(hulk_ctx* ctx, const hulk_kd_node* nodes, uint32_t n_nodes,
                        const uint32_t* leaf_polys, uint32_t n_leaf_polys);
 
