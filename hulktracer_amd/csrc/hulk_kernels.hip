@@ -114,6 +114,7 @@ __device__ __forceinline__ uint32_t frame_ray(const FrameDev* F, uint32_t n, uin
     return f;
 }
 
+constexpr float kNoStop = -__builtin_inff();
 constexpr uint32_t kRecBytes = sizeof(TriRec);
 static_assert(kRecBytes == 64, "triangle records are 64 B");
 
@@ -125,7 +126,8 @@ constexpr int kLeafMt = 3;
 // Leaf loop.  tstop: a shadow ray only needs to know whether its nearest
 // hit lies below the light distance; once best.t < tstop that answer is
 // fixed (best.t only decreases), so the ray stops.  Primary and mirror rays
-// pass tstop = -FLT_MAX, which never stops them.
+// pass kNoStop = -inf, which never stops them: no float is below it, so where
+// it is a constant after inlining the compiler drops the compares.
 template <bool COUNT, int LB>
 __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, uint32_t rec_bytes, const uint4& n,
                                           const Ray& r, Best& best, Cnt& c, float tstop) {
@@ -797,7 +799,7 @@ __device__ void render_ray(const TraceParams& P, const uint4* lds_nodes, uint32_
         w = w * 0.8f;
         best_init(best);
         uint32_t mf = 0;
-        trace_one<VARIANT, COUNT, LB>(P, lds_nodes, stk, stride, r, kFltMax, -kFltMax, best, c, mf);
+        trace_one<VARIANT, COUNT, LB>(P, lds_nodes, stk, stride, r, kFltMax, kNoStop, best, c, mf);
         if (COUNT) {
             crays++;
             chits += best.tri != kNoHit ? 1 : 0;
@@ -888,7 +890,7 @@ k_trace(TraceParams P) {
             Ray r;
             uint32_t o = i;  // output / bound index: the pixel index for camera rays
             uint32_t max_bounce = P.max_bounce;
-            float tstop = -kFltMax;
+            float tstop = kNoStop;
             if (P.source == kSourceCamera) {
                 uint32_t x, y;
                 tile_pixel(i, P.W, P.H, x, y);
@@ -903,7 +905,7 @@ k_trace(TraceParams P) {
                 r.o = v3(ray.start.x, ray.start.y, ray.start.z);
                 r.d = v3(ray.dir.x, ray.dir.y, ray.dir.z);
                 // shadow queue: start.w carries the light distance
-                if (P.stop_at_w) tstop = ray.start.w;
+                if (!RENDER && P.stop_at_w) tstop = ray.start.w;
             }
             set_inv(r);
             Best best;
@@ -1877,7 +1879,7 @@ __global__ void __launch_bounds__(256) k_trace_streamed(StreamParams S) {
                 const uint4 n = S.nodes[pending];
                 const uint2 lt = S.leaf_table[n.w];
                 if (lt.y) {
-                    test_leaf<false, 1>(S.pool, S.pool_bytes, make_uint4(n.x, lt.x, n.z, n.w), r, best, c, -kFltMax);
+                    test_leaf<false, 1>(S.pool, S.pool_bytes, make_uint4(n.x, lt.x, n.z, n.w), r, best, c, kNoStop);
                     pending = 0;
                 } else {
                     S.request[n.w] = 1u;
@@ -1907,7 +1909,7 @@ __global__ void __launch_bounds__(256) k_trace_streamed(StreamParams S) {
                 if (n.x & kNodeLeaf) {
                     const uint2 lt = S.leaf_table[n.w];
                     if (lt.y) {
-                        test_leaf<false, 1>(S.pool, S.pool_bytes, make_uint4(n.x, lt.x, n.z, n.w), r, best, c, -kFltMax);
+                        test_leaf<false, 1>(S.pool, S.pool_bytes, make_uint4(n.x, lt.x, n.z, n.w), r, best, c, kNoStop);
                     } else {
                         S.request[n.w] = 1u;
                         pending = node;
