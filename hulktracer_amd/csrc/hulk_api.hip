@@ -69,6 +69,7 @@ struct hulk_ctx {
     KdBuilder* kdb = nullptr;
     KdBuildResult kdb_last{};
     uint32_t tree_gen = 0;  // bumped whenever the tree or the vertices change
+    uint32_t lights_gen = 0;  // bumped whenever the lights change
 
     // out-of-core streaming: per-leaf extents of the records, their pinned
     // host copy (the backing store) and the device pool / tables
@@ -844,6 +845,7 @@ extern "C" int hulk_upload_kdtree(hulk_ctx* c, const hulk_kd_node* nodes, uint32
 extern "C" int hulk_set_lights(hulk_ctx* c, const hulk_light* lights, uint32_t n) {
     if (!c || (n && !lights)) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
     c->lights.assign(lights, lights + n);
+    c->lights_gen++;
     std::vector<float4> pts;
     for (uint32_t k = 0; k < n; k++)
         if (lights[k].color.w == HULK_LIGHT_POINT)
@@ -1386,6 +1388,9 @@ struct hulk_batch {
     uint32_t n = 0, max_bounce = 0;
     uint32_t* rgba = nullptr;
     const float* tbound = nullptr;
+    // context state the recorded work depends on (tree / vertex pointers,
+    // rope slots, light buffers and the queue sizes derived from them)
+    uint32_t tree_gen = 0, lights_gen = 0;
     bool graph_ready = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1465,6 +1470,8 @@ extern "C" int hulk_batch_create(hulk_ctx* c, const hulk_frame_desc* frames, uin
         hulk_batch_destroy(b);
         return hip_fail(c, e, "batch upload");
     }
+    b->tree_gen = c->tree_gen;
+    b->lights_gen = c->lights_gen;
     // one eager run: builds ropes if needed and checks the launch shapes
     if ((r = batch_record(b, false))) {
         hulk_batch_destroy(b);
@@ -1504,6 +1511,9 @@ extern "C" int hulk_batch_create(hulk_ctx* c, const hulk_frame_desc* frames, uin
 extern "C" int hulk_batch_launch(hulk_batch* b, uint32_t flags) {
     if (!b) return HULK_E_ILLEGAL_INPUT;
     hulk_ctx* c = b->c;
+    if (b->tree_gen != c->tree_gen || b->lights_gen != c->lights_gen)
+        return fail(c, HULK_E_ILLEGAL_INPUT,
+                    "the context's scene, tree or lights changed after the batch was created: create it again");
     const bool count = (flags & HULK_FLAG_COUNT) != 0;
     int r;
     if (count && (r = zero_stats(c))) return r;

@@ -170,7 +170,7 @@ __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ recs, uint3
 
 // ------------------------------------------------------- traversals -----
 
-// Full stack: trace_ray of GPUTrace.cl:77-161 (CPU twin KDTreeCPU::
+// Full stack: trace_ray of GPUTrace.cl:6-90 (CPU twin KDTreeCPU::
 // searchKDTree, KDTreeCPU.cpp:285-361).  The stack lives in LDS, one
 // column per lane.  Entries keep (node, tStart) only: the tEnd of entry k
 // equals the tStart of entry k-1 (tEnd0 for the bottom entry), because tEnd
@@ -351,7 +351,7 @@ __device__ void trav_shortstack_private(const TraceParams& P, const uint4* lds_n
     }
 }
 
-// kd-restart: trace_ray of GPUTrace_stackless.cl:182-296 with every leaf
+// kd-restart: trace_ray of GPUTrace_stackless.cl:6-120 with every leaf
 // resident.  ThreadContext starts zeroed, so tmin = tmax = 0 (the box entry
 // is not used); each leaf is followed by a restart from the root; there is
 // no early exit after a hit.
@@ -710,7 +710,7 @@ __device__ __forceinline__ uint32_t to_u8(float c) {
 }
 
 // Lambert term of one light at hit point Pp with normal nn (fused
-// semantics, GPUTrace.cl:193-228); for a point light, L and its distance
+// semantics, GPUTrace.cl:122-160); for a point light, L and its distance
 // are returned for the shadow ray.
 __device__ __forceinline__ float light_intensity(const float4& col, const float4& dat, const V3& Pp, const V3& nn,
                                                  V3& L, float& dist) {
@@ -1487,7 +1487,7 @@ __global__ void __launch_bounds__(1024) k_scan_pairs(const uint2* counts, uint32
 
 // ------------------------------------------------------------ shading --
 
-// Lambert shading of generation g (fused semantics, GPUTrace.cl:193-228):
+// Lambert shading of generation g (fused semantics, GPUTrace.cl:122-160):
 // c = 0.1 + sum_k color_k * max(I_k, 0), I = n.dir (directional) or n.L
 // halved when dist > t_shadow (point).  acc[pixel] += c * keep, keep = w*0.2
 // for a mirror hit that spawned a reflection, else w.
@@ -1563,7 +1563,7 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
-// ShadePicture, shadePicture.cl:73-121: base 0.1 for every pixel; a hit
+// ShadePicture, shadePicture.cl:7-55: base 0.1 for every pixel; a hit
 // (t != FLT_MAX and t > 0) adds, per light, n.dir (directional) or
 // (n.L)/2 (point, no shadow ray).
 __global__ void k_shade_picture(ShadeParams S, uint32_t* out) {

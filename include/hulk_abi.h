@@ -4,7 +4,7 @@
  *
  * This header is the drop-in boundary for hulkTracer's GPU trace stage.
  * It replaces the OpenCL seam that `GLVisitorGPUTrace` drives:
- *   - kernel args of `gpu_trace` (HulkEngine/src/GPUTrace_stackless.cl:298-309,
+ *   - kernel args of `gpu_trace` (HulkEngine/src/GPUTrace_stackless.cl:122-133,
  *     bound at HulkEngine/src/GLVisitorGPUTrace.cpp:342-378),
  *   - `ShadePicture` (HulkEngine/src/shadePicture.cl:7-11, bound at
  *     GLVisitorGPUTrace.cpp:385-398),
@@ -81,7 +81,7 @@ typedef struct hulk_light { hulk_float4 color; hulk_float4 data; } hulk_light;
 typedef struct hulk_ray { hulk_float4 start; hulk_float4 dir; } hulk_ray;
 
 /* Hit record.  The reference writes float4(normal.xyz, t)
- * (GPUTrace_stackless.cl:339-341); this record adds the global triangle id
+ * (GPUTrace_stackless.cl:163-165); this record adds the global triangle id
  * (m_leafPolygons[start+i], KDTreeCPU.cpp:267) and the barycentrics
  * u = s1/det, v = s2/det of the triangle test (SharedFunctions.hpp:88-91).
  * Miss: t = FLT_MAX, tri = HULK_NO_HIT. */
@@ -165,7 +165,7 @@ typedef enum hulk_pipeline {
 #define HULK_FLAG_NO_SYNC      8u   /* do not synchronize the stream on exit */
 
 /* Shading modes for hulk_shade / hulk_render. */
-#define HULK_SHADE_FUSED    0u  /* GPUTrace.cl:193-228: shadow rays, miss = 0      */
+#define HULK_SHADE_FUSED    0u  /* GPUTrace.cl:122-160: shadow rays, miss = 0      */
 #define HULK_SHADE_PICTURE  1u  /* shadePicture.cl:14-54: no shadow, miss = 0.1    */
 
 typedef enum hulk_status {
@@ -253,7 +253,7 @@ int hulk_trace_rays(hulk_ctx* ctx, const hulk_ray* rays, uint32_t n_rays,
 
 /* Shade a hit buffer produced by hulk_trace_camera with the same camera:
  * Lambert + ambient 0.1, point-light shadow rays from P + 0.001*L
- * (HULK_SHADE_FUSED, GPUTrace.cl:193-228) or the live ShadePicture rule
+ * (HULK_SHADE_FUSED, GPUTrace.cl:122-160) or the live ShadePicture rule
  * (HULK_SHADE_PICTURE, shadePicture.cl:14-54).  Output packed
  * R | G<<8 | B<<16 per pixel (FrameBuffer.hpp:196-217). */
 int hulk_shade(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height,
@@ -292,7 +292,12 @@ typedef struct hulk_batch hulk_batch;
  * the options the context had at creation.  rgba_dev / tbound_dev are
  * device pointers; frame f's pixels start at the sum of the earlier
  * frames' width*height.  Replaces the per-frame traceScene loop
- * (GLVisitorGPUTrace.cpp:296-410) for offline / multi-view rendering. */
+ * (GLVisitorGPUTrace.cpp:296-410) for offline / multi-view rendering.
+ * A batch records the context's scene, tree and lights as they are at
+ * creation: after hulk_upload_scene, hulk_update_vertices,
+ * hulk_upload_kdtree, hulk_build_kdtree_device or hulk_set_lights on its
+ * context, hulk_batch_launch returns HULK_E_ILLEGAL_INPUT without running
+ * anything, and the batch must be created again. */
 int hulk_batch_create(hulk_ctx* ctx, const hulk_frame_desc* frames, uint32_t n_frames,
                       const uint8_t* mirror_mask, const float* tbound_dev, uint32_t* rgba_dev,
                       uint32_t use_graph, hulk_batch** out);

@@ -290,7 +290,7 @@ static inline f4 inv_dir(const ray_t* r) {
     return i;
 }
 
-/* Full-stack traversal, trace_ray of GPUTrace.cl:77-161 (twin of
+/* Full-stack traversal, trace_ray of GPUTrace.cl:6-90 (twin of
  * KDTreeCPU::searchKDTree, KDTreeCPU.cpp:285-361).  tclip (FLT_MAX = none)
  * clips the box exit. */
 static void trav_fullstack(const scene_t* s, const ray_t* ray, float tclip, best_t* best, cnt_t* c) {
@@ -389,7 +389,7 @@ static void trav_restart(const scene_t* s, const ray_t* ray, float tclip, best_t
     if (tclip < te) te = tclip;
     f4 inv = inv_dir(ray);
     float tmax = 0.0f, tmin = 0.0f;
-    if (best->t < tmax) return; /* GPUTrace_stackless.cl:223 */
+    if (best->t < tmax) return; /* GPUTrace_stackless.cl:46-51 */
     while (tmax < te) {
         tmin = tmax; tmax = te;
         uint32_t node = 0;
@@ -649,7 +649,7 @@ static inline uint32_t to_u8(float c) {
 }
 uint32_t orc_pack_rgb(float r, float g, float b) { return to_u8(r) | (to_u8(g) << 8) | (to_u8(b) << 16); }
 
-/* Shading of one hit point.  Fused semantics, GPUTrace.cl:193-228 (CPU twin
+/* Shading of one hit point.  Fused semantics, GPUTrace.cl:122-160 (CPU twin
  * GLVisitorCPUTrace.cpp:165-194): ambient 0.1; directional I = n.dir; point:
  * L = pos - P (w = 0), dist = sqrt(L.L), L = normalize(L), shadow ray
  * (P + 0.001*L, L) traced nearest-hit with the same variant, I = n.L, halved

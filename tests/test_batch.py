@@ -103,3 +103,38 @@ def test_autotune_whole_frames():
     assert best["pipeline"] != 3 or best["variant"] == H.ROPES
     ref, _ = oracle_scene(2, 1).render(var, cam, 128, 128, max_bounce=0, depth=best["short_stack_depth"])
     assert np.array_equal(img, ref)
+
+
+def test_batch_refuses_a_changed_context():
+    """A batch records the context's scene, tree and lights at creation: a
+    launch after any of them changed is refused on the host (the recorded
+    graph is not replayed), and a new batch renders the new state."""
+    import torch
+
+    s, nodes, lp = scene_and_tree(3, 6, 5, 4)
+    ctx = gpu_ctx(s, nodes, lp, H.ROPES, pipeline=2)
+    frames = _frames(s, [(1, 64, 64, 1), (2, 96, 64, 0)])
+    rgba = torch.zeros(64 * 64 + 96 * 64, dtype=torch.int32, device="cuda")
+    changes = [("lights", lambda: ctx.set_lights(s.lights)),
+               ("tree", lambda: ctx.upload_kdtree(nodes, lp)),
+               ("vertices", lambda: (ctx.update_vertices(s.vertices, s.box), ctx.upload_kdtree(nodes, lp))),
+               ("scene", lambda: (ctx.upload_scene(s), ctx.upload_kdtree(nodes, lp)))]
+    for what, change in changes:
+        b = ctx.batch(frames, s.mirror, rgba.data_ptr(), use_graph=True)
+        b.launch()
+        before = rgba.cpu().numpy().copy()
+        change()
+        rgba.zero_()
+        torch.cuda.synchronize()
+        with pytest.raises(H.HulkError) as e:
+            b.launch()
+        assert e.value.code == 4 and "create it again" in str(e.value), what
+        assert not rgba.any(), f"{what}: the refused launch ran"
+        with pytest.raises(H.HulkError):
+            b.launch(4)  # the counted (eager) path is refused as well
+        b.close()
+        b2 = ctx.batch(frames, s.mirror, rgba.data_ptr(), use_graph=True)
+        b2.launch()
+        assert np.array_equal(rgba.cpu().numpy(), before), what
+        b2.close()
+    ctx.close()

@@ -20,9 +20,19 @@ How each oracle function is pinned (parity status):
       determinant signs, edges inclusive, the parallel cut and behind cases), hit /
       miss agreement with checkRayToPoly except within 1e-5 of an edge, the
       FP64 twin, and brute-force agreement of every traversal in MT mode.
-  orc_shade / orc_pack_rgb (GPUTrace.cl:193-228, FrameBuffer.hpp:196-217)
+  orc_cast_ray / orc_cast_rays (castRay, GPU_common.cl:10-30)
+      closed form on an asymmetric camera and a non-square image with
+      dyadic values (every term exact): vF scales x / W, hF scales y / H,
+      index y * W + x; the x <-> y, W <-> H and vF <-> hF swaps are shown
+      to differ.
+  orc_shade mode 0 / orc_pack_rgb (GPUTrace.cl:122-160, FrameBuffer.hpp:196-217)
       closed-form Lambert + shadow halving on a one-triangle scene;
       round-half-even packing.
+  orc_shade mode 1 (ShadePicture, shadePicture.cl:20-54)
+      closed form on a quad with a blocker: base 0.1 everywhere, point
+      light c * max(n.L, 0) / 2 with no shadow ray (blocked points are not
+      halved), directional c * max(n.dir, 0), hit iff t != FLT_MAX and t > 0
+      (t = 0, t < 0 and FLT_MAX shade 0.1 grey whatever the triangle id).
   orc_render (mirror generations; no reference counterpart)
       a mirror facing the camera reflects rays back: second-generation hit
       on the triangle behind the camera, weights 0.2 / 0.8.
@@ -447,6 +457,84 @@ def test_shade_closed_form():
         want = 0.1 + 0.9 * max(cosang, 0) * (0.5 if blocker else 1.0)
         got = img[i] & 0xFF
         assert abs(got - round(min(want, 1) * 255)) <= 1
+
+
+def test_shade_picture_closed_form():
+    """ShadePicture (shadePicture.cl:20-54): every pixel starts at 0.1 grey;
+    a pixel counts as hit iff t != FLT_MAX and t > 0; a point light adds
+    c * max(n.L^, 0) / 2 with no shadow ray; a directional light adds
+    c * max(n.dir, 0).  A dropped /2, a shadow ray, a missing base or a
+    miss shaded black breaks it."""
+    osc = _one_quad_scene(True)  # the blocker sits between the quad and the point light
+    pc = np.array([0.9, 0.6, 0.3], F32)
+    dc = np.array([0.25, 0.5, 0.125], F32)
+    d_dir = np.array([0.0, 0.6, 0.8], F32)  # n.dir = 0.8 on the quad (n = +z)
+    osc.lights = np.array([[*pc, 1.0, 0.0, 0.0, 3.0, 1.0], [*dc, 0.0, *d_dir, 0.0]], F32)
+    cam = H.make_camera((0.0, 0.0, 5.0), fov=30.0)
+    W, Hh = 48, 40
+    hits = osc.trace_camera(O.V_FULLSTACK, cam, W, Hh)
+    img = osc.shade(O.V_FULLSTACK, cam, W, Hh, hits, mode=1)
+    rays = O.cast_rays(cam, W, Hh).astype(np.float64)
+    rgb = img.view(np.uint8).reshape(-1, 4)[:, :3].astype(int)
+    grey = O.pack_rgb(0.1, 0.1, 0.1)
+    checked_quad = shadowed = 0
+    for i in range(W * Hh):
+        h = hits[i]
+        if h["tri"] == O.NO_HIT:
+            assert img[i] == grey, f"miss pixel {i} is not 0.1 grey"
+            continue
+        n = np.array([h["nx"], h["ny"], h["nz"]], np.float64)
+        P = rays[i, :3] + rays[i, 4:7] * float(h["t"])
+        L = np.array([0.0, 0.0, 3.0]) - P
+        want = 0.1 + pc * max(float(n @ (L / np.linalg.norm(L))), 0.0) / 2 + dc * max(float(n @ d_dir), 0.0)
+        assert np.all(np.abs(rgb[i] - np.round(np.minimum(want, 1.0) * 255)) <= 1), (i, rgb[i], want * 255)
+        if h["tri"] in (0, 1):
+            checked_quad += 1
+            # a quad point whose segment to the light crosses the blocker (z = 1, where the
+            # segment is at 2/3 P + 1/3 light): a shadow ray would halve the point term
+            qx, qy = 2.0 / 3.0 * P[0], 2.0 / 3.0 * P[1]
+            if qy > -0.5 and qy < 0.5 - 2.0 * abs(qx):
+                shadowed += 1
+    assert checked_quad > 100 and shadowed > 0
+    # the hit rule looks at t only: t == 0, t < 0 and t == FLT_MAX give 0.1 grey, a hit id notwithstanding
+    i = next(k for k in range(W * Hh) if hits[k]["tri"] in (0, 1))
+    for t in (0.0, -1.0, float(O.FLT_MAX)):
+        hh = hits.copy()
+        hh["t"][i] = t
+        assert osc.shade(O.V_FULLSTACK, cam, W, Hh, hh, mode=1)[i] == grey, t
+    # and a tiny positive t shades the pixel (no lower cut above zero)
+    hh = hits.copy()
+    hh["t"][i] = np.float32(1e-30)
+    assert osc.shade(O.V_FULLSTACK, cam, W, Hh, hh, mode=1)[i] != grey
+
+
+def test_cast_rays_orientation_closed_form():
+    """castRay (GPU_common.cl:10-30, called as castRay(row = y, column = x)
+    from GPUTrace.cl:105-116): d = lowerLeft + vF * (x / W) + hF * (y / H) -
+    from, ray index y * W + x.  An asymmetric camera on a non-square image
+    with dyadic values makes every term exact, so an x <-> y, W <-> H or
+    vF <-> hF swap fails."""
+    frm = np.array([0.5, 0.25, 1.0, 0.0])
+    vf = np.array([4.0, 0.5, -0.25, 0.0])
+    hf = np.array([0.25, 2.0, 0.125, 0.0])
+    ll = np.array([-2.0, -1.0, -4.0, 0.0])
+    cam = np.concatenate([frm, vf, hf, ll]).astype(F32)
+    W, Hh = 8, 4
+    rays = O.cast_rays(cam, W, Hh).astype(np.float64)
+    assert np.array_equal(rays[0, 4:7], (ll - frm)[:3]), "pixel (0, 0) must be lowerLeft - from"
+    x, y = np.meshgrid(np.arange(W), np.arange(Hh))  # index y * W + x
+    x, y = x.ravel()[:, None], y.ravel()[:, None]
+    want = ll[:3] + vf[:3] * (x / W) + hf[:3] * (y / Hh) - frm[:3]
+    assert np.array_equal(rays[:, 4:7], want)
+    assert np.array_equal(rays[:, 0:3], np.broadcast_to(frm[:3], (W * Hh, 3)))
+    # the swapped readings really differ on this camera
+    for swapped in (ll[:3] + vf[:3] * (y / Hh) + hf[:3] * (x / W) - frm[:3],
+                    ll[:3] + hf[:3] * (x / W) + vf[:3] * (y / Hh) - frm[:3],
+                    ll[:3] + vf[:3] * (x / Hh) + hf[:3] * (y / W) - frm[:3]):
+        assert not np.array_equal(rays[:, 4:7], swapped)
+    # pixel (x = 3, y = 1): d = ll + vf * 3/8 + hf * 1/4 - from
+    assert np.array_equal(rays[1 * W + 3, 4:7], [-2.0 + 1.5 + 0.0625 - 0.5, -1.0 + 0.1875 + 0.5 - 0.25,
+                                               -4.0 - 0.09375 + 0.03125 - 1.0])
 
 
 def test_mirror_generation_reflects_back():
