@@ -36,6 +36,7 @@ SHADE_FUSED = 0
 SHADE_PICTURE = 1
 NO_HIT = 0xFFFFFFFF
 HITF_BOUNDED_FALLBACK = 1
+STATIC_MAX_RAYS = 262144  # HULK_STATIC_MAX_RAYS
 
 SCENE_RANDOM_TRIS = 1
 SCENE_TERRAIN_SPHERES = 2
@@ -114,6 +115,8 @@ EXPORTS = {
                              C.c_uint32]),
     "hulk_render_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
+    "hulk_render_frame_static": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "hulk_get_counters": (C.c_int, [C.c_void_p, C.POINTER(Counters)]),
     "hulk_reset_counters": (C.c_int, [C.c_void_p]),
     "hulk_batch_create": (C.c_int, [C.c_void_p, C.POINTER(FrameDesc), C.c_uint32, C.c_void_p, C.c_void_p,

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -33,12 +34,25 @@ namespace {
 
 constexpr uint32_t kTopLevelsMax = 12;
 
+// Device scratch of the allocation-free frame call (hulk_render_frame_static):
+// the persistent waves' work counters, then the primary hit records of
+// frames whose caller passes no hit buffer, up to kStaticMaxRays of them.
+constexpr uint32_t kStaticMaxRays = 1u << 18;
+constexpr size_t kScratchCounterBytes = kWorkCounterWords * 4u;
+constexpr size_t kScratchBytes = kScratchCounterBytes + size_t(kStaticMaxRays) * sizeof(hulk_hit);
+constexpr int kMaxDevices = 64;
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
 };
 
 }  // namespace
+
+// One copy per GPU, held until the process ends.  Host code never uses this
+// name as an address: each device's address comes from hipGetSymbolAddress
+// (scratch_address below).
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 // Buffers of one frame pipeline (primary hits, shadow queue, mirror queues,
 // accumulation).  The context owns one set; every batch owns its own, so a
@@ -100,6 +114,8 @@ struct hulk_ctx {
     unsigned long long* host_stats = nullptr;  // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     hulk_counters cnt{};
+    void* scratch = nullptr;  // this device's address of `scratch` (null: lookup failed, see scratch_err)
+    hipError_t scratch_err = hipSuccess;
 };
 
 namespace {
@@ -144,6 +160,28 @@ void release(DevBuf& b) {
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
+}
+
+// Address of `scratch` on `device`, which must be the current device: looked
+// up once per device and kept for the life of the process.
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+hipError_t scratch_address(int device, void** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_err[device] = e;
+        g_scratch_addr[device] = e == hipSuccess ? p : nullptr;
+        g_scratch_done[device] = true;
+    }
+    *out = g_scratch_addr[device];
+    return g_scratch_err[device];
 }
 
 int check_opts(hulk_ctx* c, const hulk_opts& o) {
@@ -614,6 +652,9 @@ extern "C" int hulk_create(hulk_ctx** out, const hulk_opts* opts) {
         c->num_cus = prop.multiProcessorCount;
         c->max_lds = std::min<size_t>(prop.sharedMemPerBlock, 65536);
     }
+    // the device is current here: look up its scratch address (reported by
+    // hulk_render_frame_static if the lookup failed)
+    c->scratch_err = scratch_address(o.device, &c->scratch);
     if (o.stream) {
         c->stream = static_cast<hipStream_t>(o.stream);
     } else {
@@ -1015,6 +1056,47 @@ extern "C" int hulk_render_frame(hulk_ctx* c, const hulk_camera* cam, uint32_t W
         if (primary_hits)
             HIP_TRY(c, hipMemcpyAsync(primary_hits, h0, (size_t)n * sizeof(hulk_hit), hipMemcpyDeviceToHost, c->stream));
     }
+    if (count && (r = read_stats(c))) return r;
+    return finish(c, flags, true);
+}
+
+extern "C" int hulk_render_frame_static(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H,
+                                        uint32_t max_bounce, const uint8_t* mirror_dev, const float* tbound_dev,
+                                        uint32_t* rgba_dev, hulk_hit* primary_hits_dev, uint32_t flags) {
+    if (!c || !cam || !rgba_dev) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    if (max_bounce > 8) return fail(c, HULK_E_ILLEGAL_INPUT, "max_bounce must be <= 8");
+    if ((uint64_t)W * H > 0xFFFFFFC0ull) return fail(c, HULK_E_ILLEGAL_INPUT, "image too large");
+    if (!c->scratch) return hip_fail(c, c->scratch_err, "hipGetSymbolAddress(scratch)");
+    if (c->opts.pipeline == HULK_PIPELINE_QUEUED)
+        return fail(c, HULK_E_UNSUPPORTED,
+                    "the queued pipeline keeps per-frame ray queues: use the fused or refill pipeline here");
+    const uint32_t n = W * H;
+    if (!primary_hits_dev && n > kStaticMaxRays)
+        return fail(c, HULK_E_ILLEGAL_INPUT,
+                    "without a primary hit buffer a frame may have at most 262144 rays (HULK_STATIC_MAX_RAYS): "
+                    "the static scratch holds their hit records");
+    if (c->opts.variant == HULK_VARIANT_ROPES && c->have_tree && !c->ropes_ready)
+        return fail(c, HULK_E_UNSUPPORTED,
+                    "the rope slots of this tree are not built yet (the first rope trace through another entry "
+                    "point builds them)");
+    const bool count = (flags & HULK_FLAG_COUNT) != 0;
+    FrameSet fs;
+    fs.cam = to_cam(*cam);
+    fs.W = W;
+    fs.H = H;
+    fs.n = n;
+    Work w;  // nothing allocated: only the work counters are used, in the scratch
+    w.wq.p = c->scratch;
+    w.wq.cap = kScratchCounterBytes;
+    hulk_hit* h0 = primary_hits_dev
+                       ? primary_hits_dev
+                       : reinterpret_cast<hulk_hit*>(static_cast<unsigned char*>(c->scratch) + kScratchCounterBytes);
+    const bool bounces = max_bounce > 0 && mirror_dev;
+    int r;
+    if (count && (r = zero_stats(c))) return r;
+    if ((r = render_frame_launch(c, c->opts, fs, w, tbound_dev, h0, bounces ? max_bounce : 0,
+                                 bounces ? mirror_dev : nullptr, rgba_dev, count, true)))
+        return r;
     if (count && (r = read_stats(c))) return r;
     return finish(c, flags, true);
 }

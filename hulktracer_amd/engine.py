@@ -446,6 +446,16 @@ class GPUTrace:
                                         ptr(mirror_dev), ptr(tbound_dev), ptr(rgba_dev), ptr(hits_dev),
                                         flags | L.FLAG_DEVICE_PTRS))
 
+    def render_frame_static_dev(self, cam: np.ndarray, W: int, H: int, max_bounce: int, mirror_dev: int | None,
+                                rgba_dev: int, hits_dev: int | None = None, tbound_dev: int | None = None,
+                                flags: int = 0):
+        """hulk_render_frame_static: the frame of render_frame_dev, allocating
+        nothing (device pointers only; without hits_dev at most
+        STATIC_MAX_RAYS rays)."""
+        self._c(lib().hulk_render_frame_static(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, max_bounce,
+                                               ptr(mirror_dev), ptr(tbound_dev), ptr(rgba_dev), ptr(hits_dev),
+                                               flags))
+
     def sync(self):
         self._c(lib().hulk_sync(self.h))
 

@@ -268,6 +268,26 @@ int hulk_render_frame(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uin
                       const float* tbound, uint32_t* rgba8, hulk_hit* primary_hits,
                       uint32_t flags);
 
+/* Allocation-free whole frame.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The frame of
+ * hulk_render_frame (same pixels, primary hit records and counters) with the
+ * context's FUSED or REFILL pipeline; QUEUED options give
+ * HULK_E_UNSUPPORTED.  Every pointer is a device pointer; rgba8 is required.
+ * The call allocates nothing: the persistent waves' work counters live in a
+ * static device array of the library (one per GPU, held until the process
+ * ends), which also holds the primary hit records when primary_hits is NULL;
+ * such a frame may have at most HULK_STATIC_MAX_RAYS rays (else
+ * HULK_E_ILLEGAL_INPUT).  The array is shared by every context on a GPU and
+ * keeps the last call's contents: calls that overlap on one GPU (two
+ * streams, two threads) race.  With the rope variant the tree's rope slots
+ * must already exist (any earlier rope trace on the context builds them),
+ * else HULK_E_UNSUPPORTED. */
+#define HULK_STATIC_MAX_RAYS 262144u
+int hulk_render_frame_static(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height,
+                             uint32_t max_bounce, const uint8_t* mirror_mask_dev, const float* tbound_dev,
+                             uint32_t* rgba8_dev, hulk_hit* primary_hits_dev, uint32_t flags);
+
 int hulk_get_counters(const hulk_ctx* ctx, hulk_counters* out);
 int hulk_reset_counters(hulk_ctx* ctx);
 
