@@ -48,13 +48,16 @@ _TUNE_KEYS = ("variant", "block_size", "short_stack_depth", "lds_top_levels", "p
 
 
 def tune_batch(ctx: GPUTrace, frames, mirror, rgba_dev: int, table, stream, top: int = 6, reps: int = 5,
-               rounds: int = 2):
+               rounds: int = 2, prefer=(), tol: float = 0.01):
     """Batch stage of the autotuner: the `top` fastest distinct shapes of a
     per-frame autotune table are re-timed on the graph-captured batch of
     `frames` itself (a launch over every frame's rays behaves unlike one
     frame: more waves in flight, other tails), `rounds` times each in turn,
     keeping each shape's best time.  The fastest shape becomes the
-    context's options; returns (options dict, {shape: ms})."""
+    context's options, except that among the shapes within `tol` of the
+    fastest (run-to-run noise of this stage) one listed in `prefer` (tuples
+    of the option values in _TUNE_KEYS order) is taken first.  Returns
+    (options dict, {shape: ms})."""
     torch = _torch()
     seen, cands = set(), []
     for row in sorted(table, key=lambda r: r["ms"]):
@@ -64,6 +67,10 @@ def tune_batch(ctx: GPUTrace, frames, mirror, rgba_dev: int, table, stream, top:
             cands.append(k)
         if len(cands) == top:
             break
+    for k in sorted(map(tuple, prefer)):  # preferred shapes are always timed too
+        if k not in seen:
+            seen.add(k)
+            cands.append(k)
     if not cands:
         raise ValueError("empty autotune table")
     batches = []
@@ -86,6 +93,9 @@ def tune_batch(ctx: GPUTrace, frames, mirror, rgba_dev: int, table, stream, top:
     for b in batches:
         b.close()
     best = min(cands, key=lambda k: times[k])
+    near = [k for k in cands if times[k] <= times[best] * (1.0 + tol) and tuple(k) in set(map(tuple, prefer))]
+    if near:
+        best = min(near, key=lambda k: times[k])
     opts = dict(zip(_TUNE_KEYS, best))
     ctx.set_options(**opts)
     return dict(opts, device=int(ctx.opts.device)), {",".join(map(str, k)): round(v, 4) for k, v in times.items()}
