@@ -138,6 +138,8 @@ EXPORTS = {
                                      C.c_void_p]),
     "hulk_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
                                       C.POINTER(C.c_float)]),
+    "hulk_mlp_bound_cameras": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
+                                         C.c_float, C.c_uint32, C.c_void_p, C.c_uint32]),
     "hulk_mlp_bound_camera": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
                                         C.c_uint32, C.c_void_p, C.c_uint32]),
     "hulk_ctx_scene_box": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -65,12 +65,16 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
 struct FwdParams {
     uint32_t camera;
     Camera cam;
+    const Camera* cams;    // bound mode over several frames: camera of frame f (null: cam)
+    uint32_t per_frame;    // evaluations per frame (bound mode)
+    uint32_t pf_m, wq_m;   // division multipliers of per_frame and the evaluations per row (udiv)
     uint32_t W, H;
     const hulk_ray* rays;
     uint32_t n;
     float c[3], ih[3];
     const float* fourier;  // 32 x 6
-    const uint16_t *W1, *W2, *W3, *w4;
+    const uint4* Wf;       // W1, W2, W3 as 80 MFMA operand fragments (k_refresh)
+    const uint16_t* w4;
     const float* p;        // master params (biases)
     float* pred;
     float* tbound;
@@ -80,41 +84,108 @@ struct FwdParams {
     uint32_t ld;
 };
 
-constexpr uint32_t kLdsRow = kHid + 8;  // bf16 per LDS row (pad against bank conflicts)
-constexpr uint32_t kTiles = 4;          // ray tiles of 16 per wave: 64 rays share each weight fragment
+// Ray tiles of 16 per wave and group: 32 rays share each weight fragment
+// read from LDS; at 4 tiles the activations of two layers (8 x 4 x 2
+// VGPRs each) no longer fit in 256 VGPRs beside the accumulators.
+constexpr uint32_t kTiles = 2;
+constexpr uint32_t kTrainTiles = 1;  // training also writes every activation: one tile keeps it out of scratch
+constexpr uint32_t kFwdWaves = 8;  // waves per forward block (two per SIMD)
+// Weight fragments in MFMA operand order: fragment f of a layer is the A
+// operand of output tile t = f / chunks and K chunk kk = f % chunks; lane l
+// holds 8 contiguous bf16 (16 B), so a wave reads a fragment as 1 KiB.
+constexpr uint32_t kFrag1 = 8 * 2, kFrag2 = 8 * 4, kFragAll = kFrag1 + 2 * kFrag2;  // 80 fragments, 80 KiB
 
-__device__ __forceinline__ void store4(uint16_t* dst, const uint16_t* hv) {
-    uint2 w2;
-    w2.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
-    w2.y = (uint32_t)hv[2] | ((uint32_t)hv[3] << 16);
-    *reinterpret_cast<uint2*>(dst) = w2;
+// Input feature of K position p (0..7) of lane group g in K chunk kk of
+// layers 2 and 3.  The accumulator of the layer before holds neurons
+// 16 t + 4 g + i (i = 0..3) of output tile t in lane group g, so a B
+// fragment made of tiles 2 kk and 2 kk + 1 carries neurons 32 kk + 4 g + i
+// and 32 kk + 16 + 4 g + i; the weights are stored with the same K order,
+// which keeps the activations in registers from layer to layer.
+__host__ __device__ __forceinline__ uint32_t hidden_k(uint32_t kk, uint32_t g, uint32_t p) {
+    return 32u * kk + (p < 4u ? 4u * g + p : 16u + 4u * g + (p - 4u));
 }
 
-// Fused forward: one wave (64 threads, one block) handles 64 rays per step,
-// as 4 column tiles of 16; every weight fragment is loaded once and fed to
-// 4 MFMAs.  Activations go through LDS (ray-major rows); when training the
-// feature-major copies go to HBM.  Bound mode with stride s evaluates the
-// ray of pixel (s*qx, s*qy) and writes its bound to the s x s pixel block.
-__global__ void __launch_bounds__(64) k_mlp_fwd(FwdParams P) {
-    __shared__ __attribute__((aligned(16))) uint16_t buf[2][kTiles * 16 * kLdsRow];
-    const uint32_t l = threadIdx.x, g = l >> 4, r = l & 15u;
-    const uint32_t groups = (P.n + 63u) / 64u;
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// a / d for a divisor d >= 1 that is the same for every lane, with
+// m = floor((2^32 - 1) / d) from the host (udiv_multiplier): umulhi(a, m)
+// is at most two below the quotient.
+__host__ __device__ inline uint32_t udiv_multiplier(uint32_t d) { return 0xFFFFFFFFu / d; }
+__device__ __forceinline__ uint32_t udiv(uint32_t a, uint32_t d, uint32_t m) {
+    uint32_t q = __umulhi(a, m);
+    while (a - q * d >= d) q++;
+    return q;
+}
+
+// Two floats to packed BF16, round to nearest even (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint32_t cvt2(float lo, float hi) {
+    const bf16x2 v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, v);
+}
+// acc + 2 products of packed BF16 pairs in FP32 (v_dot2_f32_bf16).
+__device__ __forceinline__ float dot2(uint32_t a, uint32_t b, float acc) {
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, a), __builtin_bit_cast(bf16x2, b), acc, false);
+}
+__device__ __forceinline__ bf16x8 join(uint2 lo, uint2 hi) {
+    return __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
+}
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// ReLU + BF16 of accumulator rows 16 t + 4 g + (0..3) of one ray tile (the
+// accumulator starts at the bias); the feature-major copy goes to hT when
+// training.
+__device__ __forceinline__ uint2 activate(const f32x4& acc, uint16_t* hT, uint32_t ld, uint32_t row0, uint32_t ray,
+                                          bool valid) {
+    const uint2 h = make_uint2(cvt2(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f)),
+                               cvt2(fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f)));
+    if (hT && valid) {
+        hT[(size_t)(row0 + 0) * ld + ray] = (uint16_t)h.x;
+        hT[(size_t)(row0 + 1) * ld + ray] = (uint16_t)(h.x >> 16);
+        hT[(size_t)(row0 + 2) * ld + ray] = (uint16_t)h.y;
+        hT[(size_t)(row0 + 3) * ld + ray] = (uint16_t)(h.y >> 16);
+    }
+    return h;
+}
+
+// Fused forward.  A block of kFwdWaves waves stages the 80 weight
+// fragments (W1, W2, W3 in MFMA operand order, 80 KiB) in LDS once; each
+// wave then takes groups of 64 rays (4 tiles of 16) until none is left.
+// Per group: the Fourier encoding in registers (B operand of layer 1),
+// then three layers whose activations never leave the registers (see
+// hidden_k); layer 3 folds the output dot with w4.  Every weight fragment
+// read from LDS feeds 4 MFMAs.  Bound mode with stride s evaluates the ray
+// of pixel (s*qx, s*qy) and writes its bound to the s x s pixel block.
+template <bool TRAIN, uint32_t TILES>
+__global__ void __launch_bounds__(64 * kFwdWaves) k_mlp_fwd(FwdParams P) {
+    __shared__ uint4 wl[kFragAll * 64];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t l = tid & 63u, g = l >> 4, r = l & 15u;
+    for (uint32_t i = tid; i < kFragAll * 64; i += 64 * kFwdWaves) wl[i] = P.Wf[i];
+    __syncthreads();
+    const uint32_t groups = (P.n + 16u * TILES - 1u) / (16u * TILES);
     const uint32_t s = P.stride ? P.stride : 1u;
     const uint32_t Wq = (P.W + s - 1) / s;
-    for (uint32_t gi = blockIdx.x; gi < groups; gi += gridDim.x) {
-        uint32_t ray[kTiles];
-        bool valid[kTiles];
-        bf16x8 b1[kTiles][2];
+    const uint32_t frame_px = P.W * P.H;
+    for (uint32_t gi = blockIdx.x * kFwdWaves + (tid >> 6); gi < groups; gi += gridDim.x * kFwdWaves) {
+        uint32_t ray[TILES];
+        bool valid[TILES];
+        bf16x8 b1[TILES][2];
+        float x[TILES][6];
+        uint32_t fbase[TILES], qx[TILES], qy[TILES];  // bound mode: frame's first pixel, block coordinates
 #pragma unroll
-        for (uint32_t j = 0; j < kTiles; j++) {
-            ray[j] = gi * 64u + 16u * j + r;
+        for (uint32_t j = 0; j < TILES; j++) {
+            ray[j] = gi * 16u * TILES + 16u * j + r;
             valid[j] = ray[j] < P.n;
             float o[3] = {0, 0, 0}, d[3] = {0, 0, 1};
             if (valid[j]) {
                 if (P.camera) {
                     Ray rr;
-                    const uint32_t qy = ray[j] / Wq, qx = ray[j] - qy * Wq;
-                    cast_ray(P.cam, qx * s, qy * s, P.W, P.H, rr);
+                    const uint32_t f = P.cams ? udiv(ray[j], P.per_frame, P.pf_m) : 0u;
+                    const uint32_t loc = ray[j] - f * P.per_frame;
+                    qy[j] = udiv(loc, Wq, P.wq_m);
+                    qx[j] = loc - qy[j] * Wq;
+                    fbase[j] = f * frame_px;
+                    cast_ray(P.cams ? P.cams[f] : P.cam, qx[j] * s, qy[j] * s, P.W, P.H, rr);
                     o[0] = rr.o.x; o[1] = rr.o.y; o[2] = rr.o.z;
                     d[0] = rr.d.x; d[1] = rr.d.y; d[2] = rr.d.z;
                 } else {
@@ -123,98 +194,102 @@ __global__ void __launch_bounds__(64) k_mlp_fwd(FwdParams P) {
                     d[0] = q.dir.x; d[1] = q.dir.y; d[2] = q.dir.z;
                 }
             }
-            float x[6];
-            for (int i = 0; i < 3; i++) x[i] = (o[i] - P.c[i]) * P.ih[i];
+            for (int i = 0; i < 3; i++) x[j][i] = (o[i] - P.c[i]) * P.ih[i];
             const float dd = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
             const float inv = 1.0f / __builtin_sqrtf(dd);
-            for (int i = 0; i < 3; i++) x[3 + i] = d[i] * inv;
-            uint16_t es[8], ec[8];
+            for (int i = 0; i < 3; i++) x[j][3 + i] = d[i] * inv;
+        }
+        // Fourier features: row 8 g + k of F against each tile's ray, sin and cos as BF16 pairs
+        uint32_t es[TILES][4], ec[TILES][4];
 #pragma unroll
-            for (uint32_t k = 0; k < 8; k++) {
-                const float* F = P.fourier + (8 * g + k) * 6;
-                float z = F[0] * x[0];
-                z = z + F[1] * x[1];
-                z = z + F[2] * x[2];
-                z = z + F[3] * x[3];
-                z = z + F[4] * x[4];
-                z = z + F[5] * x[5];
-                es[k] = f2bf(__sinf(z));
-                ec[k] = f2bf(__cosf(z));
-                if (P.encT && valid[j]) {
-                    P.encT[(size_t)(8 * g + k) * P.ld + ray[j]] = es[k];
-                    P.encT[(size_t)(32 + 8 * g + k) * P.ld + ray[j]] = ec[k];
+        for (uint32_t k = 0; k < 8; k += 2) {
+            float zs[TILES][2];
+#pragma unroll
+            for (uint32_t kq = 0; kq < 2; kq++) {
+                const float* F = P.fourier + (8 * g + k + kq) * 6;
+                const float f0 = F[0], f1 = F[1], f2 = F[2], f3 = F[3], f4 = F[4], f5 = F[5];
+#pragma unroll
+                for (uint32_t j = 0; j < TILES; j++) {
+                    float z = f0 * x[j][0];
+                    z = __builtin_fmaf(f1, x[j][1], z);
+                    z = __builtin_fmaf(f2, x[j][2], z);
+                    z = __builtin_fmaf(f3, x[j][3], z);
+                    z = __builtin_fmaf(f4, x[j][4], z);
+                    z = __builtin_fmaf(f5, x[j][5], z);
+                    zs[j][kq] = z;
                 }
             }
-            b1[j][0] = pack8(es);
-            b1[j][1] = pack8(ec);
+#pragma unroll
+            for (uint32_t j = 0; j < TILES; j++) {
+                es[j][k / 2] = cvt2(__sinf(zs[j][0]), __sinf(zs[j][1]));
+                ec[j][k / 2] = cvt2(__cosf(zs[j][0]), __cosf(zs[j][1]));
+                if (TRAIN && valid[j]) {
+                    for (uint32_t kq = 0; kq < 2; kq++) {
+                        P.encT[(size_t)(8 * g + k + kq) * P.ld + ray[j]] = (uint16_t)(es[j][k / 2] >> (16 * kq));
+                        P.encT[(size_t)(32 + 8 * g + k + kq) * P.ld + ray[j]] = (uint16_t)(ec[j][k / 2] >> (16 * kq));
+                    }
+                }
+            }
         }
-        // layer 1 (K = 64) from registers
+#pragma unroll
+        for (uint32_t j = 0; j < TILES; j++) {
+            b1[j][0] = as_bf16x8(make_uint4(es[j][0], es[j][1], es[j][2], es[j][3]));
+            b1[j][1] = as_bf16x8(make_uint4(ec[j][0], ec[j][1], ec[j][2], ec[j][3]));
+        }
+        // layer 1 (K = 64: sin, then cos features)
+        uint2 h[8][TILES];
 #pragma unroll
         for (uint32_t t = 0; t < 8; t++) {
-            const bf16x8 a0 = ld8(P.W1 + (16 * t + r) * kIn + 8 * g);
-            const bf16x8 a1 = ld8(P.W1 + (16 * t + r) * kIn + 32 + 8 * g);
+            const bf16x8 a0 = as_bf16x8(wl[(2 * t) * 64 + l]);
+            const bf16x8 a1 = as_bf16x8(wl[(2 * t + 1) * 64 + l]);
+            const float4 bias = *reinterpret_cast<const float4*>(P.p + ob1 + 16 * t + 4 * g);
 #pragma unroll
-            for (uint32_t j = 0; j < kTiles; j++) {
-                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (uint32_t j = 0; j < TILES; j++) {
+                f32x4 acc = {bias.x, bias.y, bias.z, bias.w};
                 acc = mfma(a0, b1[j][0], acc);
                 acc = mfma(a1, b1[j][1], acc);
-                uint16_t hv[4];
-#pragma unroll
-                for (uint32_t i = 0; i < 4; i++) {
-                    const uint32_t nrn = 16 * t + 4 * g + i;
-                    float v = acc[i] + P.p[ob1 + nrn];
-                    v = v > 0.0f ? v : 0.0f;
-                    hv[i] = f2bf(v);
-                    if (P.h1T && valid[j]) P.h1T[(size_t)nrn * P.ld + ray[j]] = hv[i];
-                }
-                store4(buf[0] + (16 * j + r) * kLdsRow + 16 * t + 4 * g, hv);
+                h[t][j] = activate(acc, TRAIN ? P.h1T : nullptr, P.ld, 16 * t + 4 * g, ray[j], valid[j]);
             }
         }
-        __syncthreads();
-        // layers 2 and 3 (K = 128) through LDS; layer 3 folds the output dot
-        float part[kTiles] = {0.0f, 0.0f, 0.0f, 0.0f};
+        // layers 2 and 3 (K = 128) from registers; layer 3 folds the output dot
+        float part[TILES];
+#pragma unroll
+        for (uint32_t j = 0; j < TILES; j++) part[j] = 0.0f;
 #pragma unroll
         for (uint32_t layer = 0; layer < 2; layer++) {
-            const uint16_t* Wb = layer == 0 ? P.W2 : P.W3;
-            const float* bias = P.p + (layer == 0 ? ob2 : ob3);
-            uint16_t* hT = layer == 0 ? P.h2T : P.h3T;
-            const uint16_t* xin = buf[layer];
-            uint16_t* xout = buf[layer ^ 1];
-            bf16x8 b[kTiles][4];
-#pragma unroll
-            for (uint32_t j = 0; j < kTiles; j++)
-#pragma unroll
-                for (uint32_t k = 0; k < 4; k++) b[j][k] = ld8(xin + (16 * j + r) * kLdsRow + 32 * k + 8 * g);
+            const uint32_t f0 = kFrag1 + layer * kFrag2;
+            const float* bias_l = P.p + (layer == 0 ? ob2 : ob3);
+            uint16_t* hT = TRAIN ? (layer == 0 ? P.h2T : P.h3T) : nullptr;
+            uint2 hn[8][TILES];
 #pragma unroll
             for (uint32_t t = 0; t < 8; t++) {
-                f32x4 acc[kTiles];
+                const float4 bias = *reinterpret_cast<const float4*>(bias_l + 16 * t + 4 * g);
+                f32x4 acc[TILES];
 #pragma unroll
-                for (uint32_t j = 0; j < kTiles; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                for (uint32_t j = 0; j < TILES; j++) acc[j] = f32x4{bias.x, bias.y, bias.z, bias.w};
 #pragma unroll
-                for (uint32_t k = 0; k < 4; k++) {
-                    const bf16x8 a = ld8(Wb + (16 * t + r) * kHid + 32 * k + 8 * g);
+                for (uint32_t kk = 0; kk < 4; kk++) {
+                    const bf16x8 a = as_bf16x8(wl[(f0 + 4 * t + kk) * 64 + l]);
 #pragma unroll
-                    for (uint32_t j = 0; j < kTiles; j++) acc[j] = mfma(a, b[j][k], acc[j]);
+                    for (uint32_t j = 0; j < TILES; j++)
+                        acc[j] = mfma(a, join(h[2 * kk][j], h[2 * kk + 1][j]), acc[j]);
                 }
+                const uint2 w4p = *reinterpret_cast<const uint2*>(P.w4 + 16 * t + 4 * g);
 #pragma unroll
-                for (uint32_t j = 0; j < kTiles; j++) {
-                    uint16_t hv[4];
-#pragma unroll
-                    for (uint32_t i = 0; i < 4; i++) {
-                        const uint32_t nrn = 16 * t + 4 * g + i;
-                        float v = acc[j][i] + bias[nrn];
-                        v = v > 0.0f ? v : 0.0f;
-                        hv[i] = f2bf(v);
-                        if (hT && valid[j]) hT[(size_t)nrn * P.ld + ray[j]] = hv[i];
-                        if (layer == 1) part[j] = part[j] + bf2f(P.w4[nrn]) * bf2f(hv[i]);
-                    }
-                    if (layer == 0) store4(xout + (16 * j + r) * kLdsRow + 16 * t + 4 * g, hv);
+                for (uint32_t j = 0; j < TILES; j++) {
+                    hn[t][j] = activate(acc[j], hT, P.ld, 16 * t + 4 * g, ray[j], valid[j]);
+                    if (layer == 1) part[j] = dot2(hn[t][j].y, w4p.y, dot2(hn[t][j].x, w4p.x, part[j]));
                 }
             }
-            __syncthreads();
+            if (layer == 0) {
+#pragma unroll
+                for (uint32_t t = 0; t < 8; t++)
+#pragma unroll
+                    for (uint32_t j = 0; j < TILES; j++) h[t][j] = hn[t][j];
+            }
         }
 #pragma unroll
-        for (uint32_t j = 0; j < kTiles; j++) {
+        for (uint32_t j = 0; j < TILES; j++) {
             float pj = part[j];
             pj += __shfl_xor(pj, 16, 64);
             pj += __shfl_xor(pj, 32, 64);
@@ -225,10 +300,9 @@ __global__ void __launch_bounds__(64) k_mlp_fwd(FwdParams P) {
                     const float e = y + P.margin;
                     const float tb = (e < P.max_log) ? __expf(e) : kFltMax;
                     if (P.camera) {
-                        const uint32_t qy = ray[j] / Wq, qx = ray[j] - qy * Wq;
-                        for (uint32_t yy = qy * s; yy < min(P.H, qy * s + s); yy++)
-                            for (uint32_t xx = qx * s; xx < min(P.W, qx * s + s); xx++)
-                                P.tbound[(size_t)yy * P.W + xx] = tb;
+                        for (uint32_t yy = qy[j] * s; yy < min(P.H, qy[j] * s + s); yy++)
+                            for (uint32_t xx = qx[j] * s; xx < min(P.W, qx[j] * s + s); xx++)
+                                P.tbound[(size_t)fbase[j] + (size_t)yy * P.W + xx] = tb;
                     } else {
                         P.tbound[ray[j]] = tb;
                     }
@@ -343,9 +417,27 @@ __global__ void k_sum(const float* v, uint32_t n, float* out) {
     if (threadIdx.x == 0) out[0] += red[0];
 }
 
+// BF16 copies of the master weights: W1, W2, W3 and w4 row-major, W2 and
+// W3 transposed (backward), and W1, W2, W3 as the forward kernel's MFMA
+// operand fragments (Wf: fragment f, lane l = 16 g + r, 8 bf16; layer 1
+// rows 16 t + r, K 32 kk + 8 g + p; layers 2 and 3 K order hidden_k).
 __global__ void k_refresh(const float* p, uint16_t* W1, uint16_t* W2, uint16_t* W3, uint16_t* w4, uint16_t* W2T,
-                          uint16_t* W3T) {
+                          uint16_t* W3T, uint16_t* Wf) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kFragAll * 64 * 8) {
+        const uint32_t pp = i & 7u, lane = (i >> 3) & 63u, f = i >> 9;
+        const uint32_t g = lane >> 4, r = lane & 15u;
+        float w;
+        if (f < kFrag1) {
+            const uint32_t t = f >> 1, kk = f & 1u;
+            w = p[oW1 + (16 * t + r) * kIn + 32 * kk + 8 * g + pp];
+        } else {
+            const uint32_t f2 = (f - kFrag1) % kFrag2, t = f2 >> 2, kk = f2 & 3u;
+            const uint32_t base = f < kFrag1 + kFrag2 ? oW2 : oW3;
+            w = p[base + (16 * t + r) * kHid + hidden_k(kk, g, pp)];
+        }
+        Wf[i] = f2bf(w);
+    }
     if (i < kHid * kIn) W1[i] = f2bf(p[oW1 + i]);
     if (i < kHid * kHid) {
         const uint32_t a = i / kHid, b = i % kHid;
@@ -394,6 +486,12 @@ struct hulk_mlp {
     float* v = nullptr;
     float* fourier = nullptr;
     uint16_t *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *w4 = nullptr, *W2T = nullptr, *W3T = nullptr;
+    uint16_t* Wf = nullptr;  // forward operand fragments (k_refresh)
+    int num_cus = 256;
+    Camera* d_cams = nullptr;  // cameras of a multi-frame bound call (device, pinned staging)
+    Camera* h_cams = nullptr;
+    uint32_t cam_cap = 0;
+    hipEvent_t cams_ev = nullptr;
     uint16_t *encT = nullptr, *h1T = nullptr, *h2T = nullptr, *h3T = nullptr;
     uint16_t *dz3T = nullptr, *dz3 = nullptr, *dz2T = nullptr, *dz2 = nullptr, *dz1T = nullptr, *dz1 = nullptr;
     float *pred = nullptr, *dy = nullptr, *loss = nullptr, *host_loss = nullptr;
@@ -431,8 +529,8 @@ struct Rng {
     } while (0)
 
 int mlp_refresh(hulk_mlp* m) {
-    hipLaunchKernelGGL(k_refresh, dim3((kHid * kHid + 255) / 256), dim3(256), 0, m->stream, m->p, m->W1, m->W2, m->W3,
-                       m->w4, m->W2T, m->W3T);
+    hipLaunchKernelGGL(k_refresh, dim3((kFragAll * 64 * 8 + 255) / 256), dim3(256), 0, m->stream, m->p, m->W1, m->W2,
+                       m->W3, m->w4, m->W2T, m->W3T, m->Wf);
     MLP_TRY(m, hipGetLastError());
     return HULK_OK;
 }
@@ -445,18 +543,24 @@ FwdParams fwd_params(hulk_mlp* m) {
         P.ih[i] = m->ih[i];
     }
     P.fourier = m->fourier;
-    P.W1 = m->W1;
-    P.W2 = m->W2;
-    P.W3 = m->W3;
+    P.Wf = reinterpret_cast<const uint4*>(m->Wf);
     P.w4 = m->w4;
     P.p = m->p;
     return P;
 }
 
 int launch_fwd(hulk_mlp* m, const FwdParams& P) {
-    const uint32_t groups = (P.n + 63u) / 64u;
-    const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(groups, 8192u));
-    hipLaunchKernelGGL(k_mlp_fwd, dim3(grid), dim3(64), 0, m->stream, P);
+    // one block per CU (the 80 KiB of weight fragments in LDS), fewer when
+    // there are fewer groups of 64 rays than waves
+    const bool train = P.encT != nullptr;
+    const uint32_t per_group = 16u * (train ? kTrainTiles : kTiles);
+    const uint32_t groups = (P.n + per_group - 1) / per_group;
+    const uint32_t grid = std::max<uint32_t>(
+        1u, std::min<uint32_t>((groups + kFwdWaves - 1) / kFwdWaves, static_cast<uint32_t>(m->num_cus)));
+    if (train)
+        hipLaunchKernelGGL((k_mlp_fwd<true, kTrainTiles>), dim3(grid), dim3(64 * kFwdWaves), 0, m->stream, P);
+    else
+        hipLaunchKernelGGL((k_mlp_fwd<false, kTiles>), dim3(grid), dim3(64 * kFwdWaves), 0, m->stream, P);
     MLP_TRY(m, hipGetLastError());
     return HULK_OK;
 }
@@ -543,12 +647,15 @@ extern "C" uint32_t hulk_mlp_param_count(void) { return kParams; }
 extern "C" void hulk_mlp_destroy(hulk_mlp* m) {
     if (!m) return;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
-    void* bufs[] = {m->p, m->grad, m->m, m->v, m->fourier, m->W1, m->W2, m->W3, m->w4, m->W2T, m->W3T,
+    void* bufs[] = {m->p, m->grad, m->m, m->v, m->fourier, m->W1, m->W2, m->W3, m->w4, m->W2T, m->W3T, m->Wf,
                     m->encT, m->h1T, m->h2T, m->h3T, m->dz3T, m->dz3, m->dz2T, m->dz2, m->dz1T, m->dz1,
                     m->pred, m->dy, m->loss};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (m->host_loss) (void)hipHostFree(m->host_loss);
+    if (m->d_cams) (void)hipFree(m->d_cams);
+    if (m->h_cams) (void)hipHostFree(m->h_cams);
+    if (m->cams_ev) (void)hipEventDestroy(m->cams_ev);
     delete m;
 }
 
@@ -564,6 +671,10 @@ extern "C" int hulk_mlp_create(hulk_ctx* ctx, uint64_t seed, uint32_t max_batch,
     m->ctx = ctx;
     m->stream = static_cast<hipStream_t>(hulk_stream(ctx));
     m->max_batch = max_batch;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        m->num_cus = prop.multiProcessorCount;
     for (int i = 0; i < 3; i++) {
         m->c[i] = (box[i] + box[3 + i]) * 0.5f;
         m->ih[i] = 1.0f / ((box[3 + i] - box[i]) * 0.5f);
@@ -576,7 +687,8 @@ extern "C" int hulk_mlp_create(hulk_ctx* ctx, uint64_t seed, uint32_t max_batch,
         {(void**)&m->p, kParams * 4}, {(void**)&m->grad, kParams * 4}, {(void**)&m->m, kParams * 4},
         {(void**)&m->v, kParams * 4}, {(void**)&m->fourier, 32 * 6 * 4}, {(void**)&m->W1, kHid * kIn * 2},
         {(void**)&m->W2, kHid * kHid * 2}, {(void**)&m->W3, kHid * kHid * 2}, {(void**)&m->w4, kHid * 2},
-        {(void**)&m->W2T, kHid * kHid * 2}, {(void**)&m->W3T, kHid * kHid * 2}, {(void**)&m->encT, kIn * nb * 2},
+        {(void**)&m->W2T, kHid * kHid * 2}, {(void**)&m->W3T, kHid * kHid * 2}, {(void**)&m->Wf, kFragAll * 64 * 16},
+        {(void**)&m->encT, kIn * nb * 2},
         {(void**)&m->h1T, kHid * nb * 2}, {(void**)&m->h2T, kHid * nb * 2}, {(void**)&m->h3T, kHid * nb * 2},
         {(void**)&m->dz3T, kHid * nb * 2}, {(void**)&m->dz3, kHid * nb * 2}, {(void**)&m->dz2T, kHid * nb * 2},
         {(void**)&m->dz2, kHid * nb * 2}, {(void**)&m->dz1T, kHid * nb * 2}, {(void**)&m->dz1, kHid * nb * 2},
@@ -588,7 +700,8 @@ extern "C" int hulk_mlp_create(hulk_ctx* ctx, uint64_t seed, uint32_t max_batch,
             return HULK_E_ALLOC;
         }
     }
-    if (hipHostMalloc(reinterpret_cast<void**>(&m->host_loss), 64, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&m->host_loss), 64, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&m->cams_ev, hipEventDisableTiming) != hipSuccess) {
         hulk_mlp_destroy(m);
         return HULK_E_ALLOC;
     }
@@ -676,22 +789,59 @@ extern "C" int hulk_mlp_train_step(hulk_mlp* m, const hulk_ray* rays_dev, const 
     return HULK_OK;
 }
 
-extern "C" int hulk_mlp_bound_camera(hulk_mlp* m, const hulk_camera* cam, uint32_t W, uint32_t H, float margin,
-                                     float max_log, uint32_t stride, float* tbound_dev, uint32_t flags) {
-    if (!m || !cam || !tbound_dev) return HULK_E_ILLEGAL_INPUT;
+namespace {
+
+Camera to_camera(const hulk_camera& c) {
+    Camera k;
+    k.from[0] = c.from.x; k.from[1] = c.from.y; k.from[2] = c.from.z;
+    k.vf[0] = c.vertical_factor.x; k.vf[1] = c.vertical_factor.y; k.vf[2] = c.vertical_factor.z;
+    k.hf[0] = c.horizontal_factor.x; k.hf[1] = c.horizontal_factor.y; k.hf[2] = c.horizontal_factor.z;
+    k.ll[0] = c.lower_left.x; k.ll[1] = c.lower_left.y; k.ll[2] = c.lower_left.z;
+    return k;
+}
+
+// Bounds of n_cams frames of W x H pixels in one launch: frame f's bounds
+// start at tbound + f * W * H.
+int bound_frames(hulk_mlp* m, const hulk_camera* cams, uint32_t n_cams, uint32_t W, uint32_t H, float margin,
+                 float max_log, uint32_t stride, float* tbound_dev, uint32_t flags) {
+    if (!m || !cams || !tbound_dev || n_cams == 0) return HULK_E_ILLEGAL_INPUT;
+    if (stride == 0 || stride > 8 || W == 0 || H == 0) {
+        m->err = "stride must be in [1, 8] and the frame non-empty";
+        return HULK_E_ILLEGAL_INPUT;
+    }
+    const uint64_t per_frame = (uint64_t)((W + stride - 1) / stride) * ((H + stride - 1) / stride);
+    if (per_frame * n_cams > 0xFFFFFFC0ull || (uint64_t)W * H * n_cams > 0xFFFFFFFFull) {
+        m->err = "too many pixels in one call";
+        return HULK_E_ILLEGAL_INPUT;
+    }
     FwdParams P = fwd_params(m);
     P.camera = 1;
-    Camera k;
-    k.from[0] = cam->from.x; k.from[1] = cam->from.y; k.from[2] = cam->from.z;
-    k.vf[0] = cam->vertical_factor.x; k.vf[1] = cam->vertical_factor.y; k.vf[2] = cam->vertical_factor.z;
-    k.hf[0] = cam->horizontal_factor.x; k.hf[1] = cam->horizontal_factor.y; k.hf[2] = cam->horizontal_factor.z;
-    k.ll[0] = cam->lower_left.x; k.ll[1] = cam->lower_left.y; k.ll[2] = cam->lower_left.z;
-    P.cam = k;
-    if (stride == 0 || stride > 8) return HULK_E_ILLEGAL_INPUT;
+    P.cam = to_camera(cams[0]);
+    if (n_cams > 1) {
+        if (n_cams > m->cam_cap) {
+            if (m->d_cams) (void)hipFree(m->d_cams);
+            if (m->h_cams) (void)hipHostFree(m->h_cams);
+            m->d_cams = nullptr;
+            m->h_cams = nullptr;
+            m->cam_cap = 0;
+            MLP_TRY(m, hipMalloc(reinterpret_cast<void**>(&m->d_cams), n_cams * sizeof(Camera)));
+            MLP_TRY(m, hipHostMalloc(reinterpret_cast<void**>(&m->h_cams), n_cams * sizeof(Camera), hipHostMallocDefault));
+            m->cam_cap = n_cams;
+        }
+        // the pinned staging copy may still feed the previous call's upload
+        MLP_TRY(m, hipEventSynchronize(m->cams_ev));
+        for (uint32_t f = 0; f < n_cams; f++) m->h_cams[f] = to_camera(cams[f]);
+        MLP_TRY(m, hipMemcpyAsync(m->d_cams, m->h_cams, n_cams * sizeof(Camera), hipMemcpyHostToDevice, m->stream));
+        MLP_TRY(m, hipEventRecord(m->cams_ev, m->stream));
+        P.cams = m->d_cams;
+    }
     P.W = W;
     P.H = H;
     P.stride = stride;
-    P.n = ((W + stride - 1) / stride) * ((H + stride - 1) / stride);
+    P.per_frame = static_cast<uint32_t>(per_frame);
+    P.pf_m = udiv_multiplier(P.per_frame);
+    P.wq_m = udiv_multiplier((W + stride - 1) / stride);
+    P.n = static_cast<uint32_t>(per_frame * n_cams);
     P.tbound = tbound_dev;
     P.margin = margin;
     P.max_log = max_log;
@@ -699,4 +849,17 @@ extern "C" int hulk_mlp_bound_camera(hulk_mlp* m, const hulk_camera* cam, uint32
     if (r) return r;
     if (!(flags & HULK_FLAG_NO_SYNC)) MLP_TRY(m, hipStreamSynchronize(m->stream));
     return HULK_OK;
+}
+
+}  // namespace
+
+extern "C" int hulk_mlp_bound_camera(hulk_mlp* m, const hulk_camera* cam, uint32_t W, uint32_t H, float margin,
+                                     float max_log, uint32_t stride, float* tbound_dev, uint32_t flags) {
+    return bound_frames(m, cam, 1, W, H, margin, max_log, stride, tbound_dev, flags);
+}
+
+extern "C" int hulk_mlp_bound_cameras(hulk_mlp* m, const hulk_camera* cams, uint32_t n_cams, uint32_t W, uint32_t H,
+                                      float margin, float max_log, uint32_t stride, float* tbound_dev,
+                                      uint32_t flags) {
+    return bound_frames(m, cams, n_cams, W, H, margin, max_log, stride, tbound_dev, flags);
 }

@@ -100,6 +100,17 @@ class HitDistanceMLP:
         self._c(lib().hulk_mlp_bound_camera(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, m, max_log,
                                             st, ptr(tbound_dev), 0 if sync else L.FLAG_NO_SYNC))
 
+    def bound_cameras(self, cams, W: int, H: int, tbound_dev, margin: float | None = None, max_log: float = 60.0,
+                      stride: int | None = None, sync: bool = True):
+        """Bounds of several frames (same size) in one launch: frame f's at
+        tbound_dev[f * W * H:]."""
+        self._order()
+        m = self.margin if margin is None else margin
+        st = self.stride if stride is None else stride
+        arr = np.ascontiguousarray(np.stack([np.asarray(c, np.float32) for c in cams]), np.float32)
+        self._c(lib().hulk_mlp_bound_cameras(self.h, ptr(arr), len(cams), W, H, m, max_log, st, ptr(tbound_dev),
+                                             0 if sync else L.FLAG_NO_SYNC))
+
     # ------------------------------------------------------------ training --
 
     def fit(self, rays, y, steps: int, batch: int = 16384, lr: float = 2e-3, seed: int = 0, log_every: int = 0):

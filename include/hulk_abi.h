@@ -390,6 +390,10 @@ int hulk_mlp_train_step(hulk_mlp* mlp, const hulk_ray* rays, const float* target
  * the bound. */
 int hulk_mlp_bound_camera(hulk_mlp* mlp, const hulk_camera* cam, uint32_t width, uint32_t height, float margin,
                           float max_log, uint32_t stride, float* tbound, uint32_t flags);
+/* The bounds of n_cams frames of the same size in one launch; frame f's
+ * bounds start at tbound + f * width * height. */
+int hulk_mlp_bound_cameras(hulk_mlp* mlp, const hulk_camera* cams, uint32_t n_cams, uint32_t width, uint32_t height,
+                           float margin, float max_log, uint32_t stride, float* tbound, uint32_t flags);
 /* Primary rays of a camera as ray records (device) and, optionally, each
  * ray's scene-box exit time (0 if it misses the box): training input of
  * the bound (castRay, GPU_common.cl:10-30; box test SharedFunctions.hpp:245-264). */

@@ -194,3 +194,24 @@ def test_fit_generalizes_on_held_out_pairs():
     loss, _ = mlp.loss_grad(hr, hy, 16384)
     var = float(hy.var())
     assert loss < 0.1 * var, (loss, var)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stride,W,H", [(2, 256, 192), (1, 96, 64), (3, 100, 70)])
+def test_bound_cameras_equals_per_frame_calls(stride, W, H):
+    """hulk_mlp_bound_cameras (every frame in one launch) writes each frame's
+    bounds exactly as hulk_mlp_bound_camera does for that frame alone,
+    ragged frame sizes and strides included."""
+    import torch
+
+    s, ctx, mlp, _, _ = _gpu_setup()
+    cams = [s.camera(k, H / W) for k in (1, 5, 9, 13)]
+    n = W * H
+    one = torch.full((len(cams) * n,), -1.0, dtype=torch.float32, device="cuda")
+    sep = torch.full((len(cams) * n,), -2.0, dtype=torch.float32, device="cuda")
+    mlp.bound_cameras(cams, W, H, one, margin=0.1, stride=stride)
+    for k, cam in enumerate(cams):
+        mlp.bound_camera(cam, W, H, sep[k * n:(k + 1) * n], margin=0.1, stride=stride)
+    a, b = one.cpu().numpy(), sep.cpu().numpy()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert (a > 0).all()  # every pixel of every frame written
