@@ -38,6 +38,8 @@
 // quat.cpp), restated below from those steps.  The camera follows the
 // "camera" transform (AFFParser::getTransformation, :1541-1563, and
 // GLVisitorCPUTrace::updateCamera, GLVisitorCPUTrace.cpp:105-148).
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -105,196 +107,203 @@ hulk_float4 apply(const hulk_float4& v, const Mat& M) {
 }
 
 // --------------------------------------------------- key-frame splines --
+//
+// Kochanek-Bartels splines (D. Kochanek and R. Bartels, "Interpolating
+// splines with local tension, continuity, and bias control", SIGGRAPH 1984).
+// A key has a time T, a value V and parameters (tension t, continuity c,
+// bias b).  At key i, with the chords a = V_i - V_{i-1} and e = V_{i+1} - V_i,
+// the tangent that leaves the key (start of the segment i -> i+1) and the one
+// that arrives at it (end of the segment i-1 -> i) are
+//   leave  = (1-t)(1+c)(1+b)/2 a + (1-t)(1-c)(1-b)/2 e
+//   arrive = (1-t)(1-c)(1+b)/2 a + (1-t)(1+c)(1-b)/2 e
+// and, keys being unevenly spaced, each is rescaled to the span of the
+// segment it belongs to (section 5 of the paper): leave by
+// 2 (T_{i+1} - T_i) / (T_{i+1} - T_{i-1}), arrive by 2 (T_i - T_{i-1}) /
+// (T_{i+1} - T_{i-1}).  Only keys with two neighbours get tangents, so n keys
+// make n - 3 cubic Hermite segments over [T_1, T_{n-2}]; earlier and later
+// times clamp to the ends.
+//
+// Rotations take the same tangents in the tangent space of unit quaternions
+// (chords log(q_{i-1}^-1 q_i)) and are interpolated with Shoemake's
+// spherical quadrangle (K. Shoemake, "Animating rotation with quaternion
+// curves", SIGGRAPH 1985): between q_i and q_{i+1}, with the chord
+// h = log(q_i^-1 q_{i+1}), the inner control points are
+// q_i exp((leave_i - h) / 2) and q_{i+1} exp((h - arrive_{i+1}) / 2).
+// Everything is evaluated in double precision.
 
 struct Key {
     double t, v[4];  // value: xyz (+ angle in radians for rotations)
     double tension, continuity, bias;
 };
 
-// Tangent multipliers of a Kochanek-Bartels key with the time adjustment
-// of unevenly spaced keys (kbsplpos.cpp / kbsplrot.cpp, KB_*Initialize).
-void multipliers(const Key& k0, const Key& k1, const Key& k2, const Key& k3, double& out0, double& out1,
-                 double& in0, double& in1) {
-    const double omt0 = 1 - k1.tension, omc0 = 1 - k1.continuity, opc0 = 1 + k1.continuity;
-    const double omb0 = 1 - k1.bias, opb0 = 1 + k1.bias;
-    const double adj0 = 2 * (k2.t - k1.t) / (k2.t - k0.t);
-    out0 = 0.5 * adj0 * omt0 * opc0 * opb0;
-    out1 = 0.5 * adj0 * omt0 * omc0 * omb0;
-    const double omt1 = 1 - k2.tension, omc1 = 1 - k2.continuity, opc1 = 1 + k2.continuity;
-    const double omb1 = 1 - k2.bias, opb1 = 1 + k2.bias;
-    const double adj1 = 2 * (k2.t - k1.t) / (k3.t - k1.t);
-    in0 = 0.5 * adj1 * omt1 * omc1 * opb1;
-    in1 = 0.5 * adj1 * omt1 * opc1 * omb1;
+// A value in a spline's tangent space: a point or a quaternion log.
+struct Vec3 {
+    double c[3];
+    Vec3 operator+(const Vec3& o) const { return {{c[0] + o.c[0], c[1] + o.c[1], c[2] + o.c[2]}}; }
+    Vec3 operator-(const Vec3& o) const { return {{c[0] - o.c[0], c[1] - o.c[1], c[2] - o.c[2]}}; }
+    Vec3 operator*(double k) const { return {{c[0] * k, c[1] * k, c[2] * k}}; }
+    double norm() const { return std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]); }
+};
+
+// Leaving and arriving tangents of key `at` (time-scaled), from the chords
+// to and from its neighbours.
+struct KeyTangents {
+    Vec3 leave, arrive;
+};
+KeyTangents kb_tangents(const Key& prev, const Key& at, const Key& next, const Vec3& chord_in, const Vec3& chord_out) {
+    const double slack = 1.0 - at.tension, cp = 1.0 + at.continuity, cm = 1.0 - at.continuity;
+    const double bp = 1.0 + at.bias, bm = 1.0 - at.bias;
+    const double span = next.t - prev.t;
+    const double to_next = 2.0 * (next.t - at.t) / span, to_prev = 2.0 * (at.t - prev.t) / span;
+    KeyTangents k;
+    k.leave = (chord_in * (0.5 * slack * cp * bp) + chord_out * (0.5 * slack * cm * bm)) * to_next;
+    k.arrive = (chord_in * (0.5 * slack * cm * bp) + chord_out * (0.5 * slack * cp * bm)) * to_prev;
+    return k;
 }
 
-// Segment i covers [keys[i+1].t, keys[i+2].t]; times outside the first /
-// last segment clamp to its start / end.
-template <class Seg>
-void locate(const std::vector<Seg>& seg, double t, size_t& i, double& u) {
-    if (seg[0].tmin < t) {
-        if (t < seg.back().tmax) {
-            for (i = 0; i < seg.size(); i++)
-                if (t < seg[i].tmax) break;
-            u = (t - seg[i].tmin) / seg[i].trange;
-        } else {
-            i = seg.size() - 1;
-            u = 1.0;
-        }
-    } else {
-        i = 0;
+// Segment index and local parameter in [0, 1] of time `time` over segments
+// starting at `starts` (ascending) and the last one ending at `end`.
+void find_segment(const std::vector<double>& starts, double end, double time, size_t& seg, double& u) {
+    if (!(time > starts.front())) {
+        seg = 0;
         u = 0.0;
+        return;
     }
+    if (!(time < end)) {
+        seg = starts.size() - 1;
+        u = 1.0;
+        return;
+    }
+    seg = static_cast<size_t>(std::upper_bound(starts.begin(), starts.end(), time) - starts.begin()) - 1;
+    const double stop = seg + 1 < starts.size() ? starts[seg + 1] : end;
+    u = (time - starts[seg]) / (stop - starts[seg]);
 }
 
+// Translation / scale: one cubic p(u) = P + u (S + u (C2 + u C3)) per segment.
 struct PosSpline {
-    struct Seg {
-        double c0[3], c1[3], c2[3], c3[3];
-        double tmin, tmax, trange;
-    };
-    std::vector<Seg> seg;
-    void init(const std::vector<Key>& k) {
-        for (size_t i0 = 0; i0 + 3 < k.size(); i0++) {
-            const Key &k0 = k[i0], &k1 = k[i0 + 1], &k2 = k[i0 + 2], &k3 = k[i0 + 3];
-            double out0, out1, in0, in1;
-            multipliers(k0, k1, k2, k3, out0, out1, in0, in1);
-            Seg s;
-            for (int a = 0; a < 3; a++) {
-                const double dp = k2.v[a] - k1.v[a];
-                const double tout = out1 * dp + out0 * (k1.v[a] - k0.v[a]);
-                const double tin = in1 * (k3.v[a] - k2.v[a]) + in0 * dp;
-                s.c0[a] = k1.v[a];
-                s.c1[a] = tout;
-                s.c2[a] = 3 * dp - 2 * tout - tin;
-                s.c3[a] = -2 * dp + tout + tin;
-            }
-            s.tmin = k1.t;
-            s.tmax = k2.t;
-            s.trange = s.tmax - s.tmin;
-            seg.push_back(s);
+    std::vector<double> starts;
+    double end = 0.0;
+    std::vector<std::array<Vec3, 4>> cubic;  // P, S, C2, C3
+    void init(const std::vector<Key>& keys) {
+        const size_t n = keys.size();
+        auto value = [&](size_t i) { return Vec3{{keys[i].v[0], keys[i].v[1], keys[i].v[2]}}; };
+        std::vector<KeyTangents> tan(n);
+        for (size_t i = 1; i + 1 < n; i++)
+            tan[i] = kb_tangents(keys[i - 1], keys[i], keys[i + 1], value(i) - value(i - 1), value(i + 1) - value(i));
+        for (size_t i = 1; i + 2 < n; i++) {
+            const Vec3 p0 = value(i), p1 = value(i + 1), step = p1 - p0;
+            const Vec3& m0 = tan[i].leave;
+            const Vec3& m1 = tan[i + 1].arrive;
+            // Hermite to power basis: C2 = 3 (p1 - p0) - 2 m0 - m1, C3 = 2 (p0 - p1) + m0 + m1
+            cubic.push_back({p0, m0, step * 3.0 - m0 * 2.0 - m1, step * -2.0 + m0 + m1});
+            starts.push_back(keys[i].t);
         }
+        end = keys[n - 2].t;
     }
-    void eval(double t, double out[3]) const {
-        size_t i;
+    void eval(double time, double out[3]) const {
+        size_t k;
         double u;
-        locate(seg, t, i, u);
-        const Seg& s = seg[i];
-        for (int a = 0; a < 3; a++) out[a] = s.c0[a] + u * (s.c1[a] + u * (s.c2[a] + u * s.c3[a]));
+        find_segment(starts, end, time, k, u);
+        const auto& q = cubic[k];
+        for (int a = 0; a < 3; a++) out[a] = q[0].c[a] + u * (q[1].c[a] + u * (q[2].c[a] + u * q[3].c[a]));
     }
 };
 
-// Quaternions (w, x, y, z) with the operations of quat.cpp.
-struct Quat {
-    double w, x, y, z;
+// Unit quaternion w + (x, y, z).
+struct Quaternion {
+    double w;
+    Vec3 v;
+    Quaternion operator*(const Quaternion& o) const {
+        const Vec3& a = v;
+        const Vec3& b = o.v;
+        const Vec3 cross = {{a.c[1] * b.c[2] - a.c[2] * b.c[1], a.c[2] * b.c[0] - a.c[0] * b.c[2],
+                             a.c[0] * b.c[1] - a.c[1] * b.c[0]}};
+        const double dot = a.c[0] * b.c[0] + a.c[1] * b.c[1] + a.c[2] * b.c[2];
+        return {w * o.w - dot, b * w + a * o.w + cross};
+    }
+    Quaternion conj() const { return {w, v * -1.0}; }
+    double dot(const Quaternion& o) const { return w * o.w + v.c[0] * o.v.c[0] + v.c[1] * o.v.c[1] + v.c[2] * o.v.c[2]; }
 };
-constexpr double kQuatEps = 1e-03;  // sin(angle) cut-off of Log / Exp / Slerp
 
-Quat from_angle_axis(double angle, double ax, double ay, double az) {
-    const double h = 0.5 * angle, sn = std::sin(h);
-    return {std::cos(h), sn * ax, sn * ay, sn * az};
+// sin(x) / x, by its series where the quotient loses digits.
+double sinc(double x) { return std::fabs(x) < 1e-4 ? 1.0 - x * x / 6.0 : std::sin(x) / x; }
+
+// log of a unit quaternion (cos h, sin h n) = h n, h in [0, pi]; atan2 keeps
+// h accurate for small and for nearly opposite rotations alike.
+Vec3 qlog(const Quaternion& q) {
+    const double s = q.v.norm();
+    if (s == 0.0) return Vec3{{0.0, 0.0, 0.0}};
+    return q.v * (std::atan2(s, q.w) / s);
 }
-void to_angle_axis(const Quat& q, double& angle, double& ax, double& ay, double& az) {
-    const double l2 = q.x * q.x + q.y * q.y + q.z * q.z;
-    if (l2 > 0.0) {
-        angle = 2.0 * std::acos(q.w);
-        const double inv = 1.0 / std::sqrt(l2);
-        ax = q.x * inv;
-        ay = q.y * inv;
-        az = q.z * inv;
+// exp of a pure quaternion h n = (cos h, sin h n).
+Quaternion qexp(const Vec3& x) {
+    const double h = x.norm();
+    return {std::cos(h), x * sinc(h)};
+}
+Quaternion from_axis_angle(const Key& k) {
+    const double half = 0.5 * k.v[3];
+    return {std::cos(half), Vec3{{k.v[0], k.v[1], k.v[2]}} * std::sin(half)};
+}
+// Great-arc interpolation from a (u = 0) to b (u = 1); nearly equal
+// quaternions blend linearly and renormalise.
+Quaternion slerp(const Quaternion& a, const Quaternion& b, double u) {
+    if (u == 0.0) return a;
+    if (u == 1.0) return b;
+    const double c = std::max(-1.0, std::min(1.0, a.dot(b)));
+    const double ang = std::acos(c);
+    double ka, kb;
+    if (std::sin(ang) < 1e-9) {
+        ka = 1.0 - u;
+        kb = u;
     } else {
-        angle = 0;
-        ax = 1.0;
-        ay = 0.0;
-        az = 0.0;
+        ka = std::sin((1.0 - u) * ang) / std::sin(ang);
+        kb = std::sin(u * ang) / std::sin(ang);
     }
-}
-Quat qmul(const Quat& a, const Quat& b) {
-    return {a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
-            a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z, a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x};
-}
-Quat unit_inverse(const Quat& q) { return {q.w, -q.x, -q.y, -q.z}; }
-double qdot(const Quat& a, const Quat& b) { return a.w * b.w + a.x * b.x + a.y * b.y + a.z * b.z; }
-Quat qscale(const Quat& q, double c) { return {q.w * c, q.x * c, q.y * c, q.z * c}; }
-Quat qadd(const Quat& a, const Quat& b) { return {a.w + b.w, a.x + b.x, a.y + b.y, a.z + b.z}; }
-Quat qsub(const Quat& a, const Quat& b) { return {a.w - b.w, a.x - b.x, a.y - b.y, a.z - b.z}; }
-Quat qlog(const Quat& q) {
-    Quat r{0.0, q.x, q.y, q.z};
-    if (std::fabs(q.w) < 1.0) {
-        const double angle = std::acos(q.w), sn = std::sin(angle);
-        if (std::fabs(sn) >= kQuatEps) {
-            const double coeff = angle / sn;
-            r.x = coeff * q.x;
-            r.y = coeff * q.y;
-            r.z = coeff * q.z;
-        }
-    }
-    return r;
-}
-Quat qexp(const Quat& q) {
-    const double angle = std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z), sn = std::sin(angle);
-    Quat r{std::cos(angle), q.x, q.y, q.z};
-    if (std::fabs(sn) >= kQuatEps) {
-        const double coeff = sn / angle;
-        r.x = coeff * q.x;
-        r.y = coeff * q.y;
-        r.z = coeff * q.z;
-    }
-    return r;
-}
-Quat slerp(double t, const Quat& p, const Quat& q) {
-    double c = qdot(p, q);
-    c = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
-    const double angle = std::acos(c), sn = std::sin(angle);
-    if (sn < kQuatEps) return p;
-    const double inv = 1.0 / sn;
-    const double c0 = std::sin((1.0 - t) * angle) * inv, c1 = std::sin(t * angle) * inv;
-    return {c0 * p.w + c1 * q.w, c0 * p.x + c1 * q.x, c0 * p.y + c1 * q.y, c0 * p.z + c1 * q.z};
-}
-Quat squad(double t, const Quat& p, const Quat& a, const Quat& b, const Quat& q) {
-    return slerp(2 * t * (1 - t), slerp(t, p, q), slerp(t, a, b));
+    Quaternion r{a.w * ka + b.w * kb, a.v * ka + b.v * kb};
+    const double len = std::sqrt(r.dot(r));
+    return {r.w / len, r.v * (1.0 / len)};
 }
 
 struct RotSpline {
-    struct Seg {
-        Quat p, a, b, q;
-        double tmin, tmax, trange;
-    };
-    std::vector<Seg> seg;
-    // keys: v = axis xyz, angle (radians); like the reference, a key whose
-    // quaternion is flipped to keep consecutive keys within 90 degrees is
-    // rewritten in place and the later segments use the rewritten key
-    void init(std::vector<Key> k) {
-        for (size_t i0 = 0; i0 + 3 < k.size(); i0++) {
-            Key &k0 = k[i0], &k1 = k[i0 + 1], &k2 = k[i0 + 2], &k3 = k[i0 + 3];
-            const Quat q0 = from_angle_axis(k0.v[3], k0.v[0], k0.v[1], k0.v[2]);
-            const Quat q1 = from_angle_axis(k1.v[3], k1.v[0], k1.v[1], k1.v[2]);
-            Quat q2 = from_angle_axis(k2.v[3], k2.v[0], k2.v[1], k2.v[2]);
-            const Quat q3 = from_angle_axis(k3.v[3], k3.v[0], k3.v[1], k3.v[2]);
-            if (qdot(q1, q2) < 0.0) {
-                q2 = qscale(q2, -1.0);
-                to_angle_axis(q2, k2.v[3], k2.v[0], k2.v[1], k2.v[2]);
-            }
-            const Quat logdq = qlog(qmul(unit_inverse(q1), q2));
-            double out0, out1, in0, in1;
-            multipliers(k0, k1, k2, k3, out0, out1, in0, in1);
-            const Quat tout = qadd(qscale(logdq, out1), qscale(qlog(qmul(unit_inverse(q0), q1)), out0));
-            const Quat tin = qadd(qscale(logdq, in0), qscale(qlog(qmul(unit_inverse(q2), q3)), in1));
-            Seg s;
-            s.p = q1;
-            s.q = q2;
-            s.a = qmul(q1, qexp(qscale(qsub(tout, logdq), 0.5)));
-            s.b = qmul(q2, qexp(qscale(qsub(logdq, tin), 0.5)));
-            s.tmin = k1.t;
-            s.tmax = k2.t;
-            s.trange = s.tmax - s.tmin;
-            seg.push_back(s);
+    std::vector<double> starts;
+    double end = 0.0;
+    std::vector<std::array<Quaternion, 4>> quad;  // q_i, inner i, inner i+1, q_{i+1}
+    void init(const std::vector<Key>& keys) {
+        const size_t n = keys.size();
+        std::vector<Quaternion> q(n);
+        for (size_t i = 0; i < n; i++) {
+            q[i] = from_axis_angle(keys[i]);
+            // the same orientation on the near hemisphere: consecutive keys
+            // at most 180 degrees apart along the arc
+            if (i > 0 && q[i - 1].dot(q[i]) < 0.0) q[i] = {-q[i].w, q[i].v * -1.0};
         }
+        std::vector<Vec3> chord(n);  // chord[i] = log(q_{i-1}^-1 q_i)
+        for (size_t i = 1; i < n; i++) chord[i] = qlog(q[i - 1].conj() * q[i]);
+        std::vector<KeyTangents> tan(n);
+        for (size_t i = 1; i + 1 < n; i++) tan[i] = kb_tangents(keys[i - 1], keys[i], keys[i + 1], chord[i], chord[i + 1]);
+        for (size_t i = 1; i + 2 < n; i++) {
+            const Vec3& h = chord[i + 1];
+            quad.push_back({q[i], q[i] * qexp((tan[i].leave - h) * 0.5), q[i + 1] * qexp((h - tan[i + 1].arrive) * 0.5),
+                            q[i + 1]});
+            starts.push_back(keys[i].t);
+        }
+        end = keys[n - 2].t;
     }
-    void eval(double t, double out[4]) const {
-        size_t i;
+    // out: axis xyz, angle (radians)
+    void eval(double time, double out[4]) const {
+        size_t k;
         double u;
-        locate(seg, t, i, u);
-        const Seg& s = seg[i];
-        to_angle_axis(squad(u, s.p, s.a, s.b, s.q), out[3], out[0], out[1], out[2]);
+        find_segment(starts, end, time, k, u);
+        const auto& c = quad[k];
+        const Quaternion r = slerp(slerp(c[0], c[3], u), slerp(c[1], c[2], u), 2.0 * u * (1.0 - u));
+        const double s = r.v.norm();
+        out[3] = 2.0 * std::atan2(s, r.w);
+        if (s > 0.0) {
+            for (int a = 0; a < 3; a++) out[a] = r.v.c[a] / s;
+        } else {
+            out[0] = 1.0;
+            out[1] = out[2] = 0.0;
+        }
     }
 };
 

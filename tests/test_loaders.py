@@ -240,3 +240,61 @@ def test_missing_aff_file():
     with pytest.raises(H.HulkError) as e:
         H.Scene.from_aff(GOLDEN / "museum" / "none.aff")
     assert e.value.code == 2
+
+
+def _translation_file(tmp_path, keys, tension=0.0):
+    body = "\n".join(f"  {t} {x} 0 0  {tension} 0 0" for t, x in keys)
+    return _write(tmp_path, "kb.aff", "x mover\n{\nm vertices 3\n 0 0 0\n 1 0 0\n 0 1 0\ntriangles 1\n 0 1 2\n}\n"
+                                      f"k mover\n{{\n transl {len(keys)}\n{body}\n}}\n")
+
+
+def test_kochanek_bartels_closed_forms(tmp_path):
+    """Kochanek-Bartels with tension = continuity = bias = 0 on evenly spaced
+    keys is the Catmull-Rom spline: at the middle of the segment P1 -> P2 the
+    value is (-P0 + 9 P1 + 9 P2 - P3) / 16, at a quarter
+    (-9 P0 + 111 P1 + 29 P2 - 3 P3) / 128.  Tension 1 zeroes every tangent,
+    leaving the smoothstep blend P1 + (P2 - P1)(3u^2 - 2u^3).  Uneven key
+    spacing rescales the tangents to their segment (the paper's adjustment):
+    keys at 0, 1, 3, 4 still give the closed form of the rescaled tangents."""
+    xs = [0.0, 2.0, 5.0, 4.0, 4.0, 7.0]
+    p = _translation_file(tmp_path, list(zip(range(6), xs)))
+    for seg in range(1, 4):
+        x0, x1, x2, x3 = xs[seg - 1:seg + 3]
+        got = H.Scene.from_aff(p, seg + 0.5).vertices[0, 0]
+        assert abs(got - (-x0 + 9 * x1 + 9 * x2 - x3) / 16) < 1e-5, seg
+        got = H.Scene.from_aff(p, seg + 0.25).vertices[0, 0]
+        assert abs(got - (-9 * x0 + 111 * x1 + 29 * x2 - 3 * x3) / 128) < 1e-5, seg
+    p = _translation_file(tmp_path, list(zip(range(6), xs)), tension=1.0)
+    for u in (0.25, 0.5, 0.8):
+        got = H.Scene.from_aff(p, 2 + u).vertices[0, 0]
+        assert abs(got - (xs[2] + (xs[3] - xs[2]) * (3 * u * u - 2 * u ** 3))) < 1e-5, u
+    # uneven spacing: keys at times 0, 1, 3, 4; the segment 1 -> 3 (span 2)
+    # leaves key 1 with 2 * 2 / 3 of the mean chord (P2 - P0) / 2 and arrives
+    # at key 2 with 2 * 2 / 3 of (P3 - P1) / 2, per unit of its parameter
+    ts, vs = [0.0, 1.0, 3.0, 4.0], [0.0, 1.0, 5.0, 2.0]
+    p = _translation_file(tmp_path, list(zip(ts, vs)))
+    m0 = 4.0 / 3.0 * (vs[2] - vs[0]) / 2
+    m1 = 4.0 / 3.0 * (vs[3] - vs[1]) / 2
+    for u in (0.3, 0.5, 0.9):
+        h = (2 * u ** 3 - 3 * u * u + 1) * vs[1] + (u ** 3 - 2 * u * u + u) * m0 + \
+            (-2 * u ** 3 + 3 * u * u) * vs[2] + (u ** 3 - u * u) * m1
+        assert abs(H.Scene.from_aff(p, 1 + 2 * u).vertices[0, 0] - h) < 1e-5, u
+
+
+def test_coaxial_rotation_keys_interpolate_linearly(tmp_path):
+    """Rotation keys about one axis in equal steps (30 degrees per unit of
+    time, tension = continuity = bias = 0): every tangent equals the chord,
+    the inner control points coincide with the keys and the spherical
+    quadrangle reduces to the great arc, so the angle grows linearly: vertex
+    (1, 0, 0) sits at 30 (1 + u) degrees about y between keys 1 and 2.  A
+    step that crosses 180 degrees between keys (key quaternions on opposite
+    hemispheres) still takes the short way."""
+    for step in (30.0, 100.0):
+        body = "\n".join(f"  {t} 0 1 0 {step * t} 0 0 0" for t in range(5))
+        p = _write(tmp_path, "r.aff", "x spin\n{\nm vertices 3\n 1 0 0\n 0 1 0\n 0 0 1\ntriangles 1\n 0 1 2\n}\n"
+                                      f"k spin\n{{\n rot 5\n{body}\n}}\n")
+        for u in (0.0, 0.3, 0.5, 1.0):
+            v = H.Scene.from_aff(p, 1 + u).vertices[0, :3].astype(np.float64)
+            ang = np.degrees(np.arctan2(-v[2], v[0])) % 360.0
+            want = (step * (1 + u)) % 360.0
+            assert abs(((ang - want + 180.0) % 360.0) - 180.0) < 1e-3, (step, u, ang, want)
