@@ -121,6 +121,9 @@ EXPORTS = {
     "hulk_reset_counters": (C.c_int, [C.c_void_p]),
     "hulk_batch_create": (C.c_int, [C.c_void_p, C.POINTER(FrameDesc), C.c_uint32, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "hulk_batch_create_scenes": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(FrameDesc), C.c_uint32,
+                                           C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_uint32,
+                                           C.POINTER(C.c_void_p)]),
     "hulk_batch_launch": (C.c_int, [C.c_void_p, C.c_uint32]),
     "hulk_batch_ray_count": (C.c_uint32, [C.c_void_p]),
     "hulk_batch_destroy": (None, [C.c_void_p]),

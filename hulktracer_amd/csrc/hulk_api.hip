@@ -1473,6 +1473,14 @@ struct hulk_batch {
     // context state the recorded work depends on (tree / vertex pointers,
     // rope slots, light buffers and the queue sizes derived from them)
     uint32_t tree_gen = 0, lights_gen = 0;
+    // frames of several scenes (hulk_batch_create_scenes): the context of
+    // each scene-table entry, its state at creation, its mirror flags, and
+    // the device table; empty for a one-scene batch
+    std::vector<hulk_ctx*> scene_ctx;
+    std::vector<uint32_t> scene_tree_gen, scene_lights_gen;
+    std::vector<DevBuf> scene_mirror;
+    DevBuf d_scenes;
+    uint32_t max_stack = 1;
     bool graph_ready = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1480,7 +1488,55 @@ struct hulk_batch {
 
 namespace {
 
+// Every frame of a batch over several scenes in one fused launch: the
+// options and counters of the first context, the scene of each chunk of 64
+// rays from the scene table (no LDS top tree: a block's chunks may come from
+// different scenes).
+int render_scenes(hulk_batch* b, bool count) {
+    hulk_ctx* c = b->c;
+    const hulk_opts& o = b->opts;
+    TraceParams P;
+    uint32_t block;
+    size_t lds;
+    int r = trace_setup(c, o, P, block, lds);
+    if (r) return r;
+    P.lds_node_count = 0;
+    size_t stack_words = 0;
+    if (o.variant == HULK_VARIANT_FULLSTACK) {
+        P.stack_depth = b->max_stack;
+        stack_words = 2u * P.stack_depth;
+    } else if (o.variant == HULK_VARIANT_LDSSTACK) {
+        stack_words = 3u * P.stack_depth;
+    }
+    lds = stack_words * 4u * block;
+    if (lds > c->max_lds) return fail(c, HULK_E_ILLEGAL_INPUT, "stack columns exceed the block's LDS");
+    P.source = kSourceFrames;
+    P.frames = static_cast<const FrameDev*>(b->d_frames.p);
+    P.n_frames = static_cast<uint32_t>(b->frames.size());
+    P.n = b->n;
+    P.tbound = b->tbound;
+    P.hits = static_cast<hulk_hit*>(b->w.hits.p);
+    P.rgba = b->rgba;
+    P.scenes = static_cast<const SceneDev*>(b->d_scenes.p);
+    P.persistent = o.persistent ? 1u : 0u;
+    P.work_counter = static_cast<uint32_t*>(b->w.wq.p);
+    uint32_t grid;
+    if (o.persistent) {
+        HIP_TRY(c, launch_fill_u32(P.work_counter, kWorkCounterWords, 0u, c->stream));
+        const uint32_t per_cu = std::max<uint32_t>(1u, 2048u / block);
+        grid = std::max<uint32_t>(1u, std::min<uint32_t>((b->n + block - 1) / block, c->num_cus * per_cu));
+        const uint64_t waves = uint64_t(grid) * (block / 64);
+        const uint64_t chunks = (uint64_t(b->n) + 63) / 64;
+        P.fetch = static_cast<uint32_t>(std::min<uint64_t>(8, std::max<uint64_t>(1, chunks / (waves * 4))));
+    } else {
+        grid = std::max<uint32_t>(1u, (b->n + block - 1) / block);
+    }
+    HIP_TRY(c, launch_trace_scenes(o.variant, P, count, P.leaf_batch, grid, block, lds, c->stream));
+    return HULK_OK;
+}
+
 int batch_record(hulk_batch* b, bool count) {
+    if (!b->scene_ctx.empty()) return render_scenes(b, count);
     hulk_ctx* c = b->c;
     FrameSet fs;
     fs.frames = static_cast<const FrameDev*>(b->d_frames.p);
@@ -1514,6 +1570,7 @@ extern "C" int hulk_batch_create(hulk_ctx* c, const hulk_frame_desc* frames, uin
             return fail(c, HULK_E_ILLEGAL_INPUT, "frame needs width, height > 0 and max_bounce <= 8");
         }
         FrameDev fd;
+        fd.scene = 0;
         fd.cam = to_cam(d.cam);
         fd.W = d.width;
         fd.H = d.height;
@@ -1593,7 +1650,11 @@ extern "C" int hulk_batch_create(hulk_ctx* c, const hulk_frame_desc* frames, uin
 extern "C" int hulk_batch_launch(hulk_batch* b, uint32_t flags) {
     if (!b) return HULK_E_ILLEGAL_INPUT;
     hulk_ctx* c = b->c;
-    if (b->tree_gen != c->tree_gen || b->lights_gen != c->lights_gen)
+    bool stale = b->tree_gen != c->tree_gen || b->lights_gen != c->lights_gen;
+    for (size_t i = 0; i < b->scene_ctx.size(); i++)
+        stale = stale || b->scene_tree_gen[i] != b->scene_ctx[i]->tree_gen ||
+                b->scene_lights_gen[i] != b->scene_ctx[i]->lights_gen;
+    if (stale)
         return fail(c, HULK_E_ILLEGAL_INPUT,
                     "the context's scene, tree or lights changed after the batch was created: create it again");
     const bool count = (flags & HULK_FLAG_COUNT) != 0;
@@ -1627,8 +1688,165 @@ extern "C" void hulk_batch_destroy(hulk_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     release(b->d_frames);
     release(b->d_mirror);
+    release(b->d_scenes);
+    for (DevBuf& m : b->scene_mirror) release(m);
     release_work(b->w);
     delete b;
+}
+
+extern "C" int hulk_batch_create_scenes(hulk_ctx* const* ctxs, uint32_t n_ctxs, const hulk_frame_desc* frames,
+                                        uint32_t n_frames, const uint8_t* const* mirror_masks,
+                                        const float* tbound_dev, uint32_t* rgba_dev, uint32_t use_graph,
+                                        hulk_batch** out) {
+    if (!ctxs || !n_ctxs || !ctxs[0]) return HULK_E_ILLEGAL_INPUT;
+    hulk_ctx* c = ctxs[0];
+    if (!frames || !n_frames || !rgba_dev || !out) return fail(c, HULK_E_ILLEGAL_INPUT, "null argument");
+    *out = nullptr;
+    const hulk_opts& o = c->opts;
+    if (o.pipeline != HULK_PIPELINE_FUSED && o.pipeline != HULK_PIPELINE_AUTO)
+        return fail(c, HULK_E_UNSUPPORTED, "a batch over several scenes runs the fused frame pipeline");
+    if (o.tri_test == HULK_TRI_MOLLER_TRUMBORE)
+        return fail(c, HULK_E_UNSUPPORTED, "a batch over several scenes runs the reference triangle test");
+    for (uint32_t i = 0; i < n_ctxs; i++) {
+        if (!ctxs[i]) return fail(c, HULK_E_ILLEGAL_INPUT, "null context");
+        if (ctxs[i]->opts.device != o.device)
+            return fail(c, HULK_E_ILLEGAL_INPUT, "the scenes of a batch must live on one device");
+    }
+    hulk_batch* b = new (std::nothrow) hulk_batch;
+    if (!b) return HULK_E_ALLOC;
+    b->c = c;
+    b->opts = o;
+    b->opts.lds_top_levels = 0;  // a block's chunks may belong to different scenes
+    uint64_t off = 0;
+    for (uint32_t f = 0; f < n_frames; f++) {
+        const hulk_frame_desc& d = frames[f];
+        const uint64_t px = (uint64_t)d.width * d.height;
+        if (!d.width || !d.height || d.max_bounce > 8 || d.reserved >= n_ctxs || (px % 64) != 0) {
+            delete b;
+            return fail(c, HULK_E_ILLEGAL_INPUT,
+                        "frame needs width, height > 0, a multiple of 64 pixels, max_bounce <= 8 and a scene index "
+                        "(reserved) below the number of contexts");
+        }
+        FrameDev fd;
+        fd.cam = to_cam(d.cam);
+        fd.W = d.width;
+        fd.H = d.height;
+        fd.offset = static_cast<uint32_t>(off);
+        fd.max_bounce = mirror_masks && mirror_masks[d.reserved] ? d.max_bounce : 0;
+        fd.scene = d.reserved;
+        b->frames.push_back(fd);
+        b->max_bounce = std::max(b->max_bounce, fd.max_bounce);
+        off += px;
+        if (off > 0xFFFFFFC0ull) {
+            delete b;
+            return fail(c, HULK_E_ILLEGAL_INPUT, "batch has too many rays");
+        }
+    }
+    b->n = static_cast<uint32_t>(off);
+    b->rgba = rgba_dev;
+    b->tbound = tbound_dev;
+    b->tree_gen = c->tree_gen;
+    b->lights_gen = c->lights_gen;
+    b->scene_ctx.assign(ctxs, ctxs + n_ctxs);
+    b->scene_mirror.resize(n_ctxs);
+    std::vector<SceneDev> table(n_ctxs);
+    int r;
+    for (uint32_t i = 0; i < n_ctxs; i++) {
+        hulk_ctx* s = ctxs[i];
+        TraceParams Pi;
+        uint32_t block;
+        size_t lds;
+        if ((r = trace_setup(s, b->opts, Pi, block, lds))) {  // builds rope slots where needed
+            if (s != c) c->err = s->err;
+            hulk_batch_destroy(b);
+            return r;
+        }
+        SceneDev& e = table[i];
+        std::memset(&e, 0, sizeof(e));
+        e.nodes = Pi.nodes;
+        e.recs = Pi.recs;
+        e.rope_nodes = Pi.rope_nodes;
+        e.rope_slots = Pi.rope_slots;
+        e.lights = static_cast<const float4*>(s->d_lights.p);
+        e.n_lights = static_cast<uint32_t>(s->lights.size());
+        e.verts = Pi.verts;
+        e.tris = Pi.tris;
+        e.n_tris = Pi.n_tris;
+        e.box = Pi.box;
+        e.node_bytes = Pi.node_bytes;
+        e.rec_bytes = Pi.rec_bytes;
+        e.rope_node_bytes = Pi.rope_node_bytes;
+        e.guard = Pi.guard;
+        e.stack_depth = Pi.stack_depth;
+        b->max_stack = std::max(b->max_stack, e.stack_depth);
+        if (mirror_masks && mirror_masks[i] && b->max_bounce) {
+            if ((r = ensure(c, b->scene_mirror[i], std::max<uint32_t>(s->n_tris, 1)))) {
+                hulk_batch_destroy(b);
+                return r;
+            }
+            hipError_t em = s->n_tris ? hipMemcpyAsync(b->scene_mirror[i].p, mirror_masks[i], s->n_tris,
+                                                       hipMemcpyHostToDevice, c->stream)
+                                      : hipSuccess;
+            if (em != hipSuccess) {
+                hulk_batch_destroy(b);
+                return hip_fail(c, em, "mirror upload");
+            }
+            e.mirror = static_cast<const uint8_t*>(b->scene_mirror[i].p);
+        }
+        b->scene_tree_gen.push_back(s->tree_gen);
+        b->scene_lights_gen.push_back(s->lights_gen);
+    }
+    if ((r = ensure(c, b->d_frames, b->frames.size() * sizeof(FrameDev))) ||
+        (r = ensure(c, b->d_scenes, table.size() * sizeof(SceneDev))) ||
+        (r = ensure_work(c, b->w, b->n, 0, false, false))) {
+        hulk_batch_destroy(b);
+        return r;
+    }
+    hipError_t e = hipMemcpyAsync(b->d_frames.p, b->frames.data(), b->frames.size() * sizeof(FrameDev),
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_scenes.p, table.data(), table.size() * sizeof(SceneDev), hipMemcpyHostToDevice,
+                           c->stream);
+    for (uint32_t i = 0; i < n_ctxs && e == hipSuccess; i++)
+        if (ctxs[i] != c) e = hipStreamSynchronize(ctxs[i]->stream);  // their uploads and rope builds
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        hulk_batch_destroy(b);
+        return hip_fail(c, e, "batch upload");
+    }
+    if ((r = batch_record(b, false))) {
+        hulk_batch_destroy(b);
+        return r;
+    }
+    if (use_graph) {
+        e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            r = batch_record(b, false);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+            if (r) {
+                if (g) (void)hipGraphDestroy(g);
+                hulk_batch_destroy(b);
+                return r;
+            }
+            e = e2;
+            b->graph = g;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hulk_batch_destroy(b);
+            return hip_fail(c, e, "graph capture");
+        }
+        b->graph_ready = true;
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        hulk_batch_destroy(b);
+        return hip_fail(c, e, "batch warm-up");
+    }
+    *out = b;
+    return HULK_OK;
 }
 
 // --------------------------------------------------------------- autotune --

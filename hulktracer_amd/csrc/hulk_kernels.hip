@@ -823,7 +823,31 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
 // The LDS short stack with one record in flight takes 86 VGPRs (5 waves);
 // at 6 waves (80 VGPRs, a few spills outside the traversal loops) its
 // 10-frame terrain batch runs 0.6-1.4 % faster.
-template <int VARIANT, bool COUNT, bool RENDER, int LB>
+// Scene of the chunk of 64 rays starting at ray i0 of a batch over several
+// scenes, bound into P (every lane of the wave loads the same entry, so the
+// fields stay scalar).
+__device__ __forceinline__ void bind_scene(TraceParams& P, uint32_t i0) {
+    const uint32_t f = __builtin_amdgcn_readfirstlane(find_frame(P.frames, P.n_frames, i0));
+    const SceneDev& S = P.scenes[__builtin_amdgcn_readfirstlane(P.frames[f].scene)];
+    P.nodes = S.nodes;
+    P.recs = S.recs;
+    P.rope_nodes = S.rope_nodes;
+    P.rope_slots = S.rope_slots;
+    P.lights = S.lights;
+    P.mirror = S.mirror;
+    P.verts = S.verts;
+    P.tris = S.tris;
+    P.box = S.box;
+    P.node_bytes = S.node_bytes;
+    P.rec_bytes = S.rec_bytes;
+    P.rope_node_bytes = S.rope_node_bytes;
+    P.guard = S.guard;
+    P.n_lights = S.n_lights;
+    P.n_tris = S.n_tris;
+    P.stack_depth = S.stack_depth;
+}
+
+template <int VARIANT, bool COUNT, bool RENDER, int LB, bool MULTI = false>
 __global__ void __launch_bounds__(1024)
     __attribute__((amdgpu_waves_per_eu(LB == 2 ? 5 : (VARIANT == HULK_VARIANT_LDSSTACK ? 6 : 1))))
 k_trace(TraceParams P) {
@@ -886,6 +910,7 @@ k_trace(TraceParams P) {
     }
     while (have) {
         const uint32_t i = chunk * 64u + lane_id;
+        if (MULTI) bind_scene(P, chunk * 64u);  // frames hold multiples of 64 rays
         if (i < n_rays) {
             Ray r;
             uint32_t o = i;  // output / bound index: the pixel index for camera rays
@@ -1769,6 +1794,39 @@ hipError_t launch_trace(int variant, const TraceParams& P, bool count, bool rend
             return launch_trace_v<HULK_VARIANT_RESTART>(P, count, render, leaf_batch, grid, block, lds, s);
         case HULK_VARIANT_ROPES:
             return launch_trace_v<HULK_VARIANT_ROPES>(P, count, render, leaf_batch, grid, block, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int V>
+static hipError_t launch_trace_scenes_v(const TraceParams& P, bool count, uint32_t leaf_batch, uint32_t grid,
+                                        uint32_t block, size_t lds, hipStream_t s) {
+    if (count) {
+        if (leaf_batch == 2) hipLaunchKernelGGL((k_trace<V, true, true, 2, true>), dim3(grid), dim3(block), lds, s, P);
+        else hipLaunchKernelGGL((k_trace<V, true, true, 1, true>), dim3(grid), dim3(block), lds, s, P);
+    } else {
+        if (leaf_batch == 2) hipLaunchKernelGGL((k_trace<V, false, true, 2, true>), dim3(grid), dim3(block), lds, s, P);
+        else hipLaunchKernelGGL((k_trace<V, false, true, 1, true>), dim3(grid), dim3(block), lds, s, P);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_scenes(int variant, const TraceParams& P, bool count, uint32_t leaf_batch, uint32_t grid,
+                               uint32_t block, size_t lds, hipStream_t s) {
+    if (leaf_batch == kLeafMt || !P.scenes || !P.frames) return hipErrorInvalidValue;
+    switch (variant) {
+        case HULK_VARIANT_BRUTEFORCE:
+            return launch_trace_scenes_v<HULK_VARIANT_BRUTEFORCE>(P, count, leaf_batch, grid, block, lds, s);
+        case HULK_VARIANT_FULLSTACK:
+            return launch_trace_scenes_v<HULK_VARIANT_FULLSTACK>(P, count, leaf_batch, grid, block, lds, s);
+        case HULK_VARIANT_SHORTSTACK:
+            return launch_trace_scenes_v<HULK_VARIANT_SHORTSTACK>(P, count, leaf_batch, grid, block, lds, s);
+        case HULK_VARIANT_LDSSTACK:
+            return launch_trace_scenes_v<HULK_VARIANT_LDSSTACK>(P, count, leaf_batch, grid, block, lds, s);
+        case HULK_VARIANT_RESTART:
+            return launch_trace_scenes_v<HULK_VARIANT_RESTART>(P, count, leaf_batch, grid, block, lds, s);
+        case HULK_VARIANT_ROPES:
+            return launch_trace_scenes_v<HULK_VARIANT_ROPES>(P, count, leaf_batch, grid, block, lds, s);
         default: return hipErrorInvalidValue;
     }
 }

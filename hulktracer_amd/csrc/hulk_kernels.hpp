@@ -30,10 +30,28 @@ struct RopeSlot {
 };
 
 // One frame of a batch: rays offset .. offset + W*H - 1 of the batch, pixel
-// index y*W + x inside the frame.
+// index y*W + x inside the frame; scene: its entry of the batch's scene
+// table (batches over several scenes).
 struct FrameDev {
     Camera cam;
     uint32_t W, H, offset, max_bounce;
+    uint32_t scene;
+};
+
+// Everything a traversal and its shading read from one scene, for batches
+// whose frames come from several scenes (TraceParams fields of the same
+// names).
+struct SceneDev {
+    const uint4* nodes;
+    const TriRec* recs;
+    const uint4* rope_nodes;
+    const RopeSlot* rope_slots;
+    const float4* lights;
+    const uint8_t* mirror;
+    const float4* verts;
+    const uint4* tris;
+    SceneBox box;
+    uint32_t node_bytes, rec_bytes, rope_node_bytes, guard, n_lights, n_tris, stack_depth, pad;
 };
 
 struct TraceParams {
@@ -82,6 +100,9 @@ struct TraceParams {
                             // record with the Moller-Trumbore test)
     uint32_t coop_lanes;    // refill kernel: wave-cooperative leaf tests at <= this many busy lanes
     uint32_t stop_at_w;  // shadow queue: stop once best.t < ray.start.w (the light distance)
+    // several scenes (frame source, whole-frame launch): the frame's scene
+    // entry replaces the scene fields above for each chunk of 64 rays
+    const SceneDev* scenes;
 };
 
 constexpr uint32_t kWorkCounterStride = 64;  // 256 B between the per-XCD counters
@@ -160,6 +181,10 @@ struct StreamParams {
 // leaf_batch: triangle records in flight per lane in the leaf loop (1 or 2)
 hipError_t launch_trace(int variant, const TraceParams& P, bool count, bool render, uint32_t leaf_batch,
                         uint32_t grid, uint32_t block, size_t lds, hipStream_t s);
+// Whole frames of a batch over several scenes (P.scenes, P.frames): the
+// fused frame kernel with each chunk's scene bound from the table.
+hipError_t launch_trace_scenes(int variant, const TraceParams& P, bool count, uint32_t leaf_batch, uint32_t grid,
+                               uint32_t block, size_t lds, hipStream_t s);
 hipError_t launch_build_records(const float4* verts, const uint4* tris, const uint32_t* lp, uint32_t n,
                                 TriRec* recs, hipStream_t s);
 hipError_t launch_gen_secondary(const GenParams& G, uint32_t n_max, hipStream_t s);

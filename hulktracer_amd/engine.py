@@ -506,6 +506,27 @@ class GPUTrace:
         self._c(lib().hulk_reset_counters(self.h))
 
 
+def batch_scenes(ctxs, frames, mirrors, rgba_dev: int, tbound_dev: int | None = None,
+                 use_graph: bool = True) -> "Batch":
+    """hulk_batch_create_scenes: frames of several scenes in one launch.
+    ctxs: one GPUTrace per scene (ctxs[0]'s options and stream run the
+    batch); frames: list of (scene index, camera, width, height,
+    max_bounce); mirrors: per scene a mirror flag array or None."""
+    arr = (L.FrameDesc * len(frames))()
+    for i, (k, cam, w, h, mb) in enumerate(frames):
+        arr[i].cam[:] = [float(x) for x in np.asarray(cam, np.float32)]
+        arr[i].width, arr[i].height, arr[i].max_bounce, arr[i].reserved = int(w), int(h), int(mb), int(k)
+    handles = (C.c_void_p * len(ctxs))(*[cx.h.value for cx in ctxs])
+    keep = [None if m is None else np.ascontiguousarray(m, np.uint8) for m in mirrors]
+    mptr = (C.c_void_p * len(ctxs))(*[None if m is None else m.ctypes.data for m in keep])
+    hb = C.c_void_p()
+    ctxs[0]._c(lib().hulk_batch_create_scenes(handles, len(ctxs), arr, len(frames), mptr, ptr(tbound_dev),
+                                              ptr(rgba_dev), int(use_graph), C.byref(hb)))
+    b = Batch(ctxs[0], hb)
+    b.scene_ctxs = list(ctxs)  # the batch reads every context's buffers
+    return b
+
+
 class Batch:
     """A hulk_batch: many frames in one (graph-captured) pipeline."""
 

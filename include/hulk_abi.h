@@ -298,7 +298,7 @@ typedef struct hulk_frame_desc {
     hulk_camera cam;
     uint32_t width, height;
     uint32_t max_bounce;
-    uint32_t reserved;
+    uint32_t reserved;   /* hulk_batch_create_scenes: scene index; else 0 */
 } hulk_frame_desc;
 
 typedef struct hulk_batch hulk_batch;
@@ -321,6 +321,19 @@ typedef struct hulk_batch hulk_batch;
 int hulk_batch_create(hulk_ctx* ctx, const hulk_frame_desc* frames, uint32_t n_frames,
                       const uint8_t* mirror_mask, const float* tbound_dev, uint32_t* rgba_dev,
                       uint32_t use_graph, hulk_batch** out);
+/* Frames from several scenes in one launch: ctxs[k] holds scene k (scene,
+ * tree, lights; all on one device), frame f renders scene frames[f].reserved
+ * with mirror flags mirror_masks[k] (nullable array, nullable entries).  The
+ * batch runs on ctxs[0]'s stream with ctxs[0]'s options, which must use the
+ * fused frame pipeline and the reference triangle test (else
+ * HULK_E_UNSUPPORTED); every frame needs a multiple of 64 pixels, so that
+ * each wave's 64 rays belong to one scene, and the LDS top tree is not used.
+ * Otherwise as hulk_batch_create, including the refusal to launch after any
+ * of the contexts' scene, tree or lights changed.  Replaces a traceScene
+ * loop over frames of different scenes (GLVisitorGPUTrace.cpp:296-410). */
+int hulk_batch_create_scenes(hulk_ctx* const* ctxs, uint32_t n_ctxs, const hulk_frame_desc* frames, uint32_t n_frames,
+                             const uint8_t* const* mirror_masks, const float* tbound_dev, uint32_t* rgba_dev,
+                             uint32_t use_graph, hulk_batch** out);
 int hulk_batch_launch(hulk_batch* batch, uint32_t flags);
 uint32_t hulk_batch_ray_count(const hulk_batch* batch);
 /* Copy the primary hit records of the last launch (frames back to back). */

@@ -138,3 +138,92 @@ def test_batch_refuses_a_changed_context():
         assert np.array_equal(rgba.cpu().numpy(), before), what
         b2.close()
     ctx.close()
+
+
+def _config4_setup(variant, **opts):
+    from hulktracer_amd import workloads as WL
+
+    specs = WL.batch_specs()  # 64 frames: (scene 1 | 2, pose, side 256..1024, bounce depth 0..3)
+    ctxs, scenes = [], []
+    for kind in (1, 2):
+        s, nodes, lp = scene_and_tree(kind, 1)
+        ctxs.append(gpu_ctx(s, nodes, lp, variant, pipeline=2, **opts))
+        scenes.append(s)
+    frames = [(kind - 1, scenes[kind - 1].camera(pose, 1.0), r, r, mb) for kind, pose, r, mb in specs]
+    return specs, ctxs, scenes, frames
+
+
+@pytest.mark.parametrize("variant,opts", [(H.LDSSTACK, dict(persistent=True, block_size=64)),
+                                          (H.ROPES, dict(block_size=128, leaf_batch=2)),
+                                          (H.FULLSTACK, dict(persistent=True, block_size=256))])
+def test_batch_over_two_scenes_in_one_launch(variant, opts):
+    """Config 4: the 64 frames of the random-triangle and terrain scenes
+    (256^2 - 1024^2, bounce depth 0-3) in one graph-captured launch over
+    both scenes.  Every frame's pixels equal the frame rendered alone on
+    its scene's context; a subset (both scenes, the largest frames, the
+    deepest bounces) equals the oracle."""
+    import torch
+
+    specs, ctxs, scenes, frames = _config4_setup(variant, **opts)
+    total = sum(w * h for _, _, w, h, _ in frames)
+    rgba = torch.zeros(total, dtype=torch.int32, device="cuda")
+    b = H.batch_scenes(ctxs, frames, [s.mirror for s in scenes], rgba.data_ptr(), use_graph=True)
+    assert b.rays == total
+    b.launch()
+    first = rgba.cpu().numpy().view(np.uint32).copy()
+    rgba.zero_()
+    b.launch()  # a replay after eager work gives the same frame
+    got = rgba.cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, first)
+    hits = b.primary_hits()
+    off = 0
+    checked = []
+    for i, (k, cam, w, h, mb) in enumerate(frames):
+        single, sh = ctxs[k].render_frame(cam, w, h, max_bounce=mb, mirror=scenes[k].mirror)
+        assert np.array_equal(got[off:off + w * h], single), f"frame {i} (scene {k + 1}, {w}x{h}, bounce {mb})"
+        assert len(hit_mismatch(hits[off:off + w * h], sh)) == 0, f"frame {i} primary hits"
+        checked.append((i, k, w, mb, off))
+        off += w * h
+    picks = set()
+    for k in (0, 1):
+        mine = [c for c in checked if c[1] == k]
+        picks.add(max(mine, key=lambda c: (c[2], c[3]))[0])   # largest, then deepest
+        picks.add(max(mine, key=lambda c: (c[3], -c[2]))[0])  # deepest, then smallest
+    ov = {H.LDSSTACK: O.V_LDSSTACK, H.ROPES: O.V_ROPES, H.FULLSTACK: O.V_FULLSTACK}[variant]
+    for i, k, w, mb, off in checked:
+        if i not in picks:
+            continue
+        _, cam, _, _, _ = frames[i]
+        ref, _ = oracle_scene(k + 1, 1).render(ov, cam, w, w, max_bounce=mb)
+        assert np.array_equal(got[off:off + w * w], ref), f"frame {i} against the oracle"
+    b.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_batch_over_scenes_refusals():
+    import torch
+
+    specs, ctxs, scenes, frames = _config4_setup(H.ROPES)
+    rgba = torch.zeros(sum(w * h for _, _, w, h, _ in frames), dtype=torch.int32, device="cuda")
+    mir = [s.mirror for s in scenes]
+    for pipeline in (1, 3):
+        ctxs[0].set_options(pipeline=pipeline)
+        with pytest.raises(H.HulkError) as e:
+            H.batch_scenes(ctxs, frames[:4], mir, rgba.data_ptr())
+        assert e.value.code == 103
+    ctxs[0].set_options(pipeline=2)
+    bad = [(0, frames[0][1], 100, 30, 0)]  # 3000 pixels: not a multiple of 64
+    with pytest.raises(H.HulkError):
+        H.batch_scenes(ctxs, bad, mir, rgba.data_ptr())
+    with pytest.raises(H.HulkError):
+        H.batch_scenes(ctxs, [(2, frames[0][1], 64, 64, 0)], mir, rgba.data_ptr())  # no scene 2
+    b = H.batch_scenes(ctxs, frames[:6], mir, rgba.data_ptr())
+    b.launch()
+    ctxs[1].set_lights(scenes[1].lights)  # the second scene changed
+    with pytest.raises(H.HulkError) as e:
+        b.launch()
+    assert e.value.code == 4
+    b.close()
+    for c in ctxs:
+        c.close()
