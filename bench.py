@@ -214,13 +214,15 @@ class OracleFrames:
                 "rays_checked": total}
 
 
-def classify_vs_reference(osc, cams, W, H, hits, ref_hits, scene):
-    """Primary triangle ids of `hits` against the oracle's full-stack
-    traversal (the reference's GPUTrace.cl trace_ray): every ray whose id
-    differs is classified with the FP64 twin of the triangle test as a tie
-    (the two triangles' t within 1e-6 relative) or a hit point within 1e-6
-    of an edge of either triangle (shared edges and vertices); the rest are
-    unexplained."""
+def classify_vs_reference(osc, cams, W, H, hits, ref_hits, scene, tol: float = 1e-6):
+    """Primary triangle ids of `hits` against reference hits (the oracle's
+    full-stack traversal, the reference's GPUTrace.cl trace_ray): every ray
+    whose id differs is classified with the FP64 twin of the triangle test
+    as a tie (the two triangles' t within `tol` relative) or a hit point
+    within `tol` of an edge of either triangle (shared edges and vertices);
+    the rest are unexplained.  The same arithmetic breaks ties alike, so
+    1e-6 is ample there; another triangle test (Moller-Trumbore) rounds t
+    up to 1e-4 apart at grazing angles."""
     import numpy as np
 
     import oracle as O
@@ -247,10 +249,10 @@ def classify_vs_reference(osc, cams, W, H, hits, ref_hits, scene):
                     continue
                 p = V[T[tri, :3]]
                 dd.append(O.tri_test_f64(r, p[0], p[1], p[2]))
-            if all(d is not None for d in dd) and abs(dd[0][0] - dd[1][0]) <= 1e-6 * abs(dd[1][0]):
+            if all(d is not None for d in dd) and abs(dd[0][0] - dd[1][0]) <= tol * abs(dd[1][0]):
                 tie += 1
                 kind = "tie"
-            elif any(d is not None and min(d[1], d[2], 1 - d[1] - d[2]) < 1e-6 for d in dd):
+            elif any(d is not None and min(d[1], d[2], 1 - d[1] - d[2]) < tol for d in dd):
                 edge += 1
                 kind = "edge"
             else:
@@ -458,10 +460,10 @@ def run_workload(name: str, key: str, scene, W: int, H: int, stream, args, learn
     tt = time.time()
     best, table = ctx.autotune(cams[0], W, H, reps=2, frame=True)
     frames = [(cam, W, H, 0) for cam in cams]
-    # among shapes within 1 % (the stage's run-to-run noise) prefer a profiled one, so that
+    # among shapes within 2 % (the stage's run-to-run noise) prefer a profiled one, so that
     # the roofline's counters are those of the launch that runs
     bbest, btimes = WL.tune_batch(ctx, frames, None, rgba_tmp.data_ptr(), table, stream,
-                                  prefer=profiled_shapes(key), tol=0.01)
+                                  prefer=profiled_shapes(key), tol=0.02)
     log(f"[{name}] autotune {time.time() - tt:.1f}s -> {best}; batch stage {btimes} -> {bbest}")
     ctx.set_options(**{k: int(v) for k, v in best.items() if k in OPT_KEYS})
     out["options_single_frame"] = {k: int(best[k]) for k in OPT_KEYS}
@@ -756,7 +758,7 @@ def extras_run(args, W, H, stream, headline):
         osc = O.OracleScene(sc.vertices, sc.triangles, sc.box, nodes, lp, sc.lights)
         out["tri_id_differences"] = classify_vs_reference(osc, cams, W, H, hits_by["moller_trumbore"],
                                                           [hits_by["reference"][k * W * H:(k + 1) * W * H]
-                                                           for k in range(len(cams))], sc)
+                                                           for k in range(len(cams))], sc, tol=1e-4)
         out["options"] = opts
         log(f"triangle test: {out}")
         return out
