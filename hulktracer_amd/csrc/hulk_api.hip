@@ -1481,6 +1481,7 @@ struct hulk_batch {
     std::vector<DevBuf> scene_mirror;
     DevBuf d_scenes;
     uint32_t max_stack = 1;
+    uint32_t max_lds_nodes = 0;  // largest scene's LDS top tree (0: none)
     bool graph_ready = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1490,8 +1491,9 @@ namespace {
 
 // Every frame of a batch over several scenes in one fused launch: the
 // options and counters of the first context, the scene of each chunk of 64
-// rays from the scene table (no LDS top tree: a block's chunks may come from
-// different scenes).
+// rays from the scene table; an LDS top tree only for static launches whose
+// frames hold multiples of the block's rays (each block then lies in one
+// frame and stages its scene's top levels).
 int render_scenes(hulk_batch* b, bool count) {
     hulk_ctx* c = b->c;
     const hulk_opts& o = b->opts;
@@ -1500,7 +1502,7 @@ int render_scenes(hulk_batch* b, bool count) {
     size_t lds;
     int r = trace_setup(c, o, P, block, lds);
     if (r) return r;
-    P.lds_node_count = 0;
+    P.lds_node_count = b->max_lds_nodes;
     size_t stack_words = 0;
     if (o.variant == HULK_VARIANT_FULLSTACK) {
         P.stack_depth = b->max_stack;
@@ -1508,8 +1510,8 @@ int render_scenes(hulk_batch* b, bool count) {
     } else if (o.variant == HULK_VARIANT_LDSSTACK) {
         stack_words = 3u * P.stack_depth;
     }
-    lds = stack_words * 4u * block;
-    if (lds > c->max_lds) return fail(c, HULK_E_ILLEGAL_INPUT, "stack columns exceed the block's LDS");
+    lds = (size_t)b->max_lds_nodes * 16u + stack_words * 4u * block;
+    if (lds > c->max_lds) return fail(c, HULK_E_ILLEGAL_INPUT, "top tree and stack columns exceed the block's LDS");
     P.source = kSourceFrames;
     P.frames = static_cast<const FrameDev*>(b->d_frames.p);
     P.n_frames = static_cast<uint32_t>(b->frames.size());
@@ -1716,7 +1718,12 @@ extern "C" int hulk_batch_create_scenes(hulk_ctx* const* ctxs, uint32_t n_ctxs, 
     if (!b) return HULK_E_ALLOC;
     b->c = c;
     b->opts = o;
-    b->opts.lds_top_levels = 0;  // a block's chunks may belong to different scenes
+    // an LDS top tree needs every block inside one frame: static blocks and
+    // frames holding multiples of the block's rays
+    bool lds_ok = !o.persistent;
+    for (uint32_t f = 0; f < n_frames; f++)
+        lds_ok = lds_ok && ((uint64_t)frames[f].width * frames[f].height) % o.block_size == 0;
+    if (!lds_ok) b->opts.lds_top_levels = 0;
     uint64_t off = 0;
     for (uint32_t f = 0; f < n_frames; f++) {
         const hulk_frame_desc& d = frames[f];
@@ -1778,7 +1785,9 @@ extern "C" int hulk_batch_create_scenes(hulk_ctx* const* ctxs, uint32_t n_ctxs, 
         e.rope_node_bytes = Pi.rope_node_bytes;
         e.guard = Pi.guard;
         e.stack_depth = Pi.stack_depth;
+        e.lds_node_count = Pi.lds_node_count;
         b->max_stack = std::max(b->max_stack, e.stack_depth);
+        b->max_lds_nodes = std::max(b->max_lds_nodes, e.lds_node_count);
         if (mirror_masks && mirror_masks[i] && b->max_bounce) {
             if ((r = ensure(c, b->scene_mirror[i], std::max<uint32_t>(s->n_tris, 1)))) {
                 hulk_batch_destroy(b);

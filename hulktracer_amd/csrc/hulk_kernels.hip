@@ -845,6 +845,20 @@ __device__ __forceinline__ void bind_scene(TraceParams& P, uint32_t i0) {
     P.n_lights = S.n_lights;
     P.n_tris = S.n_tris;
     P.stack_depth = S.stack_depth;
+    P.lds_node_count = S.lds_node_count;
+}
+
+// Chunk range owner of a static (non-persistent) launch: XCD-aware order
+// (blocks are dispatched round-robin over the 8 XCDs, so consecutive chunks
+// go to blocks on the same XCD).  When the block count is not a multiple of
+// 8 some mapped ids would overflow, and a queue whose length is only known
+// on the device fills its low chunks, which round-robin dispatch already
+// spreads over the XCDs: plain block order then.
+__device__ __forceinline__ uint32_t static_block(const TraceParams& P) {
+    const uint32_t nblocks = gridDim.x, b = blockIdx.x;
+    if ((nblocks & 7u) != 0 || P.n_dev) return b;
+    const uint32_t per_xcd = (nblocks + 7u) >> 3;
+    return (b & 7u) * per_xcd + (b >> 3);
 }
 
 template <int VARIANT, bool COUNT, bool RENDER, int LB, bool MULTI = false>
@@ -858,6 +872,10 @@ k_trace(TraceParams P) {
     uint32_t* stk = reinterpret_cast<uint32_t*>(smem + P.lds_node_count) + tid;
     const uint32_t stride = block;
 
+    // Several scenes with a top tree in LDS (static launches whose blocks
+    // each lie in one frame): the block's scene first.  P.lds_node_count is
+    // the largest scene's count here, which placed the stack columns above.
+    if (MULTI && P.lds_node_count) bind_scene(P, static_block(P) * block);
     // Stage the top tree levels in LDS.
     for (uint32_t i = tid; i < P.lds_node_count; i += block) lds_nodes[i] = P.nodes[i];
     if (P.lds_node_count) __syncthreads();
@@ -896,16 +914,7 @@ k_trace(TraceParams P) {
     if (P.persistent) {
         have = next_chunks();
     } else {
-        const uint32_t nblocks = gridDim.x;
-        const uint32_t b = blockIdx.x;
-        const uint32_t per_xcd = (nblocks + 7u) >> 3;
-        const uint32_t xcd = b & 7u, k = b >> 3;
-        uint32_t mapped = xcd * per_xcd + k;
-        // tail: when nblocks is not a multiple of 8 some mapped ids overflow;
-        // a queue whose length is only known on the device fills its low
-        // chunks, which round-robin dispatch already spreads over the XCDs
-        if ((nblocks & 7u) != 0 || P.n_dev) mapped = b;
-        chunk = mapped * waves_per_block + wave;
+        chunk = static_block(P) * waves_per_block + wave;
         have = chunk < nchunks;
     }
     while (have) {

@@ -51,7 +51,8 @@ struct SceneDev {
     const float4* verts;
     const uint4* tris;
     SceneBox box;
-    uint32_t node_bytes, rec_bytes, rope_node_bytes, guard, n_lights, n_tris, stack_depth, pad;
+    uint32_t node_bytes, rec_bytes, rope_node_bytes, guard, n_lights, n_tris, stack_depth;
+    uint32_t lds_node_count;  // top-tree nodes a block stages in LDS (static launches only)
 };
 
 struct TraceParams {

@@ -305,7 +305,7 @@ def batched(stream=None, specs=None, reps: int = 3, autotune: bool = True, log=p
     and one launch over both scenes (hulk_batch_create_scenes, the fused
     frame kernel binding each wave's scene).  The one-launch options come
     from a batch stage on that launch: the fastest fused shapes of both
-    scenes' per-frame tables, LDS top tree off."""
+    scenes' per-frame tables."""
     from .engine import batch_scenes
 
     torch = _torch()
@@ -383,7 +383,7 @@ def batched(stream=None, specs=None, reps: int = 3, autotune: bool = True, log=p
             if taken == 5:
                 break
             k = dict((f, int(row[f])) for f in _TUNE_KEYS)
-            k.update(pipeline=2, lds_top_levels=0, coop_lanes=0)
+            k.update(pipeline=2, coop_lanes=0)
             key = tuple(k[f] for f in _TUNE_KEYS)
             if key not in seen:
                 seen.add(key)

@@ -327,7 +327,9 @@ int hulk_batch_create(hulk_ctx* ctx, const hulk_frame_desc* frames, uint32_t n_f
  * batch runs on ctxs[0]'s stream with ctxs[0]'s options, which must use the
  * fused frame pipeline and the reference triangle test (else
  * HULK_E_UNSUPPORTED); every frame needs a multiple of 64 pixels, so that
- * each wave's 64 rays belong to one scene, and the LDS top tree is not used.
+ * each wave's 64 rays belong to one scene.  The LDS top tree is used only
+ * by static launches (persistent = 0) whose frames all hold a multiple of
+ * block_size pixels, so that each block lies in one frame; else it is off.
  * Otherwise as hulk_batch_create, including the refusal to launch after any
  * of the contexts' scene, tree or lights changed.  Replaces a traceScene
  * loop over frames of different scenes (GLVisitorGPUTrace.cpp:296-410). */

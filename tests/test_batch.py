@@ -154,14 +154,17 @@ def _config4_setup(variant, **opts):
 
 
 @pytest.mark.parametrize("variant,opts", [(H.LDSSTACK, dict(persistent=True, block_size=64)),
+                                          (H.LDSSTACK, dict(block_size=128, lds_top_levels=8, leaf_batch=2)),
                                           (H.ROPES, dict(block_size=128, leaf_batch=2)),
-                                          (H.FULLSTACK, dict(persistent=True, block_size=256))])
+                                          (H.FULLSTACK, dict(persistent=True, block_size=256)),
+                                          (H.FULLSTACK, dict(block_size=64, lds_top_levels=10))])
 def test_batch_over_two_scenes_in_one_launch(variant, opts):
     """Config 4: the 64 frames of the random-triangle and terrain scenes
     (256^2 - 1024^2, bounce depth 0-3) in one graph-captured launch over
-    both scenes.  Every frame's pixels equal the frame rendered alone on
-    its scene's context; a subset (both scenes, the largest frames, the
-    deepest bounces) equals the oracle."""
+    both scenes (static launches also stage each block's scene top tree in
+    LDS).  Every frame's pixels equal the frame rendered alone on its
+    scene's context; a subset (both scenes, the largest frames, the deepest
+    bounces) equals the oracle."""
     import torch
 
     specs, ctxs, scenes, frames = _config4_setup(variant, **opts)
