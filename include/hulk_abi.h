@@ -21,7 +21,8 @@
  *     case every pointer argument of that call is a device pointer.
  *   - One context = one HIP stream.  Calls on one context are serialized on
  *     that stream; a context must not be used from two threads at once.
- *     Contexts are independent; the library keeps no global mutable state.
+ *     Contexts are independent; the only state shared between them is the
+ *     per-GPU scratch array of hulk_render_frame_static (see there).
  *   - Structs below are binary-identical to the reference's shared structs
  *     (HulkEngine/public/SharedStructs.hpp), so the output of the reference's
  *     KDTreeCPU passes through unchanged.
