@@ -399,6 +399,10 @@ int trace_primary(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, 
         P.source = kSourceCamera;
         P.cam = fs.cam;
         P.W = fs.W;
+        P.w_m = udiv_multiplier(fs.W);
+        P.tx_m = udiv_multiplier(fs.W >> 3);
+        P.fW = static_cast<float>(fs.W);
+        P.fH = static_cast<float>(fs.H);
         P.H = fs.H;
     }
     P.n = fs.n;
@@ -428,6 +432,10 @@ int render_refill(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, 
         P.source = kSourceCamera;
         P.cam = fs.cam;
         P.W = fs.W;
+        P.w_m = udiv_multiplier(fs.W);
+        P.tx_m = udiv_multiplier(fs.W >> 3);
+        P.fW = static_cast<float>(fs.W);
+        P.fH = static_cast<float>(fs.H);
         P.H = fs.H;
     }
     P.n = fs.n;
@@ -468,6 +476,10 @@ int render_fused(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, c
         P.source = kSourceCamera;
         P.cam = fs.cam;
         P.W = fs.W;
+        P.w_m = udiv_multiplier(fs.W);
+        P.tx_m = udiv_multiplier(fs.W >> 3);
+        P.fW = static_cast<float>(fs.W);
+        P.fH = static_cast<float>(fs.H);
         P.H = fs.H;
     }
     P.n = fs.n;

@@ -266,6 +266,18 @@ __device__ __forceinline__ void cast_ray(const Camera& c, uint32_t x, uint32_t y
     r.d = sub(add(add(ll, scale(vf, rc)), scale(hf, rr)), r.o);
 }
 
+// castRay with the image size already as floats (fW = (float)W, fH =
+// (float)H): the same divisions, with W and H as scalar kernel arguments.
+__device__ __forceinline__ void cast_ray_f(const Camera& c, uint32_t x, uint32_t y, float fW, float fH, Ray& r) {
+    const float rr = (float)y / fH;
+    const float rc = (float)x / fW;
+    r.o = v3(c.from[0], c.from[1], c.from[2]);
+    const V3 ll = v3(c.ll[0], c.ll[1], c.ll[2]);
+    const V3 vf = v3(c.vf[0], c.vf[1], c.vf[2]);
+    const V3 hf = v3(c.hf[0], c.hf[1], c.hf[2]);
+    r.d = sub(add(add(ll, scale(vf, rc)), scale(hf, rr)), r.o);
+}
+
 __device__ __forceinline__ void set_inv(Ray& r) {
     r.inv = v3(rcp_exact(r.d.x), rcp_exact(r.d.y), rcp_exact(r.d.z));
 }

@@ -93,6 +93,33 @@ __device__ __forceinline__ void tile_pixel(uint32_t i, uint32_t W, uint32_t H, u
     }
 }
 
+__device__ __forceinline__ uint32_t udiv(uint32_t a, uint32_t d, uint32_t m) {
+    uint32_t q = __umulhi(a, m);
+    while (a - q * d >= d) q++;
+    return q;
+}
+
+// tile_pixel with the divisions by W and W / 8 done by host multipliers
+// (camera launches: W is the same for every ray, and the multipliers stay
+// kernel arguments instead of per-wave reciprocals held in registers).
+__device__ __forceinline__ void tile_pixel_m(uint32_t i, uint32_t W, uint32_t H, uint32_t w_m, uint32_t tx_m,
+                                             uint32_t& x, uint32_t& y) {
+    if (((W | H) & 7u) == 0) {
+        // the lane's place in its tile, recomputed per ray: hoisted out of
+        // the ray loop it would be one more register held (and spilled)
+        // across every traversal
+        uint32_t k = i & 63u;
+        asm volatile("" : "+v"(k));
+        const uint32_t tile = i >> 6, tiles_x = W >> 3;
+        const uint32_t ty = udiv(tile, tiles_x, tx_m), tx = tile - ty * tiles_x;
+        x = (tx << 3) | (k & 7u);
+        y = (ty << 3) | (k >> 3);
+    } else {
+        y = udiv(i, W, w_m);
+        x = i - y * W;
+    }
+}
+
 // Batched primary ray i: pixel index (frame offset + y*W + x) returned,
 // 8x8 tiles inside a frame when its sides are multiples of 8.
 __device__ __forceinline__ uint32_t frame_ray_tiled(const FrameDev* F, uint32_t n, uint32_t i, Ray& r,
@@ -927,9 +954,9 @@ k_trace(TraceParams P) {
             float tstop = kNoStop;
             if (P.source == kSourceCamera) {
                 uint32_t x, y;
-                tile_pixel(i, P.W, P.H, x, y);
+                tile_pixel_m(i, P.W, P.H, P.w_m, P.tx_m, x, y);
                 o = y * P.W + x;
-                cast_ray(P.cam, x, y, P.W, P.H, r);
+                cast_ray_f(P.cam, x, y, P.fW, P.fH, r);
             } else if (P.source == kSourceFrames) {
                 uint32_t f;
                 o = frame_ray_tiled(P.frames, P.n_frames, i, r, f);
@@ -1107,9 +1134,9 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
                 uint32_t o, mb = P.max_bounce;
                 if (P.source == kSourceCamera) {
                     uint32_t x, y;
-                    tile_pixel(i, P.W, P.H, x, y);
+                    tile_pixel_m(i, P.W, P.H, P.w_m, P.tx_m, x, y);
                     o = y * P.W + x;
-                    cast_ray(P.cam, x, y, P.W, P.H, tr);
+                    cast_ray_f(P.cam, x, y, P.fW, P.fH, tr);
                 } else {
                     uint32_t f;
                     o = frame_ray_tiled(P.frames, P.n_frames, i, tr, f);

@@ -74,6 +74,8 @@ struct TraceParams {
     uint32_t source;
     Camera cam;
     uint32_t W, H;
+    uint32_t w_m, tx_m;  // camera source: udiv multipliers of W and W / 8 (udiv_multiplier)
+    float fW, fH;        // camera source: (float)W, (float)H
     const hulk_ray* rays;
     const FrameDev* frames;
     uint32_t n_frames;
@@ -108,6 +110,10 @@ struct TraceParams {
 
 constexpr uint32_t kWorkCounterStride = 64;  // 256 B between the per-XCD counters
 constexpr uint32_t kWorkCounterWords = 8 * kWorkCounterStride;
+
+// a / d for a divisor d >= 1 shared by every lane: m = udiv_multiplier(d)
+// from the host, umulhi(a, m) is at most two below the quotient.
+inline uint32_t udiv_multiplier(uint32_t d) { return d ? 0xFFFFFFFFu / d : 0u; }
 
 struct GenParams {
     const hulk_hit* hits;
