@@ -623,70 +623,84 @@ __device__ __forceinline__ bool trav_exit(const TraceParams& P, const Ray& r, Tr
 }
 
 // The leaves of the lanes in `want` (a wave-uniform ballot; every lane of
-// the wave executes this call) tested by the whole wave, one owner lane at a
-// time, 64 records per pass: lane k tests record k of the pass against the
-// owner's ray and the owner's best t before the pass.  The pass keeps the
+// the wave executes this call) tested by the whole wave, 64 records per
+// pass, the records of all owner lanes packed back to back in owner lane
+// order: lane k of a pass tests record k of the pass against its owner's
+// ray and that owner's best t before the pass.  Per owner the pass keeps the
 // smallest t, and of equal t the lowest leaf position, which is the hit the
 // owner's own loop keeps (strict '<' in leaf order: the first of equal t
 // wins, GPU_common.cl:62-68), with the same per-test arithmetic
-// (test_record).  A shadow ray (best.t < tstop ends it) stops after the pass
-// in which that happens, where its own loop stops inside the pass: its hit
-// may then be another triangle closer than the light, which leaves the
-// shading decision (light distance > t) unchanged.  Used where few lanes of
-// a wave are still at work, so that a long leaf costs one memory latency per
-// 64 records instead of one per record.
+// (test_record); `keys` (one 64-bit LDS word per lane of the wave) holds
+// each owner's (t bits, packed index) minimum of the pass.  A shadow ray
+// (best.t < tstop ends it) may test more records after its first hit closer
+// than the light, where its own loop would stop: its hit may then be another
+// triangle closer than the light, which leaves the shading decision (light
+// distance > t) unchanged.  Used where few lanes of a wave are still at
+// work, so that a wave's long leaves cost one memory latency per 64 records
+// of all of them instead of one per record or two.
 template <bool MT>
 __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, const uint4& n, const Ray& r,
-                                            Best& best, float tstop, uint64_t want, uint32_t lane_id) {
-    while (want) {
-        const uint32_t L = (uint32_t)__builtin_ctzll(want);
-        want &= want - 1;
-        Ray q;
-        q.o = v3(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.o.x), L)),
-                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.o.y), L)),
-                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.o.z), L)));
-        q.d = v3(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.d.x), L)),
-                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.d.y), L)),
-                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.d.z), L)));
-        const float ts = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tstop), L));
-        const uint32_t start = (uint32_t)__builtin_amdgcn_readlane((int)n.y, L);
-        const uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane((int)n.z, L);
-        float bt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(best.t), L));
-        for (uint32_t base = 0; base < cnt; base += 64u) {
-            Best b;
-            b.t = bt;
-            bool hit = false;
-            if (base + lane_id < cnt) {
-                const TriRec rec = recs[start + base + lane_id];
-                hit = test_any<MT>(q, rec, b);
-            }
-            // t >= 0 here (or -0): the bits without the sign order like the value
-            uint32_t key = hit ? (__float_as_uint(b.t) & 0x7FFFFFFFu) : 0xFFFFFFFFu;
+                                            Best& best, uint64_t want, uint32_t lane_id,
+                                            unsigned long long* keys, uint32_t* tab, uint32_t tab_stride) {
+    const bool own = (want >> lane_id) & 1u;
+    const uint32_t cnt = own ? n.z : 0u;
+    uint32_t pre = cnt;  // inclusive scan of the owners' record counts in lane order
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) key = min(key, (uint32_t)__shfl_xor((int)key, off, 64));
-            key = (uint32_t)__builtin_amdgcn_readfirstlane((int)key);
-            if (key == 0xFFFFFFFFu) continue;
-            const uint64_t at = __ballot(hit && (__float_as_uint(b.t) & 0x7FFFFFFFu) == key);
-            const uint32_t w = (uint32_t)__builtin_ctzll(at);
-            bt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.t), w));
-            const float s1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.s1), w));
-            const float s2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.s2), w));
-            const float det = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.det), w));
-            const uint32_t tri = (uint32_t)__builtin_amdgcn_readlane((int)b.tri, w);
-            const float nx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.nx), w));
-            const float ny = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.ny), w));
-            const float nz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.nz), w));
-            if (lane_id == L) {
-                best.t = bt;
-                best.s1 = s1;
-                best.s2 = s2;
-                best.det = det;
-                best.tri = tri;
-                best.nx = nx;
-                best.ny = ny;
-                best.nz = nz;
-            }
-            if (bt < ts) break;
+    for (uint32_t off = 1; off < 64u; off <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)pre, off, 64);
+        if (lane_id >= off) pre += v;
+    }
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)pre, 63);
+    const uint32_t first = pre - cnt;  // this owner's first packed index
+    // owner table by rank (owners in lane order): lane, first packed index, first record
+    if (own) {
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0u));
+        tab[rank] = lane_id;
+        tab[tab_stride + rank] = first;
+        tab[2u * tab_stride + rank] = n.y;
+    }
+    for (uint32_t base = 0; base < total; base += 64u) {
+        const uint32_t g = base + lane_id;
+        // owner of packed record g: owners that began before this pass, plus
+        // those whose first index lies in the pass at or below g
+        const uint32_t before = (uint32_t)__builtin_popcountll(__ballot(own && first < base));
+        if (lane_id == 63u) keys[63] = 0ull;
+        if (own && first >= base && first < base + 64u) atomicOr(keys + 63, 1ull << (first - base));
+        const unsigned long long starts = keys[63];
+        const unsigned long long upto = lane_id == 63u ? ~0ull : (2ull << lane_id) - 1ull;
+        const uint32_t cnt_before = before + (uint32_t)__builtin_popcountll(starts & upto);
+        const uint32_t rank = cnt_before ? cnt_before - 1u : 0u;
+        const uint32_t L = tab[rank], gL = tab[tab_stride + rank], sL = tab[2u * tab_stride + rank];
+        Ray q;
+        q.o = v3(__shfl(r.o.x, L, 64), __shfl(r.o.y, L, 64), __shfl(r.o.z, L, 64));
+        q.d = v3(__shfl(r.d.x, L, 64), __shfl(r.d.y, L, 64), __shfl(r.d.z, L, 64));
+        Best b;
+        best_init(b);
+        b.t = __shfl(best.t, L, 64);
+        bool hit = false;
+        if (own) keys[lane_id] = ~0ull;
+        if (g < total) {
+            const TriRec rec = recs[sL + (g - gL)];
+            hit = test_any<MT>(q, rec, b);
+        }
+        // t >= 0 here (or -0): the bits without the sign order like the value
+        if (hit)
+            atomicMin(keys + L, ((unsigned long long)(__float_as_uint(b.t) & 0x7FFFFFFFu) << 32) | (g - base));
+        const unsigned long long k = own ? keys[lane_id] : ~0ull;
+        const uint32_t w = (uint32_t)k & 63u;  // the lane of this pass that tested the winner
+        const float t = __shfl(b.t, w, 64), s1 = __shfl(b.s1, w, 64), s2 = __shfl(b.s2, w, 64),
+                    det = __shfl(b.det, w, 64), nx = __shfl(b.nx, w, 64), ny = __shfl(b.ny, w, 64),
+                    nz = __shfl(b.nz, w, 64);
+        const uint32_t tri = (uint32_t)__shfl((int)b.tri, w, 64);
+        if (k != ~0ull) {
+            best.t = t;
+            best.s1 = s1;
+            best.s2 = s2;
+            best.det = det;
+            best.tri = tri;
+            best.nx = nx;
+            best.ny = ny;
+            best.nz = nz;
         }
     }
 }
@@ -1070,9 +1084,12 @@ enum LaneField : uint32_t {
     kFW, kFAr, kFAg, kFAb, kFCr, kFCg, kFCb,              // weight, pixel sums, generation sums
     kFIntensity, kFTb,                                    // current light's term, primary bound
     kUO, kUMb, kUG, kUK, kUTri, kUFlags,                  // pixel, max bounce, generation, light, hit tri, flags
+    kUTile,                                               // the pixel's 8x8 tile (cost measurement)
+    kUCoop, kUCoop1, kUCoop2,                             // cooperative leaf tests: the wave's owner table
     kLaneFields
 };
-constexpr uint32_t kRefillLdsWords = kLaneFields;
+// lane fields, then one 64-bit word per lane for the cooperative leaf tests
+constexpr uint32_t kRefillLdsWords = kLaneFields + 2u;
 
 // No wave cap: the refill loop's lane state measured fastest unspilled.
 // LDS: the top tree levels (LDS short stack), the lane fields, then the
@@ -1084,7 +1101,8 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
     const uint32_t lane_id = tid & 63u;
     uint4* lds_nodes = smem;
     uint32_t* lane_mem = reinterpret_cast<uint32_t*>(smem + P.lds_node_count);
-    uint32_t* stk = lane_mem + kLaneFields * block + tid;
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(lane_mem + kLaneFields * block) + (tid - lane_id);
+    uint32_t* stk = lane_mem + kRefillLdsWords * block + tid;
     for (uint32_t i = tid; i < P.lds_node_count; i += block) lds_nodes[i] = P.nodes[i];
     if (P.lds_node_count) __syncthreads();
     auto F = [&](uint32_t f) -> float& { return reinterpret_cast<float*>(lane_mem)[f * block + tid]; };
@@ -1098,6 +1116,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
 
     // traversal state (registers)
     uint32_t phase = kLaneIdle, tfl = 0;
+    uint32_t steps = 0;  // loop steps of the lane's current path (tile cost)
     float tstop = -kFltMax;
     bool clipped = false, active = false;  // active: a traversal is in progress
     Ray tr;
@@ -1130,7 +1149,10 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
             if (phase == kLaneIdle && rank < avail) {
-                const uint32_t i = lo + got + rank;
+                uint32_t i = lo + got + rank;
+                if (P.tile_order) i = (P.tile_order[i >> 6] << 6) | (i & 63u);
+                U(kUTile) = i >> 6;
+                steps = 0;
                 uint32_t o, mb = P.max_bounce;
                 if (P.source == kSourceCamera) {
                     uint32_t x, y;
@@ -1171,6 +1193,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
         // (the counted instance keeps every test per lane, so its counters
         // equal the other kernels').
         const bool busy = phase != kLaneIdle;
+        steps += busy ? 1u : 0u;
         bool done = false;
         if (COUNT || P.coop_lanes == 0) {
             if (busy) {
@@ -1187,7 +1210,8 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             if (walk) ln = trav_descend<TV, COUNT>(P, lds_nodes, stk, block, tr, st, c);
             const uint64_t at_leaf = __ballot(walk && ln.z != 0u);
             if (at_leaf && (uint32_t)__builtin_popcountll(at_leaf) <= P.coop_lanes)
-                coop_leaves<LB == kLeafMt>(P.recs, ln, tr, bt, tstop, at_leaf, lane_id);
+                coop_leaves<LB == kLeafMt>(P.recs, ln, tr, bt, at_leaf, lane_id, keys,
+                                           lane_mem + kUCoop * block + (tid - lane_id), block);
             else if (walk && (ln.x & kNodeLeaf))
                 test_leaf<COUNT, LB>(P.recs, P.rec_bytes, ln, tr, bt, c, tstop);
             if (busy) done = !walk || !trav_exit<TV>(P, tr, st, ln, tstop, bt, tfl);
@@ -1245,6 +1269,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             if (shade) {
                 if (bt.tri == kNoHit) {  // the path ends: pixel
                     P.rgba[U(kUO)] = to_u8(F(kFAr)) | (to_u8(F(kFAg)) << 8) | (to_u8(F(kFAb)) << 16);
+                    if (P.tile_cost) atomicAdd(P.tile_cost + U(kUTile), steps);
                     phase = kLaneIdle;
                     active = false;
                     break;
@@ -1303,6 +1328,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             F(kFAb) += F(kFCb) * keep;
             if (!spawn) {
                 P.rgba[U(kUO)] = to_u8(F(kFAr)) | (to_u8(F(kFAg)) << 8) | (to_u8(F(kFAb)) << 16);
+                if (P.tile_cost) atomicAdd(P.tile_cost + U(kUTile), steps);
                 phase = kLaneIdle;
                 active = false;
                 break;
@@ -1622,6 +1648,54 @@ __global__ void k_fill_f32(float* p, size_t n, float v) {
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Tile order from tile costs, one block of kOrderThreads: a stable counting
+// sort on a key of two buckets per octave of the cost (descending), each
+// thread counting, then placing, one contiguous range of tiles; sorted rank
+// r goes to slice r % 8 at its position r / 8.
+constexpr uint32_t kOrderThreads = 128, kOrderKeys = 64;
+
+__device__ __forceinline__ uint32_t cost_key(uint32_t c) {
+    if (c < 2u) return c;
+    const uint32_t msb = 31u - (uint32_t)__builtin_clz(c);
+    return 2u * msb + ((c >> (msb - 1u)) & 1u);
+}
+
+__global__ void __launch_bounds__(kOrderThreads) k_tile_order(const uint32_t* cost, uint32_t n, uint32_t* order) {
+    __shared__ uint32_t hist[kOrderKeys * kOrderThreads];  // [63 - key][thread]
+    __shared__ uint32_t part[kOrderThreads];
+    const uint32_t t = threadIdx.x, T = kOrderThreads;
+    const uint32_t per = (n + T - 1u) / T, b = min(n, t * per), e = min(n, b + per);
+    for (uint32_t k = 0; k < kOrderKeys; k++) hist[k * T + t] = 0u;
+    for (uint32_t i = b; i < e; i++) hist[(kOrderKeys - 1u - cost_key(cost[i])) * T + t]++;
+    __syncthreads();
+    // exclusive scan of hist in (descending key, thread) order: thread t
+    // takes the kOrderKeys consecutive entries starting at t * kOrderKeys
+    uint32_t sum = 0;
+    for (uint32_t j = 0; j < kOrderKeys; j++) sum += hist[t * kOrderKeys + j];
+    part[t] = sum;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t j = 0; j < t; j++) base += part[j];
+    __syncthreads();
+    for (uint32_t j = 0; j < kOrderKeys; j++) {
+        const uint32_t v = hist[t * kOrderKeys + j];
+        hist[t * kOrderKeys + j] = base;
+        base += v;
+    }
+    __syncthreads();
+    const uint32_t per_slice = n >> 3;
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t r = hist[(kOrderKeys - 1u - cost_key(cost[i])) * T + t]++;
+        order[(r & 7u) * per_slice + (r >> 3)] = i;
+    }
+}
+
+hipError_t launch_tile_order(const uint32_t* cost, uint32_t n_tiles, uint32_t* order, hipStream_t s) {
+    if (n_tiles == 0 || (n_tiles & 7u) || n_tiles > kMaxOrderTiles) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, s, cost, n_tiles, order);
+    return hipGetLastError();
 }
 
 // ShadePicture, shadePicture.cl:7-55: base 0.1 for every pixel; a hit
