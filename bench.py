@@ -84,7 +84,9 @@ def kernel_name(opts: dict) -> str:
     if int(opts.get("pipeline", 0)) == 1:
         return "queued pipeline graph"
     if int(opts.get("pipeline", 0)) == 3:
-        return f"k_render_refill<{int(opts['variant'])},false,{lb}>"
+        if int(opts.get("coop_lanes", 0)) >= 64:  # every leaf step cooperative: its own instance
+            return f"k_render_refill<{int(opts['variant'])},false,{3 if int(opts.get('tri_test', 0)) else 1},true>"
+        return f"k_render_refill<{int(opts['variant'])},false,{lb},false>"
     return f"k_trace<{int(opts['variant'])},false,true,{lb}>"
 
 

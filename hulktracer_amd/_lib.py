@@ -63,8 +63,7 @@ class Opts(C.Structure):
     _fields_ = [("device", C.c_int32), ("variant", C.c_int32), ("block_size", C.c_uint32),
                 ("short_stack_depth", C.c_uint32), ("lds_top_levels", C.c_uint32),
                 ("persistent", C.c_uint32), ("pipeline", C.c_uint32), ("leaf_batch", C.c_uint32),
-                ("coop_lanes", C.c_uint32), ("tri_test", C.c_uint32), ("tile_order", C.c_uint32),
-                ("stream", C.c_void_p)]
+                ("coop_lanes", C.c_uint32), ("tri_test", C.c_uint32), ("stream", C.c_void_p)]
 
 
 class Counters(C.Structure):
@@ -118,7 +117,6 @@ EXPORTS = {
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "hulk_render_frame_static": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
-    "hulk_tile_costs": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "hulk_get_counters": (C.c_int, [C.c_void_p, C.POINTER(Counters)]),
     "hulk_reset_counters": (C.c_int, [C.c_void_p]),
     "hulk_batch_create": (C.c_int, [C.c_void_p, C.POINTER(FrameDesc), C.c_uint32, C.c_void_p, C.c_void_p,

@@ -28,7 +28,7 @@ using namespace hulk;
 static_assert(sizeof(hulk_float4) == 16 && sizeof(hulk_tri) == 16 && sizeof(hulk_kd_node) == 16, "layout");
 static_assert(sizeof(hulk_box) == 32 && sizeof(hulk_camera) == 64 && sizeof(hulk_light) == 32, "layout");
 static_assert(sizeof(hulk_ray) == 32 && sizeof(hulk_hit) == 32, "layout");
-static_assert(sizeof(hulk_opts) == 56 && sizeof(hulk_tune_entry) == 36 && sizeof(hulk_tune_request) == 96, "layout");
+static_assert(sizeof(hulk_opts) == 48 && sizeof(hulk_tune_entry) == 36 && sizeof(hulk_tune_request) == 96, "layout");
 
 namespace {
 
@@ -60,11 +60,6 @@ static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 struct Work {
     DevBuf hits, hits2, hits3, shadow_rays, shadow_slot, shadow_t, qrays[2], qpix[2], qw[2], acc, ctr, wq, bcount,
         boffset;
-    // lane-refill camera frames: per-tile path costs of the last frame and
-    // the tile order derived from them, valid for order_W x order_H
-    DevBuf tile_cost, tile_order;
-    uint32_t order_W = 0, order_H = 0;
-    bool no_alloc = false;  // a set over caller or static memory: no tile ordering
 };
 
 struct hulk_ctx {
@@ -206,7 +201,6 @@ int check_opts(hulk_ctx* c, const hulk_opts& o) {
     if (o.tri_test > HULK_TRI_MOLLER_TRUMBORE) return fail(c, HULK_E_ILLEGAL_INPUT, "unknown tri_test");
     if (o.tri_test == HULK_TRI_MOLLER_TRUMBORE && o.leaf_batch == 2)
         return fail(c, HULK_E_ILLEGAL_INPUT, "the Moller-Trumbore test runs with one record in flight");
-    if (o.tile_order > 1) return fail(c, HULK_E_ILLEGAL_INPUT, "tile_order must be 0 or 1");
     return HULK_OK;
 }
 
@@ -349,9 +343,8 @@ int finish(hulk_ctx* c, uint32_t flags, bool timed) {
 void release_work(Work& w) {
     DevBuf* bufs[] = {&w.hits, &w.hits2, &w.hits3, &w.shadow_rays, &w.shadow_slot, &w.shadow_t, &w.qrays[0],
                       &w.qrays[1], &w.qpix[0], &w.qpix[1], &w.qw[0], &w.qw[1], &w.acc, &w.ctr, &w.wq, &w.bcount,
-                      &w.boffset, &w.tile_cost, &w.tile_order};
+                      &w.boffset};
     for (DevBuf* b : bufs) release(*b);
-    w.order_W = w.order_H = 0;
 }
 
 // Size a work set for n primary rays, np point lights, with or without
@@ -455,32 +448,13 @@ int render_refill(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, 
     P.rgba = rgba;
     P.work_counter = static_cast<uint32_t*>(w.wq.p);
     P.persistent = 1;
-    // tiles heaviest first by the previous frame's costs (camera frames of
-    // whole 8x8 tiles, a multiple of 8 of them); this frame's costs are
-    // measured for the next one
-    const uint32_t n_tiles = fs.n >> 6;
-    const bool order = !fs.frames && o.tile_order == 0 && !w.no_alloc && ((fs.W | fs.H) & 7u) == 0 &&
-                       n_tiles > 0 && (n_tiles & 7u) == 0 && n_tiles <= kMaxOrderTiles;
-    if (order) {
-        const size_t bytes = (size_t)n_tiles * 4u;
-        if (w.tile_order.cap < bytes) w.order_W = w.order_H = 0;  // reallocated: no order yet
-        if ((r = ensure(c, w.tile_cost, bytes)) || (r = ensure(c, w.tile_order, bytes))) return r;
-        if (w.order_W == fs.W && w.order_H == fs.H) P.tile_order = static_cast<const uint32_t*>(w.tile_order.p);
-        P.tile_cost = static_cast<uint32_t*>(w.tile_cost.p);
-    }
-    if (timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     HIP_TRY(c, launch_fill_u32(P.work_counter, kWorkCounterWords, 0u, c->stream));
-    if (order) HIP_TRY(c, launch_fill_u32(P.tile_cost, n_tiles, 0u, c->stream));
     // one resident set of waves (2048 threads per CU), fewer for small frames
     const uint32_t per_cu = std::max<uint32_t>(1u, 2048u / block);
     const uint32_t grid =
         std::max<uint32_t>(1u, std::min<uint32_t>((fs.n + block - 1) / block, c->num_cus * per_cu));
+    if (timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     HIP_TRY(c, launch_render_refill(o.variant, P, count, grid, block, lds, c->stream));
-    if (order) {
-        HIP_TRY(c, launch_tile_order(P.tile_cost, n_tiles, static_cast<uint32_t*>(w.tile_order.p), c->stream));
-        w.order_W = fs.W;
-        w.order_H = fs.H;
-    }
     if (timed) HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return HULK_OK;
 }
@@ -1124,7 +1098,6 @@ extern "C" int hulk_render_frame_static(hulk_ctx* c, const hulk_camera* cam, uin
     fs.H = H;
     fs.n = n;
     Work w;  // nothing allocated: only the work counters are used, in the scratch
-    w.no_alloc = true;
     w.wq.p = c->scratch;
     w.wq.cap = kScratchCounterBytes;
     hulk_hit* h0 = primary_hits_dev
@@ -1138,18 +1111,6 @@ extern "C" int hulk_render_frame_static(hulk_ctx* c, const hulk_camera* cam, uin
         return r;
     if (count && (r = read_stats(c))) return r;
     return finish(c, flags, true);
-}
-
-extern "C" int hulk_tile_costs(hulk_ctx* c, uint32_t W, uint32_t H, uint32_t* cost, uint32_t* order) {
-    if (!c) return HULK_E_ILLEGAL_INPUT;
-    Work& w = c->work;
-    if (W == 0 || H == 0 || w.order_W != W || w.order_H != H)
-        return fail(c, HULK_E_ILLEGAL_INPUT, "no lane-refill camera frame of this size was rendered with tile order 0");
-    const size_t bytes = (size_t)(W / 8u) * (H / 8u) * 4u;
-    if (cost) HIP_TRY(c, hipMemcpyAsync(cost, w.tile_cost.p, bytes, hipMemcpyDeviceToHost, c->stream));
-    if (order) HIP_TRY(c, hipMemcpyAsync(order, w.tile_order.p, bytes, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return HULK_OK;
 }
 
 extern "C" int hulk_camera_rays(hulk_ctx* c, const hulk_camera* cam, uint32_t W, uint32_t H, hulk_ray* rays_dev,

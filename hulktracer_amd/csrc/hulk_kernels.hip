@@ -635,9 +635,9 @@ __device__ __forceinline__ bool trav_exit(const TraceParams& P, const Ray& r, Tr
 // (best.t < tstop ends it) may test more records after its first hit closer
 // than the light, where its own loop would stop: its hit may then be another
 // triangle closer than the light, which leaves the shading decision (light
-// distance > t) unchanged.  Used where few lanes of a wave are still at
-// work, so that a wave's long leaves cost one memory latency per 64 records
-// of all of them instead of one per record or two.
+// distance > t) unchanged.  A wave's leaves then cost one memory latency
+// per 64 records of all of them instead of one per record or two, and the
+// tests run on dense lanes; with coop_lanes 64 every leaf step is tested so.
 template <bool MT>
 __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, const uint4& n, const Ray& r,
                                             Best& best, uint64_t want, uint32_t lane_id,
@@ -1084,7 +1084,6 @@ enum LaneField : uint32_t {
     kFW, kFAr, kFAg, kFAb, kFCr, kFCg, kFCb,              // weight, pixel sums, generation sums
     kFIntensity, kFTb,                                    // current light's term, primary bound
     kUO, kUMb, kUG, kUK, kUTri, kUFlags,                  // pixel, max bounce, generation, light, hit tri, flags
-    kUTile,                                               // the pixel's 8x8 tile (cost measurement)
     kUCoop, kUCoop1, kUCoop2,                             // cooperative leaf tests: the wave's owner table
     kLaneFields
 };
@@ -1093,8 +1092,9 @@ constexpr uint32_t kRefillLdsWords = kLaneFields + 2u;
 
 // No wave cap: the refill loop's lane state measured fastest unspilled.
 // LDS: the top tree levels (LDS short stack), the lane fields, then the
-// short-stack columns.
-template <int TV, bool COUNT, int LB>
+// short-stack columns.  COOP: every leaf step is a cooperative one
+// (coop_lanes 64; no per-lane leaf loop in the instance).
+template <int TV, bool COUNT, int LB, bool COOP = false>
 __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
     extern __shared__ uint4 smem[];
     const uint32_t tid = threadIdx.x, block = blockDim.x;
@@ -1116,7 +1116,6 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
 
     // traversal state (registers)
     uint32_t phase = kLaneIdle, tfl = 0;
-    uint32_t steps = 0;  // loop steps of the lane's current path (tile cost)
     float tstop = -kFltMax;
     bool clipped = false, active = false;  // active: a traversal is in progress
     Ray tr;
@@ -1149,10 +1148,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
             if (phase == kLaneIdle && rank < avail) {
-                uint32_t i = lo + got + rank;
-                if (P.tile_order) i = (P.tile_order[i >> 6] << 6) | (i & 63u);
-                U(kUTile) = i >> 6;
-                steps = 0;
+                const uint32_t i = lo + got + rank;
                 uint32_t o, mb = P.max_bounce;
                 if (P.source == kSourceCamera) {
                     uint32_t x, y;
@@ -1193,9 +1189,8 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
         // (the counted instance keeps every test per lane, so its counters
         // equal the other kernels').
         const bool busy = phase != kLaneIdle;
-        steps += busy ? 1u : 0u;
         bool done = false;
-        if (COUNT || P.coop_lanes == 0) {
+        if (COUNT || (!COOP && P.coop_lanes == 0)) {
             if (busy) {
                 done = !active;
                 if (active) {
@@ -1209,10 +1204,10 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             uint4 ln = make_uint4(0u, 0u, 0u, 0u);
             if (walk) ln = trav_descend<TV, COUNT>(P, lds_nodes, stk, block, tr, st, c);
             const uint64_t at_leaf = __ballot(walk && ln.z != 0u);
-            if (at_leaf && (uint32_t)__builtin_popcountll(at_leaf) <= P.coop_lanes)
+            if (at_leaf && (COOP || (uint32_t)__builtin_popcountll(at_leaf) <= P.coop_lanes))
                 coop_leaves<LB == kLeafMt>(P.recs, ln, tr, bt, at_leaf, lane_id, keys,
                                            lane_mem + kUCoop * block + (tid - lane_id), block);
-            else if (walk && (ln.x & kNodeLeaf))
+            else if (!COOP && walk && (ln.x & kNodeLeaf))
                 test_leaf<COUNT, LB>(P.recs, P.rec_bytes, ln, tr, bt, c, tstop);
             if (busy) done = !walk || !trav_exit<TV>(P, tr, st, ln, tstop, bt, tfl);
         }
@@ -1269,7 +1264,6 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             if (shade) {
                 if (bt.tri == kNoHit) {  // the path ends: pixel
                     P.rgba[U(kUO)] = to_u8(F(kFAr)) | (to_u8(F(kFAg)) << 8) | (to_u8(F(kFAb)) << 16);
-                    if (P.tile_cost) atomicAdd(P.tile_cost + U(kUTile), steps);
                     phase = kLaneIdle;
                     active = false;
                     break;
@@ -1328,7 +1322,6 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             F(kFAb) += F(kFCb) * keep;
             if (!spawn) {
                 P.rgba[U(kUO)] = to_u8(F(kFAr)) | (to_u8(F(kFAg)) << 8) | (to_u8(F(kFAb)) << 16);
-                if (P.tile_cost) atomicAdd(P.tile_cost + U(kUTile), steps);
                 phase = kLaneIdle;
                 active = false;
                 break;
@@ -1371,7 +1364,11 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
 template <int TV>
 static void launch_refill_tv(const TraceParams& P, bool count, uint32_t grid, uint32_t block, size_t lds,
                              hipStream_t s) {
-    if (P.leaf_batch == 2) {
+    if (!count && P.coop_lanes >= 64) {  // every leaf step cooperative: the records in flight do not matter
+        if (P.leaf_batch == kLeafMt)
+            hipLaunchKernelGGL((k_render_refill<TV, false, kLeafMt, true>), dim3(grid), dim3(block), lds, s, P);
+        else hipLaunchKernelGGL((k_render_refill<TV, false, 1, true>), dim3(grid), dim3(block), lds, s, P);
+    } else if (P.leaf_batch == 2) {
         if (count) hipLaunchKernelGGL((k_render_refill<TV, true, 2>), dim3(grid), dim3(block), lds, s, P);
         else hipLaunchKernelGGL((k_render_refill<TV, false, 2>), dim3(grid), dim3(block), lds, s, P);
     } else if (P.leaf_batch == kLeafMt) {
@@ -1648,54 +1645,6 @@ __global__ void k_fill_f32(float* p, size_t n, float v) {
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
-}
-
-// Tile order from tile costs, one block of kOrderThreads: a stable counting
-// sort on a key of two buckets per octave of the cost (descending), each
-// thread counting, then placing, one contiguous range of tiles; sorted rank
-// r goes to slice r % 8 at its position r / 8.
-constexpr uint32_t kOrderThreads = 128, kOrderKeys = 64;
-
-__device__ __forceinline__ uint32_t cost_key(uint32_t c) {
-    if (c < 2u) return c;
-    const uint32_t msb = 31u - (uint32_t)__builtin_clz(c);
-    return 2u * msb + ((c >> (msb - 1u)) & 1u);
-}
-
-__global__ void __launch_bounds__(kOrderThreads) k_tile_order(const uint32_t* cost, uint32_t n, uint32_t* order) {
-    __shared__ uint32_t hist[kOrderKeys * kOrderThreads];  // [63 - key][thread]
-    __shared__ uint32_t part[kOrderThreads];
-    const uint32_t t = threadIdx.x, T = kOrderThreads;
-    const uint32_t per = (n + T - 1u) / T, b = min(n, t * per), e = min(n, b + per);
-    for (uint32_t k = 0; k < kOrderKeys; k++) hist[k * T + t] = 0u;
-    for (uint32_t i = b; i < e; i++) hist[(kOrderKeys - 1u - cost_key(cost[i])) * T + t]++;
-    __syncthreads();
-    // exclusive scan of hist in (descending key, thread) order: thread t
-    // takes the kOrderKeys consecutive entries starting at t * kOrderKeys
-    uint32_t sum = 0;
-    for (uint32_t j = 0; j < kOrderKeys; j++) sum += hist[t * kOrderKeys + j];
-    part[t] = sum;
-    __syncthreads();
-    uint32_t base = 0;
-    for (uint32_t j = 0; j < t; j++) base += part[j];
-    __syncthreads();
-    for (uint32_t j = 0; j < kOrderKeys; j++) {
-        const uint32_t v = hist[t * kOrderKeys + j];
-        hist[t * kOrderKeys + j] = base;
-        base += v;
-    }
-    __syncthreads();
-    const uint32_t per_slice = n >> 3;
-    for (uint32_t i = b; i < e; i++) {
-        const uint32_t r = hist[(kOrderKeys - 1u - cost_key(cost[i])) * T + t]++;
-        order[(r & 7u) * per_slice + (r >> 3)] = i;
-    }
-}
-
-hipError_t launch_tile_order(const uint32_t* cost, uint32_t n_tiles, uint32_t* order, hipStream_t s) {
-    if (n_tiles == 0 || (n_tiles & 7u) || n_tiles > kMaxOrderTiles) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, s, cost, n_tiles, order);
-    return hipGetLastError();
 }
 
 // ShadePicture, shadePicture.cl:7-55: base 0.1 for every pixel; a hit

@@ -102,11 +102,6 @@ struct TraceParams {
     uint32_t leaf_batch;    // host side: picks the kernel instance (1 or 2 records in flight, 3 = one
                             // record with the Moller-Trumbore test)
     uint32_t coop_lanes;    // refill kernel: wave-cooperative leaf tests at <= this many busy lanes
-    // refill kernel, camera source: queue position q takes the 64 rays of
-    // 8x8 tile tile_order[q >> 6] (null: tile q >> 6), and each finished
-    // pixel adds its path's loop steps to tile_cost[its tile] (null: none)
-    const uint32_t* tile_order;
-    uint32_t* tile_cost;
     uint32_t stop_at_w;  // shadow queue: stop once best.t < ray.start.w (the light distance)
     // several scenes (frame source, whole-frame launch): the frame's scene
     // entry replaces the scene fields above for each chunk of 64 rays
@@ -115,14 +110,6 @@ struct TraceParams {
 
 constexpr uint32_t kWorkCounterStride = 64;  // 256 B between the per-XCD counters
 constexpr uint32_t kWorkCounterWords = 8 * kWorkCounterStride;
-
-// Tile order of a lane-refill frame from the per-tile path costs of an
-// earlier frame of the same size: the tiles sorted by cost, heaviest first
-// (ties by tile index), dealt round-robin over the 8 queue slices so that
-// every slice starts on its share of the heaviest tiles.  n_tiles: a
-// multiple of 8, at most kMaxOrderTiles.
-constexpr uint32_t kMaxOrderTiles = 1u << 22;
-hipError_t launch_tile_order(const uint32_t* cost, uint32_t n_tiles, uint32_t* order, hipStream_t s);
 
 // a / d for a divisor d >= 1 shared by every lane: m = udiv_multiplier(d)
 // from the host, umulhi(a, m) is at most two below the quotient.

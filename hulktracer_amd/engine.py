@@ -307,24 +307,20 @@ class GPUTrace:
 
     def __init__(self, variant: int | str = L.FULLSTACK, block_size: int = 256, short_stack_depth: int = 6,
                  lds_top_levels: int = 0, persistent: bool = False, device: int = 0, stream: int | None = None,
-                 pipeline: int = 0, leaf_batch: int = 1, coop_lanes: int = 0, tri_test: int = 0,
-                 tile_order: int = 0):
+                 pipeline: int = 0, leaf_batch: int = 1, coop_lanes: int = 0, tri_test: int = 0):
         """pipeline: 0 = automatic, 1 = queued shadow / mirror launches,
         2 = one fused launch per frame, 3 = fused with lane refill
         (hulk_pipeline).  leaf_batch: triangle records in flight per lane
         (1 or 2).  coop_lanes: lane-refill pipeline only; when at most this
         many lanes of a wave are at a leaf, the whole wave tests their
         leaves (0 = never).  tri_test: 0 = the reference's triangle test,
-        1 = Moller-Trumbore (hulk_tri_test).  tile_order: lane-refill
-        camera frames, 0 = heaviest tiles of the previous frame of the same
-        size first, 1 = pixel order (results do not depend on it)."""
+        1 = Moller-Trumbore (hulk_tri_test)."""
         if isinstance(variant, str):
             variant = L.VARIANTS[variant]
         self.opts = L.Opts(device=device, variant=variant, block_size=block_size,
                            short_stack_depth=short_stack_depth, lds_top_levels=lds_top_levels,
                            persistent=int(persistent), pipeline=int(pipeline), leaf_batch=int(leaf_batch),
-                           coop_lanes=int(coop_lanes), tri_test=int(tri_test), tile_order=int(tile_order),
-                           stream=stream)
+                           coop_lanes=int(coop_lanes), tri_test=int(tri_test), stream=stream)
         self.h = C.c_void_p()
         _check(None, lib().hulk_create(C.byref(self.h), C.byref(self.opts)))
 
@@ -459,16 +455,6 @@ class GPUTrace:
         self._c(lib().hulk_render_frame_static(self.h, ptr(np.ascontiguousarray(cam, np.float32)), W, H, max_bounce,
                                                ptr(mirror_dev), ptr(tbound_dev), ptr(rgba_dev), ptr(hits_dev),
                                                flags))
-
-    def tile_costs(self, W: int, H: int) -> tuple[np.ndarray, np.ndarray]:
-        """hulk_tile_costs: per-8x8-tile path costs of the last lane-refill
-        camera frame of W x H (tiles row-major) and the tile order the next
-        such frame takes."""
-        n = (W // 8) * (H // 8)
-        cost = np.zeros(n, np.uint32)
-        order = np.zeros(n, np.uint32)
-        self._c(lib().hulk_tile_costs(self.h, W, H, cost.ctypes.data, order.ctypes.data))
-        return cost, order
 
     def sync(self):
         self._c(lib().hulk_sync(self.h))

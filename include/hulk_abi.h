@@ -131,15 +131,6 @@ typedef struct hulk_opts {
                                    HULK_TRI_MOLLER_TRUMBORE (optional; one
                                    record in flight, not with the streamed
                                    trace)                                    */
-    uint32_t tile_order;        /* lane-refill pipeline, camera frames whose
-                                   sides are multiples of 8 and whose 8x8
-                                   tile count is a multiple of 8: 0 = the
-                                   context takes the tiles heaviest first by
-                                   the per-tile path costs of its previous
-                                   such frame of the same size (pixel order
-                                   for the first one); 1 = pixel order.
-                                   Pixels, hit records and counters do not
-                                   depend on the order.                      */
     void*    stream;            /* hipStream_t to use; NULL = own stream     */
 } hulk_opts;
 
@@ -298,17 +289,6 @@ int hulk_render_frame(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uin
 int hulk_render_frame_static(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height,
                              uint32_t max_bounce, const uint8_t* mirror_mask_dev, const float* tbound_dev,
                              uint32_t* rgba8_dev, hulk_hit* primary_hits_dev, uint32_t flags);
-
-/* Per-tile path costs of the context's last lane-refill camera frame of
- * width x height rendered with tile_order 0.  No counterpart in the
- * reference: a cost map of the frame's divergence, and the tile order that
- * the next such frame takes.  cost[t]: the loop steps of the paths of the
- * 64 pixels of 8x8 tile t (tiles row-major over the image); order[q]: the
- * tile that queue position q takes next frame.  Host arrays of
- * (width / 8) * (height / 8) entries, either nullable; synchronizes the
- * context's stream.  HULK_E_ILLEGAL_INPUT when the context rendered no such
- * frame of that size (the allocation-free call measures nothing). */
-int hulk_tile_costs(hulk_ctx* ctx, uint32_t width, uint32_t height, uint32_t* cost, uint32_t* order);
 
 int hulk_get_counters(const hulk_ctx* ctx, hulk_counters* out);
 int hulk_reset_counters(hulk_ctx* ctx);

@@ -52,7 +52,7 @@ def test_struct_sizes_match_reference_layout():
     # CameraPos 64 B, Light 32 B, Ray 32 B
     assert H.HIT_DTYPE.itemsize == 32
     assert H.RAY_DTYPE.itemsize == 32
-    assert C.sizeof(_lib.Opts) == 56  # 11 x 4 B, padding, stream pointer
+    assert C.sizeof(_lib.Opts) == 48  # 10 x 4 B + stream pointer
     assert C.sizeof(_lib.TuneRequest) == 96  # camera 64 B + 6 x 4 B + mirror pointer
     assert C.sizeof(_lib.TuneEntry) == 36
 

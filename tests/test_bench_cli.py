@@ -32,7 +32,9 @@ def test_modes_and_helpers_exist():
     assert b.kernel_name({"variant": 3, "pipeline": 2, "leaf_batch": 2}) == "k_trace<3,false,true,2>"
     assert b.kernel_name({"variant": 5, "pipeline": 0}) == "k_trace<5,false,true,1>"
     assert b.kernel_name({"variant": 5, "pipeline": 1}) == "queued pipeline graph"
-    assert b.kernel_name({"variant": 5, "pipeline": 3, "leaf_batch": 2}) == "k_render_refill<5,false,2>"
+    assert b.kernel_name({"variant": 5, "pipeline": 3, "leaf_batch": 2}) == "k_render_refill<5,false,2,false>"
+    assert b.kernel_name({"variant": 5, "pipeline": 3, "leaf_batch": 2, "coop_lanes": 64}) == \
+        "k_render_refill<5,false,1,true>"
     threads, aff, model = b.host_cpus()
     assert 1 <= threads <= aff and isinstance(model, str)
 
@@ -61,7 +63,7 @@ def test_roofline_picks_the_most_loaded_resource(tmp_path):
     (tmp_path / "profiles").mkdir()
     # 1 ms launch: 307.2 G wave instructions/s (half the VALU peak), 1 TB/s HBM (1/8 of peak)
     (tmp_path / "profiles" / "y.txt").write_text(
-        f"counters k_render_refill<5,false,2> {b.opt_string(opts)} w_1024x1024 "
+        f"counters k_render_refill<5,false,2,false> {b.opt_string(opts)} w_1024x1024 "
         "fetch_bytes=600000000 write_bytes=400000000 valu_wave_instr=307200000\n")
     b.ROOT = tmp_path
     wl = {"options_batched": opts, "timing": {"batched": (1.01, 1.0)}, "workload_key": "w_1024x1024",

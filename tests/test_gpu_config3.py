@@ -82,85 +82,3 @@ def test_config3_frame_matches_oracle(frame, opts):
     assert cnt["rays"] == int(rcounts[0]), (cnt["rays"], int(rcounts[0]))
     assert cnt["rays"] > W * Hh  # the mirror generations traced rays
     assert (hits["tri"] != H.NO_HIT).mean() > 0.5
-
-
-def _expected_order(cost: np.ndarray) -> np.ndarray:
-    """The tile order the library derives from tile costs: stable sort on a
-    key of two buckets per octave (descending), rank r dealt to slice r % 8
-    at position r / 8."""
-    c = cost.astype(np.int64)
-    msb = np.array([max(int(x).bit_length() - 1, 0) for x in c], np.int64)
-    key = np.where(c < 2, c, 2 * msb + ((c >> np.maximum(msb - 1, 0)) & 1))
-    ranked = np.argsort(-key, kind="stable")
-    n = len(c)
-    order = np.empty(n, np.int64)
-    r = np.arange(n)
-    order[(r % 8) * (n // 8) + r // 8] = ranked
-    return order
-
-
-def test_config3_tile_order_keeps_every_pixel():
-    """Lane refill takes a frame's 8x8 tiles heaviest first by the previous
-    frame's per-tile path costs (tile_order 0): frame 0 in pixel order, then
-    frame 11 (rebuilt tree) in the order frame 0 measured, on one context.
-    Frame 11 equals the oracle bit for bit, and the order is the sorted,
-    slice-dealt permutation of frame 0's costs."""
-    opts = dict(OPTION_SETS[0])
-    variant = opts.pop("variant")
-    s0, n0, lp0 = scene_and_tree(3, 7, 0, 0)
-    s11, n11, lp11 = scene_and_tree(3, 7, 0, 11)
-    print(f"config 3 tile order frames 0 -> 11 variant {variant} {opts}", flush=True)
-    ctx = gpu_ctx(s0, n0, lp0, variant, **opts)
-    try:
-        ctx.render_frame(s0.camera(0, Hh / W), W, Hh, max_bounce=2, mirror=s0.mirror)
-        cost0, order0 = ctx.tile_costs(W, Hh)
-        ctx.upload_scene(s11)
-        ctx.upload_kdtree(n11, lp11)
-        img, hits = ctx.render_frame(s11.camera(11, Hh / W), W, Hh, max_bounce=2, mirror=s11.mirror, count=True)
-        cnt = ctx.counters()
-        cost11, _ = ctx.tile_costs(W, Hh)
-    finally:
-        ctx.close()
-    n_tiles = (W // 8) * (Hh // 8)
-    assert sorted(order0.tolist()) == list(range(n_tiles))
-    assert np.array_equal(order0, _expected_order(cost0))
-    per = n_tiles // 8
-    heads = cost0[order0.reshape(8, per)[:, 0]]
-    assert heads.min() >= np.percentile(cost0, 99)  # every slice starts on a heaviest tile
-    assert cost0.sum() > 0 and cost11.sum() > 0
-    rimg, rcounts, rhits = _oracle_frame(11, ORACLE_OF[variant])
-    bad = hit_mismatch(hits, rhits)
-    assert len(bad) == 0, f"{len(bad)} primary hit records differ\n{describe(hits, rhits, bad)}"
-    diff = img != rimg
-    assert not diff.any(), f"{int(diff.sum())} of {W * Hh} pixels differ, first at {np.nonzero(diff)[0][:5]}"
-    assert cnt["rays"] == int(rcounts[0]), (cnt["rays"], int(rcounts[0]))
-
-
-def test_tile_order_option_and_sizes():
-    """tile_order 1 keeps pixel order (no cost map); a frame whose tile count
-    is not a multiple of 8 measures none; the map belongs to the last size."""
-    opts = dict(OPTION_SETS[0])
-    variant = opts.pop("variant")
-    s, nodes, lp = scene_and_tree(3, 7, 0, 0)
-    cam = s.camera(0, 1.0)
-    ctx = gpu_ctx(s, nodes, lp, variant, tile_order=1, **opts)
-    try:
-        a, _ = ctx.render_frame(cam, 128, 128, max_bounce=2, mirror=s.mirror)
-        with pytest.raises(H.HulkError):
-            ctx.tile_costs(128, 128)
-    finally:
-        ctx.close()
-    ctx = gpu_ctx(s, nodes, lp, variant, **opts)
-    try:
-        ctx.render_frame(cam, 136, 120, max_bounce=2, mirror=s.mirror)  # 17 x 15 tiles: not a multiple of 8
-        with pytest.raises(H.HulkError):
-            ctx.tile_costs(136, 120)
-        for _ in range(2):
-            b, _ = ctx.render_frame(cam, 128, 128, max_bounce=2, mirror=s.mirror)
-            assert np.array_equal(a, b)
-        cost, order = ctx.tile_costs(128, 128)
-        assert sorted(order.tolist()) == list(range(256)) and cost.sum() > 0
-        with pytest.raises(H.HulkError):
-            ctx.tile_costs(256, 256)
-    finally:
-        ctx.close()
