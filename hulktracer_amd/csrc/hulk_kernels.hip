@@ -466,20 +466,27 @@ __device__ __forceinline__ uint4 ropes_descend(const TraceParams& P, const Ray& 
     return n;
 }
 
-// After the leaf of s.slot was tested: follow the rope out of it.  False
-// once the traversal is over.
-__device__ __forceinline__ bool ropes_exit(const TraceParams& P, const Ray& r, RopeState& s, float tstop,
-                                           const Best& best, uint32_t& flags) {
-    if (best.t < tstop) return false;
-    const RopeSlot& rs = P.rope_slots[s.slot];
-    const float4 lo = rs.lo, hi = rs.hi;
-    const float mn[3] = {lo.x, lo.y, lo.z};
-    const float mx[3] = {hi.x, hi.y, hi.z};
+// The exit of the leaf at s.slot from its rope slot: the face the ray
+// leaves through (d[a] > 0: max face, d[a] < 0: min face, ties to the lowest
+// axis), its slab time and the rope across it.  Independent of the leaf's
+// hits, so the lane-refill loop computes it before the leaf is tested.
+struct RopeExit {
+    float tout;
+    uint32_t nxt;
+    int face;
+};
+
+__device__ __forceinline__ RopeExit rope_exit_of(const Ray& r, const RopeSlot* slots, uint32_t slot) {
+    const uint4* p = reinterpret_cast<const uint4*>(slots + slot);
+    const uint4 lo4 = p[0], hi4 = p[1];
+    const float mn[3] = {__uint_as_float(lo4.x), __uint_as_float(lo4.y), __uint_as_float(lo4.z)};
+    const float mx[3] = {__uint_as_float(hi4.x), __uint_as_float(hi4.y), __uint_as_float(hi4.z)};
     const float o[3] = {r.o.x, r.o.y, r.o.z};
     const float d[3] = {r.d.x, r.d.y, r.d.z};
     const float iv[3] = {r.inv.x, r.inv.y, r.inv.z};
-    float tout = kFltMax;
-    int face = -1;
+    RopeExit e;
+    e.tout = kFltMax;
+    e.face = -1;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         float ta;
@@ -493,16 +500,24 @@ __device__ __forceinline__ bool ropes_exit(const TraceParams& P, const Ray& r, R
         } else {
             continue;
         }
-        if (ta < tout) {
-            tout = ta;
-            face = f;
+        if (ta < e.tout) {
+            e.tout = ta;
+            e.face = f;
         }
     }
-    if (best.t <= tout || tout >= s.te || face < 0) return false;
-    const uint32_t nxt = rs.rope[face];
-    if (nxt == kRopeNone) return false;
-    if (tout > s.t) s.t = tout;
-    s.slot = nxt;
+    e.nxt = e.face < 0 ? kRopeNone : slots[slot].rope[e.face];
+    return e;
+}
+
+// After the leaf of s.slot was tested: follow the rope out of it.  False
+// once the traversal is over.
+__device__ __forceinline__ bool ropes_exit(const TraceParams& P, RopeState& s, const RopeExit& e, float tstop,
+                                           const Best& best, uint32_t& flags) {
+    if (best.t < tstop) return false;
+    if (best.t <= e.tout || e.tout >= s.te || e.face < 0) return false;
+    if (e.nxt == kRopeNone) return false;
+    if (e.tout > s.t) s.t = e.tout;
+    s.slot = e.nxt;
     if (++s.guard > P.guard) {
         flags |= kHitFlagGuard;
         return false;
@@ -517,7 +532,8 @@ __device__ __forceinline__ bool ropes_step(const TraceParams& P, const Ray& r, R
                                            Cnt& c, uint32_t& flags) {
     const uint4 n = ropes_descend<COUNT>(P, r, s, c);
     test_leaf<COUNT, LB>(P.recs, P.rec_bytes, n, r, best, c, tstop);
-    return ropes_exit(P, r, s, tstop, best, flags);
+    if (best.t < tstop) return false;
+    return ropes_exit(P, s, rope_exit_of(r, P.rope_slots, s.slot), tstop, best, flags);
 }
 
 // The LDS short stack of trav_shortstack split at its leaves, so that a
@@ -599,6 +615,7 @@ __device__ __forceinline__ bool ss_exit(const TraceParams& P, SsState& s, const 
 // LDS short stack (TV = HULK_VARIANT_LDSSTACK).
 struct TravState {
     RopeState rope;
+    RopeExit ex;  // ropes: the exit of the leaf the last descent reached
     SsState ss;
 };
 
@@ -612,14 +629,16 @@ template <int TV, bool COUNT>
 __device__ __forceinline__ uint4 trav_descend(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk,
                                               uint32_t stride, const Ray& r, TravState& s, Cnt& c) {
     if (TV == HULK_VARIANT_LDSSTACK) return ss_descend<COUNT>(P, lds_nodes, stk, stride, r, s.ss, c);
-    return ropes_descend<COUNT>(P, r, s.rope, c);
+    const uint4 n = ropes_descend<COUNT>(P, r, s.rope, c);
+    s.ex = rope_exit_of(r, P.rope_slots, s.rope.slot);
+    return n;
 }
 
 template <int TV>
 __device__ __forceinline__ bool trav_exit(const TraceParams& P, const Ray& r, TravState& s, const uint4& leaf,
                                           float tstop, const Best& best, uint32_t& flags) {
     if (TV == HULK_VARIANT_LDSSTACK) return ss_exit(P, s.ss, leaf, tstop, best, flags);
-    return ropes_exit(P, r, s.rope, tstop, best, flags);
+    return ropes_exit(P, s.rope, s.ex, tstop, best, flags);
 }
 
 // The leaves of the lanes in `want` (a wave-uniform ballot; every lane of
