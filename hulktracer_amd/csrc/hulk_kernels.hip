@@ -657,49 +657,66 @@ __device__ __forceinline__ bool trav_exit(const TraceParams& P, const Ray& r, Tr
 // distance > t) unchanged.  A wave's leaves then cost one memory latency
 // per 64 records of all of them instead of one per record or two, and the
 // tests run on dense lanes; with coop_lanes 64 every leaf step is tested so.
+// Inclusive prefix sum over the 64 lanes of a wave with DPP: row shifts by
+// 1, 2, 4 and 8 (zero from outside the row of 16), then the last lane of
+// row 0 added to row 1 and of row 2 to row 3, then the last lane of row 1 to
+// rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return v;
+}
+
+// Per-wave LDS of the cooperative leaf tests (kCoopWaveWords words): 64
+// owner entries of three 16-B words (lane, first packed index, first record,
+// best t; ray origin, d.x; d.y, d.z), then one 64-bit word of segment-start
+// bits.
+constexpr uint32_t kCoopWaveWords = 13u * 64u;
+
 template <bool MT>
 __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, const uint4& n, const Ray& r,
                                             Best& best, uint64_t want, uint32_t lane_id,
-                                            unsigned long long* keys, uint32_t* tab, uint32_t tab_stride) {
+                                            unsigned long long* keys, uint4* otab) {
     const bool own = (want >> lane_id) & 1u;
     const uint32_t cnt = own ? n.z : 0u;
-    uint32_t pre = cnt;  // inclusive scan of the owners' record counts in lane order
-#pragma unroll
-    for (uint32_t off = 1; off < 64u; off <<= 1) {
-        const uint32_t v = (uint32_t)__shfl_up((int)pre, off, 64);
-        if (lane_id >= off) pre += v;
-    }
+    const uint32_t pre = wave_inclusive_sum(cnt);  // the owners' record counts, scanned in lane order
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)pre, 63);
     const uint32_t first = pre - cnt;  // this owner's first packed index
-    // owner table by rank (owners in lane order): lane, first packed index, first record
+    const uint32_t my_rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0u));
     if (own) {
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0u));
-        tab[rank] = lane_id;
-        tab[tab_stride + rank] = first;
-        tab[2u * tab_stride + rank] = n.y;
+        otab[3u * my_rank] = make_uint4(lane_id, first, n.y, __float_as_uint(best.t));
+        otab[3u * my_rank + 1u] = make_uint4(__float_as_uint(r.o.x), __float_as_uint(r.o.y), __float_as_uint(r.o.z),
+                                             __float_as_uint(r.d.x));
+        otab[3u * my_rank + 2u] = make_uint4(__float_as_uint(r.d.y), __float_as_uint(r.d.z), 0u, 0u);
     }
+    unsigned long long* starts_word = reinterpret_cast<unsigned long long*>(otab + 192);
+    Best b;
+    best_init(b);
     for (uint32_t base = 0; base < total; base += 64u) {
         const uint32_t g = base + lane_id;
-        // owner of packed record g: owners that began before this pass, plus
-        // those whose first index lies in the pass at or below g
+        // owner of packed record g: the owners that began before the pass,
+        // plus those whose first index lies in the pass at or below g
         const uint32_t before = (uint32_t)__builtin_popcountll(__ballot(own && first < base));
-        if (lane_id == 63u) keys[63] = 0ull;
-        if (own && first >= base && first < base + 64u) atomicOr(keys + 63, 1ull << (first - base));
-        const unsigned long long starts = keys[63];
+        if (lane_id == 0u) *starts_word = 0ull;
+        if (own && first >= base && first < base + 64u) atomicOr(starts_word, 1ull << (first - base));
+        const unsigned long long starts = *starts_word;
         const unsigned long long upto = lane_id == 63u ? ~0ull : (2ull << lane_id) - 1ull;
         const uint32_t cnt_before = before + (uint32_t)__builtin_popcountll(starts & upto);
         const uint32_t rank = cnt_before ? cnt_before - 1u : 0u;
-        const uint32_t L = tab[rank], gL = tab[tab_stride + rank], sL = tab[2u * tab_stride + rank];
+        const uint4 e0 = otab[3u * rank], e1 = otab[3u * rank + 1u], e2 = otab[3u * rank + 2u];
+        const uint32_t L = e0.x;
         Ray q;
-        q.o = v3(__shfl(r.o.x, L, 64), __shfl(r.o.y, L, 64), __shfl(r.o.z, L, 64));
-        q.d = v3(__shfl(r.d.x, L, 64), __shfl(r.d.y, L, 64), __shfl(r.d.z, L, 64));
-        Best b;
-        best_init(b);
-        b.t = __shfl(best.t, L, 64);
+        q.o = v3(__uint_as_float(e1.x), __uint_as_float(e1.y), __uint_as_float(e1.z));
+        q.d = v3(__uint_as_float(e1.w), __uint_as_float(e2.x), __uint_as_float(e2.y));
+        b.t = __uint_as_float(e0.w);  // the owner's best t before the pass
         bool hit = false;
         if (own) keys[lane_id] = ~0ull;
         if (g < total) {
-            const TriRec rec = recs[sL + (g - gL)];
+            const TriRec rec = recs[e0.z + (g - e0.y)];
             hit = test_any<MT>(q, rec, b);
         }
         // t >= 0 here (or -0): the bits without the sign order like the value
@@ -720,6 +737,7 @@ __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, con
             best.nx = nx;
             best.ny = ny;
             best.nz = nz;
+            otab[3u * my_rank].w = __float_as_uint(t);
         }
     }
 }
@@ -1103,11 +1121,11 @@ enum LaneField : uint32_t {
     kFW, kFAr, kFAg, kFAb, kFCr, kFCg, kFCb,              // weight, pixel sums, generation sums
     kFIntensity, kFTb,                                    // current light's term, primary bound
     kUO, kUMb, kUG, kUK, kUTri, kUFlags,                  // pixel, max bounce, generation, light, hit tri, flags
-    kUCoop, kUCoop1, kUCoop2,                             // cooperative leaf tests: the wave's owner table
     kLaneFields
 };
-// lane fields, then one 64-bit word per lane for the cooperative leaf tests
-constexpr uint32_t kRefillLdsWords = kLaneFields + 2u;
+// lane fields, one 64-bit word per lane and kCoopWaveWords per wave for
+// the cooperative leaf tests
+constexpr uint32_t kRefillLdsWords = kLaneFields + 2u + kCoopWaveWords / 64u;
 
 // No wave cap: the refill loop's lane state measured fastest unspilled.
 // LDS: the top tree levels (LDS short stack), the lane fields, then the
@@ -1121,6 +1139,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
     uint4* lds_nodes = smem;
     uint32_t* lane_mem = reinterpret_cast<uint32_t*>(smem + P.lds_node_count);
     unsigned long long* keys = reinterpret_cast<unsigned long long*>(lane_mem + kLaneFields * block) + (tid - lane_id);
+    uint4* otab = reinterpret_cast<uint4*>(lane_mem + (kLaneFields + 2u) * block + (tid >> 6) * kCoopWaveWords);
     uint32_t* stk = lane_mem + kRefillLdsWords * block + tid;
     for (uint32_t i = tid; i < P.lds_node_count; i += block) lds_nodes[i] = P.nodes[i];
     if (P.lds_node_count) __syncthreads();
@@ -1224,8 +1243,7 @@ __global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
             if (walk) ln = trav_descend<TV, COUNT>(P, lds_nodes, stk, block, tr, st, c);
             const uint64_t at_leaf = __ballot(walk && ln.z != 0u);
             if (at_leaf && (COOP || (uint32_t)__builtin_popcountll(at_leaf) <= P.coop_lanes))
-                coop_leaves<LB == kLeafMt>(P.recs, ln, tr, bt, at_leaf, lane_id, keys,
-                                           lane_mem + kUCoop * block + (tid - lane_id), block);
+                coop_leaves<LB == kLeafMt>(P.recs, ln, tr, bt, at_leaf, lane_id, keys, otab);
             else if (!COOP && walk && (ln.x & kNodeLeaf))
                 test_leaf<COUNT, LB>(P.recs, P.rec_bytes, ln, tr, bt, c, tstop);
             if (busy) done = !walk || !trav_exit<TV>(P, tr, st, ln, tstop, bt, tfl);
