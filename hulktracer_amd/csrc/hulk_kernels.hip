@@ -1123,28 +1123,35 @@ enum LaneField : uint32_t {
     kUO, kUMb, kUG, kUK, kUTri, kUFlags,                  // pixel, max bounce, generation, light, hit tri, flags
     kLaneFields
 };
-// lane fields, one 64-bit word per lane and kCoopWaveWords per wave for
-// the cooperative leaf tests
+// per lane: the lane fields, one 64-bit key and the lane's share of the
+// kCoopWaveWords of the cooperative leaf tests
 constexpr uint32_t kRefillLdsWords = kLaneFields + 2u + kCoopWaveWords / 64u;
+static_assert(kRefillLdsWords * 64u * 4u % 16u == 0u && (kLaneFields + 2u) * 64u * 4u % 16u == 0u,
+              "the owner table of each wave is 16-B aligned");
 
 // No wave cap: the refill loop's lane state measured fastest unspilled.
 // LDS: the top tree levels (LDS short stack), the lane fields, then the
 // short-stack columns.  COOP: every leaf step is a cooperative one
 // (coop_lanes 64; no per-lane leaf loop in the instance).
 template <int TV, bool COUNT, int LB, bool COOP = false>
-__global__ void __launch_bounds__(256) k_render_refill(TraceParams P) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COOP ? 4 : 1))) k_render_refill(TraceParams P) {
     extern __shared__ uint4 smem[];
     const uint32_t tid = threadIdx.x, block = blockDim.x;
     const uint32_t lane_id = tid & 63u;
     uint4* lds_nodes = smem;
     uint32_t* lane_mem = reinterpret_cast<uint32_t*>(smem + P.lds_node_count);
-    unsigned long long* keys = reinterpret_cast<unsigned long long*>(lane_mem + kLaneFields * block) + (tid - lane_id);
-    uint4* otab = reinterpret_cast<uint4*>(lane_mem + (kLaneFields + 2u) * block + (tid >> 6) * kCoopWaveWords);
+    // one region of kRefillLdsWords * 64 words per wave: lane field f of
+    // lane l at word 64 f + l, so that every field is one base register and
+    // an immediate offset; then the owners' keys and the owner table
+    uint32_t* wave_mem = lane_mem + (tid >> 6) * (kRefillLdsWords * 64u);
+    uint32_t* fields = wave_mem + lane_id;
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(wave_mem + kLaneFields * 64u);
+    uint4* otab = reinterpret_cast<uint4*>(wave_mem + (kLaneFields + 2u) * 64u);
     uint32_t* stk = lane_mem + kRefillLdsWords * block + tid;
     for (uint32_t i = tid; i < P.lds_node_count; i += block) lds_nodes[i] = P.nodes[i];
     if (P.lds_node_count) __syncthreads();
-    auto F = [&](uint32_t f) -> float& { return reinterpret_cast<float*>(lane_mem)[f * block + tid]; };
-    auto U = [&](uint32_t f) -> uint32_t& { return lane_mem[f * block + tid]; };
+    auto F = [&](uint32_t f) -> float& { return reinterpret_cast<float*>(fields)[f * 64u]; };
+    auto U = [&](uint32_t f) -> uint32_t& { return fields[f * 64u]; };
     const uint32_t n_rays = P.n;
     uint64_t cfb = 0, chits = 0, crays = 0, cshadow = 0;
 
