@@ -625,6 +625,20 @@ __device__ __forceinline__ bool trav_begin(const TraceParams& P, const Ray& r, f
     return ropes_begin(P, r, tclip, s.rope);
 }
 
+// Rope traversal of a ray whose origin lies in the scene box (entry t 0)
+// started at leaf slot `hint` instead of the root when the origin lies
+// strictly inside the hint's cell: with t 0 every descent decision takes the
+// origin's side of the plane (o != plane), so the root descent ends there.
+__device__ __forceinline__ void ropes_hint(const TraceParams& P, const Ray& r, RopeState& s, uint32_t hint) {
+    if (s.t != 0.0f || hint == 0u) return;
+    const uint4* p = reinterpret_cast<const uint4*>(P.rope_slots + hint);
+    const uint4 lo = p[0], hi = p[1];
+    const bool in = __uint_as_float(lo.x) < r.o.x && r.o.x < __uint_as_float(hi.x) &&
+                    __uint_as_float(lo.y) < r.o.y && r.o.y < __uint_as_float(hi.y) &&
+                    __uint_as_float(lo.z) < r.o.z && r.o.z < __uint_as_float(hi.z);
+    if (in) s.slot = hint;
+}
+
 template <int TV, bool COUNT>
 __device__ __forceinline__ uint4 trav_descend(const TraceParams& P, const uint4* lds_nodes, uint32_t* stk,
                                               uint32_t stride, const Ray& r, TravState& s, Cnt& c) {
@@ -641,22 +655,6 @@ __device__ __forceinline__ bool trav_exit(const TraceParams& P, const Ray& r, Tr
     return ropes_exit(P, s.rope, s.ex, tstop, best, flags);
 }
 
-// The leaves of the lanes in `want` (a wave-uniform ballot; every lane of
-// the wave executes this call) tested by the whole wave, 64 records per
-// pass, the records of all owner lanes packed back to back in owner lane
-// order: lane k of a pass tests record k of the pass against its owner's
-// ray and that owner's best t before the pass.  Per owner the pass keeps the
-// smallest t, and of equal t the lowest leaf position, which is the hit the
-// owner's own loop keeps (strict '<' in leaf order: the first of equal t
-// wins, GPU_common.cl:62-68), with the same per-test arithmetic
-// (test_record); `keys` (one 64-bit LDS word per lane of the wave) holds
-// each owner's (t bits, packed index) minimum of the pass.  A shadow ray
-// (best.t < tstop ends it) may test more records after its first hit closer
-// than the light, where its own loop would stop: its hit may then be another
-// triangle closer than the light, which leaves the shading decision (light
-// distance > t) unchanged.  A wave's leaves then cost one memory latency
-// per 64 records of all of them instead of one per record or two, and the
-// tests run on dense lanes; with coop_lanes 64 every leaf step is tested so.
 // Inclusive prefix sum over the 64 lanes of a wave with DPP: row shifts by
 // 1, 2, 4 and 8 (zero from outside the row of 16), then the last lane of
 // row 0 added to row 1 and of row 2 to row 3, then the last lane of row 1 to
@@ -673,10 +671,28 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 
 // Per-wave LDS of the cooperative leaf tests (kCoopWaveWords words): 64
 // owner entries of three 16-B words (lane, first packed index, first record,
-// best t; ray origin, d.x; d.y, d.z), then one 64-bit word of segment-start
-// bits.
+// best t before the leaf; ray origin, d.x; d.y, d.z), then one 64-bit word of
+// segment-start bits.
 constexpr uint32_t kCoopWaveWords = 13u * 64u;
 
+// The leaves of the lanes in `want` (a wave-uniform ballot; every lane of
+// the wave executes this call) tested by the whole wave, 64 records per
+// pass, the records of all owner lanes packed back to back in owner lane
+// order: lane k of a pass tests packed record k against its owner's ray and
+// that owner's best t before the leaf.  `keys` (one 64-bit LDS word per lane
+// of the wave) keeps each owner's minimum of (t bits, packed index) over all
+// passes: the smallest t, and of equal t the lowest leaf position, which is
+// the hit the owner's own loop keeps (strict '<' in leaf order: the first of
+// equal t wins, GPU_common.cl:62-68), with the same per-test arithmetic
+// (test_record).  After a pass in which some lane hit, each owner whose
+// minimum lies in that pass takes the hit fields of the lane that tested
+// it; a pass without hits ends at its tests.  A shadow ray (best.t <
+// tstop ends it) may test more records after its first hit closer than the
+// light, where its own loop would stop: its hit may then be another triangle
+// closer than the light, which leaves the shading decision (light distance
+// > t) unchanged.  A wave's leaves then cost one memory latency per 64
+// records of all of them instead of one per record or two, and the tests
+// run on dense lanes; with coop_lanes 64 every leaf step is tested so.
 template <bool MT>
 __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, const uint4& n, const Ray& r,
                                             Best& best, uint64_t want, uint32_t lane_id,
@@ -692,10 +708,9 @@ __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, con
         otab[3u * my_rank + 1u] = make_uint4(__float_as_uint(r.o.x), __float_as_uint(r.o.y), __float_as_uint(r.o.z),
                                              __float_as_uint(r.d.x));
         otab[3u * my_rank + 2u] = make_uint4(__float_as_uint(r.d.y), __float_as_uint(r.d.z), 0u, 0u);
+        keys[lane_id] = ~0ull;
     }
     unsigned long long* starts_word = reinterpret_cast<unsigned long long*>(otab + 192);
-    Best b;
-    best_init(b);
     for (uint32_t base = 0; base < total; base += 64u) {
         const uint32_t g = base + lane_id;
         // owner of packed record g: the owners that began before the pass,
@@ -707,28 +722,29 @@ __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, con
         const unsigned long long upto = lane_id == 63u ? ~0ull : (2ull << lane_id) - 1ull;
         const uint32_t cnt_before = before + (uint32_t)__builtin_popcountll(starts & upto);
         const uint32_t rank = cnt_before ? cnt_before - 1u : 0u;
-        const uint4 e0 = otab[3u * rank], e1 = otab[3u * rank + 1u], e2 = otab[3u * rank + 2u];
-        const uint32_t L = e0.x;
-        Ray q;
-        q.o = v3(__uint_as_float(e1.x), __uint_as_float(e1.y), __uint_as_float(e1.z));
-        q.d = v3(__uint_as_float(e1.w), __uint_as_float(e2.x), __uint_as_float(e2.y));
-        b.t = __uint_as_float(e0.w);  // the owner's best t before the pass
+        Best b;
         bool hit = false;
-        if (own) keys[lane_id] = ~0ull;
         if (g < total) {
+            const uint4 e0 = otab[3u * rank], e1 = otab[3u * rank + 1u], e2 = otab[3u * rank + 2u];
+            Ray q;
+            q.o = v3(__uint_as_float(e1.x), __uint_as_float(e1.y), __uint_as_float(e1.z));
+            q.d = v3(__uint_as_float(e1.w), __uint_as_float(e2.x), __uint_as_float(e2.y));
+            b.t = __uint_as_float(e0.w);  // the owner's best t before the leaf
             const TriRec rec = recs[e0.z + (g - e0.y)];
             hit = test_any<MT>(q, rec, b);
+            // t >= 0 here (or -0): the bits without the sign order like the value
+            if (hit) atomicMin(keys + e0.x, ((unsigned long long)(__float_as_uint(b.t) & 0x7FFFFFFFu) << 32) | g);
         }
-        // t >= 0 here (or -0): the bits without the sign order like the value
-        if (hit)
-            atomicMin(keys + L, ((unsigned long long)(__float_as_uint(b.t) & 0x7FFFFFFFu) << 32) | (g - base));
+        if (__ballot(hit) == 0ull) continue;
+        // an owner whose minimum now lies in this pass takes the hit fields
+        // of the lane of the pass that tested it
         const unsigned long long k = own ? keys[lane_id] : ~0ull;
-        const uint32_t w = (uint32_t)k & 63u;  // the lane of this pass that tested the winner
+        const uint32_t w = ((uint32_t)k - base) & 63u;
         const float t = __shfl(b.t, w, 64), s1 = __shfl(b.s1, w, 64), s2 = __shfl(b.s2, w, 64),
                     det = __shfl(b.det, w, 64), nx = __shfl(b.nx, w, 64), ny = __shfl(b.ny, w, 64),
                     nz = __shfl(b.nz, w, 64);
         const uint32_t tri = (uint32_t)__shfl((int)b.tri, w, 64);
-        if (k != ~0ull) {
+        if (k != ~0ull && (uint32_t)k >= base) {
             best.t = t;
             best.s1 = s1;
             best.s2 = s2;
@@ -737,7 +753,6 @@ __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, con
             best.nx = nx;
             best.ny = ny;
             best.nz = nz;
-            otab[3u * my_rank].w = __float_as_uint(t);
         }
     }
 }
@@ -1121,6 +1136,8 @@ enum LaneField : uint32_t {
     kFW, kFAr, kFAg, kFAb, kFCr, kFCg, kFCb,              // weight, pixel sums, generation sums
     kFIntensity, kFTb,                                    // current light's term, primary bound
     kUO, kUMb, kUG, kUK, kUTri, kUFlags,                  // pixel, max bounce, generation, light, hit tri, flags
+                                                          // (primary hit record; then the rope slot of the
+                                                          // generation's hit leaf)
     kLaneFields
 };
 // per lane: the lane fields, one 64-bit key and the lane's share of the
@@ -1129,12 +1146,15 @@ constexpr uint32_t kRefillLdsWords = kLaneFields + 2u + kCoopWaveWords / 64u;
 static_assert(kRefillLdsWords * 64u * 4u % 16u == 0u && (kLaneFields + 2u) * 64u * 4u % 16u == 0u,
               "the owner table of each wave is 16-B aligned");
 
-// No wave cap: the refill loop's lane state measured fastest unspilled.
-// LDS: the top tree levels (LDS short stack), the lane fields, then the
-// short-stack columns.  COOP: every leaf step is a cooperative one
-// (coop_lanes 64; no per-lane leaf loop in the instance).
+// Waves per SIMD: the cooperative instances (no per-lane leaf loop) fit 128
+// VGPRs and run 4 waves per SIMD; the others measured fastest uncapped.
+// LDS: the top tree levels (LDS short stack), one region per wave (lane
+// fields, keys, owner table), then the short-stack columns.  COOP: every
+// leaf step is a cooperative one (coop_lanes 64; no per-lane leaf loop in
+// the instance).
 template <int TV, bool COUNT, int LB, bool COOP = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COOP ? 4 : 1))) k_render_refill(TraceParams P) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COOP ? 4 : 1)))
+k_render_refill(TraceParams P) {
     extern __shared__ uint4 smem[];
     const uint32_t tid = threadIdx.x, block = blockDim.x;
     const uint32_t lane_id = tid & 63u;
@@ -1323,6 +1343,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COOP ?
                 F(kFDy) = tr.d.y;
                 F(kFDz) = tr.d.z;
                 U(kUTri) = bt.tri;
+                if (TV == HULK_VARIANT_ROPES) U(kUFlags) = st.rope.slot;
                 F(kFCr) = F(kFCg) = F(kFCb) = 0.1f;
                 U(kUK) = 0;
             }
@@ -1346,6 +1367,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COOP ?
                     tstop = dist;
                     phase = kLaneShadow;
                     active = trav_begin<TV>(P, tr, kFltMax, st);
+                    if (TV == HULK_VARIANT_ROPES && !COUNT && active) ropes_hint(P, tr, st.rope, U(kUFlags));
                     done = !active;
                     traced = true;
                     break;
@@ -1383,6 +1405,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COOP ?
             tstop = -kFltMax;
             phase = kLaneMirror;
             active = trav_begin<TV>(P, tr, kFltMax, st);
+            if (TV == HULK_VARIANT_ROPES && !COUNT && active) ropes_hint(P, tr, st.rope, U(kUFlags));
             done = !active;
         }
     }
