@@ -671,16 +671,16 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 
 // Per-wave LDS of the cooperative leaf tests (kCoopWaveWords words): 64
 // owner entries of three 16-B words (lane, first packed index, first record,
-// best t before the leaf; ray origin, d.x; d.y, d.z), then one 64-bit word of
-// segment-start bits.
+// best t before the leaf; ray origin, d.x; d.y, d.z, 64-bit key), then one
+// 64-bit word of segment-start bits.
 constexpr uint32_t kCoopWaveWords = 13u * 64u;
 
 // The leaves of the lanes in `want` (a wave-uniform ballot; every lane of
 // the wave executes this call) tested by the whole wave, 64 records per
 // pass, the records of all owner lanes packed back to back in owner lane
 // order: lane k of a pass tests packed record k against its owner's ray and
-// that owner's best t before the leaf.  `keys` (one 64-bit LDS word per lane
-// of the wave) keeps each owner's minimum of (t bits, packed index) over all
+// that owner's best t before the leaf.  The owner's key (the last 8 B of
+// its table entry) keeps its minimum of (t bits, packed index) over all
 // passes: the smallest t, and of equal t the lowest leaf position, which is
 // the hit the owner's own loop keeps (strict '<' in leaf order: the first of
 // equal t wins, GPU_common.cl:62-68), with the same per-test arithmetic
@@ -695,8 +695,7 @@ constexpr uint32_t kCoopWaveWords = 13u * 64u;
 // run on dense lanes; with coop_lanes 64 every leaf step is tested so.
 template <bool MT>
 __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, const uint4& n, const Ray& r,
-                                            Best& best, uint64_t want, uint32_t lane_id,
-                                            unsigned long long* keys, uint4* otab) {
+                                            Best& best, uint64_t want, uint32_t lane_id, uint4* otab) {
     const bool own = (want >> lane_id) & 1u;
     const uint32_t cnt = own ? n.z : 0u;
     const uint32_t pre = wave_inclusive_sum(cnt);  // the owners' record counts, scanned in lane order
@@ -707,8 +706,7 @@ __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, con
         otab[3u * my_rank] = make_uint4(lane_id, first, n.y, __float_as_uint(best.t));
         otab[3u * my_rank + 1u] = make_uint4(__float_as_uint(r.o.x), __float_as_uint(r.o.y), __float_as_uint(r.o.z),
                                              __float_as_uint(r.d.x));
-        otab[3u * my_rank + 2u] = make_uint4(__float_as_uint(r.d.y), __float_as_uint(r.d.z), 0u, 0u);
-        keys[lane_id] = ~0ull;
+        otab[3u * my_rank + 2u] = make_uint4(__float_as_uint(r.d.y), __float_as_uint(r.d.z), ~0u, ~0u);
     }
     unsigned long long* starts_word = reinterpret_cast<unsigned long long*>(otab + 192);
     for (uint32_t base = 0; base < total; base += 64u) {
@@ -733,12 +731,15 @@ __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, con
             const TriRec rec = recs[e0.z + (g - e0.y)];
             hit = test_any<MT>(q, rec, b);
             // t >= 0 here (or -0): the bits without the sign order like the value
-            if (hit) atomicMin(keys + e0.x, ((unsigned long long)(__float_as_uint(b.t) & 0x7FFFFFFFu) << 32) | g);
+            if (hit)
+                atomicMin(reinterpret_cast<unsigned long long*>(otab + 3u * rank + 2u) + 1,
+                          ((unsigned long long)(__float_as_uint(b.t) & 0x7FFFFFFFu) << 32) | g);
         }
         if (__ballot(hit) == 0ull) continue;
         // an owner whose minimum now lies in this pass takes the hit fields
         // of the lane of the pass that tested it
-        const unsigned long long k = own ? keys[lane_id] : ~0ull;
+        const unsigned long long k =
+            own ? reinterpret_cast<const unsigned long long*>(otab + 3u * my_rank + 2u)[1] : ~0ull;
         const uint32_t w = ((uint32_t)k - base) & 63u;
         const float t = __shfl(b.t, w, 64), s1 = __shfl(b.s1, w, 64), s2 = __shfl(b.s2, w, 64),
                     det = __shfl(b.det, w, 64), nx = __shfl(b.nx, w, 64), ny = __shfl(b.ny, w, 64),
@@ -1140,16 +1141,16 @@ enum LaneField : uint32_t {
                                                           // generation's hit leaf)
     kLaneFields
 };
-// per lane: the lane fields, one 64-bit key and the lane's share of the
-// kCoopWaveWords of the cooperative leaf tests
-constexpr uint32_t kRefillLdsWords = kLaneFields + 2u + kCoopWaveWords / 64u;
-static_assert(kRefillLdsWords * 64u * 4u % 16u == 0u && (kLaneFields + 2u) * 64u * 4u % 16u == 0u,
+// per lane: the lane fields and the lane's share of the kCoopWaveWords of
+// the cooperative leaf tests
+constexpr uint32_t kRefillLdsWords = kLaneFields + kCoopWaveWords / 64u;
+static_assert(kRefillLdsWords * 64u * 4u % 16u == 0u && kLaneFields * 64u * 4u % 16u == 0u,
               "the owner table of each wave is 16-B aligned");
 
 // Waves per SIMD: the cooperative instances (no per-lane leaf loop) fit 128
 // VGPRs and run 4 waves per SIMD; the others measured fastest uncapped.
 // LDS: the top tree levels (LDS short stack), one region per wave (lane
-// fields, keys, owner table), then the short-stack columns.  COOP: every
+// fields, owner table), then the short-stack columns.  COOP: every
 // leaf step is a cooperative one (coop_lanes 64; no per-lane leaf loop in
 // the instance).
 template <int TV, bool COUNT, int LB, bool COOP = false>
@@ -1162,11 +1163,10 @@ k_render_refill(TraceParams P) {
     uint32_t* lane_mem = reinterpret_cast<uint32_t*>(smem + P.lds_node_count);
     // one region of kRefillLdsWords * 64 words per wave: lane field f of
     // lane l at word 64 f + l, so that every field is one base register and
-    // an immediate offset; then the owners' keys and the owner table
+    // an immediate offset; then the owner table
     uint32_t* wave_mem = lane_mem + (tid >> 6) * (kRefillLdsWords * 64u);
     uint32_t* fields = wave_mem + lane_id;
-    unsigned long long* keys = reinterpret_cast<unsigned long long*>(wave_mem + kLaneFields * 64u);
-    uint4* otab = reinterpret_cast<uint4*>(wave_mem + (kLaneFields + 2u) * 64u);
+    uint4* otab = reinterpret_cast<uint4*>(wave_mem + kLaneFields * 64u);
     uint32_t* stk = lane_mem + kRefillLdsWords * block + tid;
     for (uint32_t i = tid; i < P.lds_node_count; i += block) lds_nodes[i] = P.nodes[i];
     if (P.lds_node_count) __syncthreads();
@@ -1270,7 +1270,7 @@ k_render_refill(TraceParams P) {
             if (walk) ln = trav_descend<TV, COUNT>(P, lds_nodes, stk, block, tr, st, c);
             const uint64_t at_leaf = __ballot(walk && ln.z != 0u);
             if (at_leaf && (COOP || (uint32_t)__builtin_popcountll(at_leaf) <= P.coop_lanes))
-                coop_leaves<LB == kLeafMt>(P.recs, ln, tr, bt, at_leaf, lane_id, keys, otab);
+                coop_leaves<LB == kLeafMt>(P.recs, ln, tr, bt, at_leaf, lane_id, otab);
             else if (!COOP && walk && (ln.x & kNodeLeaf))
                 test_leaf<COUNT, LB>(P.recs, P.rec_bytes, ln, tr, bt, c, tstop);
             if (busy) done = !walk || !trav_exit<TV>(P, tr, st, ln, tstop, bt, tfl);
