@@ -437,16 +437,23 @@ struct RopeState {
     float t, te;
 };
 
-// Start of a rope traversal; false when the ray misses the scene box.
-__device__ __forceinline__ bool ropes_begin(const TraceParams& P, const Ray& r, float tclip, RopeState& s) {
-    float ts = 0.0f, te = kFltMax;
-    if (!ray_box(P.box, r, ts, te)) return false;
+// Start of a rope traversal from the ray's scene-box interval [ts, te]
+// (inbox false: the ray misses the box, and so does the traversal).
+__device__ __forceinline__ bool ropes_begin_box(bool inbox, float ts, float te, float tclip, RopeState& s) {
+    if (!inbox) return false;
     if (tclip < te) te = tclip;
     s.t = ts;
     s.te = te;
     s.slot = 0;
     s.guard = 0;
     return true;
+}
+
+// Start of a rope traversal; false when the ray misses the scene box.
+__device__ __forceinline__ bool ropes_begin(const TraceParams& P, const Ray& r, float tclip, RopeState& s) {
+    float ts = 0.0f, te = kFltMax;
+    const bool inbox = ray_box(P.box, r, ts, te);
+    return ropes_begin_box(inbox, ts, te, tclip, s);
 }
 
 // Descent of a rope traversal from the current slot to a leaf; returns the
@@ -547,15 +554,20 @@ struct SsState {
     uint32_t guard;
 };
 
-__device__ __forceinline__ bool ss_begin(const TraceParams& P, const Ray& r, float tclip, SsState& s) {
-    float ts = 0.0f, te = kFltMax;
-    if (!ray_box(P.box, r, ts, te)) return false;
+__device__ __forceinline__ bool ss_begin_box(bool inbox, float ts, float te, float tclip, SsState& s) {
+    if (!inbox) return false;
     if (tclip < te) te = tclip;
     s.tmin = s.tmax = ts;
     s.te = te;
     s.size = s.off = 0;
     s.guard = 0;
     return s.tmax < s.te;
+}
+
+__device__ __forceinline__ bool ss_begin(const TraceParams& P, const Ray& r, float tclip, SsState& s) {
+    float ts = 0.0f, te = kFltMax;
+    const bool inbox = ray_box(P.box, r, ts, te);
+    return ss_begin_box(inbox, ts, te, tclip, s);
 }
 
 template <bool COUNT>
@@ -623,6 +635,13 @@ template <int TV>
 __device__ __forceinline__ bool trav_begin(const TraceParams& P, const Ray& r, float tclip, TravState& s) {
     if (TV == HULK_VARIANT_LDSSTACK) return ss_begin(P, r, tclip, s.ss);
     return ropes_begin(P, r, tclip, s.rope);
+}
+
+// trav_begin from the ray's scene-box interval, computed by the caller.
+template <int TV>
+__device__ __forceinline__ bool trav_begin_box(bool inbox, float ts, float te, float tclip, TravState& s) {
+    if (TV == HULK_VARIANT_LDSSTACK) return ss_begin_box(inbox, ts, te, tclip, s.ss);
+    return ropes_begin_box(inbox, ts, te, tclip, s.rope);
 }
 
 // Rope traversal of a ray whose origin lies in the scene box (entry t 0)
@@ -1235,13 +1254,15 @@ k_render_refill(TraceParams P) {
                 F(kFAr) = F(kFAg) = F(kFAb) = 0.0f;
                 const float tb = P.tbound ? P.tbound[o] : kFltMax;
                 F(kFTb) = tb;
-                float bts = 0.0f, bte = 0.0f;
+                // one box test for the bounded pass's clip and the traversal
+                float bts = 0.0f, bte = kFltMax;
+                const bool inbox = ray_box(P.box, tr, bts, bte);
                 const float tclip = tb * (1.0f + 1.0f / 1024.0f);
-                clipped = tb < kFltMax && ray_box(P.box, tr, bts, bte) && tclip < bte;
+                clipped = tb < kFltMax && inbox && tclip < bte;
                 phase = kLanePrimary;
                 tstop = -kFltMax;
                 tfl = 0;
-                active = trav_begin<TV>(P, tr, clipped ? tclip : kFltMax, st);
+                active = trav_begin_box<TV>(inbox, bts, bte, clipped ? tclip : kFltMax, st);
             }
         }
         if (__ballot(phase != kLaneIdle) == 0) {
@@ -1279,134 +1300,137 @@ k_render_refill(TraceParams P) {
         // traversal may end at once when it misses the box: loop)
         while (done) {
             done = false;
-            bool shade = false;  // start shading the generation's hit in bt
-            if (phase == kLanePrimary && clipped && !(bt.tri != kNoHit && bt.t < F(kFTb))) {
-                best_init(bt);
-                U(kUFlags) |= tfl | HULK_HITF_BOUNDED_FALLBACK;
-                if (COUNT) cfb++;
-                phase = kLaneFallback;
-                tfl = 0;
-                active = trav_begin<TV>(P, tr, kFltMax, st);
-                done = !active;
-                continue;
-            }
-            if (phase == kLanePrimary || phase == kLaneFallback) {
-                const uint32_t flags = U(kUFlags) | tfl;
-                float u = 0.0f, v = 0.0f;
-                if (bt.tri != kNoHit) {
-                    u = bt.s1 / bt.det;
-                    v = bt.s2 / bt.det;
+            // a lane that starts a new traversal of tr (the unbounded re-run
+            // of a primary ray, a shadow ray or a mirror ray) begins it after
+            // this block, so that the wave runs one box test for all of them
+            bool begin = false;
+            do {
+                bool shade = false;  // start shading the generation's hit in bt
+                if (phase == kLanePrimary && clipped && !(bt.tri != kNoHit && bt.t < F(kFTb))) {
+                    best_init(bt);
+                    U(kUFlags) |= tfl | HULK_HITF_BOUNDED_FALLBACK;
+                    if (COUNT) cfb++;
+                    phase = kLaneFallback;
+                    tfl = 0;
+                    begin = true;
+                    break;
                 }
-                float4* h = reinterpret_cast<float4*>(P.hits + U(kUO));
-                h[0] = make_float4(bt.t, u, v, __uint_as_float(bt.tri));
-                h[1] = make_float4(bt.nx, bt.ny, bt.nz, __uint_as_float(flags));
-                if (COUNT) {
-                    crays++;
-                    chits += bt.tri != kNoHit ? 1 : 0;
+                if (phase == kLanePrimary || phase == kLaneFallback) {
+                    const uint32_t flags = U(kUFlags) | tfl;
+                    float u = 0.0f, v = 0.0f;
+                    if (bt.tri != kNoHit) {
+                        u = bt.s1 / bt.det;
+                        v = bt.s2 / bt.det;
+                    }
+                    float4* h = reinterpret_cast<float4*>(P.hits + U(kUO));
+                    h[0] = make_float4(bt.t, u, v, __uint_as_float(bt.tri));
+                    h[1] = make_float4(bt.nx, bt.ny, bt.nz, __uint_as_float(flags));
+                    if (COUNT) {
+                        crays++;
+                        chits += bt.tri != kNoHit ? 1 : 0;
+                    }
+                    shade = true;
+                } else if (phase == kLaneMirror) {
+                    if (COUNT) {
+                        crays++;
+                        chits += bt.tri != kNoHit ? 1 : 0;
+                    }
+                    shade = true;
+                } else {  // kLaneShadow: point light k of the generation
+                    if (COUNT) {
+                        cshadow++;
+                        chits += bt.tri != kNoHit ? 1 : 0;
+                    }
+                    float intensity = F(kFIntensity);
+                    if (tstop > bt.t) intensity *= 0.5f;  // tstop holds the light distance
+                    const float4 col = P.lights[2 * U(kUK)];
+                    const float m = intensity > 0.0f ? intensity : 0.0f;
+                    F(kFCr) += col.x * m;
+                    F(kFCg) += col.y * m;
+                    F(kFCb) += col.z * m;
+                    U(kUK) += 1;
                 }
-                shade = true;
-            } else if (phase == kLaneMirror) {
-                if (COUNT) {
-                    crays++;
-                    chits += bt.tri != kNoHit ? 1 : 0;
+                if (shade) {
+                    if (bt.tri == kNoHit) {  // the path ends: pixel
+                        P.rgba[U(kUO)] = to_u8(F(kFAr)) | (to_u8(F(kFAg)) << 8) | (to_u8(F(kFAb)) << 16);
+                        phase = kLaneIdle;
+                        active = false;
+                        break;
+                    }
+                    const V3 hp = add(tr.o, scale(tr.d, bt.t));
+                    F(kFPx) = hp.x;
+                    F(kFPy) = hp.y;
+                    F(kFPz) = hp.z;
+                    F(kFNx) = bt.nx;
+                    F(kFNy) = bt.ny;
+                    F(kFNz) = bt.nz;
+                    F(kFDx) = tr.d.x;
+                    F(kFDy) = tr.d.y;
+                    F(kFDz) = tr.d.z;
+                    U(kUTri) = bt.tri;
+                    if (TV == HULK_VARIANT_ROPES) U(kUFlags) = st.rope.slot;
+                    F(kFCr) = F(kFCg) = F(kFCb) = 0.1f;
+                    U(kUK) = 0;
                 }
-                shade = true;
-            } else {  // kLaneShadow: point light k of the generation
-                if (COUNT) {
-                    cshadow++;
-                    chits += bt.tri != kNoHit ? 1 : 0;
+                // lights from k on: directional ones inline, the next point light as a shadow ray
+                const V3 Pp = v3(F(kFPx), F(kFPy), F(kFPz)), nn = v3(F(kFNx), F(kFNy), F(kFNz));
+                for (uint32_t k = U(kUK); k < P.n_lights; k++) {
+                    const float4 col = P.lights[2 * k];
+                    const float4 dat = P.lights[2 * k + 1];
+                    V3 L = v3(0.0f, 0.0f, 0.0f);
+                    float dist = 0.0f;
+                    const float intensity = light_intensity(col, dat, Pp, nn, L, dist);
+                    if (col.w == 1.0f && intensity > 0.0f) {  // facing away: no shadow ray, term 0
+                        U(kUK) = k;
+                        F(kFIntensity) = intensity;
+                        tr.o = add(Pp, scale(L, 0.001f));
+                        tr.d = L;
+                        tstop = dist;
+                        phase = kLaneShadow;
+                        begin = true;
+                        break;
+                    }
+                    const float m = intensity > 0.0f ? intensity : 0.0f;
+                    F(kFCr) += col.x * m;
+                    F(kFCg) += col.y * m;
+                    F(kFCb) += col.z * m;
                 }
-                float intensity = F(kFIntensity);
-                if (tstop > bt.t) intensity *= 0.5f;  // tstop holds the light distance
-                const float4 col = P.lights[2 * U(kUK)];
-                const float m = intensity > 0.0f ? intensity : 0.0f;
-                F(kFCr) += col.x * m;
-                F(kFCg) += col.y * m;
-                F(kFCb) += col.z * m;
-                U(kUK) += 1;
-            }
-            if (shade) {
-                if (bt.tri == kNoHit) {  // the path ends: pixel
+                if (begin) break;
+                // generation complete: accumulate, then a mirror ray or the pixel
+                const uint32_t g = U(kUG);
+                const bool spawn = P.mirror && g < U(kUMb) && P.mirror[U(kUTri)];
+                const float w = F(kFW);
+                const float keep = spawn ? w * 0.2f : w;
+                F(kFAr) += F(kFCr) * keep;
+                F(kFAg) += F(kFCg) * keep;
+                F(kFAb) += F(kFCb) * keep;
+                if (!spawn) {
                     P.rgba[U(kUO)] = to_u8(F(kFAr)) | (to_u8(F(kFAg)) << 8) | (to_u8(F(kFAb)) << 16);
                     phase = kLaneIdle;
                     active = false;
                     break;
                 }
-                const V3 hp = add(tr.o, scale(tr.d, bt.t));
-                F(kFPx) = hp.x;
-                F(kFPy) = hp.y;
-                F(kFPz) = hp.z;
-                F(kFNx) = bt.nx;
-                F(kFNy) = bt.ny;
-                F(kFNz) = bt.nz;
-                F(kFDx) = tr.d.x;
-                F(kFDy) = tr.d.y;
-                F(kFDz) = tr.d.z;
-                U(kUTri) = bt.tri;
-                if (TV == HULK_VARIANT_ROPES) U(kUFlags) = st.rope.slot;
-                F(kFCr) = F(kFCg) = F(kFCb) = 0.1f;
-                U(kUK) = 0;
+                const V3 dn = normalize(v3(F(kFDx), F(kFDy), F(kFDz)));
+                const float kk = 2.0f * dot(dn, nn);
+                const V3 rr = sub(dn, scale(nn, kk));
+                tr.o = add(Pp, scale(rr, 0.001f));
+                tr.d = rr;
+                F(kFW) = w * 0.8f;
+                U(kUG) = g + 1;
+                tstop = -kFltMax;
+                phase = kLaneMirror;
+                begin = true;
+            } while (false);
+            if (begin) {
+                if (phase != kLaneFallback) set_inv(tr);
+                best_init(bt);
+                tfl = 0;
+                active = trav_begin<TV>(P, tr, kFltMax, st);
+                // shadow and mirror rays leave the generation's hit point
+                if (TV == HULK_VARIANT_ROPES && !COUNT && active && phase != kLaneFallback)
+                    ropes_hint(P, tr, st.rope, U(kUFlags));
+                done = !active;
             }
-            // lights from k on: directional ones inline, the next point light as a shadow ray
-            const V3 Pp = v3(F(kFPx), F(kFPy), F(kFPz)), nn = v3(F(kFNx), F(kFNy), F(kFNz));
-            bool traced = false;
-            for (uint32_t k = U(kUK); k < P.n_lights; k++) {
-                const float4 col = P.lights[2 * k];
-                const float4 dat = P.lights[2 * k + 1];
-                V3 L = v3(0.0f, 0.0f, 0.0f);
-                float dist = 0.0f;
-                const float intensity = light_intensity(col, dat, Pp, nn, L, dist);
-                if (col.w == 1.0f && intensity > 0.0f) {  // facing away: no shadow ray, term 0
-                    U(kUK) = k;
-                    F(kFIntensity) = intensity;
-                    tr.o = add(Pp, scale(L, 0.001f));
-                    tr.d = L;
-                    set_inv(tr);
-                    best_init(bt);
-                    tfl = 0;
-                    tstop = dist;
-                    phase = kLaneShadow;
-                    active = trav_begin<TV>(P, tr, kFltMax, st);
-                    if (TV == HULK_VARIANT_ROPES && !COUNT && active) ropes_hint(P, tr, st.rope, U(kUFlags));
-                    done = !active;
-                    traced = true;
-                    break;
-                }
-                const float m = intensity > 0.0f ? intensity : 0.0f;
-                F(kFCr) += col.x * m;
-                F(kFCg) += col.y * m;
-                F(kFCb) += col.z * m;
-            }
-            if (traced) continue;
-            // generation complete: accumulate, then a mirror ray or the pixel
-            const uint32_t g = U(kUG);
-            const bool spawn = P.mirror && g < U(kUMb) && P.mirror[U(kUTri)];
-            const float w = F(kFW);
-            const float keep = spawn ? w * 0.2f : w;
-            F(kFAr) += F(kFCr) * keep;
-            F(kFAg) += F(kFCg) * keep;
-            F(kFAb) += F(kFCb) * keep;
-            if (!spawn) {
-                P.rgba[U(kUO)] = to_u8(F(kFAr)) | (to_u8(F(kFAg)) << 8) | (to_u8(F(kFAb)) << 16);
-                phase = kLaneIdle;
-                active = false;
-                break;
-            }
-            const V3 dn = normalize(v3(F(kFDx), F(kFDy), F(kFDz)));
-            const float kk = 2.0f * dot(dn, nn);
-            const V3 rr = sub(dn, scale(nn, kk));
-            tr.o = add(Pp, scale(rr, 0.001f));
-            tr.d = rr;
-            set_inv(tr);
-            F(kFW) = w * 0.8f;
-            U(kUG) = g + 1;
-            best_init(bt);
-            tfl = 0;
-            tstop = -kFltMax;
-            phase = kLaneMirror;
-            active = trav_begin<TV>(P, tr, kFltMax, st);
-            if (TV == HULK_VARIANT_ROPES && !COUNT && active) ropes_hint(P, tr, st.rope, U(kUFlags));
-            done = !active;
         }
     }
     if (COUNT) {
