@@ -689,18 +689,19 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 }
 
 // Per-wave LDS of the cooperative leaf tests (kCoopWaveWords words): 64
-// owner entries of three 16-B words (lane, first packed index, first record,
-// best t before the leaf; ray origin, d.x; d.y, d.z, 64-bit key), then one
-// 64-bit word of segment-start bits.
-constexpr uint32_t kCoopWaveWords = 13u * 64u;
+// owner entries of two 16-B words (first record minus first packed index,
+// best t before the leaf, ray origin x, y; origin z, direction), the owners'
+// 64-bit keys, then one 64-bit word of segment-start bits.
+constexpr uint32_t kCoopWaveWords = 11u * 64u;
+constexpr uint32_t kCoopKeys = 128u;    // uint4 index of the keys (word 512)
+constexpr uint32_t kCoopStarts = 160u;  // uint4 index of the segment-start word (word 640)
 
 // The leaves of the lanes in `want` (a wave-uniform ballot; every lane of
 // the wave executes this call) tested by the whole wave, 64 records per
 // pass, the records of all owner lanes packed back to back in owner lane
 // order: lane k of a pass tests packed record k against its owner's ray and
-// that owner's best t before the leaf.  The owner's key (the last 8 B of
-// its table entry) keeps its minimum of (t bits, packed index) over all
-// passes: the smallest t, and of equal t the lowest leaf position, which is
+// that owner's best t before the leaf.  The owner's key keeps its minimum
+// of (t bits, packed index) over all passes: the smallest t, and of equal t the lowest leaf position, which is
 // the hit the owner's own loop keeps (strict '<' in leaf order: the first of
 // equal t wins, GPU_common.cl:62-68), with the same per-test arithmetic
 // (test_record).  After a pass in which some lane hit, each owner whose
@@ -721,13 +722,15 @@ __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, con
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)pre, 63);
     const uint32_t first = pre - cnt;  // this owner's first packed index
     const uint32_t my_rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0u));
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(otab + kCoopKeys);
     if (own) {
-        otab[3u * my_rank] = make_uint4(lane_id, first, n.y, __float_as_uint(best.t));
-        otab[3u * my_rank + 1u] = make_uint4(__float_as_uint(r.o.x), __float_as_uint(r.o.y), __float_as_uint(r.o.z),
-                                             __float_as_uint(r.d.x));
-        otab[3u * my_rank + 2u] = make_uint4(__float_as_uint(r.d.y), __float_as_uint(r.d.z), ~0u, ~0u);
+        otab[2u * my_rank] = make_uint4(n.y - first, __float_as_uint(best.t), __float_as_uint(r.o.x),
+                                        __float_as_uint(r.o.y));
+        otab[2u * my_rank + 1u] = make_uint4(__float_as_uint(r.o.z), __float_as_uint(r.d.x), __float_as_uint(r.d.y),
+                                             __float_as_uint(r.d.z));
+        keys[my_rank] = ~0ull;
     }
-    unsigned long long* starts_word = reinterpret_cast<unsigned long long*>(otab + 192);
+    unsigned long long* starts_word = reinterpret_cast<unsigned long long*>(otab + kCoopStarts);
     for (uint32_t base = 0; base < total; base += 64u) {
         const uint32_t g = base + lane_id;
         // owner of packed record g: the owners that began before the pass,
@@ -742,23 +745,20 @@ __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, con
         Best b;
         bool hit = false;
         if (g < total) {
-            const uint4 e0 = otab[3u * rank], e1 = otab[3u * rank + 1u], e2 = otab[3u * rank + 2u];
+            const uint4 e0 = otab[2u * rank], e1 = otab[2u * rank + 1u];
             Ray q;
-            q.o = v3(__uint_as_float(e1.x), __uint_as_float(e1.y), __uint_as_float(e1.z));
-            q.d = v3(__uint_as_float(e1.w), __uint_as_float(e2.x), __uint_as_float(e2.y));
-            b.t = __uint_as_float(e0.w);  // the owner's best t before the leaf
-            const TriRec rec = recs[e0.z + (g - e0.y)];
+            q.o = v3(__uint_as_float(e0.z), __uint_as_float(e0.w), __uint_as_float(e1.x));
+            q.d = v3(__uint_as_float(e1.y), __uint_as_float(e1.z), __uint_as_float(e1.w));
+            b.t = __uint_as_float(e0.y);  // the owner's best t before the leaf
+            const TriRec rec = recs[e0.x + g];
             hit = test_any<MT>(q, rec, b);
             // t >= 0 here (or -0): the bits without the sign order like the value
-            if (hit)
-                atomicMin(reinterpret_cast<unsigned long long*>(otab + 3u * rank + 2u) + 1,
-                          ((unsigned long long)(__float_as_uint(b.t) & 0x7FFFFFFFu) << 32) | g);
+            if (hit) atomicMin(keys + rank, ((unsigned long long)(__float_as_uint(b.t) & 0x7FFFFFFFu) << 32) | g);
         }
         if (__ballot(hit) == 0ull) continue;
         // an owner whose minimum now lies in this pass takes the hit fields
         // of the lane of the pass that tested it
-        const unsigned long long k =
-            own ? reinterpret_cast<const unsigned long long*>(otab + 3u * my_rank + 2u)[1] : ~0ull;
+        const unsigned long long k = own ? keys[my_rank] : ~0ull;
         const uint32_t w = ((uint32_t)k - base) & 63u;
         const float t = __shfl(b.t, w, 64), s1 = __shfl(b.s1, w, 64), s2 = __shfl(b.s2, w, 64),
                     det = __shfl(b.det, w, 64), nx = __shfl(b.nx, w, 64), ny = __shfl(b.ny, w, 64),
@@ -1154,8 +1154,9 @@ enum LanePhase : uint32_t { kLaneIdle = 0, kLanePrimary, kLaneFallback, kLaneSha
 enum LaneField : uint32_t {
     kFPx, kFPy, kFPz, kFNx, kFNy, kFNz, kFDx, kFDy, kFDz,  // generation hit point, normal, ray direction
     kFW, kFAr, kFAg, kFAb, kFCr, kFCg, kFCb,              // weight, pixel sums, generation sums
-    kFIntensity, kFTb,                                    // current light's term, primary bound
-    kUO, kUMb, kUG, kUK, kUTri, kUFlags,                  // pixel, max bounce, generation, light, hit tri, flags
+    kFIntensity,                                          // current light's term
+    kUO, kURem, kUK, kUFlags,                             // pixel, mirror bounces left (bit 31: the hit surface
+                                                          // is a mirror), light, flags
                                                           // (primary hit record; then the rope slot of the
                                                           // generation's hit leaf)
     kLaneFields
@@ -1166,8 +1167,10 @@ constexpr uint32_t kRefillLdsWords = kLaneFields + kCoopWaveWords / 64u;
 static_assert(kRefillLdsWords * 64u * 4u % 16u == 0u && kLaneFields * 64u * 4u % 16u == 0u,
               "the owner table of each wave is 16-B aligned");
 
-// Waves per SIMD: the cooperative instances (no per-lane leaf loop) fit 128
-// VGPRs and run 4 waves per SIMD; the others measured fastest uncapped.
+// Waves per SIMD: the cooperative instances (no per-lane leaf loop) run 4
+// waves per SIMD without spilling (5, at 96 VGPRs, spill and measured
+// slower); the others measured fastest uncapped.  A wave's LDS region is
+// 8 KiB (32 words per lane).
 // LDS: the top tree levels (LDS short stack), one region per wave (lane
 // fields, owner table), then the short-stack columns.  COOP: every
 // leaf step is a cooperative one (coop_lanes 64; no per-lane leaf loop in
@@ -1247,13 +1250,11 @@ k_render_refill(TraceParams P) {
                 set_inv(tr);
                 best_init(bt);
                 U(kUO) = o;
-                U(kUMb) = mb;
-                U(kUG) = 0;
+                U(kURem) = mb < 0x7FFFFFFFu ? mb : 0x7FFFFFFFu;
                 U(kUFlags) = 0;
                 F(kFW) = 1.0f;
                 F(kFAr) = F(kFAg) = F(kFAb) = 0.0f;
                 const float tb = P.tbound ? P.tbound[o] : kFltMax;
-                F(kFTb) = tb;
                 // one box test for the bounded pass's clip and the traversal
                 float bts = 0.0f, bte = kFltMax;
                 const bool inbox = ray_box(P.box, tr, bts, bte);
@@ -1306,7 +1307,8 @@ k_render_refill(TraceParams P) {
             bool begin = false;
             do {
                 bool shade = false;  // start shading the generation's hit in bt
-                if (phase == kLanePrimary && clipped && !(bt.tri != kNoHit && bt.t < F(kFTb))) {
+                // clipped: the lane's bound (P.tbound, below FLT_MAX) clipped its pass
+                if (phase == kLanePrimary && clipped && !(bt.tri != kNoHit && bt.t < P.tbound[U(kUO)])) {
                     best_init(bt);
                     U(kUFlags) |= tfl | HULK_HITF_BOUNDED_FALLBACK;
                     if (COUNT) cfb++;
@@ -1367,7 +1369,8 @@ k_render_refill(TraceParams P) {
                     F(kFDx) = tr.d.x;
                     F(kFDy) = tr.d.y;
                     F(kFDz) = tr.d.z;
-                    U(kUTri) = bt.tri;
+                    const uint32_t rem = U(kURem) & 0x7FFFFFFFu;
+                    U(kURem) = rem | ((P.mirror && rem && P.mirror[bt.tri]) ? 0x80000000u : 0u);
                     if (TV == HULK_VARIANT_ROPES) U(kUFlags) = st.rope.slot;
                     F(kFCr) = F(kFCg) = F(kFCb) = 0.1f;
                     U(kUK) = 0;
@@ -1397,8 +1400,8 @@ k_render_refill(TraceParams P) {
                 }
                 if (begin) break;
                 // generation complete: accumulate, then a mirror ray or the pixel
-                const uint32_t g = U(kUG);
-                const bool spawn = P.mirror && g < U(kUMb) && P.mirror[U(kUTri)];
+                const uint32_t rem = U(kURem);
+                const bool spawn = rem > 0x80000000u;  // a mirror, and bounces left
                 const float w = F(kFW);
                 const float keep = spawn ? w * 0.2f : w;
                 F(kFAr) += F(kFCr) * keep;
@@ -1416,7 +1419,7 @@ k_render_refill(TraceParams P) {
                 tr.o = add(Pp, scale(rr, 0.001f));
                 tr.d = rr;
                 F(kFW) = w * 0.8f;
-                U(kUG) = g + 1;
+                U(kURem) = (rem & 0x7FFFFFFFu) - 1u;
                 tstop = -kFltMax;
                 phase = kLaneMirror;
                 begin = true;
