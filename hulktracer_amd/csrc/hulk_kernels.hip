@@ -706,7 +706,8 @@ constexpr uint32_t kCoopStarts = 160u;  // uint4 index of the segment-start word
 // equal t wins, GPU_common.cl:62-68), with the same per-test arithmetic
 // (test_record).  After a pass in which some lane hit, each owner whose
 // minimum lies in that pass takes the hit fields of the lane that tested
-// it; a pass without hits ends at its tests.  A shadow ray (best.t <
+// it; a pass without hits ends at its tests.  The records of the next pass
+// load while a pass is tested (two record buffers in turn).  A shadow ray (best.t <
 // tstop ends it) may test more records after its first hit closer than the
 // light, where its own loop would stop: its hit may then be another triangle
 // closer than the light, which leaves the shading decision (light distance
@@ -731,31 +732,39 @@ __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, con
         keys[my_rank] = ~0ull;
     }
     unsigned long long* starts_word = reinterpret_cast<unsigned long long*>(otab + kCoopStarts);
-    for (uint32_t base = 0; base < total; base += 64u) {
-        const uint32_t g = base + lane_id;
-        // owner of packed record g: the owners that began before the pass,
-        // plus those whose first index lies in the pass at or below g
+    // owner rank of packed record base + lane: the owners that began before
+    // the pass, plus those whose first index lies in the pass at or below it
+    auto owner = [&](uint32_t base) -> uint32_t {
         const uint32_t before = (uint32_t)__builtin_popcountll(__ballot(own && first < base));
         if (lane_id == 0u) *starts_word = 0ull;
         if (own && first >= base && first < base + 64u) atomicOr(starts_word, 1ull << (first - base));
         const unsigned long long starts = *starts_word;
         const unsigned long long upto = lane_id == 63u ? ~0ull : (2ull << lane_id) - 1ull;
         const uint32_t cnt_before = before + (uint32_t)__builtin_popcountll(starts & upto);
-        const uint32_t rank = cnt_before ? cnt_before - 1u : 0u;
+        return cnt_before ? cnt_before - 1u : 0u;
+    };
+    // lanes past the last record load the last one (their owner is the last
+    // owner), so that every path consumes the loads
+    auto fetch = [&](uint32_t rank, uint32_t g, TriRec& rec) {
+        rec = recs[otab[2u * rank].x + (g < total ? g : total - 1u)];
+    };
+    // the tests of the pass at `base` on its records, loaded one pass ahead
+    auto pass = [&](uint32_t base, uint32_t rank, const TriRec& rec) {
+        const uint32_t g = base + lane_id;
         Best b;
-        bool hit = false;
-        if (g < total) {
-            const uint4 e0 = otab[2u * rank], e1 = otab[2u * rank + 1u];
-            Ray q;
-            q.o = v3(__uint_as_float(e0.z), __uint_as_float(e0.w), __uint_as_float(e1.x));
-            q.d = v3(__uint_as_float(e1.y), __uint_as_float(e1.z), __uint_as_float(e1.w));
-            b.t = __uint_as_float(e0.y);  // the owner's best t before the leaf
-            const TriRec rec = recs[e0.x + g];
-            hit = test_any<MT>(q, rec, b);
-            // t >= 0 here (or -0): the bits without the sign order like the value
-            if (hit) atomicMin(keys + rank, ((unsigned long long)(__float_as_uint(b.t) & 0x7FFFFFFFu) << 32) | g);
-        }
-        if (__ballot(hit) == 0ull) continue;
+        const uint4 e0 = otab[2u * rank], e1 = otab[2u * rank + 1u];
+        Ray q;
+        q.o = v3(__uint_as_float(e0.z), __uint_as_float(e0.w), __uint_as_float(e1.x));
+        q.d = v3(__uint_as_float(e1.y), __uint_as_float(e1.z), __uint_as_float(e1.w));
+        b.t = __uint_as_float(e0.y);  // the owner's best t before the leaf
+        const bool hit = test_any<MT>(q, rec, b) && g < total;
+        // every part of the record is consumed on every path (the test may
+        // stop before it reads the edges), so that a load in flight of the
+        // other buffer is never waited for here
+        asm volatile("" ::"v"(rec.a.x), "v"(rec.b.x), "v"(rec.c.x), "v"(rec.d.x));
+        // t >= 0 here (or -0): the bits without the sign order like the value
+        if (hit) atomicMin(keys + rank, ((unsigned long long)(__float_as_uint(b.t) & 0x7FFFFFFFu) << 32) | g);
+        if (__ballot(hit) == 0ull) return;
         // an owner whose minimum now lies in this pass takes the hit fields
         // of the lane of the pass that tested it
         const unsigned long long k = own ? keys[my_rank] : ~0ull;
@@ -774,6 +783,25 @@ __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, con
             best.ny = ny;
             best.nz = nz;
         }
+    };
+    // two record buffers in turn, so that no copy waits for a load in flight;
+    // the next pass's records are loaded unconditionally (past the last
+    // pass, the last record again), so that every path has the same loads
+    // in flight and the test of a pass waits only for its own
+    TriRec ra, rb;
+    uint32_t rank_a = owner(0u), rank_b = 0u;
+    fetch(rank_a, lane_id, ra);
+    for (uint32_t base = 0u;;) {
+        rank_b = owner(base + 64u);
+        fetch(rank_b, base + 64u + lane_id, rb);
+        pass(base, rank_a, ra);
+        base += 64u;
+        if (base >= total) break;
+        rank_a = owner(base + 64u);
+        fetch(rank_a, base + 64u + lane_id, ra);
+        pass(base, rank_b, rb);
+        base += 64u;
+        if (base >= total) break;
     }
 }
 
