@@ -337,6 +337,17 @@ def compare_methods(ctx, scene, osc_frames, cams, train_cams, W: int, H: int, st
     bopts = {k: int(v) for k, v in bbest.items() if k in OPT_KEYS}
     # kd-restart runs in the fused frame kernel (lane refill is a rope / LDS-stack pipeline)
     ropts = dict(bopts, variant=HT.RESTART, pipeline=2 if bopts.get("pipeline") == 3 else bopts.get("pipeline", 2))
+    if bopts.get("pipeline") == 3:
+        # the tuned refill shape leaves records in flight open (its cooperative
+        # instance ignores them): take kd-restart's faster of one and two
+        def batch_ms(opts):
+            ctx.set_options(**opts)
+            b = ctx.batch(frames, None, rgba_all.data_ptr(), use_graph=True)
+            ms, _ = timer(lambda: b.launch(L.FLAG_NO_SYNC), 3, 1)
+            b.close()
+            return ms
+
+        ropts = min((dict(ropts, leaf_batch=lb) for lb in (1, 2)), key=batch_ms)
     plain(topts, "plain", args.steps)
     batched(bopts, "batched", args.steps)
     learned_info = None
