@@ -201,14 +201,45 @@ __device__ __forceinline__ void test_vertices(const Ray& r, V3 p1, V3 p2, V3 p3,
 }
 
 // isRayToAbovePlane / isRayToBelowPlane / checkRayToBoundingBoxGPU
-// (SharedFunctions.hpp:154-264), one IEEE quotient per plane and no branch:
-// for d > 0 the reference tests ti > t1 (miss) and raises t0 to ti; for
-// d < 0 it tests ti < t0 (miss) and lowers t1 to ti; for d == 0 the plane
-// only decides by the origin's side.  A plane checks only the bound it does
-// not move, so the order of check and update within a plane is free, and
-// once a plane misses the box test's result no longer depends on t0 / t1.
-// Divergent rays of a wave then pay one division per plane instead of one
-// for each sign of d.
+// (SharedFunctions.hpp:154-264).
+__device__ __forceinline__ bool above_plane(float p0, float d0, float e0, float& t0, float& t1) {
+    if (d0 > 0.0f) {
+        float ti = (e0 - p0) / d0;
+        if (ti > t1) return false;
+        if (ti > t0) t0 = ti;
+        return true;
+    } else if (d0 < 0.0f) {
+        float ti = (e0 - p0) / d0;
+        if (ti < t0) return false;
+        if (ti < t1) t1 = ti;
+        return true;
+    }
+    return e0 < p0;
+}
+__device__ __forceinline__ bool below_plane(float p0, float d0, float e0, float& t0, float& t1) {
+    if (d0 < 0.0f) {
+        float ti = (e0 - p0) / d0;
+        if (ti > t1) return false;
+        if (ti > t0) t0 = ti;
+        return true;
+    } else if (d0 > 0.0f) {
+        float ti = (e0 - p0) / d0;
+        if (ti < t0) return false;
+        if (ti < t1) t1 = ti;
+        return true;
+    }
+    return e0 > p0;
+}
+
+// The same plane test with one IEEE quotient and no branch: for d > 0 the
+// reference tests ti > t1 (miss) and raises t0 to ti; for d < 0 it tests
+// ti < t0 (miss) and lowers t1 to ti; for d == 0 the plane only decides by
+// the origin's side.  A plane checks only the bound it does not move, so
+// the order of check and update within a plane is free, and once a plane
+// misses, the box test's result no longer depends on t0 / t1.  Divergent
+// rays of a wave then pay one division per plane instead of one for each
+// sign of d; the branches of the reference's form hold fewer registers,
+// which the frame kernels under a wave cap need.
 __device__ __forceinline__ void box_plane(float p0, float d0, float e0, bool above, bool& ok, float& t0,
                                           float& t1) {
     const float ti = (e0 - p0) / d0;
@@ -225,6 +256,22 @@ struct SceneBox {
 };
 
 __device__ __forceinline__ bool ray_box(const SceneBox& b, const Ray& r, float& ts, float& te) {
+    float t0 = 0.0f, t1 = kFltMax;
+    bool ok = above_plane(r.o.x, r.d.x, b.mn[0], t0, t1);
+    ok &= below_plane(r.o.x, r.d.x, b.mx[0], t0, t1);
+    ok &= above_plane(r.o.y, r.d.y, b.mn[1], t0, t1);
+    ok &= below_plane(r.o.y, r.d.y, b.mx[1], t0, t1);
+    ok &= above_plane(r.o.z, r.d.z, b.mn[2], t0, t1);
+    ok &= below_plane(r.o.z, r.d.z, b.mx[2], t0, t1);
+    if (ok) {
+        ts = t0;
+        te = t1;
+    }
+    return ok;
+}
+
+// ray_box with the branch-free plane test: the same result bit for bit.
+__device__ __forceinline__ bool ray_box_nb(const SceneBox& b, const Ray& r, float& ts, float& te) {
     float t0 = 0.0f, t1 = kFltMax;
     bool ok = true;
     box_plane(r.o.x, r.d.x, b.mn[0], true, ok, t0, t1);

@@ -631,17 +631,19 @@ struct TravState {
     SsState ss;
 };
 
-template <int TV>
-__device__ __forceinline__ bool trav_begin(const TraceParams& P, const Ray& r, float tclip, TravState& s) {
-    if (TV == HULK_VARIANT_LDSSTACK) return ss_begin(P, r, tclip, s.ss);
-    return ropes_begin(P, r, tclip, s.rope);
-}
-
 // trav_begin from the ray's scene-box interval, computed by the caller.
 template <int TV>
 __device__ __forceinline__ bool trav_begin_box(bool inbox, float ts, float te, float tclip, TravState& s) {
     if (TV == HULK_VARIANT_LDSSTACK) return ss_begin_box(inbox, ts, te, tclip, s.ss);
     return ropes_begin_box(inbox, ts, te, tclip, s.rope);
+}
+
+// Start of the lane-refill kernel's traversal (branch-free box test).
+template <int TV>
+__device__ __forceinline__ bool trav_begin(const TraceParams& P, const Ray& r, float tclip, TravState& s) {
+    float ts = 0.0f, te = kFltMax;
+    const bool inbox = ray_box_nb(P.box, r, ts, te);
+    return trav_begin_box<TV>(inbox, ts, te, tclip, s);
 }
 
 // Rope traversal of a ray whose origin lies in the scene box (entry t 0)
@@ -1285,7 +1287,7 @@ k_render_refill(TraceParams P) {
                 const float tb = P.tbound ? P.tbound[o] : kFltMax;
                 // one box test for the bounded pass's clip and the traversal
                 float bts = 0.0f, bte = kFltMax;
-                const bool inbox = ray_box(P.box, tr, bts, bte);
+                const bool inbox = ray_box_nb(P.box, tr, bts, bte);
                 const float tclip = tb * (1.0f + 1.0f / 1024.0f);
                 clipped = tb < kFltMax && inbox && tclip < bte;
                 phase = kLanePrimary;
