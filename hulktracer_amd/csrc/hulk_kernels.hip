@@ -747,8 +747,11 @@ __device__ __forceinline__ void coop_leaves(const TriRec* __restrict__ recs, con
     };
     // lanes past the last record load the last one (their owner is the last
     // owner), so that every path consumes the loads
+    // The entry's base (first record minus first packed index) may wrap
+    // below zero: the record index is its 32-bit sum with the packed index.
     auto fetch = [&](uint32_t rank, uint32_t g, TriRec& rec) {
-        rec = recs[otab[2u * rank].x + (g < total ? g : total - 1u)];
+        const uint32_t idx = otab[2u * rank].x + (g < total ? g : total - 1u);
+        rec = recs[idx];
     };
     // the tests of the pass at `base` on its records, loaded one pass ahead
     auto pass = [&](uint32_t base, uint32_t rank, const TriRec& rec) {
