@@ -476,11 +476,14 @@ __device__ __forceinline__ uint4 ropes_descend(const TraceParams& P, const Ray& 
 // The exit of the leaf at s.slot from its rope slot: the face the ray
 // leaves through (d[a] > 0: max face, d[a] < 0: min face, ties to the lowest
 // axis), its slab time and the rope across it.  Independent of the leaf's
-// hits, so the lane-refill loop computes it before the leaf is tested.
+// hits, so the lane-refill loop computes it before the leaf is tested.  It
+// keeps the leaf's cell, which the start-leaf check of the next shadow or
+// mirror ray reads when that ray leaves the same leaf.
 struct RopeExit {
     float tout;
-    uint32_t nxt;
-    int face;
+    uint32_t nxt;        // kRopeNone when the ray leaves through no face
+    uint32_t slot;       // the leaf slot of this exit, and its cell
+    float lo[3], hi[3];
 };
 
 __device__ __forceinline__ RopeExit rope_exit_of(const Ray& r, const RopeSlot* slots, uint32_t slot) {
@@ -493,7 +496,13 @@ __device__ __forceinline__ RopeExit rope_exit_of(const Ray& r, const RopeSlot* s
     const float iv[3] = {r.inv.x, r.inv.y, r.inv.z};
     RopeExit e;
     e.tout = kFltMax;
-    e.face = -1;
+    int face = -1;
+    e.slot = slot;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        e.lo[a] = mn[a];
+        e.hi[a] = mx[a];
+    }
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         float ta;
@@ -509,10 +518,10 @@ __device__ __forceinline__ RopeExit rope_exit_of(const Ray& r, const RopeSlot* s
         }
         if (ta < e.tout) {
             e.tout = ta;
-            e.face = f;
+            face = f;
         }
     }
-    e.nxt = e.face < 0 ? kRopeNone : slots[slot].rope[e.face];
+    e.nxt = face < 0 ? kRopeNone : slots[slot].rope[face];
     return e;
 }
 
@@ -521,8 +530,7 @@ __device__ __forceinline__ RopeExit rope_exit_of(const Ray& r, const RopeSlot* s
 __device__ __forceinline__ bool ropes_exit(const TraceParams& P, RopeState& s, const RopeExit& e, float tstop,
                                            const Best& best, uint32_t& flags) {
     if (best.t < tstop) return false;
-    if (best.t <= e.tout || e.tout >= s.te || e.face < 0) return false;
-    if (e.nxt == kRopeNone) return false;
+    if (best.t <= e.tout || e.tout >= s.te || e.nxt == kRopeNone) return false;
     if (e.tout > s.t) s.t = e.tout;
     s.slot = e.nxt;
     if (++s.guard > P.guard) {
@@ -650,13 +658,29 @@ __device__ __forceinline__ bool trav_begin(const TraceParams& P, const Ray& r, f
 // started at leaf slot `hint` instead of the root when the origin lies
 // strictly inside the hint's cell: with t 0 every descent decision takes the
 // origin's side of the plane (o != plane), so the root descent ends there.
-__device__ __forceinline__ void ropes_hint(const TraceParams& P, const Ray& r, RopeState& s, uint32_t hint) {
+// The cell comes from the last rope exit when that was the hint's leaf.
+__device__ __forceinline__ void ropes_hint(const TraceParams& P, const Ray& r, RopeState& s, uint32_t hint,
+                                           const RopeExit& ex) {
     if (s.t != 0.0f || hint == 0u) return;
-    const uint4* p = reinterpret_cast<const uint4*>(P.rope_slots + hint);
-    const uint4 lo = p[0], hi = p[1];
-    const bool in = __uint_as_float(lo.x) < r.o.x && r.o.x < __uint_as_float(hi.x) &&
-                    __uint_as_float(lo.y) < r.o.y && r.o.y < __uint_as_float(hi.y) &&
-                    __uint_as_float(lo.z) < r.o.z && r.o.z < __uint_as_float(hi.z);
+    float lo[3], hi[3];
+    if (hint == ex.slot) {  // the cell the last rope exit read
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo[a] = ex.lo[a];
+            hi[a] = ex.hi[a];
+        }
+    } else {
+        const uint4* p = reinterpret_cast<const uint4*>(P.rope_slots + hint);
+        const uint4 l4 = p[0], h4 = p[1];
+        lo[0] = __uint_as_float(l4.x);
+        lo[1] = __uint_as_float(l4.y);
+        lo[2] = __uint_as_float(l4.z);
+        hi[0] = __uint_as_float(h4.x);
+        hi[1] = __uint_as_float(h4.y);
+        hi[2] = __uint_as_float(h4.z);
+    }
+    const bool in = lo[0] < r.o.x && r.o.x < hi[0] && lo[1] < r.o.y && r.o.y < hi[1] && lo[2] < r.o.z &&
+                    r.o.z < hi[2];
     if (in) s.slot = hint;
 }
 
@@ -1464,7 +1488,7 @@ k_render_refill(TraceParams P) {
                 active = trav_begin<TV>(P, tr, kFltMax, st);
                 // shadow and mirror rays leave the generation's hit point
                 if (TV == HULK_VARIANT_ROPES && !COUNT && active && phase != kLaneFallback)
-                    ropes_hint(P, tr, st.rope, U(kUFlags));
+                    ropes_hint(P, tr, st.rope, U(kUFlags), st.ex);
                 done = !active;
             }
         }
