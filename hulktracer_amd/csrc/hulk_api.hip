@@ -1590,6 +1590,10 @@ extern "C" int hulk_batch_create(hulk_ctx* c, const hulk_frame_desc* frames, uin
         fd.H = d.height;
         fd.offset = static_cast<uint32_t>(off);
         fd.max_bounce = d.max_bounce;
+        fd.w_m = udiv_multiplier(d.width);
+        fd.tx_m = udiv_multiplier(d.width >> 3);
+        fd.fW = static_cast<float>(d.width);
+        fd.fH = static_cast<float>(d.height);
         b->frames.push_back(fd);
         b->max_bounce = std::max(b->max_bounce, d.max_bounce);
         off += (uint64_t)d.width * d.height;
@@ -1753,6 +1757,10 @@ extern "C" int hulk_batch_create_scenes(hulk_ctx* const* ctxs, uint32_t n_ctxs, 
         fd.offset = static_cast<uint32_t>(off);
         fd.max_bounce = mirror_masks && mirror_masks[d.reserved] ? d.max_bounce : 0;
         fd.scene = d.reserved;
+        fd.w_m = udiv_multiplier(d.width);
+        fd.tx_m = udiv_multiplier(d.width >> 3);
+        fd.fW = static_cast<float>(d.width);
+        fd.fH = static_cast<float>(d.height);
         b->frames.push_back(fd);
         b->max_bounce = std::max(b->max_bounce, fd.max_bounce);
         off += px;

@@ -120,16 +120,30 @@ __device__ __forceinline__ void tile_pixel_m(uint32_t i, uint32_t W, uint32_t H,
     }
 }
 
-// Batched primary ray i: pixel index (frame offset + y*W + x) returned,
-// 8x8 tiles inside a frame when its sides are multiples of 8.
+// Ray i of frame fr: pixel index (frame offset + y*W + x) returned, 8x8
+// tiles inside the frame when its sides are multiples of 8, the divisions
+// by W and W / 8 done by the frame's multipliers.
+__device__ __forceinline__ uint32_t frame_pixel_ray(const FrameDev& fr, uint32_t i, Ray& r) {
+    uint32_t x, y;
+    tile_pixel_m(i - fr.offset, fr.W, fr.H, fr.w_m, fr.tx_m, x, y);
+    cast_ray_f(fr.cam, x, y, fr.fW, fr.fH, r);
+    return fr.offset + y * fr.W + x;
+}
+
+// Batched primary ray i (f: its frame).  The active lanes take consecutive
+// rays, nearly always of one frame: the frame of the first active lane's ray
+// is searched once for the wave (wave-uniform, so scalar loads), and the
+// lanes search their own frames only when some ray lies outside it.
 __device__ __forceinline__ uint32_t frame_ray_tiled(const FrameDev* F, uint32_t n, uint32_t i, Ray& r,
                                                     uint32_t& f) {
+    const uint32_t f0 = find_frame(F, n, __builtin_amdgcn_readfirstlane(i));
+    const uint32_t lo = F[f0].offset, hi = f0 + 1u < n ? F[f0 + 1u].offset : 0xFFFFFFFFu;
+    if (__ballot(i < lo || i >= hi) == 0ull) {
+        f = f0;
+        return frame_pixel_ray(F[f0], i, r);
+    }
     f = find_frame(F, n, i);
-    const FrameDev& fr = F[f];
-    uint32_t x, y;
-    tile_pixel(i - fr.offset, fr.W, fr.H, x, y);
-    cast_ray(fr.cam, x, y, fr.W, fr.H, r);
-    return fr.offset + y * fr.W + x;
+    return frame_pixel_ray(F[f], i, r);
 }
 
 __device__ __forceinline__ uint32_t frame_ray(const FrameDev* F, uint32_t n, uint32_t i, Ray& r) {

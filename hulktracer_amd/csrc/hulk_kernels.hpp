@@ -36,6 +36,8 @@ struct FrameDev {
     Camera cam;
     uint32_t W, H, offset, max_bounce;
     uint32_t scene;
+    uint32_t w_m, tx_m;  // udiv_multiplier(W), udiv_multiplier(W / 8)
+    float fW, fH;        // W and H as floats
 };
 
 // Everything a traversal and its shading read from one scene, for batches
