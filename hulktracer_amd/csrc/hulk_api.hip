@@ -421,7 +421,7 @@ int render_refill(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, 
     size_t lds;
     int r = trace_setup(c, o, P, block, lds);
     if (r) return r;
-    if (refill_lds_bytes(lds, block) > c->max_lds)
+    if (refill_lds_bytes(lds, block, static_cast<uint32_t>(c->lights.size())) > c->max_lds)
         return fail(c, HULK_E_ILLEGAL_INPUT, "refill LDS (lane fields + short stack + top levels) exceeds the block limit");
     if (fs.n == 0) return HULK_OK;
     if (fs.frames) {
@@ -1940,7 +1940,9 @@ extern "C" int hulk_autotune(hulk_ctx* c, const hulk_tune_request* req, hulk_tun
         uint32_t block;
         size_t lds;
         if (check_opts(c, o) || trace_setup(c, o, P, block, lds)) return 1;  // shape not allowed: skip
-        if (o.pipeline == HULK_PIPELINE_REFILL && refill_lds_bytes(lds, block) > c->max_lds) return 1;
+        if (o.pipeline == HULK_PIPELINE_REFILL &&
+            refill_lds_bytes(lds, block, static_cast<uint32_t>(c->lights.size())) > c->max_lds)
+            return 1;
         auto once = [&]() -> int {
             if (!frame) return trace_primary(c, o, fs, c->work, nullptr, out, false, false);
             if (use_fused(o, bounces))

@@ -1216,6 +1216,8 @@ k_trace(TraceParams P) {
 // rays from 8 per-XCD slices of the ray range, a count of idle lanes per
 // atomic, refilling once at least kRefillMin lanes are idle.
 constexpr uint32_t kRefillMin = 16;
+// Lights staged in the refill kernel's LDS (2 float4 each) when they fit.
+constexpr uint32_t kRefillLdsLights = 64;
 
 enum LanePhase : uint32_t { kLaneIdle = 0, kLanePrimary, kLaneFallback, kLaneShadow, kLaneMirror };
 
@@ -1242,8 +1244,9 @@ static_assert(kRefillLdsWords * 64u * 4u % 16u == 0u && kLaneFields * 64u * 4u %
 // waves per SIMD without spilling (5, at 96 VGPRs, spill and measured
 // slower); the others measured fastest uncapped.  A wave's LDS region is
 // 8 KiB (32 words per lane).
-// LDS: the top tree levels (LDS short stack), one region per wave (lane
-// fields, owner table), then the short-stack columns.  COOP: every
+// LDS: the lights (up to kRefillLdsLights; more are read from memory), the
+// top tree levels (LDS short stack), one region per wave (lane fields, owner
+// table), then the short-stack columns.  COOP: every
 // leaf step is a cooperative one (coop_lanes 64; no per-lane leaf loop in
 // the instance).
 template <int TV, bool COUNT, int LB, bool COOP = false>
@@ -1252,8 +1255,16 @@ k_render_refill(TraceParams P) {
     extern __shared__ uint4 smem[];
     const uint32_t tid = threadIdx.x, block = blockDim.x;
     const uint32_t lane_id = tid & 63u;
-    uint4* lds_nodes = smem;
-    uint32_t* lane_mem = reinterpret_cast<uint32_t*>(smem + P.lds_node_count);
+    float4* lds_lights = reinterpret_cast<float4*>(smem);
+    uint4* lds_nodes = smem + (P.n_lights <= kRefillLdsLights ? 2u * P.n_lights : 0u);
+    uint32_t* lane_mem = reinterpret_cast<uint32_t*>(lds_nodes + P.lds_node_count);
+    const bool ll_lds = P.n_lights <= kRefillLdsLights;
+    if (ll_lds) for (uint32_t i = tid; i < 2u * P.n_lights; i += block) lds_lights[i] = P.lights[i];
+    if (P.n_lights) __syncthreads();
+    auto LL = [&](uint32_t i) -> float4 {
+        if (ll_lds) return lds_lights[i];
+        return P.lights[i];
+    };
     // one region of kRefillLdsWords * 64 words per wave: lane field f of
     // lane l at word 64 f + l, so that every field is one base register and
     // an immediate offset; then the owner table
@@ -1416,7 +1427,7 @@ k_render_refill(TraceParams P) {
                     }
                     float intensity = F(kFIntensity);
                     if (tstop > bt.t) intensity *= 0.5f;  // tstop holds the light distance
-                    const float4 col = P.lights[2 * U(kUK)];
+                    const float4 col = LL(2 * U(kUK));
                     const float m = intensity > 0.0f ? intensity : 0.0f;
                     F(kFCr) += col.x * m;
                     F(kFCg) += col.y * m;
@@ -1449,8 +1460,8 @@ k_render_refill(TraceParams P) {
                 // lights from k on: directional ones inline, the next point light as a shadow ray
                 const V3 Pp = v3(F(kFPx), F(kFPy), F(kFPz)), nn = v3(F(kFNx), F(kFNy), F(kFNz));
                 for (uint32_t k = U(kUK); k < P.n_lights; k++) {
-                    const float4 col = P.lights[2 * k];
-                    const float4 dat = P.lights[2 * k + 1];
+                    const float4 col = LL(2 * k);
+                    const float4 dat = LL(2 * k + 1);
                     V3 L = v3(0.0f, 0.0f, 0.0f);
                     float dist = 0.0f;
                     const float intensity = light_intensity(col, dat, Pp, nn, L, dist);
@@ -1545,11 +1556,13 @@ static void launch_refill_tv(const TraceParams& P, bool count, uint32_t grid, ui
     }
 }
 
-size_t refill_lds_bytes(size_t trace_lds, uint32_t block) { return trace_lds + (size_t)kRefillLdsWords * block * 4; }
+size_t refill_lds_bytes(size_t trace_lds, uint32_t block, uint32_t n_lights) {
+    return trace_lds + (size_t)kRefillLdsWords * block * 4 + (n_lights <= kRefillLdsLights ? 32u * n_lights : 0u);
+}
 
 hipError_t launch_render_refill(int variant, const TraceParams& P, bool count, uint32_t grid, uint32_t block,
                                 size_t trace_lds, hipStream_t s) {
-    const size_t lds = refill_lds_bytes(trace_lds, block);
+    const size_t lds = refill_lds_bytes(trace_lds, block, P.n_lights);
     if (variant == HULK_VARIANT_LDSSTACK) launch_refill_tv<HULK_VARIANT_LDSSTACK>(P, count, grid, block, lds, s);
     else launch_refill_tv<HULK_VARIANT_ROPES>(P, count, grid, block, lds, s);
     return hipGetLastError();

@@ -208,7 +208,7 @@ hipError_t launch_selftest_rcp(unsigned long long* out, hipStream_t s);  // out:
 // whole frames, rope traversal, lanes refilled from the ray queue (work_counter: 8 slices)
 // variant: HULK_VARIANT_ROPES or HULK_VARIANT_LDSSTACK; trace_lds: the LDS of
 // the traversal (top tree levels, short-stack columns) before the lane fields
-size_t refill_lds_bytes(size_t trace_lds, uint32_t block);
+size_t refill_lds_bytes(size_t trace_lds, uint32_t block, uint32_t n_lights);
 hipError_t launch_render_refill(int variant, const TraceParams& P, bool count, uint32_t grid, uint32_t block,
                                 size_t trace_lds, hipStream_t s);
 // rope slots (3 n slots: nodes, then two empty-leaf slots per junction)
