@@ -500,14 +500,81 @@ struct RopeExit {
     float lo[3], hi[3];
 };
 
-__device__ __forceinline__ RopeExit rope_exit_of(const Ray& r, const RopeSlot* slots, uint32_t slot) {
-    const uint4* p = reinterpret_cast<const uint4*>(slots + slot);
-    const uint4 lo4 = p[0], hi4 = p[1];
-    const float mn[3] = {__uint_as_float(lo4.x), __uint_as_float(lo4.y), __uint_as_float(lo4.z)};
-    const float mx[3] = {__uint_as_float(hi4.x), __uint_as_float(hi4.y), __uint_as_float(hi4.z)};
+// v_i of (v_0, v_1, v_2), from values (no address select).
+template <typename T>
+__device__ __forceinline__ T sel3(int i, T v0, T v1, T v2) {
+    return i == 0 ? v0 : (i == 1 ? v1 : v2);
+}
+
+// Cell of rope slot s (words 0-5 of its RopeSlot).
+__device__ __forceinline__ void slot_cell(const RopeSlot* slots, uint32_t s, float lo[3], float hi[3]) {
+    const uint4* p = reinterpret_cast<const uint4*>(slots + s);
+    const uint4 a = p[0];
+    const uint2 b = reinterpret_cast<const uint2*>(p + 1)[0];
+    lo[0] = __uint_as_float(a.x);
+    lo[1] = __uint_as_float(a.y);
+    lo[2] = __uint_as_float(a.z);
+    hi[0] = __uint_as_float(a.w);
+    hi[1] = __uint_as_float(b.x);
+    hi[2] = __uint_as_float(b.y);
+}
+
+// Bit 31 of RopeState::guard: the ray may take pushed ropes.  |o| and |inv|
+// within [2^-64, FLT_MAX] and [2^-32, FLT_MAX] (every comparison with NaN is
+// false): then split(v) = (v - o) * inv is finite and nonzero for every
+// plane v != o, which the proof of the pushed ropes (rope_exit_of) needs.
+constexpr uint32_t kRopePushOk = 0x80000000u;
+__device__ __forceinline__ uint32_t rope_push_ok(const Ray& r) {
+    const float o[3] = {r.o.x, r.o.y, r.o.z}, iv[3] = {r.inv.x, r.inv.y, r.inv.z};
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float ao = __builtin_fabsf(o[a]), ai = __builtin_fabsf(iv[a]);
+        ok &= (ao >= 0x1p-64f) & (ao <= kFltMax) & (ai >= 0x1p-32f) & (ai <= kFltMax);
+    }
+    return ok ? kRopePushOk : 0u;
+}
+
+// The exit of the leaf at `slot`.  PUSHED (the lane-refill kernel): the
+// rope taken is the face's pushed rope when the descent from the plain rope
+// at the next time t = max(t_prev, t_out) provably passes through it, else
+// the plain rope; both descents end in the same leaf.  The push leaves the
+// plain rope target N through junctions split across the face (the child
+// that holds the face's extent) and junctions split on the face's axis
+// (the child toward the face).  Across the face, on an axis b whose bound
+// the ray leaves through after the face (d_b > 0: the max bound), the planes
+// pushed through lie beyond that bound, so their split exceeds the leaf's
+// slab time on b, and the descent keeps to the face's side when that time
+// exceeds t: the exit is strict, t2 > t with t2 the second of the three
+// slab times.  On the other side (the bound the ray entered through) the
+// leaf's own descent, and every earlier one of the traversal, took that
+// side at an earlier t, and split is monotone in the plane, so the planes
+// beyond it decide the same way (the rope builder never pushes through a
+// plane on the scene box, which no descent decided); kRopePushOk keeps
+// these splits nonzero.  On the face's axis the planes pushed through lie
+// beyond the face, and the nearest (RopeFace::far) decides for all: its
+// split must exceed t.  Sufficient, not necessary: ties take the plain rope.
+template <bool PUSHED = false>
+__device__ __forceinline__ RopeExit rope_exit_of(const Ray& r, const RopeSlot* slots, uint32_t slot,
+                                                 float t_prev = 0.0f, uint32_t push_ok = 0u) {
+    float mn[3], mx[3];
+    slot_cell(slots, slot, mn, mx);
     const float o[3] = {r.o.x, r.o.y, r.o.z};
     const float d[3] = {r.d.x, r.d.y, r.d.z};
     const float iv[3] = {r.inv.x, r.inv.y, r.inv.z};
+    // PUSHED: the face record of each axis the ray may leave through (the
+    // direction's sign picks it), read with the cell from the same line
+    uint32_t frope[3] = {0u, 0u, 0u}, fpush[3] = {0u, 0u, 0u};
+    float ffar[3] = {0.0f, 0.0f, 0.0f};
+    if (PUSHED) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const uint4 q = reinterpret_cast<const uint4*>(slots[slot].face)[2 * a + (d[a] > 0.0f ? 1 : 0)];
+            frope[a] = q.x;
+            fpush[a] = q.y;
+            ffar[a] = __uint_as_float(q.z);
+        }
+    }
     RopeExit e;
     e.tout = kFltMax;
     int face = -1;
@@ -517,10 +584,12 @@ __device__ __forceinline__ RopeExit rope_exit_of(const Ray& r, const RopeSlot* s
         e.lo[a] = mn[a];
         e.hi[a] = mx[a];
     }
+    float ts[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         float ta;
         int f;
+        ts[a] = __builtin_inff();
         if (d[a] > 0.0f) {
             ta = (mx[a] - o[a]) * iv[a];
             f = 2 * a + 1;
@@ -530,12 +599,23 @@ __device__ __forceinline__ RopeExit rope_exit_of(const Ray& r, const RopeSlot* s
         } else {
             continue;
         }
+        ts[a] = ta;
         if (ta < e.tout) {
             e.tout = ta;
             face = f;
         }
     }
-    e.nxt = face < 0 ? kRopeNone : slots[slot].rope[face];
+    if (!PUSHED) {
+        e.nxt = face < 0 ? kRopeNone : slots[slot].face[face].rope;
+        return e;
+    }
+    const int ax = face >> 1;  // face < 0: unused
+    const float t = e.tout > t_prev ? e.tout : t_prev;
+    const float t2 = __builtin_amdgcn_fmed3f(ts[0], ts[1], ts[2]);
+    const float sfar = (sel3(ax, ffar[0], ffar[1], ffar[2]) - sel3(ax, o[0], o[1], o[2])) * sel3(ax, iv[0], iv[1], iv[2]);
+    const bool take = (push_ok & kRopePushOk) && t2 > t && sfar > t;
+    e.nxt = face < 0 ? kRopeNone
+                     : (take ? sel3(ax, fpush[0], fpush[1], fpush[2]) : sel3(ax, frope[0], frope[1], frope[2]));
     return e;
 }
 
@@ -547,7 +627,7 @@ __device__ __forceinline__ bool ropes_exit(const TraceParams& P, RopeState& s, c
     if (best.t <= e.tout || e.tout >= s.te || e.nxt == kRopeNone) return false;
     if (e.tout > s.t) s.t = e.tout;
     s.slot = e.nxt;
-    if (++s.guard > P.guard) {
+    if ((++s.guard & ~kRopePushOk) > P.guard) {
         flags |= kHitFlagGuard;
         return false;
     }
@@ -684,14 +764,7 @@ __device__ __forceinline__ void ropes_hint(const TraceParams& P, const Ray& r, R
             hi[a] = ex.hi[a];
         }
     } else {
-        const uint4* p = reinterpret_cast<const uint4*>(P.rope_slots + hint);
-        const uint4 l4 = p[0], h4 = p[1];
-        lo[0] = __uint_as_float(l4.x);
-        lo[1] = __uint_as_float(l4.y);
-        lo[2] = __uint_as_float(l4.z);
-        hi[0] = __uint_as_float(h4.x);
-        hi[1] = __uint_as_float(h4.y);
-        hi[2] = __uint_as_float(h4.z);
+        slot_cell(P.rope_slots, hint, lo, hi);
     }
     const bool in = lo[0] < r.o.x && r.o.x < hi[0] && lo[1] < r.o.y && r.o.y < hi[1] && lo[2] < r.o.z &&
                     r.o.z < hi[2];
@@ -703,7 +776,7 @@ __device__ __forceinline__ uint4 trav_descend(const TraceParams& P, const uint4*
                                               uint32_t stride, const Ray& r, TravState& s, Cnt& c) {
     if (TV == HULK_VARIANT_LDSSTACK) return ss_descend<COUNT>(P, lds_nodes, stk, stride, r, s.ss, c);
     const uint4 n = ropes_descend<COUNT>(P, r, s.rope, c);
-    s.ex = rope_exit_of(r, P.rope_slots, s.rope.slot);
+    s.ex = rope_exit_of<!COUNT>(r, P.rope_slots, s.rope.slot, s.rope.t, s.rope.guard);
     return n;
 }
 
@@ -1346,6 +1419,7 @@ k_render_refill(TraceParams P) {
                 tstop = -kFltMax;
                 tfl = 0;
                 active = trav_begin_box<TV>(inbox, bts, bte, clipped ? tclip : kFltMax, st);
+                if (TV == HULK_VARIANT_ROPES && !COUNT) st.rope.guard = rope_push_ok(tr);
             }
         }
         if (__ballot(phase != kLaneIdle) == 0) {
@@ -1511,6 +1585,7 @@ k_render_refill(TraceParams P) {
                 best_init(bt);
                 tfl = 0;
                 active = trav_begin<TV>(P, tr, kFltMax, st);
+                if (TV == HULK_VARIANT_ROPES && !COUNT) st.rope.guard = rope_push_ok(tr);
                 // shadow and mirror rays leave the generation's hit point
                 if (TV == HULK_VARIANT_ROPES && !COUNT && active && phase != kLaneFallback)
                     ropes_hint(P, tr, st.rope, U(kUFlags), st.ex);
@@ -1974,11 +2049,58 @@ __global__ void k_rope_slots(const uint4* __restrict__ rn, const uint32_t* __res
         link = up[p];
     }
     RopeSlot r;
-    r.lo = make_float4(lo[0], lo[1], lo[2], 0.0f);
-    r.hi = make_float4(hi[0], hi[1], hi[2], 0.0f);
-    for (int k = 0; k < 6; k++) r.rope[k] = rope[k];
+    for (int a = 0; a < 3; a++) {
+        r.lo[a] = lo[a];
+        r.hi[a] = hi[a];
+    }
     r.pad[0] = r.pad[1] = 0;
+    for (int k = 0; k < 6; k++) {  // pushes: k_rope_push
+        r.face[k].rope = r.face[k].push = rope[k];
+        r.face[k].far = (k & 1) ? __builtin_inff() : -__builtin_inff();
+        r.face[k].pad = 0;
+    }
     rs[s] = r;
+}
+
+// Pushed ropes of the leaf slots: from the rope of face f (2a: the min face,
+// 2a + 1: the max face on axis a), down through every junction split across
+// a whose plane leaves the leaf's extent on one side (the child holding the
+// whole face; never a plane on the scene box, which no descent decides) and
+// every junction split on a (the child toward the face; the last such plane,
+// the nearest to the face, is RopeFace::far), to the first junction that
+// cuts the face.  rope_exit_of takes a push only when the descent from the
+// rope provably passes through it.
+__global__ void k_rope_push(const uint4* __restrict__ rn, const uint32_t* __restrict__ up, uint32_t n_slots,
+                            SceneBox box, RopeSlot* __restrict__ rs) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots || up[s] == kLinkUnused || is_junction(rn[s].x)) return;
+    float lo[3], hi[3];
+    slot_cell(rs, s, lo, hi);
+    for (uint32_t f = 0; f < 6; f++) {
+        const uint32_t a = f >> 1;
+        uint32_t d = rs[s].face[f].rope;
+        float far = rs[s].face[f].far;
+        for (uint32_t step = 0; d != kRopeNone && step < 2 * kMaxFullStack; step++) {
+            const uint4 m = rn[d];
+            if (!is_junction(m.x)) break;
+            const uint32_t b = m.x & 3u;
+            const float v = __uint_as_float(m.w);
+            if (b == a) {  // max face: the ray enters the region above it, whose side toward the face is below
+                d = (f & 1u) ? m.z : m.y;
+                far = v;
+            } else if (v == box.mn[b] || v == box.mx[b]) {
+                break;
+            } else if (v >= hi[b]) {
+                d = m.z;
+            } else if (v <= lo[b]) {
+                d = m.y;
+            } else {
+                break;
+            }
+        }
+        rs[s].face[f].push = d;
+        rs[s].face[f].far = far;
+    }
 }
 
 hipError_t launch_build_ropes(const uint4* nodes, uint32_t n, uint32_t* up, uint4* rn, RopeSlot* rs,
@@ -1988,6 +2110,7 @@ hipError_t launch_build_ropes(const uint4* nodes, uint32_t n, uint32_t* up, uint
     hipLaunchKernelGGL(k_fill_u32, dim3((n_slots + 255) / 256), dim3(256), 0, s, up, (size_t)n_slots, kLinkUnused);
     hipLaunchKernelGGL(k_rope_links, dim3((n + 255) / 256), dim3(256), 0, s, nodes, n, up, rn);
     hipLaunchKernelGGL(k_rope_slots, dim3((n_slots + 255) / 256), dim3(256), 0, s, rn, up, n_slots, box, rs);
+    hipLaunchKernelGGL(k_rope_push, dim3((n_slots + 255) / 256), dim3(256), 0, s, rn, up, n_slots, box, rs);
     return hipGetLastError();
 }
 

@@ -22,12 +22,25 @@ constexpr uint32_t kHitFlagOverflow = 1u << 30;  // full stack deeper than the u
 constexpr uint32_t kHitFlagGuard = 1u << 31;     // iteration guard reached (malformed tree)
 
 // Rope slot: cell box + six ropes (face 2*axis + 0 = min, + 1 = max).
-struct RopeSlot {
-    float4 lo;  // cell min (w unused)
-    float4 hi;  // cell max
-    uint32_t rope[6];
-    uint32_t pad[2];
+// One face of a rope slot: the rope (the slot across the face), the pushed
+// rope (k_rope_push: a node below the rope whose cell holds the whole face)
+// and the plane on the face's axis that bounds the push (+-inf: none).
+struct RopeFace {
+    uint32_t rope;
+    uint32_t push;
+    float far;
+    uint32_t pad;
 };
+
+// 128 B, one cache line: the cell, then face f (2 axis + 0: min face, + 1:
+// max face), so that a rope exit reads its cell and its faces from one line.
+struct RopeSlot {
+    float lo[3];  // cell min
+    float hi[3];  // cell max
+    uint32_t pad[2];
+    RopeFace face[6];
+};
+static_assert(sizeof(RopeSlot) == 128, "rope slots are one 128-B line");
 
 // One frame of a batch: rays offset .. offset + W*H - 1 of the batch, pixel
 // index y*W + x inside the frame; scene: its entry of the batch's scene

@@ -184,6 +184,34 @@ def test_refill_cooperative_leaves(coop):
         gpu_ctx(s, nodes, lp, H.ROPES, pipeline=3, coop_lanes=65)
 
 
+@pytest.mark.parametrize("view", ["down", "outside", "inside"])
+def test_refill_pushed_ropes(view):
+    """The lane-refill rope traversal takes a leaf face's pushed rope only
+    where the descent from the plain rope provably passes through it.  Views
+    whose centre row or column of rays has exact zero direction components
+    (no push for those rays), from above the scene box, from outside its
+    side and from inside it (traversals from t = 0), with shadow rays and 2
+    mirror generations: pixels and primary hit records equal the oracle's
+    rope traversal (orc_render, orc_trace_camera), every bit."""
+    s, nodes, lp = scene_and_tree(3, 6, 5, 4)
+    from_, aa = {"down": ((0.0, 6.0, 0.0), (1.0, 0.0, 0.0, 90.0)),
+                 "outside": ((0.0, 0.5, 6.0), (0.0, 1.0, 0.0, 0.0)),
+                 "inside": ((0.25, 0.6, 0.0), (0.0, 1.0, 0.0, 0.0))}[view]
+    cam = H.make_camera(np.array(from_, np.float32), aa, 45.0, 1.0)
+    W = H_ = 128
+    rays = O.cast_rays(cam, W, H_)
+    assert (rays[:, 4:7] == 0).any(axis=1).sum() >= W  # rays the push never applies to
+    osc = oracle_scene(3, 6, 5, 4)
+    rimg, _ = osc.render(O.V_ROPES, cam, W, H_, max_bounce=2)
+    rhits = osc.trace_camera(O.V_ROPES, cam, W, H_)
+    for block, lb in ((64, 1), (256, 2)):
+        ctx = gpu_ctx(s, nodes, lp, H.ROPES, pipeline=3, block_size=block, leaf_batch=lb, coop_lanes=64)
+        img, hits = ctx.render_frame(cam, W, H_, max_bounce=2, mirror=s.mirror)
+        assert np.array_equal(img, rimg), f"block {block}: {(img != rimg).sum()} pixels differ"
+        bad = hit_mismatch(hits, rhits)
+        assert len(bad) == 0, describe(hits, rhits, bad)
+
+
 @pytest.mark.parametrize("n", [1, 63, 65, 1000, 40000])
 def test_persistent_ray_counts(n):
     """Persistent waves over ray buffers of ragged sizes (fewer rays than
