@@ -99,6 +99,7 @@ struct hulk_ctx {
     uint32_t level_prefix[kTopLevelsMax + 1] = {0};
     bool have_tree = false;
     bool ropes_ready = false;
+    bool ropes_pushed = false;  // k_rope_push has run on the current rope slots
     DevBuf rope_nodes, rope_slots, rope_links;
     DevBuf selftest;
     uint32_t n_slots = 0;
@@ -217,9 +218,26 @@ Camera to_cam(const hulk_camera& c) {
 }
 
 // Rope slots for the rope variant, built on the device (launch_build_ropes)
-// the first time a trace needs them after a tree upload.
+// the first time a trace needs them after a tree upload; their pushed
+// ropes (launch_push_ropes) on the second trace of the same tree, so that a
+// tree rebuilt for every frame does not pay the push build (tens of us) for
+// the few us per frame it saves.  The pushed targets are the plain ropes
+// until then.  Not inside a graph capture (the build would replay with the
+// graph): a later eager trace builds them in place, where the graph reads.
 int ensure_ropes(hulk_ctx* c) {
-    if (c->ropes_ready) return HULK_OK;
+    if (c->ropes_ready) {
+        if (!c->ropes_pushed) {
+            hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+            HIP_TRY(c, hipStreamIsCapturing(c->stream, &st));
+            if (st == hipStreamCaptureStatusNone) {
+                HIP_TRY(c, launch_push_ropes(static_cast<const uint4*>(c->rope_nodes.p),
+                                             static_cast<const uint32_t*>(c->rope_links.p), c->n_nodes,
+                                             static_cast<RopeSlot*>(c->rope_slots.p), c->box, c->stream));
+                c->ropes_pushed = true;
+            }
+        }
+        return HULK_OK;
+    }
     const uint32_t n_slots = 3 * c->n_nodes;
     int r;
     if ((r = ensure(c, c->rope_nodes, (size_t)n_slots * sizeof(uint4)))) return r;
@@ -230,6 +248,7 @@ int ensure_ropes(hulk_ctx* c) {
                                   static_cast<RopeSlot*>(c->rope_slots.p), c->box, c->stream));
     c->n_slots = n_slots;
     c->ropes_ready = true;
+    c->ropes_pushed = false;
     return HULK_OK;
 }
 
@@ -1631,6 +1650,11 @@ extern "C" int hulk_batch_create(hulk_ctx* c, const hulk_frame_desc* frames, uin
     b->lights_gen = c->lights_gen;
     // one eager run: builds ropes if needed and checks the launch shapes
     if ((r = batch_record(b, false))) {
+        hulk_batch_destroy(b);
+        return r;
+    }
+    // a batch is for replays: its rope traversals take pushed ropes from the start
+    if (b->opts.variant == HULK_VARIANT_ROPES && (r = ensure_ropes(c))) {
         hulk_batch_destroy(b);
         return r;
     }

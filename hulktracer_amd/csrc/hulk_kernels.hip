@@ -2002,6 +2002,7 @@ hipError_t launch_selftest_rcp(unsigned long long* out, hipStream_t s) {
 // walking up the parent links; faces no ancestor fixes keep the scene box
 // and kRopeNone.
 constexpr uint32_t kLinkRoot = 0xFFFFFFFFu, kLinkUnused = 0xFFFFFFFEu;
+constexpr uint32_t kRopePushLevels = 6;
 
 __global__ void k_rope_links(const uint4* __restrict__ nodes, uint32_t n, uint32_t* __restrict__ up,
                              uint4* __restrict__ rn) {
@@ -2026,7 +2027,8 @@ __global__ void k_rope_slots(const uint4* __restrict__ rn, const uint32_t* __res
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_slots) return;
     uint32_t link = up[s];
-    if (link == kLinkUnused) return;
+    // traversals read the slots of leaves only (rope exits, start leaves)
+    if (link == kLinkUnused || is_junction(rn[s].x)) return;
     float lo[3] = {box.mn[0], box.mn[1], box.mn[2]}, hi[3] = {box.mx[0], box.mx[1], box.mx[2]};
     uint32_t rope[6] = {kRopeNone, kRopeNone, kRopeNone, kRopeNone, kRopeNone, kRopeNone};
     uint32_t fixed = 0;  // bit f: face f (2a = min, 2a + 1 = max on axis a) already set by a deeper split
@@ -2054,7 +2056,7 @@ __global__ void k_rope_slots(const uint4* __restrict__ rn, const uint32_t* __res
         r.hi[a] = hi[a];
     }
     r.pad[0] = r.pad[1] = 0;
-    for (int k = 0; k < 6; k++) {  // pushes: k_rope_push
+    for (int k = 0; k < 6; k++) {  // pushed ropes: the ropes themselves until k_rope_push runs
         r.face[k].rope = r.face[k].push = rope[k];
         r.face[k].far = (k & 1) ? __builtin_inff() : -__builtin_inff();
         r.face[k].pad = 0;
@@ -2070,17 +2072,22 @@ __global__ void k_rope_slots(const uint4* __restrict__ rn, const uint32_t* __res
 // the nearest to the face, is RopeFace::far), to the first junction that
 // cuts the face.  rope_exit_of takes a push only when the descent from the
 // rope provably passes through it.
+// One thread per (slot, face): thread i handles face i % 6 of slot i / 6.
+// A push goes at most kRopePushLevels levels: the build is a chain of
+// dependent node loads, and on the 1M-triangle frames 6 levels keep 98 % of
+// the descent loads an unbounded push saves (a CPU walk of the rope rules).
 __global__ void k_rope_push(const uint4* __restrict__ rn, const uint32_t* __restrict__ up, uint32_t n_slots,
                             SceneBox box, RopeSlot* __restrict__ rs) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = (uint32_t)(i / 6u), f = (uint32_t)(i % 6u);
     if (s >= n_slots || up[s] == kLinkUnused || is_junction(rn[s].x)) return;
     float lo[3], hi[3];
     slot_cell(rs, s, lo, hi);
-    for (uint32_t f = 0; f < 6; f++) {
+    {
         const uint32_t a = f >> 1;
         uint32_t d = rs[s].face[f].rope;
         float far = rs[s].face[f].far;
-        for (uint32_t step = 0; d != kRopeNone && step < 2 * kMaxFullStack; step++) {
+        for (uint32_t step = 0; d != kRopeNone && step < kRopePushLevels; step++) {
             const uint4 m = rn[d];
             if (!is_junction(m.x)) break;
             const uint32_t b = m.x & 3u;
@@ -2110,7 +2117,14 @@ hipError_t launch_build_ropes(const uint4* nodes, uint32_t n, uint32_t* up, uint
     hipLaunchKernelGGL(k_fill_u32, dim3((n_slots + 255) / 256), dim3(256), 0, s, up, (size_t)n_slots, kLinkUnused);
     hipLaunchKernelGGL(k_rope_links, dim3((n + 255) / 256), dim3(256), 0, s, nodes, n, up, rn);
     hipLaunchKernelGGL(k_rope_slots, dim3((n_slots + 255) / 256), dim3(256), 0, s, rn, up, n_slots, box, rs);
-    hipLaunchKernelGGL(k_rope_push, dim3((n_slots + 255) / 256), dim3(256), 0, s, rn, up, n_slots, box, rs);
+    return hipGetLastError();
+}
+
+hipError_t launch_push_ropes(const uint4* rn, const uint32_t* up, uint32_t n, RopeSlot* rs, const SceneBox& box,
+                             hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint32_t n_slots = 3 * n;
+    hipLaunchKernelGGL(k_rope_push, dim3((6 * n_slots + 255) / 256), dim3(256), 0, s, rn, up, n_slots, box, rs);
     return hipGetLastError();
 }
 

@@ -34,6 +34,7 @@ struct RopeFace {
 
 // 128 B, one cache line: the cell, then face f (2 axis + 0: min face, + 1:
 // max face), so that a rope exit reads its cell and its faces from one line.
+// Written for leaf slots only (the slots traversals read).
 struct RopeSlot {
     float lo[3];  // cell min
     float hi[3];  // cell max
@@ -229,5 +230,8 @@ hipError_t launch_trace_streamed(const StreamParams& S, hipStream_t s);
 hipError_t launch_collect_requests(uint32_t* request, uint32_t n, uint32_t* list, uint32_t* n_list, hipStream_t s);
 hipError_t launch_build_ropes(const uint4* nodes, uint32_t n, uint32_t* up, uint4* rn, RopeSlot* rs,
                               const SceneBox& box, hipStream_t s);
+// Pushed ropes of the slots launch_build_ropes built (k_rope_push).
+hipError_t launch_push_ropes(const uint4* rn, const uint32_t* up, uint32_t n, RopeSlot* rs, const SceneBox& box,
+                             hipStream_t s);
 
 }  // namespace hulk

@@ -71,13 +71,17 @@ def test_config3_frame_matches_oracle(frame, opts):
     try:
         img, hits = ctx.render_frame(cam, W, Hh, max_bounce=2, mirror=s.mirror, count=True)
         cnt = ctx.counters()
+        # the uncounted kernel, on the context's second trace of this tree
+        # (ropes: with the pushed ropes built)
+        img2, hits2 = ctx.render_frame(cam, W, Hh, max_bounce=2, mirror=s.mirror)
     finally:
         ctx.close()
     rimg, rcounts, rhits = _oracle_frame(frame, ORACLE_OF[variant])
-    bad = hit_mismatch(hits, rhits)
-    assert len(bad) == 0, f"{len(bad)} primary hit records differ\n{describe(hits, rhits, bad)}"
-    diff = img != rimg
-    assert not diff.any(), f"{int(diff.sum())} of {W * Hh} pixels differ, first at {np.nonzero(diff)[0][:5]}"
+    for im, ht in ((img, hits), (img2, hits2)):
+        bad = hit_mismatch(ht, rhits)
+        assert len(bad) == 0, f"{len(bad)} primary hit records differ\n{describe(ht, rhits, bad)}"
+        diff = im != rimg
+        assert not diff.any(), f"{int(diff.sum())} of {W * Hh} pixels differ, first at {np.nonzero(diff)[0][:5]}"
     # nearest-hit rays: primary plus both mirror generations, as the oracle counts them
     assert cnt["rays"] == int(rcounts[0]), (cnt["rays"], int(rcounts[0]))
     assert cnt["rays"] > W * Hh  # the mirror generations traced rays
