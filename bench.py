@@ -30,10 +30,11 @@ The roofline object prices the dominant kernel (the batched launch) against
 the resource its counters show closest to saturation (rocprofv3 summaries in
 profiles/, matched to the exact kernel, options and workload): VALU issue
 (wave64 instructions per CU per cycle) and HBM bytes, each divided by the
-live HIP-event time of the launch measured here.  The SURVEY algorithmic
-bytes per ray (32 + 16 + 16 per node visit + 64 per triangle test, counted by
-a full-stack pass) are reported as a cache-served byte rate, not as an HBM
-fraction: the scene stays in L2 / MALL.
+live HIP-event time of the launch measured here.  The algorithmic bytes per
+ray, after the reference's traversal counters (Statistics.hpp:58-60,112-120:
+32 + 16 + 16 per node visit + 64 per triangle test, counted by a full-stack
+pass), are reported as a cache-served byte rate, not as an HBM fraction: the
+scene stays in L2 / MALL.
 
 The CPU baseline is the oracle's port of the reference CPU path
 (GLVisitorCPUTrace::traceScene + KDTreeCPU::checkCollision, OpenMP over rows)
