@@ -284,7 +284,8 @@ int hulk_render_frame(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uin
  * keeps the last call's contents: calls that overlap on one GPU (two
  * streams, two threads) race.  With the rope variant the tree's rope slots
  * must already exist (any earlier rope trace on the context builds them),
- * else HULK_E_UNSUPPORTED. */
+ * else HULK_E_UNSUPPORTED; the call may launch the build of their pushed
+ * ropes (a kernel, no allocation), which a tree's second rope trace runs. */
 #define HULK_STATIC_MAX_RAYS 262144u
 int hulk_render_frame_static(hulk_ctx* ctx, const hulk_camera* cam, uint32_t width, uint32_t height,
                              uint32_t max_bounce, const uint8_t* mirror_mask_dev, const float* tbound_dev,
