@@ -56,6 +56,33 @@ def test_reference_model_trace_and_shade(name, variant):
 
 
 @pytest.mark.parametrize("name", sorted(VIEWS))
+@pytest.mark.parametrize("inside", [False, True])
+def test_reference_model_refill_ropes(name, inside):
+    """The lane-refill rope kernel on the reference's models, from the view
+    above and from inside the model's box (the box model's tree splits on
+    planes of the scene box, which the pushed ropes never pass): the first
+    trace of the tree on the plain ropes, the second on the pushed ones;
+    pixels (with shadow rays) and primary hits equal the oracle's."""
+    s, nodes, lp, cam, osc = _setup(name, 0.7)
+    if inside:
+        b = s.box
+        c = [(b[k] + b[4 + k]) * 0.5 for k in range(3)]
+        cam = H.make_camera_lookat(c, (b[4], b[1] + 0.25 * (b[5] - b[1]), b[6]), fov=70.0, aspect=1.0)
+    print(f"reference model {name} refill ropes inside={inside}", flush=True)
+    ctx = H.GPUTrace(variant=H.ROPES, pipeline=3, coop_lanes=64, block_size=64)
+    ctx.upload_scene(s)
+    ctx.upload_kdtree(nodes, lp)
+    rimg, _ = osc.render(O.V_ROPES, cam, 192, 192, max_bounce=0)
+    rhits = osc.trace_camera(O.V_ROPES, cam, 192, 192)
+    for k in range(2):
+        img, hits = ctx.render_frame(cam, 192, 192)
+        bad = hit_mismatch(hits, rhits)
+        assert len(bad) == 0, f"trace {k}: {len(bad)} rays differ\n{describe(hits, rhits, bad)}"
+        assert np.array_equal(img, rimg), f"trace {k}: {(img != rimg).sum()} pixels differ"
+    assert (hits["tri"] != H.NO_HIT).sum() > 1000
+
+
+@pytest.mark.parametrize("name", sorted(VIEWS))
 def test_reference_model_device_build(name):
     s, nodes, lp, cam, osc = _setup(name, 0.7)
     ctx = H.GPUTrace(variant=H.FULLSTACK)
