@@ -47,6 +47,41 @@ _TUNE_KEYS = ("variant", "block_size", "short_stack_depth", "lds_top_levels", "p
               "leaf_batch", "coop_lanes")
 
 
+def batch_candidates(table, top: int = 6, prefer=()) -> list:
+    """The shapes a batch stage times: the `top` fastest distinct shapes of
+    a per-frame autotune table, at most half of them from one frame pipeline
+    (then the fastest of the rest), since a pipeline that wins single frames
+    (lane refill) can fill the table's top while another wins the batch;
+    then every shape in `prefer`."""
+    seen, cands = set(), []
+    rows = sorted(table, key=lambda r: r["ms"])
+    per, cap = {}, max(1, (top + 1) // 2)
+    for row in rows:
+        k = tuple(int(row[f]) for f in _TUNE_KEYS)
+        pipe = int(row["pipeline"])
+        if k in seen or per.get(pipe, 0) >= cap:
+            continue
+        seen.add(k)
+        cands.append(k)
+        per[pipe] = per.get(pipe, 0) + 1
+        if len(cands) == top:
+            break
+    for row in rows:
+        if len(cands) >= top:
+            break
+        k = tuple(int(row[f]) for f in _TUNE_KEYS)
+        if k not in seen:
+            seen.add(k)
+            cands.append(k)
+    for k in sorted(map(tuple, prefer)):  # preferred shapes are always timed too
+        if k not in seen:
+            seen.add(k)
+            cands.append(k)
+    if not cands:
+        raise ValueError("empty autotune table")
+    return cands
+
+
 def tune_batch(ctx: GPUTrace, frames, mirror, rgba_dev: int, table, stream, top: int = 6, reps: int = 5,
                rounds: int = 2, prefer=(), tol: float = 0.01):
     """Batch stage of the autotuner: the `top` fastest distinct shapes of a
@@ -59,20 +94,7 @@ def tune_batch(ctx: GPUTrace, frames, mirror, rgba_dev: int, table, stream, top:
     of the option values in _TUNE_KEYS order) is taken first.  Returns
     (options dict, {shape: ms})."""
     torch = _torch()
-    seen, cands = set(), []
-    for row in sorted(table, key=lambda r: r["ms"]):
-        k = tuple(int(row[f]) for f in _TUNE_KEYS)
-        if k not in seen:
-            seen.add(k)
-            cands.append(k)
-        if len(cands) == top:
-            break
-    for k in sorted(map(tuple, prefer)):  # preferred shapes are always timed too
-        if k not in seen:
-            seen.add(k)
-            cands.append(k)
-    if not cands:
-        raise ValueError("empty autotune table")
+    cands = batch_candidates(table, top, prefer)
     batches = []
     for k in cands:
         ctx.set_options(**dict(zip(_TUNE_KEYS, k)))
