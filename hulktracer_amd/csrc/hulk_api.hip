@@ -420,6 +420,7 @@ int trace_primary(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, 
         P.W = fs.W;
         P.w_m = udiv_multiplier(fs.W);
         P.tx_m = udiv_multiplier(fs.W >> 3);
+        P.ty_m = udiv_multiplier(fs.H >> 3);
         P.fW = static_cast<float>(fs.W);
         P.fH = static_cast<float>(fs.H);
         P.H = fs.H;
@@ -453,6 +454,7 @@ int render_refill(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, 
         P.W = fs.W;
         P.w_m = udiv_multiplier(fs.W);
         P.tx_m = udiv_multiplier(fs.W >> 3);
+        P.ty_m = udiv_multiplier(fs.H >> 3);
         P.fW = static_cast<float>(fs.W);
         P.fH = static_cast<float>(fs.H);
         P.H = fs.H;
@@ -497,6 +499,7 @@ int render_fused(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, c
         P.W = fs.W;
         P.w_m = udiv_multiplier(fs.W);
         P.tx_m = udiv_multiplier(fs.W >> 3);
+        P.ty_m = udiv_multiplier(fs.H >> 3);
         P.fW = static_cast<float>(fs.W);
         P.fH = static_cast<float>(fs.H);
         P.H = fs.H;
@@ -1611,6 +1614,7 @@ extern "C" int hulk_batch_create(hulk_ctx* c, const hulk_frame_desc* frames, uin
         fd.max_bounce = d.max_bounce;
         fd.w_m = udiv_multiplier(d.width);
         fd.tx_m = udiv_multiplier(d.width >> 3);
+        fd.ty_m = udiv_multiplier(d.height >> 3);
         fd.fW = static_cast<float>(d.width);
         fd.fH = static_cast<float>(d.height);
         b->frames.push_back(fd);
@@ -1783,6 +1787,7 @@ extern "C" int hulk_batch_create_scenes(hulk_ctx* const* ctxs, uint32_t n_ctxs, 
         fd.scene = d.reserved;
         fd.w_m = udiv_multiplier(d.width);
         fd.tx_m = udiv_multiplier(d.width >> 3);
+        fd.ty_m = udiv_multiplier(d.height >> 3);
         fd.fW = static_cast<float>(d.width);
         fd.fH = static_cast<float>(d.height);
         b->frames.push_back(fd);

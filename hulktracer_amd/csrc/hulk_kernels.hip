@@ -102,8 +102,15 @@ __device__ __forceinline__ uint32_t udiv(uint32_t a, uint32_t d, uint32_t m) {
 // tile_pixel with the divisions by W and W / 8 done by host multipliers
 // (camera launches: W is the same for every ray, and the multipliers stay
 // kernel arguments instead of per-wave reciprocals held in registers).
+// STRIPS: the tiles in vertical strips of 16 tiles (128 pixels), row after
+// row within a strip, strip after strip, where the tile columns fill whole
+// strips (ty_m: the multiplier of H / 8).  The rays in flight on one XCD
+// (consecutive ray indices) then cover a region about as tall as wide
+// instead of a band across the image, so that they share more of the tree
+// and the triangle records in that XCD's L2.
+template <bool STRIPS = false>
 __device__ __forceinline__ void tile_pixel_m(uint32_t i, uint32_t W, uint32_t H, uint32_t w_m, uint32_t tx_m,
-                                             uint32_t& x, uint32_t& y) {
+                                             uint32_t ty_m, uint32_t& x, uint32_t& y) {
     if (((W | H) & 7u) == 0) {
         // the lane's place in its tile, recomputed per ray: hoisted out of
         // the ray loop it would be one more register held (and spilled)
@@ -111,7 +118,17 @@ __device__ __forceinline__ void tile_pixel_m(uint32_t i, uint32_t W, uint32_t H,
         uint32_t k = i & 63u;
         asm volatile("" : "+v"(k));
         const uint32_t tile = i >> 6, tiles_x = W >> 3;
-        const uint32_t ty = udiv(tile, tiles_x, tx_m), tx = tile - ty * tiles_x;
+        uint32_t tx, ty;
+        if (STRIPS && (tiles_x & 15u) == 0u) {
+            const uint32_t tiles_y = H >> 3;
+            const uint32_t strip = udiv(tile >> 4, tiles_y, ty_m);
+            const uint32_t r = tile - strip * (tiles_y << 4);
+            ty = r >> 4;
+            tx = (strip << 4) | (r & 15u);
+        } else {
+            ty = udiv(tile, tiles_x, tx_m);
+            tx = tile - ty * tiles_x;
+        }
         x = (tx << 3) | (k & 7u);
         y = (ty << 3) | (k >> 3);
     } else {
@@ -123,9 +140,10 @@ __device__ __forceinline__ void tile_pixel_m(uint32_t i, uint32_t W, uint32_t H,
 // Ray i of frame fr: pixel index (frame offset + y*W + x) returned, 8x8
 // tiles inside the frame when its sides are multiples of 8, the divisions
 // by W and W / 8 done by the frame's multipliers.
+template <bool STRIPS = false>
 __device__ __forceinline__ uint32_t frame_pixel_ray(const FrameDev& fr, uint32_t i, Ray& r) {
     uint32_t x, y;
-    tile_pixel_m(i - fr.offset, fr.W, fr.H, fr.w_m, fr.tx_m, x, y);
+    tile_pixel_m<STRIPS>(i - fr.offset, fr.W, fr.H, fr.w_m, fr.tx_m, fr.ty_m, x, y);
     cast_ray_f(fr.cam, x, y, fr.fW, fr.fH, r);
     return fr.offset + y * fr.W + x;
 }
@@ -134,16 +152,17 @@ __device__ __forceinline__ uint32_t frame_pixel_ray(const FrameDev& fr, uint32_t
 // rays, nearly always of one frame: the frame of the first active lane's ray
 // is searched once for the wave (wave-uniform, so scalar loads), and the
 // lanes search their own frames only when some ray lies outside it.
+template <bool STRIPS = false>
 __device__ __forceinline__ uint32_t frame_ray_tiled(const FrameDev* F, uint32_t n, uint32_t i, Ray& r,
                                                     uint32_t& f) {
     const uint32_t f0 = find_frame(F, n, __builtin_amdgcn_readfirstlane(i));
     const uint32_t lo = F[f0].offset, hi = f0 + 1u < n ? F[f0 + 1u].offset : 0xFFFFFFFFu;
     if (__ballot(i < lo || i >= hi) == 0ull) {
         f = f0;
-        return frame_pixel_ray(F[f0], i, r);
+        return frame_pixel_ray<STRIPS>(F[f0], i, r);
     }
     f = find_frame(F, n, i);
-    return frame_pixel_ray(F[f], i, r);
+    return frame_pixel_ray<STRIPS>(F[f], i, r);
 }
 
 __device__ __forceinline__ uint32_t frame_ray(const FrameDev* F, uint32_t n, uint32_t i, Ray& r) {
@@ -1184,7 +1203,7 @@ k_trace(TraceParams P) {
             float tstop = kNoStop;
             if (P.source == kSourceCamera) {
                 uint32_t x, y;
-                tile_pixel_m(i, P.W, P.H, P.w_m, P.tx_m, x, y);
+                tile_pixel_m(i, P.W, P.H, P.w_m, P.tx_m, P.ty_m, x, y);
                 o = y * P.W + x;
                 cast_ray_f(P.cam, x, y, P.fW, P.fH, r);
             } else if (P.source == kSourceFrames) {
@@ -1394,12 +1413,12 @@ k_render_refill(TraceParams P) {
                 uint32_t o, mb = P.max_bounce;
                 if (P.source == kSourceCamera) {
                     uint32_t x, y;
-                    tile_pixel_m(i, P.W, P.H, P.w_m, P.tx_m, x, y);
+                    tile_pixel_m<true>(i, P.W, P.H, P.w_m, P.tx_m, P.ty_m, x, y);
                     o = y * P.W + x;
                     cast_ray_f(P.cam, x, y, P.fW, P.fH, tr);
                 } else {
                     uint32_t f;
-                    o = frame_ray_tiled(P.frames, P.n_frames, i, tr, f);
+                    o = frame_ray_tiled<true>(P.frames, P.n_frames, i, tr, f);
                     mb = P.frames[f].max_bounce;
                 }
                 set_inv(tr);

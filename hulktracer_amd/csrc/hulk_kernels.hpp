@@ -50,7 +50,7 @@ struct FrameDev {
     Camera cam;
     uint32_t W, H, offset, max_bounce;
     uint32_t scene;
-    uint32_t w_m, tx_m;  // udiv_multiplier(W), udiv_multiplier(W / 8)
+    uint32_t w_m, tx_m, ty_m;  // udiv_multiplier(W), udiv_multiplier(W / 8), udiv_multiplier(H / 8)
     float fW, fH;        // W and H as floats
 };
 
@@ -90,7 +90,7 @@ struct TraceParams {
     uint32_t source;
     Camera cam;
     uint32_t W, H;
-    uint32_t w_m, tx_m;  // camera source: udiv multipliers of W and W / 8 (udiv_multiplier)
+    uint32_t w_m, tx_m, ty_m;  // camera source: udiv multipliers of W, W / 8 and H / 8 (udiv_multiplier)
     float fW, fH;        // camera source: (float)W, (float)H
     const hulk_ray* rays;
     const FrameDev* frames;
