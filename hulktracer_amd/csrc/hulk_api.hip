@@ -398,6 +398,7 @@ int ensure_work(hulk_ctx* c, Work& w, uint32_t n, uint32_t np, bool bounces, boo
 struct FrameSet {
     const FrameDev* frames = nullptr;
     uint32_t n_frames = 0;
+    uint32_t frame_w = 0, frame_h = 0;  // frames: their size when every frame has the same, else 0
     Camera cam{};
     uint32_t W = 0, H = 0;
     uint32_t n = 0;
@@ -448,6 +449,14 @@ int render_refill(hulk_ctx* c, const hulk_opts& o, const FrameSet& fs, Work& w, 
         P.source = kSourceFrames;
         P.frames = fs.frames;
         P.n_frames = fs.n_frames;
+        // frames of one size whose tile columns fill whole strips: ray order
+        // strip-major across the frames (frame_ray_xstrips)
+        const uint32_t fw = fs.frame_w, fh = fs.frame_h;
+        if (fw && fh && ((fw | fh) & 7u) == 0 && ((fw >> 3) & 15u) == 0) {
+            P.xs_rps = (fw * fh) / ((fw >> 3) >> 4);
+            P.xs_rps_m = udiv_multiplier(P.xs_rps);
+            P.xs_nf_m = udiv_multiplier(fs.n_frames);
+        }
     } else {
         P.source = kSourceCamera;
         P.cam = fs.cam;
@@ -1578,6 +1587,10 @@ int batch_record(hulk_batch* b, bool count) {
     fs.frames = static_cast<const FrameDev*>(b->d_frames.p);
     fs.n_frames = static_cast<uint32_t>(b->frames.size());
     fs.n = b->n;
+    fs.frame_w = b->frames[0].W;
+    fs.frame_h = b->frames[0].H;
+    for (const FrameDev& fd : b->frames)
+        if (fd.W != fs.frame_w || fd.H != fs.frame_h) fs.frame_w = fs.frame_h = 0;
     int r;
     hulk_hit* h0 = static_cast<hulk_hit*>(b->w.hits.p);
     const uint8_t* dm = b->max_bounce ? static_cast<const uint8_t*>(b->d_mirror.p) : nullptr;

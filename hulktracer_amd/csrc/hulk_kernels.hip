@@ -165,6 +165,32 @@ __device__ __forceinline__ uint32_t frame_ray_tiled(const FrameDev* F, uint32_t 
     return frame_pixel_ray<STRIPS>(F[f], i, r);
 }
 
+// Batched primary ray i of frames of one size (P.xs_rps rays per strip of
+// 16 tile columns): strip-major across the frames, strip s of every frame
+// before strip s + 1, tile rows within a strip.  An XCD's slice of the ray
+// indices then holds the same strips of all frames, so its L2 keeps serving
+// the geometry those strips see.  The frame of the wave's first active lane
+// is read with scalar loads when every lane shares it.
+__device__ __forceinline__ uint32_t frame_ray_xstrips(const TraceParams& P, uint32_t i, Ray& r, uint32_t& f) {
+    const uint32_t rps = P.xs_rps, nf = P.n_frames;
+    const uint32_t sg = udiv(i, rps, P.xs_rps_m);
+    const uint32_t strip = udiv(sg, nf, P.xs_nf_m);
+    f = sg - strip * nf;
+    uint32_t k = i - sg * rps;  // ray within the strip
+    const uint32_t t = k >> 6;
+    k &= 63u;
+    const uint32_t x = (((strip << 4) | (t & 15u)) << 3) | (k & 7u), y = ((t >> 4) << 3) | (k >> 3);
+    const uint32_t f0 = __builtin_amdgcn_readfirstlane(f);
+    if (__ballot(f != f0) == 0ull) {
+        const FrameDev& fr = P.frames[f0];
+        cast_ray_f(fr.cam, x, y, fr.fW, fr.fH, r);
+        return fr.offset + y * fr.W + x;
+    }
+    const FrameDev& fr = P.frames[f];
+    cast_ray_f(fr.cam, x, y, fr.fW, fr.fH, r);
+    return fr.offset + y * fr.W + x;
+}
+
 __device__ __forceinline__ uint32_t frame_ray(const FrameDev* F, uint32_t n, uint32_t i, Ray& r) {
     const uint32_t f = find_frame(F, n, i);
     const FrameDev& fr = F[f];
@@ -1418,7 +1444,8 @@ k_render_refill(TraceParams P) {
                     cast_ray_f(P.cam, x, y, P.fW, P.fH, tr);
                 } else {
                     uint32_t f;
-                    o = frame_ray_tiled<true>(P.frames, P.n_frames, i, tr, f);
+                    o = P.xs_rps ? frame_ray_xstrips(P, i, tr, f)
+                                 : frame_ray_tiled<true>(P.frames, P.n_frames, i, tr, f);
                     mb = P.frames[f].max_bounce;
                 }
                 set_inv(tr);

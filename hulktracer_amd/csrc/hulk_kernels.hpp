@@ -95,6 +95,10 @@ struct TraceParams {
     const hulk_ray* rays;
     const FrameDev* frames;
     uint32_t n_frames;
+    // frames of one size (lane-refill kernel): rays per strip of 16 tile
+    // columns (0: the frames' own order) and the multipliers of it and of
+    // n_frames
+    uint32_t xs_rps, xs_rps_m, xs_nf_m;
     const uint32_t* n_dev;
     uint32_t n;
     const float* tbound;

@@ -72,6 +72,33 @@ def test_batch_matches_oracle_two_scenes(pipeline):
         b.close()
 
 
+@pytest.mark.parametrize("size", [(128, 128), (256, 128)])
+@pytest.mark.parametrize("mb", [0, 2])
+def test_refill_batch_frames_of_one_size(size, mb):
+    """A lane-refill batch whose frames share one size takes its rays strip
+    by strip across the frames (one or two strips of 16 tile columns per
+    frame here): every frame's pixels equal the oracle's render."""
+    import torch
+
+    s, nodes, lp = scene_and_tree(3, 6, 5, 4)
+    w, h = size
+    ctx = gpu_ctx(s, nodes, lp, H.ROPES, pipeline=3, coop_lanes=64, block_size=64)
+    frames = _frames(s, [(p, w, h, mb) for p in (2, 5, 9)])
+    total = 3 * w * h
+    rgba = torch.zeros(total, dtype=torch.int32, device="cuda")
+    b = ctx.batch(frames, s.mirror, rgba.data_ptr(), use_graph=True)
+    osc = oracle_scene(3, 6, 5, 4)
+    for rep in range(2):
+        rgba.zero_()
+        b.launch()
+        got = rgba.cpu().numpy().view(np.uint32)
+        for k, (cam, fw, fh, fmb) in enumerate(frames):
+            ref, _ = osc.render(O.V_ROPES, cam, fw, fh, max_bounce=fmb)
+            part = got[k * w * h:(k + 1) * w * h]
+            assert np.array_equal(part, ref), f"replay {rep} frame {k}: {(part != ref).sum()} pixels differ"
+    b.close()
+
+
 def test_autotune_picks_a_parity_clean_config():
     s, nodes, lp = scene_and_tree(2, 1)
     ctx = gpu_ctx(s, nodes, lp, H.FULLSTACK)
