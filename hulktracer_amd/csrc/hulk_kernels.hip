@@ -1332,8 +1332,10 @@ k_trace(TraceParams P) {
 // fallback, hit record, shading, pixel), so pixels, hit records and counters
 // are identical; only the assignment of rays to lanes differs.  Waves take
 // rays from 8 per-XCD slices of the ray range, a count of idle lanes per
-// atomic, refilling once at least kRefillMin lanes are idle.
-constexpr uint32_t kRefillMin = 16;
+// atomic, refilling once at least kRefillMin lanes are idle (28 of 64:
+// with the strip ray orders, fewer and larger refills ran the 1M batch 2 %
+// faster than 16 at the same config-3 frame time; 8 and 32 slower).
+constexpr uint32_t kRefillMin = 28;
 // Lights staged in the refill kernel's LDS (2 float4 each) when they fit.
 constexpr uint32_t kRefillLdsLights = 64;
 
